@@ -1,0 +1,250 @@
+"""ctypes mirror of include/stp_capi.h and the loader of the HIP library.
+
+The structs below must match include/stp_capi.h field for field.  The library
+is loaded from fastimaging_amd/lib/libstp_hip.so (built in-tree); a missing
+library raises instead of falling back to any other implementation.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libstp_hip.so")
+
+# enums (src/stp/types.h:32-88)
+KERNEL_TOPHAT, KERNEL_TRIANGLE, KERNEL_SINC, KERNEL_GAUSSIAN, KERNEL_GAUSSIANSINC, KERNEL_PSWF = range(6)
+MEDIAN_ZEROMEDIAN, MEDIAN_BINMEDIAN, MEDIAN_BINAPPROX, MEDIAN_NTHELEMENT = range(4)
+PRECISION_FP64, PRECISION_FP32C = 0, 1
+
+STATUS_NAMES = {
+    0: "STP_OK", 1: "STP_ERR_INVALID_ARGUMENT", 2: "STP_ERR_UNSUPPORTED", 3: "STP_ERR_DEVICE",
+    4: "STP_ERR_OUT_OF_MEMORY", 5: "STP_ERR_INTERNAL", 6: "STP_ERR_CAPACITY",
+}
+
+
+class ImagerPars(ctypes.Structure):
+    """stp::ImagerPars (src/stp/types.h:93-173)."""
+    _fields_ = [
+        ("image_size", ctypes.c_uint32),
+        ("padded_image_size", ctypes.c_uint32),
+        ("cell_size", ctypes.c_double),
+        ("padding_factor", ctypes.c_double),
+        ("kernel_function", ctypes.c_int32),
+        ("kernel_support", ctypes.c_uint32),
+        ("kernel_exact", ctypes.c_int32),
+        ("oversampling", ctypes.c_uint32),
+        ("generate_beam", ctypes.c_int32),
+        ("gridding_correction", ctypes.c_int32),
+        ("analytic_gcf", ctypes.c_int32),
+        ("r_fft", ctypes.c_int32),
+    ]
+
+
+class WProjPars(ctypes.Structure):
+    """stp::W_ProjectionPars (src/stp/types.h:178-249)."""
+    _fields_ = [
+        ("num_wplanes", ctypes.c_uint32),
+        ("max_wpconv_support", ctypes.c_uint32),
+        ("undersampling_opt", ctypes.c_uint32),
+        ("kernel_trunc_perc", ctypes.c_double),
+        ("hankel_opt", ctypes.c_int32),
+        ("hankel_proj_slice", ctypes.c_int32),
+        ("interp_type", ctypes.c_int32),
+        ("wplanes_median", ctypes.c_int32),
+    ]
+
+
+class AProjPars(ctypes.Structure):
+    _fields_ = [
+        ("num_timesteps", ctypes.c_uint32),
+        ("obs_dec", ctypes.c_double),
+        ("obs_ra", ctypes.c_double),
+        ("aproj_opt", ctypes.c_int32),
+        ("aproj_mask_perc", ctypes.c_double),
+    ]
+
+
+class SourceFindPars(ctypes.Structure):
+    """Arguments of stp::SourceFindImage (src/stp/sourcefind/sourcefind.h:188-201)."""
+    _fields_ = [
+        ("detection_n_sigma", ctypes.c_double),
+        ("analysis_n_sigma", ctypes.c_double),
+        ("rms_est", ctypes.c_double),
+        ("find_negative_sources", ctypes.c_int32),
+        ("sigma_clip_iters", ctypes.c_int32),
+        ("median_method", ctypes.c_int32),
+        ("gaussian_fitting", ctypes.c_int32),
+        ("ccl_4connectivity", ctypes.c_int32),
+        ("generate_labelmap", ctypes.c_int32),
+        ("source_min_area", ctypes.c_int32),
+    ]
+
+
+class BoundingBox(ctypes.Structure):
+    _fields_ = [("top", ctypes.c_int32), ("bottom", ctypes.c_int32), ("left", ctypes.c_int32), ("right", ctypes.c_int32)]
+
+
+class Gaussian2d(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("amplitude", "x_centre", "y_centre", "semimajor", "semiminor", "theta")]
+
+
+class Island(ctypes.Structure):
+    """stp::IslandParams (src/stp/sourcefind/sourcefind.h:63-133)."""
+    _fields_ = [
+        ("label_idx", ctypes.c_int32),
+        ("sign", ctypes.c_int32),
+        ("extremum_y_idx", ctypes.c_int32),
+        ("extremum_x_idx", ctypes.c_int32),
+        ("num_samples", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("extremum_val", ctypes.c_double),
+        ("bounding_box", BoundingBox),
+        ("moments_fit", Gaussian2d),
+    ]
+
+
+class SourceFindStats(ctypes.Structure):
+    _fields_ = [
+        ("bg_level", ctypes.c_double),
+        ("rms_est", ctypes.c_double),
+        ("mean", ctypes.c_double),
+        ("sigma", ctypes.c_double),
+        ("num_labels_pos", ctypes.c_uint32),
+        ("num_labels_neg", ctypes.c_uint32),
+        ("num_valid_labels", ctypes.c_uint32),
+        ("num_islands", ctypes.c_uint32),
+    ]
+
+
+class ImagerDiag(ctypes.Structure):
+    _fields_ = [
+        ("sample_grid_total", ctypes.c_double),
+        ("num_good_vis", ctypes.c_uint64),
+        ("image_rows", ctypes.c_uint32),
+        ("image_cols", ctypes.c_uint32),
+        ("ms_total", ctypes.c_double),
+    ]
+
+
+def imager_pars(image_size, cell_size, kernel_function=KERNEL_GAUSSIANSINC, kernel_support=3,
+                kernel_exact=False, oversampling=9, generate_beam=True, gridding_correction=False,
+                analytic_gcf=False, padding_factor=1.0):
+    p = ImagerPars()
+    p.image_size = image_size
+    p.padding_factor = padding_factor
+    p.padded_image_size = int(image_size * padding_factor)
+    p.cell_size = cell_size
+    p.kernel_function = kernel_function
+    p.kernel_support = kernel_support
+    p.kernel_exact = int(kernel_exact)
+    p.oversampling = oversampling
+    p.generate_beam = int(generate_beam)
+    p.gridding_correction = int(gridding_correction)
+    p.analytic_gcf = int(analytic_gcf)
+    p.r_fft = 0
+    return p
+
+
+def wproj_pars(num_wplanes=0, max_wpconv_support=0, undersampling_opt=1, kernel_trunc_perc=0.0,
+               wplanes_median=False):
+    p = WProjPars()
+    p.num_wplanes = num_wplanes
+    p.max_wpconv_support = max_wpconv_support
+    p.undersampling_opt = undersampling_opt
+    p.kernel_trunc_perc = kernel_trunc_perc
+    p.hankel_opt = 0
+    p.hankel_proj_slice = 0
+    p.interp_type = 0
+    p.wplanes_median = int(wplanes_median)
+    return p
+
+
+def sourcefind_pars(detection_n_sigma, analysis_n_sigma, rms_est=0.0, find_negative_sources=True,
+                    sigma_clip_iters=5, median_method=MEDIAN_BINAPPROX, gaussian_fitting=False,
+                    ccl_4connectivity=False, generate_labelmap=True, source_min_area=5):
+    """Defaults of the stp::SourceFindImage constructor (sourcefind.h:188-201)."""
+    p = SourceFindPars()
+    p.detection_n_sigma = detection_n_sigma
+    p.analysis_n_sigma = analysis_n_sigma
+    p.rms_est = rms_est
+    p.find_negative_sources = int(find_negative_sources)
+    p.sigma_clip_iters = sigma_clip_iters
+    p.median_method = median_method
+    p.gaussian_fitting = int(gaussian_fitting)
+    p.ccl_4connectivity = int(ccl_4connectivity)
+    p.generate_labelmap = int(generate_labelmap)
+    p.source_min_area = source_min_area
+    return p
+
+
+def island_tuple(isl):
+    """Plain-Python view of an Island record."""
+    g = isl.moments_fit
+    return dict(label_idx=isl.label_idx, sign=isl.sign, extremum_y_idx=isl.extremum_y_idx,
+                extremum_x_idx=isl.extremum_x_idx, num_samples=isl.num_samples,
+                extremum_val=isl.extremum_val,
+                moments_fit=(g.amplitude, g.x_centre, g.y_centre, g.semimajor, g.semiminor, g.theta))
+
+
+_lib = None
+
+# exported symbols of the HIP library, with (restype, argtypes)
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_SIG = {
+    "stp_ctx_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int]),
+    "stp_ctx_destroy": (ctypes.c_int, [_P]),
+    "stp_ctx_last_error": (ctypes.c_char_p, [_P]),
+    "stp_ctx_synchronize": (ctypes.c_int, [_P]),
+    "stp_ctx_release_workspace": (ctypes.c_int, [_P]),
+    "stp_ctx_workspace_bytes": (ctypes.c_size_t, [_P]),
+    "stp_imager_pars_init": (None, [ctypes.POINTER(ImagerPars), ctypes.c_uint32, ctypes.c_double]),
+    "stp_wproj_pars_init": (None, [ctypes.POINTER(WProjPars)]),
+    "stp_sourcefind_pars_init": (None, [ctypes.POINTER(SourceFindPars), ctypes.c_double, ctypes.c_double]),
+    "stp_image_visibilities": (ctypes.c_int, [_P, ctypes.POINTER(ImagerPars), ctypes.POINTER(WProjPars),
+                                              ctypes.POINTER(AProjPars), ctypes.c_size_t, _P, _P, _P,
+                                              ctypes.c_int32, _P, _P, ctypes.POINTER(ImagerDiag)]),
+    "stp_image_visibilities_device": (ctypes.c_int, [_P, ctypes.POINTER(ImagerPars), ctypes.POINTER(WProjPars),
+                                                     ctypes.POINTER(AProjPars), ctypes.c_size_t, _P, _P, _P,
+                                                     ctypes.c_int32, _P, _P, ctypes.c_int32,
+                                                     ctypes.POINTER(ImagerDiag)]),
+    "stp_convolve_to_grid": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double, ctypes.c_uint32, ctypes.c_int32,
+                                            ctypes.c_size_t, _P, _P, _P, ctypes.c_int32, ctypes.c_uint32,
+                                            ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, _P, _D]),
+    "stp_fft_c2r": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, _P]),
+    "stp_fft_c2c": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, _P]),
+    "stp_source_find": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(SourceFindPars), _P,
+                                       ctypes.POINTER(Island), ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
+                                       ctypes.POINTER(SourceFindStats)]),
+    "stp_source_find_device": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.POINTER(SourceFindPars), _P, ctypes.POINTER(Island),
+                                              ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
+                                              ctypes.POINTER(SourceFindStats)]),
+    "stp_device_malloc": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_size_t]),
+    "stp_device_free": (ctypes.c_int, [_P, _P]),
+    "stp_memcpy_htod": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "stp_memcpy_dtoh": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "stp_ctx_set_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "stp_ctx_reset_timings": (ctypes.c_int, [_P]),
+    "stp_ctx_get_timings": (ctypes.c_int, [_P, ctypes.c_char_p, _D, ctypes.POINTER(ctypes.c_uint64),
+                                           ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+}
+
+EXPORTED_SYMBOLS = tuple(_SIG)
+
+
+def load(path=None):
+    """Loads the HIP library (raises OSError if it is not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise OSError(f"HIP library not built: {p} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in _SIG.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib

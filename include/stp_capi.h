@@ -1,0 +1,340 @@
+/*
+ * stp_capi.h - C-ABI of the MI355X (gfx950) Slow Transients Pipeline imaging and
+ * source-finding hot path.
+ *
+ * Plain C types only: POD structs, raw pointers and sizes, integer status codes.
+ * No exception ever crosses this boundary; every entry point returns an
+ * stp_status and leaves a human-readable message in stp_ctx_last_error().
+ *
+ * Matrix layout everywhere is column-major (Armadillo / Fortran order), the
+ * layout of the reference's arma::Mat (src/stp/common/matstp.h:94):
+ *   element (row r, col c) of an n_rows x n_cols matrix is at [r + c*n_rows].
+ * Complex values are interleaved (re, im) doubles, as std::complex<double>.
+ *
+ * Each entry point names the reference interface it replaces (file:line, paths
+ * relative to the reference repository root).
+ */
+#ifndef STP_CAPI_H
+#define STP_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------- */
+typedef enum stp_status {
+    STP_OK = 0,
+    STP_ERR_INVALID_ARGUMENT = 1, /* reference: assert(...) in imager.h:257-287, gridder.h:270-295 */
+    STP_ERR_UNSUPPORTED = 2,      /* feature outside this implementation (A-projection, Hankel, Ceres fit) */
+    STP_ERR_DEVICE = 3,           /* HIP runtime error */
+    STP_ERR_OUT_OF_MEMORY = 4,    /* device allocation failed or HBM budget exceeded */
+    STP_ERR_INTERNAL = 5,         /* internal consistency check failed */
+    STP_ERR_CAPACITY = 6          /* caller-provided output buffer too small */
+} stp_status;
+
+/* ------------------------------------------------------------------------- */
+/* Enumerations (same order and meaning as src/stp/types.h:32-88)            */
+/* ------------------------------------------------------------------------- */
+typedef enum stp_kernel_function {
+    STP_KERNEL_TOPHAT = 0,
+    STP_KERNEL_TRIANGLE = 1,
+    STP_KERNEL_SINC = 2,
+    STP_KERNEL_GAUSSIAN = 3,
+    STP_KERNEL_GAUSSIANSINC = 4,
+    STP_KERNEL_PSWF = 5
+} stp_kernel_function;
+
+typedef enum stp_fft_routine {
+    STP_FFT_ESTIMATE = 0,
+    STP_FFT_MEASURE = 1,
+    STP_FFT_PATIENT = 2,
+    STP_FFT_WISDOM = 3,
+    STP_FFT_WISDOM_INPLACE = 4
+} stp_fft_routine;
+
+typedef enum stp_median_method {
+    STP_MEDIAN_ZEROMEDIAN = 0,
+    STP_MEDIAN_BINMEDIAN = 1,
+    STP_MEDIAN_BINAPPROX = 2,
+    STP_MEDIAN_NTHELEMENT = 3
+} stp_median_method;
+
+/* Arithmetic of the gridder + FFT.  FP64 is the reference's USE_FLOAT=OFF build.
+ * FP32C grids in FP32 with two-float compensated accumulation and FP64 index,
+ * phase and normalisation math; the FFT then runs in FP32. */
+typedef enum stp_precision {
+    STP_PRECISION_FP64 = 0,
+    STP_PRECISION_FP32C = 1
+} stp_precision;
+
+/* ------------------------------------------------------------------------- */
+/* Parameter structs                                                          */
+/* ------------------------------------------------------------------------- */
+
+/* stp::ImagerPars (src/stp/types.h:93-173).  padded_image_size is taken as given
+ * (the reference computes image_size * padding_factor in its constructor,
+ * types.h:157); stp_imager_pars_init() fills it the same way. */
+typedef struct stp_imager_pars {
+    uint32_t image_size;
+    uint32_t padded_image_size;
+    double cell_size;        /* arcsec */
+    double padding_factor;
+    int32_t kernel_function; /* stp_kernel_function */
+    uint32_t kernel_support;
+    int32_t kernel_exact;
+    uint32_t oversampling;
+    int32_t generate_beam;
+    int32_t gridding_correction;
+    int32_t analytic_gcf;
+    int32_t r_fft;           /* stp_fft_routine; accepted and ignored (no FFTW) */
+} stp_imager_pars;
+
+/* stp::W_ProjectionPars (src/stp/types.h:178-249).  W-projection is on iff
+ * num_wplanes > 0 (types.h:233-239). */
+typedef struct stp_wproj_pars {
+    uint32_t num_wplanes;
+    uint32_t max_wpconv_support;
+    uint32_t undersampling_opt;
+    double kernel_trunc_perc;
+    int32_t hankel_opt;      /* must be 0: Hankel kernels are not implemented */
+    int32_t hankel_proj_slice;
+    int32_t interp_type;
+    int32_t wplanes_median;
+} stp_wproj_pars;
+
+/* stp::A_ProjectionPars (src/stp/types.h:254-316).  Accepted for signature
+ * compatibility; num_timesteps > 0 returns STP_ERR_UNSUPPORTED. */
+typedef struct stp_aproj_pars {
+    uint32_t num_timesteps;
+    double obs_dec;
+    double obs_ra;
+    int32_t aproj_opt;
+    double aproj_mask_perc;
+} stp_aproj_pars;
+
+/* Arguments of the stp::SourceFindImage constructor (src/stp/sourcefind/sourcefind.h:188-201). */
+typedef struct stp_sourcefind_pars {
+    double detection_n_sigma;
+    double analysis_n_sigma;
+    double rms_est;              /* <= 0: estimate from the image */
+    int32_t find_negative_sources;
+    int32_t sigma_clip_iters;
+    int32_t median_method;       /* stp_median_method */
+    int32_t gaussian_fitting;    /* must be 0 (Ceres fit not implemented) */
+    int32_t ccl_4connectivity;
+    int32_t generate_labelmap;
+    int32_t source_min_area;
+} stp_sourcefind_pars;
+
+/* Output record types (src/stp/sourcefind/fitting.h:18-152, sourcefind.h:63-133). */
+typedef struct stp_bounding_box {
+    int32_t top, bottom, left, right;
+} stp_bounding_box;
+
+typedef struct stp_gaussian2d {
+    double amplitude, x_centre, y_centre, semimajor, semiminor, theta;
+} stp_gaussian2d;
+
+typedef struct stp_island {
+    int32_t label_idx;        /* +id for positive islands, -id for negative ones */
+    int32_t sign;
+    int32_t extremum_y_idx;   /* visual (fft-shifted) coordinates, sourcefind.cpp:243-244 */
+    int32_t extremum_x_idx;
+    int32_t num_samples;
+    int32_t reserved;
+    double extremum_val;
+    stp_bounding_box bounding_box;
+    stp_gaussian2d moments_fit;
+} stp_island;
+
+/* Scalars the reference exposes as SourceFindImage members (sourcefind.h:146-164). */
+typedef struct stp_sourcefind_stats {
+    double bg_level;
+    double rms_est;
+    double mean;              /* from the one-pass mean/sigma (0 if not computed) */
+    double sigma;
+    uint32_t num_labels_pos;  /* labels produced by the CCL before detection filtering */
+    uint32_t num_labels_neg;
+    uint32_t num_valid_labels;
+    uint32_t num_islands;     /* islands with area >= source_min_area */
+} stp_sourcefind_stats;
+
+/* Diagnostics of one imaging call. */
+typedef struct stp_imager_diag {
+    double sample_grid_total;   /* 2 * sum of good weights (gridder.h:415-423) */
+    uint64_t num_good_vis;      /* visibilities passing the bounds check */
+    uint32_t image_rows;        /* 0 when sample_grid_total <= 0 (empty result, imager.h:355) */
+    uint32_t image_cols;
+    double ms_total;            /* device time of the whole call (ms) */
+} stp_imager_diag;
+
+/* ------------------------------------------------------------------------- */
+/* Context                                                                    */
+/* ------------------------------------------------------------------------- */
+typedef struct stp_ctx stp_ctx;
+
+/* Creates a context bound to HIP device `device` with its own stream and a
+ * grow-only device workspace.  Replaces the reference's process-global FFTW
+ * thread and wisdom setup (src/stp/common/fft.cpp:15-43). */
+stp_status stp_ctx_create(stp_ctx** ctx, int device);
+stp_status stp_ctx_destroy(stp_ctx* ctx);
+const char* stp_ctx_last_error(const stp_ctx* ctx);
+stp_status stp_ctx_synchronize(stp_ctx* ctx);
+/* Releases every cached device buffer of the context. */
+stp_status stp_ctx_release_workspace(stp_ctx* ctx);
+/* Bytes of device workspace the context currently holds. */
+size_t stp_ctx_workspace_bytes(const stp_ctx* ctx);
+
+/* Fills the defaults of the reference's ImagerPars full constructor
+ * (types.h:132-158) for the given size and cell. */
+void stp_imager_pars_init(stp_imager_pars* p, uint32_t image_size, double cell_size);
+void stp_wproj_pars_init(stp_wproj_pars* p);
+void stp_sourcefind_pars_init(stp_sourcefind_pars* p, double detection_n_sigma, double analysis_n_sigma);
+
+/* ------------------------------------------------------------------------- */
+/* Imaging                                                                    */
+/* ------------------------------------------------------------------------- */
+
+/* stp::image_visibilities<T>(kernel_creator, vis, vis_weights, uvw_lambda,
+ *                            img_pars, w_proj, a_proj)
+ *   src/stp/imager/imager.h:233-376 (dispatch: src/stp/imager/imager.cpp:13-69)
+ * Host buffers: vis = n_vis complex (2*n_vis doubles), vis_weights = n_vis,
+ * uvw_lambda = n_vis x 3 column-major (u column, v column, w column).
+ * image and beam: image_size x image_size column-major, caller-allocated; beam
+ * may be NULL when generate_beam == 0.  diag may be NULL.  Synchronous. */
+stp_status stp_image_visibilities(stp_ctx* ctx,
+    const stp_imager_pars* img_pars,
+    const stp_wproj_pars* w_proj,   /* NULL = disabled */
+    const stp_aproj_pars* a_proj,   /* NULL = disabled */
+    size_t n_vis,
+    const double* vis,
+    const double* vis_weights,
+    const double* uvw_lambda,
+    int32_t precision,              /* stp_precision */
+    double* image,
+    double* beam,
+    stp_imager_diag* diag);
+
+/* Same computation with every buffer already in device memory of the context's
+ * device.  Enqueued on the context stream; returns before the work finishes
+ * unless `synchronize` is non-zero.  diag->sample_grid_total is only valid
+ * after synchronisation.  Image/beam element type: double for FP64, float for
+ * FP32C. */
+stp_status stp_image_visibilities_device(stp_ctx* ctx,
+    const stp_imager_pars* img_pars,
+    const stp_wproj_pars* w_proj,
+    const stp_aproj_pars* a_proj,
+    size_t n_vis,
+    const double* d_vis,
+    const double* d_vis_weights,
+    const double* d_uvw_lambda,
+    int32_t precision,
+    void* d_image,
+    void* d_beam,
+    int32_t synchronize,
+    stp_imager_diag* diag);
+
+/* stp::convolve_to_grid<generateBeam>(kernel_creator, support, image_size, uv_lambda,
+ *     vis, vis_weights, kernel_exact, oversampling, shift_uv, halfplane_gridding)
+ *   src/stp/gridder/gridder.h:240-997 (no W-projection in this entry point).
+ * uv_lambda: n_vis x 2 column-major, already in pixel units.  Outputs:
+ * vis_grid and sampling_grid, rows x image_size complex column-major with
+ * rows = image_size/2+1 if halfplane_gridding else image_size.  sampling_grid
+ * may be NULL when generate_beam == 0. */
+stp_status stp_convolve_to_grid(stp_ctx* ctx,
+    int32_t kernel_function,
+    double kernel_param,            /* functor argument: half_base_width (TopHat/Triangle) or trunc (others) */
+    uint32_t support,
+    int32_t image_size,
+    size_t n_vis,
+    const double* uv_lambda,
+    const double* vis,
+    const double* vis_weights,
+    int32_t kernel_exact,
+    uint32_t oversampling,
+    int32_t shift_uv,
+    int32_t halfplane_gridding,
+    int32_t generate_beam,
+    double* vis_grid,
+    double* sampling_grid,
+    double* sample_grid_total);
+
+/* stp::fft_fftw_c2r(input, output, r_fft)   src/stp/common/fft.cpp:45-135
+ * Unnormalised inverse 2-D complex-to-real FFT of a half-plane grid of
+ * in_rows = n/2+1 rows and n_cols columns (column-major complex) into an
+ * n x n_cols real matrix, n = 2*(in_rows-1). */
+stp_status stp_fft_c2r(stp_ctx* ctx, const double* in, int32_t in_rows, int32_t n_cols, double* out);
+
+/* stp::fft_fftw_c2c(input, output, r_fft, forward)   src/stp/common/fft.cpp:227-323
+ * Unnormalised 2-D complex FFT (sign -1 forward, +1 inverse) of a square
+ * n x n column-major complex matrix. */
+stp_status stp_fft_c2c(stp_ctx* ctx, const double* in, int32_t n, int32_t forward, double* out);
+
+/* ------------------------------------------------------------------------- */
+/* Source finding                                                             */
+/* ------------------------------------------------------------------------- */
+
+/* stp::SourceFindImage(data, detection_n_sigma, analysis_n_sigma, rms_est,
+ *     find_negative_sources, sigma_clip_iters, median_method, gaussian_fitting,
+ *     ccl_4connectivity, generate_labelmap, source_min_area)
+ *   src/stp/sourcefind/sourcefind.cpp:156-292, sourcefind.h:188-201
+ * image: rows x cols column-major doubles (the un-shifted imager output).
+ * label_map: rows x cols int32 (may be NULL).  islands: caller array of
+ * capacity island_capacity; *n_islands receives the number found (if it
+ * exceeds the capacity the call fails with STP_ERR_CAPACITY and *n_islands
+ * holds the required count).  Synchronous. */
+stp_status stp_source_find(stp_ctx* ctx,
+    const double* image,
+    int32_t rows,
+    int32_t cols,
+    const stp_sourcefind_pars* pars,
+    int32_t* label_map,
+    stp_island* islands,
+    size_t island_capacity,
+    size_t* n_islands,
+    stp_sourcefind_stats* stats);
+
+/* Same, with the image (and label map, if not NULL) in device memory.
+ * image_precision selects the element type of d_image (double for FP64,
+ * float for FP32C).  Islands and stats are returned in host memory. */
+stp_status stp_source_find_device(stp_ctx* ctx,
+    const void* d_image,
+    int32_t image_precision,
+    int32_t rows,
+    int32_t cols,
+    const stp_sourcefind_pars* pars,
+    int32_t* d_label_map,
+    stp_island* islands,
+    size_t island_capacity,
+    size_t* n_islands,
+    stp_sourcefind_stats* stats);
+
+/* ------------------------------------------------------------------------- */
+/* Device memory helpers (so callers need no HIP headers)                     */
+/* ------------------------------------------------------------------------- */
+stp_status stp_device_malloc(stp_ctx* ctx, void** ptr, size_t bytes);
+stp_status stp_device_free(stp_ctx* ctx, void* ptr);
+stp_status stp_memcpy_htod(stp_ctx* ctx, void* dst, const void* src, size_t bytes);
+stp_status stp_memcpy_dtoh(stp_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ------------------------------------------------------------------------- */
+/* Kernel timing (HIP events on the context stream)                           */
+/* ------------------------------------------------------------------------- */
+/* When enabled, every kernel launch of the context is bracketed by HIP events
+ * recorded on the context stream, and durations accumulate per kernel name. */
+stp_status stp_ctx_set_timing(stp_ctx* ctx, int32_t enable);
+stp_status stp_ctx_reset_timings(stp_ctx* ctx);
+/* Copies up to `cap` entries: names (NUL-terminated, 64 bytes each), total
+ * milliseconds and launch counts.  *n receives the number of distinct kernels. */
+stp_status stp_ctx_get_timings(stp_ctx* ctx, char* names, double* total_ms, uint64_t* counts, size_t cap, size_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STP_CAPI_H */

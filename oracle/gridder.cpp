@@ -1,0 +1,447 @@
+/*
+ * ORACLE - TEST INFRASTRUCTURE ONLY (see oracle.h).
+ * Convolutional gridder: src/stp/gridder/gridder.h:73-105,240-997,
+ * src/stp/gridder/gridder.cpp:11-146,202-241,
+ * W-projection kernels: src/stp/gridder/aw_projection.cpp:19-111,173-210,366-474.
+ */
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <stdexcept>
+#include <thread>
+
+#include "oracle.h"
+
+namespace oracle {
+
+// populate_kernel_cache (gridder.h:73-105): cache[a * cs + b] is the
+// (2s+1)^2 column-major matrix k_a[row] * k_b[col] of the 1-D kernels at
+// oversampled offsets (a - os/2)/os (rows) and (b - os/2)/os (columns).
+std::vector<std::vector<cx>> populate_kernel_cache(const KernelFunc& kf, unsigned support, unsigned oversampling)
+{
+    const unsigned op = oversampling / 2, cs = op * 2 + 1;
+    std::vector<std::vector<double>> k1d(cs);
+    for (unsigned i = 0; i < cs; i++) {
+        const double off = (double(i) - double(op)) / double(oversampling);
+        k1d[i] = make_1d_kernel(kf, int(support), off, 1, false, true);
+    }
+    const size_t ksz = k1d[0].size();
+    std::vector<std::vector<cx>> cache(size_t(cs) * cs, std::vector<cx>(ksz * ksz));
+    for (unsigned a = 0; a < cs; a++)
+        for (unsigned b = 0; b < cs; b++)
+            for (size_t c = 0; c < ksz; c++)
+                for (size_t r = 0; r < ksz; r++)
+                    cache[a * cs + b][r + c * ksz] = cx(k1d[a][r] * k1d[b][c], 0.0);
+    return cache;
+}
+
+// convert_to_halfplane_visibilities (gridder.cpp:11-66)
+void convert_to_halfplane_visibilities(std::vector<double>& uv, size_t n, std::vector<double>* w, std::vector<cx>& vis, int support, std::vector<uint32_t>& good)
+{
+    for (size_t i = 0; i < n; i++) {
+        const double v = uv[n + i];
+        if (v < 0.0) {
+            if (v > -(support + 1))
+                good[i] = 2;
+            uv[i] *= -1;
+            uv[n + i] *= -1;
+            if (w)
+                (*w)[i] *= -1;
+            vis[i] = std::conj(vis[i]);
+        } else {
+            if (v < (support + 1))
+                good[i] = 2;
+        }
+    }
+}
+
+// bounds_check_kernel_centre_locations (gridder.cpp:202-217); kc is n x 2 column-major.
+void bounds_check_kernel_centre_locations(std::vector<uint32_t>& good, const std::vector<int>& kc, size_t n, int image_size, int support)
+{
+    for (size_t i = 0; i < n; i++) {
+        const int x = kc[i], y = kc[n + i];
+        if ((x - support) <= 0 || (y - support) <= 0 || (x + support) >= image_size || (y + support) >= image_size)
+            good[i] = 0;
+    }
+}
+
+// calculate_oversampled_kernel_indices (gridder.cpp:219-241)
+void calculate_oversampled_kernel_indices(const double* frac, size_t n, unsigned oversampling, int* out)
+{
+    const int range_max = int(oversampling / 2);
+    const int range_min = -range_max;
+    for (size_t i = 0; i < n; i++) {
+        int val = int(std::rint(frac[i] * oversampling));
+        if (val > range_max)
+            val = range_max;
+        else if (val < range_min)
+            val = range_min;
+        out[i] = val;
+    }
+}
+
+// average_w_planes (gridder.cpp:68-146)
+void average_w_planes(const std::vector<double>& w, const std::vector<uint32_t>& good, unsigned num_wplanes,
+    std::vector<double>& w_avg, std::vector<size_t>& first_idx, bool median)
+{
+    const size_t num_elems = good.size();
+    size_t num_gvis = 0;
+    for (auto g : good)
+        if (g != 0)
+            num_gvis++;
+    w_avg.assign(num_wplanes, 0.0);
+    first_idx.assign(num_wplanes, 0);
+    size_t begin = 0;
+    for (unsigned idx = 0; idx < num_wplanes; idx++) {
+        const size_t plane_size = size_t(std::ceil(double(num_gvis - begin) / (num_wplanes - idx)));
+        size_t k, counter = 0;
+        if (median) {
+            size_t idx_mid = 0, middle = plane_size / 2;
+            if ((begin + plane_size) > num_gvis)
+                middle = (num_gvis - begin) / 2;
+            for (k = begin; (counter < plane_size) && (k < num_elems); k++) {
+                if (good[k] != 0) {
+                    if (counter == middle)
+                        idx_mid = k;
+                    counter++;
+                }
+            }
+            if (counter > 0)
+                w_avg[idx] = (plane_size % 2 == 0) ? (w[idx_mid - 1] + w[idx_mid]) / 2.0 : w[idx_mid];
+            else
+                w_avg[idx] = 0.0;
+            first_idx[idx] = begin;
+            begin = k;
+        } else {
+            double sum = 0;
+            for (k = begin; (counter < plane_size) && (k < num_elems); k++) {
+                if (good[k] != 0) {
+                    sum += w[k];
+                    counter++;
+                }
+            }
+            first_idx[idx] = begin;
+            begin = k;
+            w_avg[idx] = counter > 0 ? sum / counter : 0.0;
+        }
+    }
+}
+
+// WideFieldImaging::combine_2d_kernel_value (aw_projection.cpp:455-474)
+static cx combine_2d_kernel_value(const std::vector<double>& aa, size_t px, size_t py, double w_value, double scaled_cell, unsigned ctr)
+{
+    const double dx = double(px) * scaled_cell;
+    const double dy = double(py) * scaled_cell;
+    const double r2 = dx * dx + dy * dy;
+    if (r2 >= 1.0)
+        return cx(aa[ctr + px] * aa[ctr + py], 0.0);
+    const double nn = std::sqrt(1.0 - r2);
+    const double ph = (-2.0 * M_PI) * w_value * (nn - 1.0);
+    cx v = cx(std::cos(ph), std::sin(ph)) / nn;
+    v = v * aa[ctr + px];
+    v = v * aa[ctr + py];
+    return v;
+}
+
+std::vector<std::vector<cx>> wproj_kernel_cache(double w_value, const std::vector<double>& aa, unsigned workarea_size,
+    double cell_rad, unsigned oversampling, double scaling_factor, unsigned support, int nthreads)
+{
+    // generate_image_domain_convolution_kernel (aw_projection.cpp:50-111)
+    const size_t arr = size_t(workarea_size) * oversampling;
+    const unsigned half = workarea_size / 2;
+    const double scaled_cell = cell_rad * scaling_factor;
+    std::vector<cx> comb(arr * arr, cx(0.0, 0.0));
+    auto at = [&](size_t i, size_t j) -> cx& { return comb[i + j * arr]; };
+    for (size_t i = 1; i < half; i++) {
+        const cx v = combine_2d_kernel_value(aa, i, 0, w_value, scaled_cell, half);
+        at(i, 0) = v;
+        at(arr - i, 0) = v;
+        at(0, i) = v;
+        at(0, arr - i) = v;
+    }
+    at(0, 0) = cx(aa[half] * aa[half], 0.0);
+    for (size_t j = 1; j < half; j++)
+        for (size_t i = 1; i < half; i++) {
+            const cx v = combine_2d_kernel_value(aa, i, j, w_value, scaled_cell, half);
+            at(i, j) = v;
+            at(arr - i, j) = v;
+            at(i, arr - j) = v;
+            at(arr - i, arr - j) = v;
+        }
+    // forward 2-D c2c (aw_projection.cpp:194)
+    std::vector<cx> conv(arr * arr);
+    fft_c2c_2d(comb.data(), int(arr), int(arr), true, conv.data(), nthreads);
+    // generate_kernel_cache, non-Hankel branch (aw_projection.cpp:415-443)
+    const size_t ctr_idx = arr / 2, op = oversampling / 2, cache_size = 2 * op + 1;
+    const size_t tcs = 2 * size_t(support) + 1;
+    const size_t oversamp_conv = ctr_idx - size_t(support) * oversampling + op;
+    std::vector<std::vector<cx>> cache(cache_size * cache_size, std::vector<cx>(tcs * tcs));
+    for (size_t x = 0; x < cache_size; x++) {
+        for (size_t y = 0; y < cache_size; y++) {
+            std::vector<cx>& tmp = cache[y * cache_size + x];
+            double ksum = 0;
+            for (size_t ty = 0; ty < tcs; ty++) {
+                const size_t cxi = (oversamp_conv - x + ty * oversampling + ctr_idx) % arr;
+                for (size_t tx = 0; tx < tcs; tx++) {
+                    const size_t cyi = (oversamp_conv - y + tx * oversampling + ctr_idx) % arr;
+                    const cx v = conv[cxi + cyi * arr];
+                    tmp[tx + ty * tcs] = v;
+                    ksum += v.real();
+                }
+            }
+            ksum = 1 / ksum;
+            for (auto& v : tmp)
+                v = cx(v.real() * ksum, v.imag() * ksum);
+        }
+    }
+    return cache;
+}
+
+GridderOutput convolve_to_grid(const KernelFunc& kf, unsigned support, int image_size,
+    const double* uv_in, const cx* vis_in, const double* weights_in, size_t n,
+    bool kernel_exact, unsigned oversampling, bool shift_uv, bool halfplane, bool generate_beam,
+    const WProjPars& wp, const double* w_in, double cell_size_arcsec, bool analytic_gcf, int nthreads)
+{
+    const int N = image_size;
+    const int half = N / 2;
+    int conv_support = int(support);
+    const bool use_wproj = wp.num_wplanes > 0;
+    if ((N % 2) != 0)
+        throw std::invalid_argument("image_size must be even");
+    if (use_wproj) {
+        if (wp.hankel)
+            throw std::invalid_argument("Hankel W-kernels are not restated");
+        conv_support = int(wp.max_wpconv_support);
+        kernel_exact = false;
+    }
+    if (conv_support <= 0)
+        throw std::invalid_argument("support must be > 0");
+    if (kernel_exact) {
+        oversampling = 1;
+    } else {
+        oversampling = (oversampling >> 1) << 1;
+        if (oversampling == 0)
+            oversampling = 1;
+    }
+
+    // own copies (the reference takes these by value, gridder.h:245-253)
+    std::vector<double> uv(uv_in, uv_in + 2 * n);
+    std::vector<cx> vis(vis_in, vis_in + n);
+    std::vector<double> wts(weights_in, weights_in + n);
+    std::vector<double> wl(n, 0.0);
+    if (use_wproj)
+        wl.assign(w_in, w_in + n);
+    std::vector<uint32_t> good(n, 1);
+
+    if (use_wproj) {
+        // sort by |w| ascending (gridder.h:324); stable for determinism
+        std::vector<size_t> idx(n);
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return std::fabs(wl[a]) < std::fabs(wl[b]); });
+        std::vector<double> uv2(2 * n), wl2(n), w2(n);
+        std::vector<cx> vis2(n);
+        for (size_t i = 0; i < n; i++) {
+            uv2[i] = uv[idx[i]];
+            uv2[n + i] = uv[n + idx[i]];
+            wl2[i] = wl[idx[i]];
+            vis2[i] = vis[idx[i]];
+            w2[i] = wts[idx[i]];
+        }
+        uv.swap(uv2);
+        wl.swap(wl2);
+        vis.swap(vis2);
+        wts.swap(w2);
+    }
+    if (halfplane)
+        convert_to_halfplane_visibilities(uv, n, use_wproj ? &wl : nullptr, vis, conv_support, good);
+
+    std::vector<int> kc(2 * n);
+    std::vector<double> frac(2 * n);
+    for (size_t i = 0; i < n; i++) {
+        int val = int(std::rint(uv[i]));
+        frac[i] = uv[i] - val;
+        kc[i] = val + half;
+        val = int(std::rint(uv[n + i]));
+        frac[n + i] = uv[n + i] - val;
+        kc[n + i] = val + half;
+    }
+    bounds_check_kernel_centre_locations(good, kc, n, N, conv_support);
+    const int rows = halfplane ? half + 1 : N;
+    if (shift_uv) {
+        for (size_t i = 0; i < 2 * n; i++)
+            kc[i] = (kc[i] < half) ? kc[i] + half : kc[i] - half;
+    }
+    double total = 0;
+    size_t ngood = 0;
+    for (size_t i = 0; i < n; i++)
+        if (good[i] != 0) {
+            total += wts[i];
+            ngood++;
+        }
+    total *= 2.0;
+
+    GridderOutput out;
+    out.rows = rows;
+    out.cols = N;
+    out.sample_grid_total = total;
+    out.num_good_vis = ngood;
+    out.vis_grid.assign(size_t(rows) * N, cx(0.0, 0.0));
+    if (generate_beam)
+        out.sampling_grid.assign(size_t(rows) * N, cx(0.0, 0.0));
+    cx* vg = out.vis_grid.data();
+    cx* sg = generate_beam ? out.sampling_grid.data() : nullptr;
+
+    if (!kernel_exact) {
+        unsigned num_planes = 1;
+        std::vector<double> w_avg;
+        std::vector<size_t> first_idx(1, 0);
+        std::vector<double> aa;
+        unsigned workarea = unsigned(N);
+        double scaling = 1.0;
+        // cache[(cp_y * cs + cp_x)] -> (2s+1)^2 column-major, row = y tap
+        std::vector<std::vector<cx>> cache;
+        if (use_wproj) {
+            num_planes = wp.num_wplanes;
+            std::vector<double> absw(n);
+            for (size_t i = 0; i < n; i++)
+                absw[i] = std::fabs(wl[i]);
+            average_w_planes(absw, good, num_planes, w_avg, first_idx, wp.wplanes_median);
+            const int ks = 2 * conv_support + 1;
+            const int us = wp.undersampling_opt > 0 ? int(std::pow(2, wp.undersampling_opt - 1)) : 0;
+            if (us > 0) {
+                workarea = 2;
+                while (int(workarea) / us < ks)
+                    workarea *= 2;
+            } else {
+                workarea = 2;
+                while (int(workarea) < N)
+                    workarea *= 2;
+            }
+            scaling = double(N) / double(workarea);
+            aa = img_dom_kernel(kf, workarea, false, analytic_gcf);
+        } else {
+            cache = populate_kernel_cache(kf, support, oversampling);
+        }
+        const unsigned cache_size = (oversampling / 2) * 2 + 1;
+        std::vector<int> ovs(2 * n);
+        calculate_oversampled_kernel_indices(frac.data(), 2 * n, oversampling, ovs.data());
+        for (auto& v : ovs)
+            v += int(oversampling / 2);
+        const double cell_rad = ((cell_size_arcsec / 3600.0) * (M_PI / 180.0));
+
+        for (unsigned pi = 0; pi < num_planes; pi++) {
+            const size_t vb = first_idx[pi];
+            const size_t ve = (pi == num_planes - 1) ? n : first_idx[pi + 1];
+            int cs_plane = conv_support;
+            if (use_wproj) {
+                cache = wproj_kernel_cache(w_avg[pi], aa, workarea, cell_rad, oversampling, scaling, unsigned(conv_support), nthreads);
+            }
+            const int ks = 2 * cs_plane + 1;
+            // column-owner parallelism: per pixel, contributions add in visibility order,
+            // exactly as each TBB column-residue task does (gridder.h:767-934)
+            const int T = std::max(1, std::min(ks, nthreads));
+            auto worker = [&](int t) {
+                for (size_t vi = vb; vi < ve; vi++) {
+                    const uint32_t g = good[vi];
+                    if (g == 0)
+                        continue;
+                    int gc_x = kc[vi], gc_y = kc[n + vi];
+                    int cp_x = ovs[vi], cp_y = ovs[n + vi];
+                    cx vv = vis[vi];
+                    const double wt = wts[vi];
+                    double wlv = wl[vi];
+                    for (uint32_t gv = 0; gv < g; gv++) {
+                        if (gv) {
+                            gc_x = -kc[vi] + N;
+                            gc_y = -kc[n + vi] + N;
+                            if (oversampling > 1) {
+                                cp_x = -ovs[vi] + int(oversampling);
+                                cp_y = -ovs[n + vi] + int(oversampling);
+                            }
+                            vv = std::conj(vv);
+                            wlv *= -1;
+                        }
+                        const std::vector<cx>& K = cache[size_t(cp_y) * cache_size + cp_x];
+                        const bool conjk = use_wproj && (wlv < 0.0);
+                        for (int d = -cs_plane; d <= cs_plane; d++) {
+                            int col = gc_x + d;
+                            col = ((col % N) + N) % N;
+                            if ((col % T) != t)
+                                continue;
+                            const cx* kcol = &K[size_t(cs_plane + d) * ks];
+                            int r = gc_y - cs_plane;
+                            for (int i = 0; i < ks; i++, r++) {
+                                if (r < 0)
+                                    continue;
+                                int rr = r;
+                                if (rr >= N)
+                                    rr -= N;
+                                if (rr < rows) {
+                                    const cx kv0 = conjk ? std::conj(kcol[i]) : kcol[i];
+                                    const cx kval(kv0.real() * wt, kv0.imag() * wt);
+                                    vg[rr + size_t(col) * rows] += vv * kval;
+                                    if (sg)
+                                        sg[rr + size_t(col) * rows] += kval;
+                                }
+                            }
+                        }
+                    }
+                }
+            };
+            if (T == 1) {
+                worker(0);
+            } else {
+                std::vector<std::thread> th;
+                for (int t = 0; t < T; t++)
+                    th.emplace_back(worker, t);
+                for (auto& x : th)
+                    x.join();
+            }
+        }
+    } else {
+        // exact gridder (gridder.h:943-994)
+        const int ks = 2 * conv_support + 1;
+        for (size_t vi = 0; vi < n; vi++) {
+            if (good[vi] == 0)
+                continue;
+            int gc_x = kc[vi], gc_y = kc[n + vi];
+            double fx = frac[vi], fy = frac[n + vi];
+            cx vv = vis[vi];
+            const double wt = wts[vi];
+            for (uint32_t gv = 0; gv < good[vi]; gv++) {
+                if (gv == 1) {
+                    gc_x = -kc[vi] + N;
+                    gc_y = -kc[n + vi] + N;
+                    fx *= -1;
+                    fy *= -1;
+                    vv = std::conj(vv);
+                }
+                const std::vector<double> K = make_kernel_array(kf, conv_support, fx, fy, 1, false, true);
+                for (int j = 0; j < ks; j++) {
+                    int col = gc_x - conv_support + j;
+                    if (col < 0)
+                        col += N;
+                    if (col >= N)
+                        col -= N;
+                    for (int i = 0; i < ks; i++) {
+                        int r = gc_y - conv_support + i;
+                        if (r < 0)
+                            r += N;
+                        if (r >= N)
+                            r -= N;
+                        if (r < rows) {
+                            const double kv = wt * K[i + size_t(j) * ks];
+                            const cx kval(kv, 0.0);
+                            vg[r + size_t(col) * rows] += vv * kval;
+                            if (sg)
+                                sg[r + size_t(col) * rows] += kval;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    return out;
+}
+
+} // namespace oracle

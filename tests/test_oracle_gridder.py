@@ -1,0 +1,200 @@
+"""Pins the oracle's gridder and kernel sampling against the reference's own
+known-answer tests (src/test/gridder, src/test/kernel, src/test/conv) and
+against closed forms."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from fastimaging_amd import capi
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+KAT = json.load(open(os.path.join(GOLDEN, "gridder_kat.json")))["cases"]
+KERNELS = {"TopHat": capi.KERNEL_TOPHAT, "Triangle": capi.KERNEL_TRIANGLE}
+TOL = 1e-10  # fptolerance of the FP64 build (src/stp/types.h:26)
+
+
+@pytest.mark.parametrize("case", KAT, ids=[c["name"] for c in KAT])
+def test_gridder_kat(orc, case):
+    uv = np.array(case["uv"], dtype=float)
+    vis = np.array([complex(a, b) for a, b in case["vis"]])
+    w = np.array(case["weights"], dtype=float)
+    vg, sg, total, _ = orc.convolve_to_grid(KERNELS[case["kernel"]], case["param"], case["support"],
+                                            case["image_size"], uv, vis, w, kernel_exact=case["kernel_exact"],
+                                            oversampling=case["oversampling"], shift_uv=case["shift_uv"],
+                                            halfplane=case["halfplane"], generate_beam=True)
+    if "expected_vis_grid_real" in case:
+        exp = np.array(case["expected_vis_grid_real"])
+        assert np.allclose(vg.real, exp, atol=TOL, rtol=0)
+        es = case.get("expected_sampling_grid_real")
+        if es is not None:
+            es = exp if es == "same" else np.array(es)
+            assert np.allclose(sg.real, es, atol=TOL, rtol=0)
+        # sum rule: sum(grid) == sum(vis * w) when no kernel tap is dropped
+        assert abs(vg.sum() - (vis * w).sum()) < TOL * 100
+    else:
+        # half-plane + shift: column-shifted grid equals the expected rows, and the
+        # exact gridder gives the same grid (gridder_test_HalfPlaneShiftedGridding.cpp:38-49)
+        shifted = np.roll(vg, case["column_shift"], axis=1)
+        exp = np.array(case["expected_vis_grid_real_after_column_shift"])
+        assert np.allclose(shifted.real, exp, atol=TOL, rtol=0)
+        vge, sge, _, _ = orc.convolve_to_grid(KERNELS[case["kernel"]], case["param"], case["support"],
+                                              case["image_size"], uv, vis, w, kernel_exact=True, oversampling=1,
+                                              shift_uv=True, halfplane=True, generate_beam=True)
+        assert np.allclose(vg, vge, atol=TOL, rtol=0)
+        assert np.allclose(sg, sge, atol=TOL, rtol=0)
+        # both the visibility at v=0 and its conjugate copy are gridded: total = 2 * sum(w)
+        assert total == 2.0 * w.sum()
+
+
+def test_gridder_triangle_uv_location(orc):
+    """gridder_test_Triangle.cpp:14-52: the grid block around the kernel centre equals
+    the exact kernel for the sub-pixel offset."""
+    n, s = 8, 2
+    off = (0.1, -0.15)
+    uv = np.array([[1.0 + off[0], 0.0 + off[1]]])
+    vg, _, _, _ = orc.convolve_to_grid(capi.KERNEL_TRIANGLE, 2.0, s, n, uv, np.ones(1, complex), np.ones(1),
+                                       kernel_exact=True, oversampling=1, shift_uv=False, halfplane=False)
+    k = orc.make_kernel_array(capi.KERNEL_TRIANGLE, 2.0, s, off, 1)
+    block = vg.real[n // 2 - s: n // 2 + s + 1, n // 2 + 1 - s: n // 2 + 1 + s + 1]
+    assert np.allclose(block, k / k.sum(), atol=TOL, rtol=0)
+    assert abs(vg.real.sum() - 1.0) < TOL
+
+
+def test_gridder_bounds_checking(orc):
+    """gridder_test_BoundsChecking.cpp:9-42 (default shift_uv and half-plane)."""
+    args = (capi.KERNEL_TOPHAT, 1.5, 2, 8)
+    vg, sg, _, ngood = orc.convolve_to_grid(*args, np.array([[-3.0, 0.0]]), np.ones(1, complex), np.ones(1),
+                                            kernel_exact=True, oversampling=1)
+    assert vg.real.sum() == 0 and sg.real.sum() == 0 and ngood == 0
+    g1, s1, _, _ = orc.convolve_to_grid(*args, np.array([[0.0, 0.0]]), np.ones(1, complex), np.ones(1),
+                                        kernel_exact=True, oversampling=1)
+    g2, s2, _, ngood2 = orc.convolve_to_grid(*args, np.array([[-3.0, 0.0], [0.0, 0.0]]), np.ones(2, complex),
+                                             np.ones(2), kernel_exact=True, oversampling=1)
+    assert np.array_equal(g1, g2) and np.array_equal(s1, s2) and ngood2 == 1
+
+
+def test_gridder_oversampled_sum(orc):
+    """gridder_test_OversampledGridding.cpp:13-38: the sampling grid sums to the number of visibilities."""
+    uv = np.array([[1.0, 0.0], [1.3, 0.0], [0.01, -1.32]])
+    _, sg, _, _ = orc.convolve_to_grid(capi.KERNEL_TRIANGLE, 2.0, 2, 8, uv, np.ones(3, complex), np.ones(3),
+                                       kernel_exact=False, oversampling=9, shift_uv=False, halfplane=False)
+    assert abs(sg.real.sum() - 3.0) < TOL
+
+
+IO_PAIRS = [(-0.5, -2), (-0.4999, -2), (-0.4, -2), (-0.35, -2), (-0.3, -2), (-0.2999, -1), (-0.2, -1),
+            (-0.11, -1), (-0.1, 0), (-0.09, 0), (-0.01, 0), (0.0, 0)]
+
+
+def test_oversampled_index_math(orc):
+    """gridder_test_FractionalCoordToOversampledIndexMath.cpp:6-85 and ..In2DCase.cpp:26-35."""
+    x = np.array([a for a, _ in IO_PAIRS])
+    exp = np.array([b for _, b in IO_PAIRS])
+    assert np.array_equal(orc.oversampled_indices(x, 5), exp)
+    assert np.array_equal(orc.oversampled_indices(-x, 5), -exp)
+    assert orc.oversampled_indices(np.array([0.5]), 7)[0] == 3  # rint(3.5)=4 clamped to 7//2
+    assert np.array_equal(orc.oversampled_indices(np.array([0.3, 0.3]), 5), [2, 2])
+
+
+def test_kernel_caching(orc):
+    """gridder_test_KernelCaching.cpp:22-61: the cached kernel picked for a sub-pixel offset is
+    the exact kernel at the nearest oversampled offset."""
+    os_, support = 5, 3
+    cache = orc.populate_kernel_cache(capi.KERNEL_TRIANGLE, 2.5, support, os_)
+    for step in (-0.4, -0.2, 0.0, 0.2, 0.4):
+        exact = orc.make_kernel_array(capi.KERNEL_TRIANGLE, 2.5, support, (step, 0.0))
+        idx = orc.oversampled_indices(np.array([step, 0.0]), os_)
+        assert np.allclose(cache[idx[1] + os_ // 2, idx[0] + os_ // 2].real, exact, atol=TOL, rtol=0)
+        for sub in np.linspace(-0.099999, 0.099999, 15):
+            if sub == 0:
+                continue
+            idx2 = orc.oversampled_indices(np.array([step + sub, 0.0]), os_)
+            assert np.array_equal(idx2, idx)
+            unaligned = orc.make_kernel_array(capi.KERNEL_TRIANGLE, 2.5, support, (step + sub, 0.0))
+            assert np.any(np.abs(exact - unaligned) > 1e-8)
+
+
+def test_gaussiansinc_closed_form(orc):
+    """conv_test_gaussiansinc.cpp:9-40: sinc(k/2) nodes have closed-form values."""
+    wg, ws = 2.52, 1.55
+    x = np.array([0.0, ws * 0.5, ws, ws * 1.5, ws * 2.0, ws * 2.5, 5.5])
+    exp = np.array([1.0,
+                    math.exp(-(0.5 * ws / wg) ** 2) / (0.5 * math.pi), 0.0,
+                    math.exp(-(1.5 * ws / wg) ** 2) * -1.0 / (1.5 * math.pi), 0.0,
+                    math.exp(-(2.5 * ws / wg) ** 2) / (2.5 * math.pi), 0.0])
+    got = orc.kernel_eval(capi.KERNEL_GAUSSIANSINC, 5.0, x)
+    assert np.allclose(got, exp, atol=TOL, rtol=0)
+
+
+def test_tophat_triangle_closed_forms(orc):
+    """conv_test_tophat.cpp / conv_test_triangle.cpp: piecewise definitions at break points."""
+    x = np.array([-2.0, -1.5, -1.0, -0.5, 0.0, 0.5, 1.0, 1.5, 2.0])
+    assert np.array_equal(orc.kernel_eval(capi.KERNEL_TOPHAT, 1.0, x), (np.abs(x) < 1.0).astype(float))
+    tri = orc.kernel_eval(capi.KERNEL_TRIANGLE, 2.0, x)
+    assert np.allclose(tri, [0, 0.25, 0.5, 0.75, 1, 0.75, 0.5, 0.25, 0], atol=1e-15)
+
+
+@pytest.mark.parametrize("offset,cols", [((0.0, 0.0), (4, 5, 6)), ((0.4, 0.0), (6, 7, 8))])
+def test_oversampled_pillbox_small(orc, offset, cols):
+    """kernel_test_OversampledPillboxSmall.cpp:9-62: TopHat(0.25), support 1, os 5, unnormalised."""
+    k = orc.make_kernel_array(capi.KERNEL_TOPHAT, 0.25, 1, offset, 5, False, False)
+    exp = np.zeros((11, 11))
+    for r in (4, 5, 6):
+        for c in cols:
+            exp[r, c] = 1.0
+    assert np.array_equal(k, exp)
+
+
+def test_gridder_brute_force_random(orc):
+    """Oversampled half-plane gridding equals a direct per-tap sum computed here from the
+    kernel cache on a tiny grid (rows < 0 skipped, rows >= N wrapped, columns wrapped)."""
+    rng = np.random.default_rng(3)
+    N, s, os_ = 16, 2, 8
+    n = 40
+    uv = rng.uniform(-4.5, 4.5, size=(n, 2))
+    vis = rng.normal(size=n) + 1j * rng.normal(size=n)
+    w = rng.uniform(0.5, 2.0, size=n)
+    vg, sg, total, _ = orc.convolve_to_grid(capi.KERNEL_GAUSSIANSINC, s, s, N, uv, vis, w, kernel_exact=False,
+                                            oversampling=os_, shift_uv=True, halfplane=True)
+    cache = orc.populate_kernel_cache(capi.KERNEL_GAUSSIANSINC, s, s, os_)
+    rows = N // 2 + 1
+    ref = np.zeros((rows, N), complex)
+    refb = np.zeros((rows, N), complex)
+    tot = 0.0
+    for i in range(n):
+        u, v, vv = uv[i, 0], uv[i, 1], vis[i]
+        dup = False
+        if v < 0:
+            dup = v > -(s + 1)
+            u, v, vv = -u, -v, np.conj(vv)
+        else:
+            dup = v < s + 1
+        cx_, cy_ = int(np.rint(u)), int(np.rint(v))
+        kx, ky = cx_ + N // 2, cy_ + N // 2
+        if kx - s <= 0 or ky - s <= 0 or kx + s >= N or ky + s >= N:
+            continue
+        tot += w[i]
+        ox = int(np.clip(np.rint((u - cx_) * os_), -os_ // 2, os_ // 2)) + os_ // 2
+        oy = int(np.clip(np.rint((v - cy_) * os_), -os_ // 2, os_ // 2)) + os_ // 2
+        gx, gy = (kx + N // 2) % N, (ky + N // 2) % N
+        passes = [(gx, gy, ox, oy, vv)]
+        if dup:
+            passes.append((N - gx, N - gy, os_ - ox, os_ - oy, np.conj(vv)))
+        for (px, py, qx, qy, val) in passes:
+            K = cache[qy, qx].real
+            for d in range(-s, s + 1):
+                col = (px + d) % N
+                for t in range(2 * s + 1):
+                    r = py - s + t
+                    if r < 0:
+                        continue
+                    if r >= N:
+                        r -= N
+                    if r < rows:
+                        ref[r, col] += val * K[t, s + d] * w[i]
+                        refb[r, col] += K[t, s + d] * w[i]
+    assert np.allclose(vg, ref, atol=1e-13)
+    assert np.allclose(sg, refb, atol=1e-13)
+    assert abs(total - 2 * tot) < 1e-12
