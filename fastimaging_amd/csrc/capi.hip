@@ -1,0 +1,497 @@
+// extern "C" entry points of the gfx950 STP library (include/stp_capi.h).
+// Every entry point converts exceptions into a status code and a message.
+#include <cmath>
+#include <cstring>
+#include <exception>
+
+#include "common.h"
+#include "fft.h"
+#include "gridder.h"
+#include "kernel_tables.h"
+#include "reduce.h"
+#include "sourcefind.h"
+
+using namespace stp_hip;
+
+namespace {
+
+template <typename F>
+stp_status guarded(stp_ctx* ctx, F f)
+{
+    if (!ctx)
+        return STP_ERR_INVALID_ARGUMENT;
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        f();
+        ctx->last_error.clear();
+        return STP_OK;
+    } catch (const Error& e) {
+        ctx->last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        ctx->last_error = "host allocation failed";
+        return STP_ERR_OUT_OF_MEMORY;
+    } catch (const std::invalid_argument& e) {
+        ctx->last_error = e.what();
+        return STP_ERR_INVALID_ARGUMENT;
+    } catch (const std::exception& e) {
+        ctx->last_error = e.what();
+        return STP_ERR_INTERNAL;
+    }
+}
+
+bool is_pow2(uint32_t x) { return x && !(x & (x - 1)); }
+
+// Parameter checks of image_visibilities (imager.h:257-287).
+void check_imager_pars(const stp_imager_pars* p, const stp_wproj_pars* wp, const stp_aproj_pars* ap)
+{
+    STP_REQUIRE(p != nullptr, "img_pars is NULL");
+    STP_REQUIRE(p->padding_factor >= 1.0, "padding_factor must be >= 1");
+    STP_REQUIRE(p->padded_image_size >= p->image_size, "padded_image_size < image_size");
+    STP_REQUIRE(p->padded_image_size > 0 && is_pow2(p->padded_image_size), "padded image size must be a power of two");
+    STP_REQUIRE(p->image_size >= 4 && p->image_size % 4 == 0, "image_size must be a multiple of 4");
+    STP_REQUIRE(p->kernel_support > 0, "kernel_support must be > 0");
+    STP_REQUIRE(p->cell_size > 0.0, "cell_size must be > 0");
+    STP_REQUIRE(p->kernel_exact || p->oversampling >= 1, "oversampling must be >= 1 when kernel_exact is false");
+    STP_REQUIRE(p->kernel_function >= STP_KERNEL_TOPHAT && p->kernel_function <= STP_KERNEL_PSWF, "unknown kernel function");
+    if (ap && ap->num_timesteps > 0)
+        throw Error(STP_ERR_UNSUPPORTED, "A-projection is not supported");
+    if (wp && wp->num_wplanes > 0) {
+        STP_REQUIRE(!p->kernel_exact, "W-projection requires kernel_exact == false (imager.h:287)");
+        if (wp->hankel_opt)
+            throw Error(STP_ERR_UNSUPPORTED, "Hankel-transform W-kernels are not supported");
+    }
+}
+
+struct ImagingPlan {
+    int N;        // padded size
+    int rows;     // N/2 + 1
+    size_t grid_elems;
+};
+
+// Full imaging pipeline on device buffers: grid -> c2r (image, beam) -> normalise.
+template <typename T, typename OT>
+void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp, size_t n, const double* d_vis,
+    const double* d_w, const double* d_uvw, OT* d_image, OT* d_beam, double* d_total, unsigned long long* d_ngood)
+{
+    const int N = int(p->padded_image_size);
+    const bool beam = p->generate_beam != 0;
+    STP_REQUIRE(!beam || d_beam != nullptr, "beam buffer is NULL with generate_beam");
+    if (wp && wp->num_wplanes > 0)
+        throw Error(STP_ERR_UNSUPPORTED, "W-projection is not available in this build yet");
+    GridderArgs a;
+    a.image_size = N;
+    a.support = int(p->kernel_support);
+    a.oversampling = int(p->oversampling);
+    a.kernel_exact = p->kernel_exact != 0;
+    a.shift_uv = true;
+    a.halfplane = true;
+    a.generate_beam = beam;
+    a.uv_scale = ((p->cell_size / 3600.0) * (M_PI / 180.0)) * double(N); // imager.h:293
+    a.kernel_function = p->kernel_function;
+    a.kernel_param = double(p->kernel_support); // functor trunc = kernel_support (imager.cpp:26-60)
+    const int rows = N / 2 + 1;
+    const size_t ge = size_t(rows) * N;
+    cplx<T>* grids = (cplx<T>*)ws_get(ctx, "imager_grids", sizeof(cplx<T>) * ge * (beam ? 2 : 1));
+    grid_visibilities<T>(ctx, a, n, d_uvw, d_vis, d_w, grids, beam ? grids + ge : nullptr, d_total, d_ngood);
+    NormSpec ns;
+    ns.image_size = int(p->image_size);
+    ns.total_ptr = d_total;
+    if (p->gridding_correction) {
+        const KernelFn kf = kernel_fn(p->kernel_function, double(p->kernel_support));
+        char key[160];
+        snprintf(key, sizeof(key), "gcf:%d:%.17g:%d:%d", kf.kind, kf.param, N, p->analytic_gcf);
+        ns.gcf = has_table(ctx, key) ? dev_table(ctx, key, {})
+                                     : dev_table(ctx, key, img_dom_kernel(kf, size_t(N), false, p->analytic_gcf != 0));
+    }
+    OutPtrs<OT> outs{};
+    outs.p[0] = d_image;
+    outs.p[1] = d_beam;
+    fft_c2r_2d<T, OT>(ctx, grids, beam ? 2 : 1, N, N, outs, ns);
+}
+
+} // namespace
+
+extern "C" {
+
+stp_status stp_ctx_create(stp_ctx** out, int device)
+{
+    if (!out)
+        return STP_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return STP_ERR_DEVICE;
+    if (device < 0 || device >= count)
+        return STP_ERR_INVALID_ARGUMENT;
+    stp_ctx* ctx = new (std::nothrow) stp_ctx();
+    if (!ctx)
+        return STP_ERR_OUT_OF_MEMORY;
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return STP_ERR_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+        ctx->num_cus = prop.multiProcessorCount;
+    *out = ctx;
+    return STP_OK;
+}
+
+stp_status stp_ctx_destroy(stp_ctx* ctx)
+{
+    if (!ctx)
+        return STP_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->ws)
+        if (kv.second.ptr)
+            (void)hipFree(kv.second.ptr);
+    for (auto& kv : ctx->tables)
+        if (kv.second.ptr)
+            (void)hipFree(kv.second.ptr);
+    for (auto& kv : ctx->twiddles) {
+        (void)hipFree(kv.second.d);
+        (void)hipFree(kv.second.f);
+    }
+    for (auto& p : ctx->pending) {
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+    for (auto e : ctx->event_pool)
+        (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return STP_OK;
+}
+
+const char* stp_ctx_last_error(const stp_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+stp_status stp_ctx_synchronize(stp_ctx* ctx)
+{
+    return guarded(ctx, [&] { HIP_CHECK(hipStreamSynchronize(ctx->stream)); });
+}
+
+stp_status stp_ctx_release_workspace(stp_ctx* ctx)
+{
+    return guarded(ctx, [&] {
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for (auto& kv : ctx->ws)
+            if (kv.second.ptr)
+                HIP_CHECK(hipFree(kv.second.ptr));
+        ctx->ws.clear();
+    });
+}
+
+size_t stp_ctx_workspace_bytes(const stp_ctx* ctx)
+{
+    if (!ctx)
+        return 0;
+    size_t s = 0;
+    for (auto& kv : ctx->ws)
+        s += kv.second.bytes;
+    return s;
+}
+
+void stp_imager_pars_init(stp_imager_pars* p, uint32_t image_size, double cell_size)
+{
+    if (!p)
+        return;
+    // ImagerPars full constructor defaults (types.h:132-158)
+    p->image_size = image_size;
+    p->cell_size = cell_size;
+    p->padding_factor = 1.0;
+    p->padded_image_size = uint32_t(image_size * p->padding_factor);
+    p->kernel_function = STP_KERNEL_PSWF;
+    p->kernel_support = 3;
+    p->kernel_exact = 1;
+    p->oversampling = 1;
+    p->generate_beam = 0;
+    p->gridding_correction = 1;
+    p->analytic_gcf = 1;
+    p->r_fft = STP_FFT_ESTIMATE;
+}
+
+void stp_wproj_pars_init(stp_wproj_pars* p)
+{
+    if (!p)
+        return;
+    std::memset(p, 0, sizeof(*p));
+    p->undersampling_opt = 1; // types.h:183-192
+}
+
+void stp_sourcefind_pars_init(stp_sourcefind_pars* p, double det, double ana)
+{
+    if (!p)
+        return;
+    // SourceFindImage constructor defaults (sourcefind.h:188-201)
+    p->detection_n_sigma = det;
+    p->analysis_n_sigma = ana;
+    p->rms_est = 0.0;
+    p->find_negative_sources = 1;
+    p->sigma_clip_iters = 5;
+    p->median_method = STP_MEDIAN_BINAPPROX;
+    p->gaussian_fitting = 0;
+    p->ccl_4connectivity = 0;
+    p->generate_labelmap = 1;
+    p->source_min_area = 5;
+}
+
+stp_status stp_image_visibilities_device(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp,
+    const stp_aproj_pars* ap, size_t n, const double* d_vis, const double* d_w, const double* d_uvw, int32_t precision,
+    void* d_image, void* d_beam, int32_t synchronize, stp_imager_diag* diag)
+{
+    return guarded(ctx, [&] {
+        check_imager_pars(p, wp, ap);
+        STP_REQUIRE(d_image != nullptr, "image buffer is NULL");
+        STP_REQUIRE(n == 0 || (d_vis && d_w && d_uvw), "visibility buffers are NULL");
+        double* d_total = (double*)ws_get(ctx, "imager_total", sizeof(double) * 2);
+        unsigned long long* d_ngood = (unsigned long long*)(d_total + 1);
+        LaunchTimer whole(ctx, "image_visibilities");
+        if (precision == STP_PRECISION_FP64) {
+            run_imager<double, double>(ctx, p, wp, n, d_vis, d_w, d_uvw, (double*)d_image, (double*)d_beam, d_total, d_ngood);
+        } else {
+            throw Error(STP_ERR_UNSUPPORTED, "FP32 compensated path not available in this build yet");
+        }
+        if (synchronize || diag) {
+            double tot = 0.0;
+            unsigned long long ng = 0;
+            HIP_CHECK(hipMemcpyAsync(&tot, d_total, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_CHECK(hipMemcpyAsync(&ng, d_ngood, sizeof(ng), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            if (diag) {
+                diag->sample_grid_total = tot;
+                diag->num_good_vis = ng;
+                diag->image_rows = tot > 0.0 ? p->image_size : 0;
+                diag->image_cols = tot > 0.0 ? p->image_size : 0;
+                diag->ms_total = 0.0;
+            }
+        }
+    });
+}
+
+stp_status stp_image_visibilities(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp,
+    const stp_aproj_pars* ap, size_t n, const double* vis, const double* w, const double* uvw, int32_t precision,
+    double* image, double* beam, stp_imager_diag* diag)
+{
+    return guarded(ctx, [&] {
+        check_imager_pars(p, wp, ap);
+        STP_REQUIRE(image != nullptr, "image buffer is NULL");
+        STP_REQUIRE(!p->generate_beam || beam != nullptr, "beam buffer is NULL with generate_beam");
+        STP_REQUIRE(n == 0 || (vis && w && uvw), "visibility buffers are NULL");
+        const size_t isz = size_t(p->image_size) * p->image_size;
+        const size_t osz = (precision == STP_PRECISION_FP64) ? sizeof(double) : sizeof(float);
+        double* d_in = (double*)ws_get(ctx, "host_in", sizeof(double) * (6 * n + 1));
+        char* d_out = (char*)ws_get(ctx, "host_out", osz * isz * 2);
+        if (n) {
+            HIP_CHECK(hipMemcpyAsync(d_in, vis, sizeof(double) * 2 * n, hipMemcpyHostToDevice, ctx->stream));
+            HIP_CHECK(hipMemcpyAsync(d_in + 2 * n, w, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+            HIP_CHECK(hipMemcpyAsync(d_in + 3 * n, uvw, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+        }
+        stp_imager_diag dg{};
+        const stp_status st = stp_image_visibilities_device(ctx, p, wp, ap, n, d_in, d_in + 2 * n, d_in + 3 * n,
+            precision, d_out, p->generate_beam ? d_out + osz * isz : nullptr, 1, &dg);
+        if (st != STP_OK)
+            throw Error(st, ctx->last_error);
+        if (dg.image_rows > 0) {
+            if (precision == STP_PRECISION_FP64) {
+                HIP_CHECK(hipMemcpyAsync(image, d_out, sizeof(double) * isz, hipMemcpyDeviceToHost, ctx->stream));
+                if (p->generate_beam)
+                    HIP_CHECK(hipMemcpyAsync(beam, d_out + osz * isz, sizeof(double) * isz, hipMemcpyDeviceToHost, ctx->stream));
+                HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            } else {
+                std::vector<float> tmp(isz);
+                HIP_CHECK(hipMemcpy(tmp.data(), d_out, sizeof(float) * isz, hipMemcpyDeviceToHost));
+                for (size_t i = 0; i < isz; i++)
+                    image[i] = tmp[i];
+                if (p->generate_beam) {
+                    HIP_CHECK(hipMemcpy(tmp.data(), d_out + osz * isz, sizeof(float) * isz, hipMemcpyDeviceToHost));
+                    for (size_t i = 0; i < isz; i++)
+                        beam[i] = tmp[i];
+                }
+            }
+        }
+        if (diag)
+            *diag = dg;
+    });
+}
+
+stp_status stp_convolve_to_grid(stp_ctx* ctx, int32_t kernel_function, double kernel_param, uint32_t support,
+    int32_t image_size, size_t n, const double* uv, const double* vis, const double* w, int32_t kernel_exact,
+    uint32_t oversampling, int32_t shift_uv, int32_t halfplane, int32_t generate_beam, double* vis_grid,
+    double* sampling_grid, double* sample_grid_total)
+{
+    return guarded(ctx, [&] {
+        STP_REQUIRE(vis_grid != nullptr, "vis_grid is NULL");
+        STP_REQUIRE(!generate_beam || sampling_grid != nullptr, "sampling_grid is NULL with generate_beam");
+        STP_REQUIRE(kernel_function >= STP_KERNEL_TOPHAT && kernel_function <= STP_KERNEL_PSWF, "unknown kernel");
+        STP_REQUIRE(kernel_exact || oversampling >= 1, "oversampling must be >= 1");
+        STP_REQUIRE(n == 0 || (uv && vis && w), "visibility buffers are NULL");
+        const int N = image_size;
+        const int rows = halfplane ? N / 2 + 1 : N;
+        const size_t ge = size_t(rows) * size_t(N);
+        double* d_in = (double*)ws_get(ctx, "host_in", sizeof(double) * (5 * n + 1));
+        if (n) {
+            HIP_CHECK(hipMemcpyAsync(d_in, uv, sizeof(double) * 2 * n, hipMemcpyHostToDevice, ctx->stream));
+            HIP_CHECK(hipMemcpyAsync(d_in + 2 * n, vis, sizeof(double) * 2 * n, hipMemcpyHostToDevice, ctx->stream));
+            HIP_CHECK(hipMemcpyAsync(d_in + 4 * n, w, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+        }
+        cplx<double>* grids = (cplx<double>*)ws_get(ctx, "unit_grids", sizeof(cplx<double>) * ge * 2);
+        double* d_total = (double*)ws_get(ctx, "unit_total", sizeof(double));
+        GridderArgs a;
+        a.image_size = N;
+        a.support = int(support);
+        a.oversampling = int(oversampling);
+        a.kernel_exact = kernel_exact != 0;
+        a.shift_uv = shift_uv != 0;
+        a.halfplane = halfplane != 0;
+        a.generate_beam = generate_beam != 0;
+        a.uv_scale = 1.0;
+        a.kernel_function = kernel_function;
+        a.kernel_param = kernel_param;
+        grid_visibilities<double>(ctx, a, n, d_in, d_in + 2 * n, d_in + 4 * n, grids, generate_beam ? grids + ge : nullptr,
+            d_total, nullptr);
+        HIP_CHECK(hipMemcpyAsync(vis_grid, grids, sizeof(cplx<double>) * ge, hipMemcpyDeviceToHost, ctx->stream));
+        if (generate_beam)
+            HIP_CHECK(hipMemcpyAsync(sampling_grid, grids + ge, sizeof(cplx<double>) * ge, hipMemcpyDeviceToHost, ctx->stream));
+        double tot = 0.0;
+        HIP_CHECK(hipMemcpyAsync(&tot, d_total, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (sample_grid_total)
+            *sample_grid_total = tot;
+    });
+}
+
+stp_status stp_fft_c2r(stp_ctx* ctx, const double* in, int32_t in_rows, int32_t n_cols, double* out)
+{
+    return guarded(ctx, [&] {
+        STP_REQUIRE(in && out, "NULL buffer");
+        STP_REQUIRE(in_rows >= 3, "in_rows must be n/2+1 with n >= 4");
+        const int n = 2 * (in_rows - 1);
+        const size_t ge = size_t(in_rows) * n_cols;
+        cplx<double>* g = (cplx<double>*)ws_get(ctx, "unit_fft_in", sizeof(cplx<double>) * ge);
+        double* o = (double*)ws_get(ctx, "unit_fft_out", sizeof(double) * size_t(n) * n_cols);
+        HIP_CHECK(hipMemcpyAsync(g, in, sizeof(cplx<double>) * ge, hipMemcpyHostToDevice, ctx->stream));
+        STP_REQUIRE(n == n_cols, "c2r currently needs a square output (n_rows == n_cols)");
+        NormSpec ns;
+        ns.image_size = n;
+        OutPtrs<double> outs{};
+        outs.p[0] = o;
+        fft_c2r_2d<double, double>(ctx, g, 1, n, n_cols, outs, ns);
+        HIP_CHECK(hipMemcpyAsync(out, o, sizeof(double) * size_t(n) * n_cols, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+stp_status stp_fft_c2c(stp_ctx* ctx, const double* in, int32_t n, int32_t forward, double* out)
+{
+    return guarded(ctx, [&] {
+        STP_REQUIRE(in && out, "NULL buffer");
+        const size_t ne = size_t(n) * n;
+        cplx<double>* a = (cplx<double>*)ws_get(ctx, "unit_c2c_a", sizeof(cplx<double>) * ne);
+        cplx<double>* b = (cplx<double>*)ws_get(ctx, "unit_c2c_b", sizeof(cplx<double>) * ne);
+        HIP_CHECK(hipMemcpyAsync(a, in, sizeof(cplx<double>) * ne, hipMemcpyHostToDevice, ctx->stream));
+        fft_c2c_2d<double>(ctx, a, b, n, forward != 0);
+        HIP_CHECK(hipMemcpyAsync(out, b, sizeof(cplx<double>) * ne, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+stp_status stp_source_find(stp_ctx* ctx, const double* image, int32_t rows, int32_t cols, const stp_sourcefind_pars* p,
+    int32_t* label_map, stp_island* islands, size_t cap, size_t* n_islands, stp_sourcefind_stats* stats)
+{
+    return guarded(ctx, [&] {
+        STP_REQUIRE(image && p && n_islands, "NULL argument");
+        STP_REQUIRE(rows > 0 && cols > 0, "empty image");
+        const size_t ne = size_t(rows) * cols;
+        double* d_img = (double*)ws_get(ctx, "host_sf_img", sizeof(double) * ne);
+        int32_t* d_lab = label_map ? (int32_t*)ws_get(ctx, "host_sf_lab", sizeof(int32_t) * ne) : nullptr;
+        HIP_CHECK(hipMemcpyAsync(d_img, image, sizeof(double) * ne, hipMemcpyHostToDevice, ctx->stream));
+        source_find_device(ctx, d_img, STP_PRECISION_FP64, rows, cols, *p, d_lab, islands, cap, n_islands, stats);
+        if (label_map) {
+            HIP_CHECK(hipMemcpyAsync(label_map, d_lab, sizeof(int32_t) * ne, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        }
+    });
+}
+
+stp_status stp_source_find_device(stp_ctx* ctx, const void* d_image, int32_t prec, int32_t rows, int32_t cols,
+    const stp_sourcefind_pars* p, int32_t* d_label_map, stp_island* islands, size_t cap, size_t* n_islands,
+    stp_sourcefind_stats* stats)
+{
+    return guarded(ctx, [&] {
+        STP_REQUIRE(d_image && p && n_islands, "NULL argument");
+        source_find_device(ctx, d_image, prec, rows, cols, *p, d_label_map, islands, cap, n_islands, stats);
+    });
+}
+
+stp_status stp_device_malloc(stp_ctx* ctx, void** ptr, size_t bytes)
+{
+    return guarded(ctx, [&] {
+        STP_REQUIRE(ptr != nullptr, "NULL pointer");
+        HIP_CHECK(hipMalloc(ptr, bytes ? bytes : 1));
+    });
+}
+
+stp_status stp_device_free(stp_ctx* ctx, void* ptr)
+{
+    return guarded(ctx, [&] {
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        HIP_CHECK(hipFree(ptr));
+    });
+}
+
+stp_status stp_memcpy_htod(stp_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    return guarded(ctx, [&] {
+        HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+stp_status stp_memcpy_dtoh(stp_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    return guarded(ctx, [&] {
+        HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+stp_status stp_ctx_set_timing(stp_ctx* ctx, int32_t enable)
+{
+    return guarded(ctx, [&] { ctx->timing = enable != 0; });
+}
+
+stp_status stp_ctx_reset_timings(stp_ctx* ctx)
+{
+    return guarded(ctx, [&] {
+        resolve_timings(ctx);
+        ctx->timings.clear();
+    });
+}
+
+stp_status stp_ctx_get_timings(stp_ctx* ctx, char* names, double* total_ms, uint64_t* counts, size_t cap, size_t* n)
+{
+    return guarded(ctx, [&] {
+        resolve_timings(ctx);
+        size_t i = 0;
+        for (auto& kv : ctx->timings) {
+            if (i < cap) {
+                if (names) {
+                    std::memset(names + 64 * i, 0, 64);
+                    std::strncpy(names + 64 * i, kv.first.c_str(), 63);
+                }
+                if (total_ms)
+                    total_ms[i] = kv.second.ms;
+                if (counts)
+                    counts[i] = kv.second.count;
+            }
+            i++;
+        }
+        if (n)
+            *n = i;
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,133 @@
+// Shared definitions of the gfx950 STP library: context, workspace, errors,
+// launch timing and small device helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/stp_capi.h"
+
+namespace stp_hip {
+
+// Error carried from the implementation to the C-ABI boundary.
+struct Error : std::runtime_error {
+    stp_status code;
+    Error(stp_status c, const std::string& m)
+        : std::runtime_error(m)
+        , code(c)
+    {
+    }
+};
+
+#define STP_REQUIRE(cond, msg)                                              \
+    do {                                                                    \
+        if (!(cond))                                                        \
+            throw ::stp_hip::Error(STP_ERR_INVALID_ARGUMENT, (msg));        \
+    } while (0)
+
+#define HIP_CHECK(expr)                                                                          \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            throw ::stp_hip::Error(e_ == hipErrorOutOfMemory ? STP_ERR_OUT_OF_MEMORY : STP_ERR_DEVICE, \
+                std::string(#expr) + ": " + hipGetErrorString(e_));                              \
+    } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct TimingAcc {
+    double ms = 0.0;
+    uint64_t count = 0;
+};
+
+struct PendingTiming {
+    std::string name;
+    hipEvent_t start, stop;
+};
+
+// Twiddle table exp(+2 pi i k / n), k < n, in double and float.
+struct Twiddles {
+    double* d = nullptr; // 2n doubles (re, im)
+    float* f = nullptr;  // 2n floats
+    int n = 0;
+};
+
+} // namespace stp_hip
+
+struct stp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    std::map<std::string, stp_hip::DevBuf> ws;
+    std::map<int, stp_hip::Twiddles> twiddles;
+    std::map<std::string, stp_hip::DevBuf> tables;
+    bool timing = false;
+    std::vector<stp_hip::PendingTiming> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, stp_hip::TimingAcc> timings;
+    int num_cus = 256;
+};
+
+namespace stp_hip {
+
+// Returns a device buffer of at least `bytes` for slot `name` (contents undefined).
+void* ws_get(stp_ctx* ctx, const std::string& name, size_t bytes);
+// Device copy of a host table, uploaded once per key and kept for the context's lifetime.
+const double* dev_table(stp_ctx* ctx, const std::string& key, const std::vector<double>& host);
+bool has_table(stp_ctx* ctx, const std::string& key);
+const Twiddles& twiddles_for(stp_ctx* ctx, int n);
+
+// Kernel-launch bracket: records HIP events on the context stream when timing is on.
+struct LaunchTimer {
+    stp_ctx* ctx;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    LaunchTimer(stp_ctx* c, const char* n);
+    ~LaunchTimer();
+};
+
+void resolve_timings(stp_ctx* ctx);
+
+template <typename K, typename... Args>
+inline void launch(stp_ctx* ctx, const char* name, K kernel, dim3 grid, dim3 block, size_t shmem, Args... args)
+{
+    if (grid.x == 0 || grid.y == 0 || grid.z == 0)
+        return;
+    LaunchTimer t(ctx, name);
+    hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream, args...);
+    HIP_CHECK(hipGetLastError());
+}
+
+inline unsigned div_up(size_t a, size_t b) { return unsigned((a + b - 1) / b); }
+
+} // namespace stp_hip
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+struct cplx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b)
+{
+    return { a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x };
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return { a.x + b.x, a.y + b.y }; }
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return { a.x - b.x, a.y - b.y }; }
+template <typename T>
+__device__ __forceinline__ cplx<T> cconj(cplx<T> a) { return { a.x, -a.y }; }

@@ -1,0 +1,120 @@
+// Context, workspace, twiddle tables and launch timing of the gfx950 STP library.
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+
+namespace stp_hip {
+
+void* ws_get(stp_ctx* ctx, const std::string& name, size_t bytes)
+{
+    if (bytes == 0)
+        bytes = 256;
+    DevBuf& b = ctx->ws[name];
+    if (b.bytes >= bytes)
+        return b.ptr;
+    if (b.ptr) {
+        // the old buffer may still be read by queued work
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        HIP_CHECK(hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    const size_t alloc = (bytes + 4095) & ~size_t(4095);
+    HIP_CHECK(hipMalloc(&b.ptr, alloc));
+    b.bytes = alloc;
+    return b.ptr;
+}
+
+bool has_table(stp_ctx* ctx, const std::string& key) { return ctx->tables.count(key) != 0; }
+
+const double* dev_table(stp_ctx* ctx, const std::string& key, const std::vector<double>& host)
+{
+    auto it = ctx->tables.find(key);
+    if (it != ctx->tables.end())
+        return (const double*)it->second.ptr;
+    DevBuf b;
+    b.bytes = std::max<size_t>(8, host.size() * sizeof(double));
+    HIP_CHECK(hipMalloc(&b.ptr, b.bytes));
+    if (!host.empty())
+        HIP_CHECK(hipMemcpy(b.ptr, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice));
+    ctx->tables[key] = b;
+    return (const double*)b.ptr;
+}
+
+// exp(+2 pi i k / n) for k < n, evaluated in long double and rounded once.
+const Twiddles& twiddles_for(stp_ctx* ctx, int n)
+{
+    auto it = ctx->twiddles.find(n);
+    if (it != ctx->twiddles.end())
+        return it->second;
+    std::vector<double> hd(2 * size_t(n));
+    std::vector<float> hf(2 * size_t(n));
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int k = 0; k < n; k++) {
+        const long double a = two_pi * (long double)k / (long double)n;
+        const long double c = cosl(a), s = sinl(a);
+        hd[2 * k] = double(c);
+        hd[2 * k + 1] = double(s);
+        hf[2 * k] = float(c);
+        hf[2 * k + 1] = float(s);
+    }
+    Twiddles t;
+    t.n = n;
+    HIP_CHECK(hipMalloc(&t.d, hd.size() * sizeof(double)));
+    HIP_CHECK(hipMalloc(&t.f, hf.size() * sizeof(float)));
+    HIP_CHECK(hipMemcpyAsync(t.d, hd.data(), hd.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_CHECK(hipMemcpyAsync(t.f, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return ctx->twiddles.emplace(n, t).first->second;
+}
+
+static hipEvent_t take_event(stp_ctx* ctx)
+{
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+LaunchTimer::LaunchTimer(stp_ctx* c, const char* n)
+    : ctx(c)
+    , name(n)
+{
+    if (ctx->timing) {
+        a = take_event(ctx);
+        b = take_event(ctx);
+        HIP_CHECK(hipEventRecord(a, ctx->stream));
+    }
+}
+
+LaunchTimer::~LaunchTimer()
+{
+    if (a) {
+        if (hipEventRecord(b, ctx->stream) == hipSuccess)
+            ctx->pending.push_back({ name, a, b });
+    }
+}
+
+void resolve_timings(stp_ctx* ctx)
+{
+    if (ctx->pending.empty())
+        return;
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (auto& p : ctx->pending) {
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, p.start, p.stop));
+        TimingAcc& acc = ctx->timings[p.name];
+        acc.ms += ms;
+        acc.count++;
+        ctx->event_pool.push_back(p.start);
+        ctx->event_pool.push_back(p.stop);
+    }
+    ctx->pending.clear();
+}
+
+} // namespace stp_hip

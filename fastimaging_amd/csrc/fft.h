@@ -1,0 +1,31 @@
+// Host entry points of the Stockham FFT kernels (fft.hip).
+#pragma once
+
+#include "common.h"
+
+namespace stp_hip {
+
+// Normalisation fused into the last c2r pass (imager.h:96-213, non-FFTSHIFT):
+// out(i, j) = img(oi, oj) / *total_ptr [/ (gcf[ii] * gcf[jj])], cropped from the
+// padded transform size to image_size.
+struct NormSpec {
+    int image_size = 0;
+    const double* total_ptr = nullptr; // device scalar; NULL = no scaling
+    const double* gcf = nullptr;       // device array of the padded size; NULL = none
+};
+
+template <typename OT>
+struct OutPtrs {
+    OT* p[4];
+};
+
+// Unnormalised 2-D c2r of nbatch half-plane grids ((n_rows/2+1) x N each,
+// contiguous) into nbatch outputs; the grids are overwritten.
+template <typename T, typename OT>
+void fft_c2r_2d(stp_ctx* ctx, cplx<T>* grids, int nbatch, int n_rows, int N, const OutPtrs<OT>& outs, const NormSpec& ns);
+
+// 2-D c2c of an N x N column-major matrix; `data` is overwritten, result in `out`.
+template <typename T>
+void fft_c2c_2d(stp_ctx* ctx, cplx<T>* data, cplx<T>* out, int N, bool forward);
+
+} // namespace stp_hip

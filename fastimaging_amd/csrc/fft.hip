@@ -1,0 +1,409 @@
+// Hand-written Stockham FFTs for gfx950 with the semantics of the reference's
+// FFTW calls (src/stp/common/fft.cpp:45-135 c2r, :227-323 c2c).
+//
+// Layout: a half-plane grid is (n/2+1) rows (v) x N columns (u), column-major,
+// complex.  The 2-D c2r runs
+//   U1, U2 : four-step c2c along u (N = L1 * L2), in place; element (v, x) of
+//            the transformed grid sits in column slot (x % L1) * L2 + x / L1;
+//   V      : per output column x, c2r along v of length n via one complex FFT
+//            of length n/2 (even/odd packing); the imaginary parts of the v=0
+//            and v=n/2 rows are ignored, as FFTW's c2r does.  Normalisation,
+//            padding crop and grid correction of the imager are fused here.
+// Each transform runs in LDS with 64-wide waves; a workgroup holds NT
+// transforms of length L, TPT = 256/NT threads per transform.
+#include "common.h"
+#include "fft.h"
+
+namespace {
+
+using stp_hip::NormSpec;
+using stp_hip::OutPtrs;
+
+constexpr int WG = 256;
+
+template <int L>
+struct Log2 {
+    static constexpr int value = 1 + Log2<L / 2>::value;
+};
+template <>
+struct Log2<1> {
+    static constexpr int value = 0;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> load_tw(const T* tw, int idx, bool inverse)
+{
+    cplx<T> w = { tw[2 * idx], tw[2 * idx + 1] };
+    if (!inverse)
+        w.y = -w.y;
+    return w;
+}
+
+template <typename T>
+__device__ __forceinline__ void bfly2(cplx<T>* v)
+{
+    const cplx<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+// Radix-4 DFT, y_k = sum_r v_r w^{rk}, w = exp(+-i pi/2).
+template <typename T>
+__device__ __forceinline__ void bfly4(cplx<T>* v, bool inverse)
+{
+    const cplx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    const cplx<T> a2 = cadd(v[1], v[3]);
+    const cplx<T> d = csub(v[1], v[3]);
+    // a3 = w * d with w = +i (inverse) or -i (forward)
+    const cplx<T> a3 = inverse ? cplx<T>{ -d.y, d.x } : cplx<T>{ d.y, -d.x };
+    v[0] = cadd(a0, a2);
+    v[2] = csub(a0, a2);
+    v[1] = cadd(a1, a3);
+    v[3] = csub(a1, a3);
+}
+
+// One Stockham stage of radix R on a length-L transform held at buf.
+template <typename T, int L, int TPT, int R>
+__device__ __forceinline__ void stockham_stage(cplx<T>* buf, int lt, int Ns, const T* tw, int tws, bool inverse)
+{
+    constexpr int NB = L / R;
+    constexpr int BPT = (NB + TPT - 1) / TPT;
+    cplx<T> v[BPT][R];
+#pragma unroll
+    for (int b = 0; b < BPT; b++) {
+        const int j = lt + b * TPT;
+        if (j < NB) {
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                v[b][r] = buf[j + r * NB];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < BPT; b++) {
+        const int j = lt + b * TPT;
+        if (j < NB) {
+            const int k = j & (Ns - 1);
+            if (Ns > 1) {
+                const int step = L / (Ns * R);
+#pragma unroll
+                for (int r = 1; r < R; r++)
+                    v[b][r] = cmul(v[b][r], load_tw(tw, r * k * step * tws, inverse));
+            }
+            if (R == 4)
+                bfly4(v[b], inverse);
+            else
+                bfly2(v[b]);
+            const int idxD = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                buf[idxD + r * Ns] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+// In-place unnormalised FFT of length L at buf, TPT cooperating threads (lt in [0, TPT)).
+// tw: table exp(+2 pi i m / (L*tws)); inverse selects the + sign.
+template <typename T, int L, int TPT>
+__device__ __forceinline__ void lds_fft(cplx<T>* buf, int lt, const T* tw, int tws, bool inverse)
+{
+    constexpr int LOG = Log2<L>::value;
+    int Ns = 1;
+    if (LOG & 1) {
+        stockham_stage<T, L, TPT, 2>(buf, lt, Ns, tw, tws, inverse);
+        Ns = 2;
+    }
+    if constexpr (L >= 4) {
+        for (; Ns < L; Ns *= 4)
+            stockham_stage<T, L, TPT, 4>(buf, lt, Ns, tw, tws, inverse);
+    }
+}
+
+template <int L>
+struct Shape {
+    static constexpr int TPT = (L / 4 < 1) ? 1 : ((L / 4 > WG) ? WG : L / 4);
+    static constexpr int NT = WG / TPT;
+    static constexpr int PAD = 1;
+};
+
+// U1: for each u2 and block of NT rows, FFT over u1 (stride L2 columns), then
+// multiply by exp(+-2 pi i k1 u2 / N).
+template <typename T, int L1>
+__global__ __launch_bounds__(WG) void fft_u1(cplx<T>* grids, int rows, int N, int L2, size_t grid_elems,
+    const T* tw, int inverse)
+{
+    using S = Shape<L1>;
+    __shared__ cplx<T> buf[S::NT * (L1 + S::PAD)];
+    const int u2 = blockIdx.x;
+    const int v0 = blockIdx.y * S::NT;
+    cplx<T>* g = grids + size_t(blockIdx.z) * grid_elems;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < S::NT * L1; idx += WG) {
+        const int vv = idx % S::NT, u1 = idx / S::NT;
+        const int v = v0 + vv;
+        cplx<T> x = { T(0), T(0) };
+        if (v < rows)
+            x = g[size_t(v) + size_t(u1 * L2 + u2) * rows];
+        buf[vv * (L1 + S::PAD) + u1] = x;
+    }
+    __syncthreads();
+    const int t = tid / S::TPT, lt = tid % S::TPT;
+    lds_fft<T, L1, S::TPT>(buf + t * (L1 + S::PAD), lt, tw, N / L1, inverse != 0);
+    for (int idx = tid; idx < S::NT * L1; idx += WG) {
+        const int vv = idx % S::NT, k1 = idx / S::NT;
+        const int v = v0 + vv;
+        if (v < rows) {
+            cplx<T> x = buf[vv * (L1 + S::PAD) + k1];
+            x = cmul(x, load_tw(tw, (k1 * u2) & (N - 1), inverse != 0));
+            g[size_t(v) + size_t(k1 * L2 + u2) * rows] = x;
+        }
+    }
+}
+
+// U2: for each k1 and block of rows, FFT over u2 (contiguous column slots).
+template <typename T, int L2>
+__global__ __launch_bounds__(WG) void fft_u2(cplx<T>* grids, int rows, int N, int L1, size_t grid_elems,
+    const T* tw, int inverse)
+{
+    using S = Shape<L2>;
+    __shared__ cplx<T> buf[S::NT * (L2 + S::PAD)];
+    const int k1 = blockIdx.x;
+    const int v0 = blockIdx.y * S::NT;
+    cplx<T>* g = grids + size_t(blockIdx.z) * grid_elems;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < S::NT * L2; idx += WG) {
+        const int vv = idx % S::NT, u2 = idx / S::NT;
+        const int v = v0 + vv;
+        cplx<T> x = { T(0), T(0) };
+        if (v < rows)
+            x = g[size_t(v) + size_t(k1 * L2 + u2) * rows];
+        buf[vv * (L2 + S::PAD) + u2] = x;
+    }
+    __syncthreads();
+    const int t = tid / S::TPT, lt = tid % S::TPT;
+    lds_fft<T, L2, S::TPT>(buf + t * (L2 + S::PAD), lt, tw, N / L2, inverse != 0);
+    for (int idx = tid; idx < S::NT * L2; idx += WG) {
+        const int vv = idx % S::NT, k2 = idx / S::NT;
+        const int v = v0 + vv;
+        if (v < rows)
+            g[size_t(v) + size_t(k1 * L2 + k2) * rows] = buf[vv * (L2 + S::PAD) + k2];
+    }
+}
+
+// V: c2r along v of length n = 2M for output column x (natural order), reading
+// column slot (x % L1) * L2 + x / L1.  Output value at padded (y, x) is
+//   z(y, x) * factor / (gcf[ii] * gcf[jj])   (gcf optional)
+// written to the cropped image_size x image_size output, as in
+// normalise_image_beam_result_1D (imager.h:96-213).
+template <typename T, typename OT, int M>
+__global__ __launch_bounds__(WG) void fft_v_c2r(const cplx<T>* grids, size_t grid_elems, int N, int L1, int L2,
+    const T* tw_n, OutPtrs<OT> outs, NormSpec ns)
+{
+    using S = Shape<M>;
+    constexpr int n = 2 * M;
+    constexpr int rows = M + 1;
+    __shared__ cplx<T> buf[S::NT * (M + S::PAD)];
+    const int tid = threadIdx.x;
+    const int t = tid / S::TPT, lt = tid % S::TPT;
+    const int x = blockIdx.x * S::NT + t;
+    const int gi = blockIdx.y;
+    const cplx<T>* g = grids + size_t(gi) * grid_elems;
+    const bool active = x < N;
+    const int slot = active ? ((x % L1) * L2 + x / L1) : 0;
+    cplx<T>* b = buf + t * (M + S::PAD);
+    const cplx<T>* col = g + size_t(slot) * rows;
+    // pack Z[k] = E[k] + i O[k]; thread handles the pair (k, M - k)
+    for (int k = lt; k <= M / 2; k += S::TPT) {
+        if (!active)
+            break;
+        const int km = M - k;
+        if (k == 0) {
+            const T x0 = col[0].x, xm = col[M].x; // imaginary parts of v=0 and v=n/2 ignored
+            b[0] = { x0 + xm, x0 - xm };
+            continue;
+        }
+        const cplx<T> a = col[k], c = col[km];
+        // E[k] = X[k] + conj(X[M-k]);  O[k] = (X[k] - conj(X[M-k])) W^k, W = exp(+2 pi i / n)
+        const cplx<T> e1 = { a.x + c.x, a.y - c.y };
+        const cplx<T> d1 = { a.x - c.x, a.y + c.y };
+        const cplx<T> o1 = cmul(d1, load_tw(tw_n, k, true));
+        b[k] = { e1.x - o1.y, e1.y + o1.x };
+        if (km != k) {
+            const cplx<T> e2 = { c.x + a.x, c.y - a.y };
+            const cplx<T> d2 = { c.x - a.x, c.y + a.y };
+            const cplx<T> o2 = cmul(d2, load_tw(tw_n, km, true));
+            b[km] = { e2.x - o2.y, e2.y + o2.x };
+        }
+    }
+    __syncthreads();
+    lds_fft<T, M, S::TPT>(b, lt, tw_n, 2, true);
+    if (!active)
+        return;
+    // crop mapping of the padded column x (imager.h:99-150)
+    const int hi = ns.image_size / 2, hp = n / 2;
+    int j;
+    if (x < hi)
+        j = x;
+    else if (x >= N - (ns.image_size - hi))
+        j = x - N + ns.image_size;
+    else
+        return;
+    const int joff = (j < hi) ? j : j - ns.image_size;
+    OT* out = outs.p[gi] + size_t(j) * ns.image_size;
+    const bool scale = ns.total_ptr != nullptr;
+    const double factor = scale ? (1.0 / ns.total_ptr[0]) : 1.0;
+    for (int m = lt; m < M; m += S::TPT) {
+        const cplx<T> z = b[m];
+        const int ys[2] = { 2 * m, 2 * m + 1 };
+        const T vals[2] = { z.x, z.y };
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int y = ys[q];
+            int i;
+            if (y < hi)
+                i = y;
+            else if (y >= n - (ns.image_size - hi))
+                i = y - n + ns.image_size;
+            else
+                continue;
+            double val = scale ? double(vals[q]) * factor : double(vals[q]);
+            if (ns.gcf) {
+                const int ioff = (i < hi) ? i : i - ns.image_size;
+                val = val / (ns.gcf[hp + ioff] * ns.gcf[hp + joff]);
+            }
+            out[i] = OT(val);
+        }
+    }
+}
+
+// Column c2c of length L (contiguous) reading column slot of a four-step row
+// transform and writing natural column x of out.
+template <typename T, int L>
+__global__ __launch_bounds__(WG) void fft_cols_c2c(const cplx<T>* in, int rows, int N, int L1, int L2,
+    cplx<T>* out, const T* tw_rows, int inverse)
+{
+    using S = Shape<L>;
+    __shared__ cplx<T> buf[S::NT * (L + S::PAD)];
+    const int tid = threadIdx.x;
+    const int t = tid / S::TPT, lt = tid % S::TPT;
+    const int x = blockIdx.x * S::NT + t;
+    const bool active = x < N;
+    const int slot = active ? ((x % L1) * L2 + x / L1) : 0;
+    cplx<T>* b = buf + t * (L + S::PAD);
+    if (active)
+        for (int r = lt; r < L; r += S::TPT)
+            b[r] = in[size_t(slot) * rows + r];
+    __syncthreads();
+    lds_fft<T, L, S::TPT>(b, lt, tw_rows, 1, inverse != 0);
+    if (active)
+        for (int r = lt; r < L; r += S::TPT)
+            out[size_t(x) * rows + r] = b[r];
+}
+
+template <typename T>
+void split_len(int N, int& L1, int& L2)
+{
+    int lg = 0;
+    while ((1 << lg) < N)
+        lg++;
+    L1 = 1 << ((lg + 1) / 2);
+    L2 = N / L1;
+}
+
+#define STP_POW2_CASES(X) X(2) X(4) X(8) X(16) X(32) X(64) X(128)
+
+template <typename T>
+void launch_u_passes(stp_ctx* ctx, cplx<T>* grids, int nbatch, int rows, int N, size_t grid_elems, const T* tw, bool inverse)
+{
+    int L1, L2;
+    split_len<T>(N, L1, L2);
+    using namespace stp_hip;
+    switch (L1) {
+#define U1CASE(LL)                                                                                          \
+    case LL:                                                                                                \
+        launch(ctx, "fft_u1", fft_u1<T, LL>, dim3(L2, div_up(rows, Shape<LL>::NT), nbatch), dim3(WG), 0,   \
+            grids, rows, N, L2, grid_elems, tw, int(inverse));                                              \
+        break;
+        STP_POW2_CASES(U1CASE)
+#undef U1CASE
+    default:
+        throw Error(STP_ERR_UNSUPPORTED, "FFT length not supported: " + std::to_string(N));
+    }
+    switch (L2) {
+#define U2CASE(LL)                                                                                          \
+    case LL:                                                                                                \
+        launch(ctx, "fft_u2", fft_u2<T, LL>, dim3(L1, div_up(rows, Shape<LL>::NT), nbatch), dim3(WG), 0,   \
+            grids, rows, N, L1, grid_elems, tw, int(inverse));                                              \
+        break;
+        STP_POW2_CASES(U2CASE)
+#undef U2CASE
+    default:
+        throw Error(STP_ERR_UNSUPPORTED, "FFT length not supported: " + std::to_string(N));
+    }
+}
+
+} // namespace
+
+namespace stp_hip {
+
+template <typename T, typename OT>
+void fft_c2r_2d(stp_ctx* ctx, cplx<T>* grids, int nbatch, int n_rows, int N, const OutPtrs<OT>& outs, const NormSpec& ns)
+{
+    STP_REQUIRE(N >= 4 && (N & (N - 1)) == 0, "FFT column count must be a power of two >= 4");
+    STP_REQUIRE(n_rows >= 4 && (n_rows & (n_rows - 1)) == 0, "FFT row count must be a power of two >= 4");
+    const int rows = n_rows / 2 + 1;
+    const size_t grid_elems = size_t(rows) * N;
+    const Twiddles& twu = twiddles_for(ctx, N);
+    const T* tw = std::is_same<T, double>::value ? (const T*)twu.d : (const T*)twu.f;
+    launch_u_passes<T>(ctx, grids, nbatch, rows, N, grid_elems, tw, true);
+    int L1, L2;
+    split_len<T>(N, L1, L2);
+    const Twiddles& twv = twiddles_for(ctx, n_rows);
+    const T* tvn = std::is_same<T, double>::value ? (const T*)twv.d : (const T*)twv.f;
+    STP_REQUIRE(nbatch >= 1 && nbatch <= 4, "1 to 4 grids per FFT launch");
+    const int M = n_rows / 2;
+    switch (M) {
+#define VCASE(MM)                                                                                              \
+    case MM:                                                                                                   \
+        launch(ctx, "fft_v_c2r", fft_v_c2r<T, OT, MM>, dim3(div_up(N, Shape<MM>::NT), nbatch), dim3(WG), 0,   \
+            (const cplx<T>*)grids, grid_elems, N, L1, L2, tvn, outs, ns);                                      \
+        break;
+        VCASE(2) VCASE(4) VCASE(8) VCASE(16) VCASE(32) VCASE(64) VCASE(128) VCASE(256) VCASE(512) VCASE(1024)
+        VCASE(2048) VCASE(4096)
+#undef VCASE
+    default:
+        throw Error(STP_ERR_UNSUPPORTED, "c2r length not supported: " + std::to_string(n_rows));
+    }
+}
+
+template <typename T>
+void fft_c2c_2d(stp_ctx* ctx, cplx<T>* data, cplx<T>* out, int N, bool forward)
+{
+    STP_REQUIRE(N >= 4 && (N & (N - 1)) == 0, "c2c size must be a power of two >= 4");
+    const Twiddles& twn = twiddles_for(ctx, N);
+    const T* tw = std::is_same<T, double>::value ? (const T*)twn.d : (const T*)twn.f;
+    // rows: transform along columns index (u) for every row, in place (four-step)
+    launch_u_passes<T>(ctx, data, 1, N, N, size_t(N) * N, tw, !forward);
+    int L1, L2;
+    split_len<T>(N, L1, L2);
+    switch (N) {
+#define CCASE(LL)                                                                                           \
+    case LL:                                                                                                \
+        launch(ctx, "fft_cols_c2c", fft_cols_c2c<T, LL>, dim3(div_up(N, Shape<LL>::NT)), dim3(WG), 0,      \
+            (const cplx<T>*)data, N, N, L1, L2, out, tw, int(!forward));                                    \
+        break;
+        CCASE(4) CCASE(8) CCASE(16) CCASE(32) CCASE(64) CCASE(128) CCASE(256) CCASE(512) CCASE(1024)
+        CCASE(2048) CCASE(4096)
+#undef CCASE
+    default:
+        throw Error(STP_ERR_UNSUPPORTED, "c2c size not supported: " + std::to_string(N));
+    }
+}
+
+template void fft_c2r_2d<double, double>(stp_ctx*, cplx<double>*, int, int, int, const OutPtrs<double>&, const NormSpec&);
+template void fft_c2r_2d<float, float>(stp_ctx*, cplx<float>*, int, int, int, const OutPtrs<float>&, const NormSpec&);
+template void fft_c2r_2d<float, double>(stp_ctx*, cplx<float>*, int, int, int, const OutPtrs<double>&, const NormSpec&);
+template void fft_c2c_2d<double>(stp_ctx*, cplx<double>*, cplx<double>*, int, bool);
+
+} // namespace stp_hip

@@ -1,0 +1,30 @@
+// Host entry of the tile-binned gridder (gridder.hip).
+#pragma once
+
+#include "common.h"
+
+namespace stp_hip {
+
+struct GridderArgs {
+    int image_size;      // grid width N (the padded image size for the imager)
+    int support;         // conv support s
+    int oversampling;    // as passed by the caller (forced even below)
+    bool kernel_exact;
+    bool shift_uv;
+    bool halfplane;
+    bool generate_beam;
+    double uv_scale;     // applied to u and v (imager: cell_rad * N; unit gridder: 1)
+    int kernel_function;
+    double kernel_param;
+};
+
+// Grids n visibilities into vis_grid (and sampling_grid), each rows x N complex
+// column-major with rows = N/2+1 (half-plane) or N.  uv: n x >=2 column-major
+// (u column then v column; a w column may follow).  total: device scalar that
+// receives sample_grid_total = 2 * sum of good weights.  ngood (device u64) may
+// be NULL.
+template <typename T>
+void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const double* d_uv, const double* d_vis,
+    const double* d_w, cplx<T>* d_vis_grid, cplx<T>* d_sampling_grid, double* d_total, unsigned long long* d_ngood);
+
+} // namespace stp_hip

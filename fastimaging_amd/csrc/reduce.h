@@ -1,0 +1,52 @@
+// Deterministic reductions shared by the gridder and the source finder.
+//
+// Canonical summation order (the order the CPU oracle restates): blocks of 4096
+// consecutive terms; lane t (0..255) of a block adds terms t, t+256, ...,
+// t+15*256 sequentially from 0.0; lane sums combine by the tree
+// p[t] = p[t] + p[t+off], off = 128, 64, ..., 1; block results are reduced again
+// the same way until one value is left.  Every sum is therefore bit-identical to
+// the oracle's, whatever the launch configuration.
+#pragma once
+
+#include "common.h"
+
+namespace stp_hip {
+
+constexpr int CANON_LANES = 256;
+constexpr int CANON_PER_LANE = 16;
+constexpr int CANON_BLOCK = CANON_LANES * CANON_PER_LANE;
+
+// Combines the lane pair (a, b) of all 256 threads of the block in the canonical
+// tree; thread 0 returns the block sums.  lds: 512 doubles.
+__device__ __forceinline__ void canon_tree(double& a, double& b, double* lds)
+{
+#pragma clang fp contract(off)
+    const int t = threadIdx.x;
+    lds[t] = a;
+    lds[CANON_LANES + t] = b;
+    __syncthreads();
+    for (int off = CANON_LANES / 2; off >= 1; off >>= 1) {
+        if (t < off) {
+            lds[t] = lds[t] + lds[t + off];
+            lds[CANON_LANES + t] = lds[CANON_LANES + t] + lds[CANON_LANES + t + off];
+        }
+        __syncthreads();
+    }
+    a = lds[0];
+    b = lds[CANON_LANES];
+}
+
+// Reduces m pairs (2m doubles at `partials`) to one pair written to `result`
+// (device, 2 doubles), with the canonical order.  `scratch` holds
+// canon_scratch_doubles(m) doubles.
+inline size_t canon_scratch_doubles(size_t m) { return 4 * ((m + CANON_BLOCK - 1) / CANON_BLOCK) + 4; }
+void canon_reduce_pairs(stp_ctx* ctx, const double* partials, size_t m, double* scratch, double* result);
+
+// Fill kernels (used instead of hipMemsetAsync so the work stays capturable).
+void fill_f64(stp_ctx* ctx, double* p, double v, size_t n);
+void fill_u32(stp_ctx* ctx, uint32_t* p, uint32_t v, size_t n);
+
+// Exclusive scan of m uint32 counts into offsets[0..m] and cursor[0..m).
+void exclusive_scan_u32(stp_ctx* ctx, const uint32_t* counts, int m, uint32_t* offsets, uint32_t* cursor);
+
+} // namespace stp_hip

@@ -1,0 +1,117 @@
+// Canonical-order pair reduction and exclusive scan kernels (see reduce.h).
+#include "reduce.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+using namespace stp_hip;
+
+__global__ __launch_bounds__(CANON_LANES) void canon_pairs_kernel(const double* in, size_t m, double* out)
+{
+    __shared__ double lds[2 * CANON_LANES];
+    const size_t base = size_t(blockIdx.x) * CANON_BLOCK;
+    double a = 0.0, b = 0.0;
+#pragma unroll
+    for (int j = 0; j < CANON_PER_LANE; j++) {
+        const size_t i = base + size_t(j) * CANON_LANES + threadIdx.x;
+        if (i < m) {
+            a = a + in[2 * i];
+            b = b + in[2 * i + 1];
+        }
+    }
+    canon_tree(a, b, lds);
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = a;
+        out[2 * blockIdx.x + 1] = b;
+    }
+}
+
+__global__ void fill_f64_kernel(double* p, double v, size_t n)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t n)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+constexpr int SCAN_WG = 1024;
+
+__global__ __launch_bounds__(SCAN_WG) void exclusive_scan_kernel(const uint32_t* counts, int m, uint32_t* offsets, uint32_t* cursor)
+{
+    __shared__ uint32_t part[SCAN_WG];
+    const int t = threadIdx.x;
+    const int chunk = (m + SCAN_WG - 1) / SCAN_WG;
+    const int b = t * chunk, e = min(m, b + chunk);
+    uint32_t s = 0;
+    for (int i = b; i < e; i++)
+        s += counts[i];
+    part[t] = s;
+    __syncthreads();
+    // Hillis-Steele inclusive scan of the per-thread sums
+    for (int off = 1; off < SCAN_WG; off <<= 1) {
+        const uint32_t v = (t >= off) ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;
+    for (int i = b; i < e; i++) {
+        offsets[i] = run;
+        cursor[i] = run;
+        run += counts[i];
+    }
+    if (t == SCAN_WG - 1)
+        offsets[m] = part[t];
+}
+
+} // namespace
+
+namespace stp_hip {
+
+void canon_reduce_pairs(stp_ctx* ctx, const double* partials, size_t m, double* scratch, double* result)
+{
+    if (m <= 1) {
+        if (m == 1)
+            HIP_CHECK(hipMemcpyAsync(result, partials, 2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        else
+            fill_f64(ctx, result, 0.0, 2);
+        return;
+    }
+    const double* src = partials;
+    size_t cur = m;
+    // ping-pong between the two halves of scratch; the last level writes result
+    double* bufs[2] = { scratch, scratch + 2 * ((m + CANON_BLOCK - 1) / CANON_BLOCK) };
+    int which = 0;
+    while (cur > 1) {
+        const size_t nb = (cur + CANON_BLOCK - 1) / CANON_BLOCK;
+        double* dst = (nb == 1) ? result : bufs[which];
+        launch(ctx, "canon_reduce", canon_pairs_kernel, dim3(unsigned(nb)), dim3(CANON_LANES), 0, src, cur, dst);
+        src = dst;
+        cur = nb;
+        which ^= 1;
+    }
+}
+
+void fill_f64(stp_ctx* ctx, double* p, double v, size_t n)
+{
+    launch(ctx, "fill", fill_f64_kernel, dim3(div_up(n, 256)), dim3(256), 0, p, v, n);
+}
+
+void fill_u32(stp_ctx* ctx, uint32_t* p, uint32_t v, size_t n)
+{
+    launch(ctx, "fill", fill_u32_kernel, dim3(div_up(n, 256)), dim3(256), 0, p, v, n);
+}
+
+void exclusive_scan_u32(stp_ctx* ctx, const uint32_t* counts, int m, uint32_t* offsets, uint32_t* cursor)
+{
+    launch(ctx, "tile_scan", exclusive_scan_kernel, dim3(1), dim3(SCAN_WG), 0, counts, m, offsets, cursor);
+}
+
+} // namespace stp_hip

@@ -1,0 +1,140 @@
+"""GPU parity of the imaging path through the C-ABI: gridder, FFTs and the full
+imager against the FP64 oracle and the reference's gridder known-answer tests.
+Tolerance (FP64 path): 1e-10 relative to the peak for images and grids."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from fastimaging_amd import capi, simdata
+from fastimaging_amd.stp import STPError
+from tests.helpers import peak_rel_err
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+KAT = json.load(open(os.path.join(GOLDEN, "gridder_kat.json")))["cases"]
+KERNELS = {"TopHat": capi.KERNEL_TOPHAT, "Triangle": capi.KERNEL_TRIANGLE}
+TOL = 1e-10
+
+
+@pytest.mark.parametrize("case", KAT, ids=[c["name"] for c in KAT])
+def test_gpu_gridder_kat(gpu, orc, case):
+    uv = np.array(case["uv"], dtype=float)
+    vis = np.array([complex(a, b) for a, b in case["vis"]])
+    w = np.array(case["weights"], dtype=float)
+    args = (KERNELS[case["kernel"]], case["param"], case["support"], case["image_size"], uv, vis, w)
+    kw = dict(kernel_exact=case["kernel_exact"], oversampling=case["oversampling"], shift_uv=case["shift_uv"],
+              halfplane=case["halfplane"], generate_beam=True)
+    vg, sg, total = gpu.convolve_to_grid(*args, **kw)
+    ovg, osg, ototal, _ = orc.convolve_to_grid(*args, **kw)
+    assert np.allclose(vg, ovg, atol=TOL, rtol=0) and np.allclose(sg, osg, atol=TOL, rtol=0)
+    assert abs(total - ototal) <= 1e-15 * abs(ototal)
+    if "expected_vis_grid_real" in case:
+        assert np.allclose(vg.real, np.array(case["expected_vis_grid_real"]), atol=TOL, rtol=0)
+    else:
+        exp = np.array(case["expected_vis_grid_real_after_column_shift"])
+        assert np.allclose(np.roll(vg, case["column_shift"], axis=1).real, exp, atol=TOL, rtol=0)
+
+
+@pytest.mark.parametrize("N,n,s,os_,exact,halfplane,shift", [
+    (256, 5000, 3, 9, False, True, True),
+    (512, 20000, 3, 8, False, True, True),
+    (64, 3000, 2, 16, False, False, False),
+    (128, 2000, 3, 1, True, True, True),
+    (64, 500, 5, 4, False, True, True),
+    (128, 1000, 7, 8, False, True, False),
+])
+def test_gpu_gridder_random(gpu, orc, N, n, s, os_, exact, halfplane, shift):
+    rng = np.random.default_rng(N + n + s)
+    uv = rng.uniform(-0.5 * N, 0.5 * N, size=(n, 2))
+    uv[: n // 10, 1] = rng.uniform(-s - 1.5, s + 1.5, size=n // 10)  # near v = 0: conjugate copies
+    vis = rng.normal(size=n) + 1j * rng.normal(size=n)
+    w = rng.uniform(0.1, 2.0, size=n)
+    args = (capi.KERNEL_GAUSSIANSINC, float(s), s, N, uv, vis, w)
+    kw = dict(kernel_exact=exact, oversampling=os_, shift_uv=shift, halfplane=halfplane, generate_beam=True)
+    vg, sg, total = gpu.convolve_to_grid(*args, **kw)
+    ovg, osg, ototal, _ = orc.convolve_to_grid(*args, **kw)
+    assert peak_rel_err(vg, ovg) < 1e-12
+    assert peak_rel_err(sg, osg) < 1e-12
+    assert abs(total - ototal) <= 1e-13 * ototal
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32, 64, 128, 256, 1024, 2048, 4096, 8192])
+def test_gpu_fft_c2r(gpu, orc, n):
+    rng = np.random.default_rng(n)
+    g = rng.normal(size=(n // 2 + 1, n)) + 1j * rng.normal(size=(n // 2 + 1, n))
+    out = gpu.fft_c2r(g)
+    ref = orc.fft_c2r(g) if n <= 4096 else (np.fft.irfftn(g.T) * n * n).T
+    assert peak_rel_err(out, ref) < 1e-12
+
+
+@pytest.mark.parametrize("n", [4, 16, 64, 256, 1024])
+def test_gpu_fft_c2c(gpu, n):
+    rng = np.random.default_rng(n + 1)
+    a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    assert peak_rel_err(gpu.fft_c2c(a, True), np.fft.fft2(a)) < 1e-12
+    assert peak_rel_err(gpu.fft_c2c(a, False), np.fft.ifft2(a) * n * n) < 1e-12
+
+
+def _imaging_case(orc, gpu, snap, **kw):
+    N = snap["image_size"]
+    pars = capi.imager_pars(N, snap["cell_arcsec"], **kw)
+    img, beam, diag = gpu.image_visibilities(snap["vis"], snap["weights"], snap["uvw"], pars, return_diag=True)
+    oimg, obeam, ototal, ongood = orc.image_visibilities(pars, snap["vis"], snap["weights"], snap["uvw"])
+    assert diag.num_good_vis == ongood
+    assert abs(diag.sample_grid_total - ototal) <= 1e-13 * ototal
+    return img, beam, oimg, obeam
+
+
+def test_gpu_imager_c1(gpu, orc):
+    """Workload c1: N=1024, 50k visibilities, GaussianSinc s=3, os 9, beam."""
+    snap = simdata.workload_snapshot("c1")
+    img, beam, oimg, obeam = _imaging_case(orc, gpu, snap)
+    assert peak_rel_err(img, oimg) < TOL
+    assert peak_rel_err(beam, obeam) < TOL
+
+
+def test_gpu_imager_exact_and_gcf(gpu, orc):
+    """Exact kernels with analytic grid correction (Gaussian) and padding 2 (crop)."""
+    snap = simdata.snapshot(128, 2.0, 4000, 3, seed=77)
+    img, beam, oimg, obeam = _imaging_case(orc, gpu, snap, kernel_function=capi.KERNEL_GAUSSIAN,
+                                           kernel_exact=True, oversampling=1, gridding_correction=True,
+                                           analytic_gcf=True)
+    assert peak_rel_err(img, oimg) < 1e-9 and peak_rel_err(beam, obeam) < 1e-9
+    snap = simdata.snapshot(128, 2.0, 4000, 3, seed=78)
+    img, beam, oimg, obeam = _imaging_case(orc, gpu, snap, padding_factor=2.0, gridding_correction=True,
+                                           analytic_gcf=False)
+    assert img.shape == (128, 128)
+    assert peak_rel_err(img, oimg) < TOL and peak_rel_err(beam, obeam) < TOL
+
+
+def test_gpu_imager_empty_and_errors(gpu, orc):
+    """All visibilities out of bounds -> sample_grid_total 0 -> empty images (imager.h:355);
+    malformed parameters -> status codes, not crashes."""
+    N = 64
+    uvw = np.zeros((10, 3), order="F")
+    uvw[:, 0] = 1e12  # far outside the grid
+    pars = capi.imager_pars(N, 1.0)
+    img, beam = gpu.image_visibilities(np.ones(10, complex), np.ones(10), uvw, pars)
+    assert img.size == 0
+    bad = capi.imager_pars(96, 1.0)  # not a power of two
+    with pytest.raises(STPError) as e:
+        gpu.image_visibilities(np.ones(10, complex), np.ones(10), uvw, bad)
+    assert e.value.status == 1
+    ap = capi.AProjPars()
+    ap.num_timesteps = 3
+    with pytest.raises(STPError) as e:
+        gpu.image_visibilities(np.ones(10, complex), np.ones(10), uvw, pars, a_proj=ap)
+    assert e.value.status == 2
+
+
+def test_gpu_imager_c2_full_size(gpu, orc):
+    """Workload c2 at its full size: N=4096, 1M visibilities, 50 sources."""
+    snap = simdata.workload_snapshot("c2")
+    img, beam, oimg, obeam = _imaging_case(orc, gpu, snap)
+    assert peak_rel_err(img, oimg) < TOL
+    assert peak_rel_err(beam, obeam) < TOL
+    # beam peak at the phase centre is 1 after normalisation
+    assert abs(beam[0, 0] - 1.0) < 1e-12
