@@ -1,0 +1,136 @@
+"""GPU parity of the source finder through the C-ABI.  Bar: label maps, label
+counts, island ids/positions/areas and the statistics (mean, sigma, median,
+RMS) bit-exact against the oracle on the same image; moments within 1e-9."""
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+from fastimaging_amd import capi, simdata
+from tests.helpers import fftshift, gaussian2d, noise
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+LAB = json.load(open(os.path.join(GOLDEN, "labeling_kat.json")))
+ISLAND_EXACT = ("label_idx", "sign", "extremum_y_idx", "extremum_x_idx", "num_samples", "extremum_val")
+
+
+def compare(gpu_res, orc_res, moments_tol=1e-9):
+    lm, isl, st = orc_res
+    assert np.array_equal(gpu_res.label_map, lm)
+    g = gpu_res.stats
+    for f in ("bg_level", "rms_est", "mean", "sigma"):
+        a, b = getattr(g, f), getattr(st, f)
+        assert a == b or (np.isnan(a) and np.isnan(b)), (f, a, b)  # bit-exact (NaN where the algebra gives NaN)
+    for f in ("num_labels_pos", "num_labels_neg", "num_valid_labels", "num_islands"):
+        assert getattr(g, f) == getattr(st, f), f
+    assert len(gpu_res.islands) == len(isl)
+    for a, b in zip(gpu_res.islands, isl):
+        for f in ISLAND_EXACT:
+            assert a[f] == b[f], f
+        ma, mb = np.array(a["moments_fit"]), np.array(b["moments_fit"])
+        assert np.allclose(ma, mb, rtol=moments_tol, atol=moments_tol)
+
+
+@pytest.mark.parametrize("case", LAB["cases"], ids=[c["name"] for c in LAB["cases"]])
+def test_gpu_labeling_kat(gpu, case):
+    img = np.array(case["image"], dtype=float)
+    exp = np.array(case["expected_map_nonfftshift"])
+    p = LAB["params"]
+    pars = capi.sourcefind_pars(p["detection_n_sigma"], p["analysis_n_sigma"], rms_est=p["rms_est"],
+                                median_method=capi.MEDIAN_ZEROMEDIAN, ccl_4connectivity=case["ccl_4connectivity"],
+                                generate_labelmap=False, source_min_area=1)
+    for sgn in (1, -1):
+        r = gpu.source_find(fftshift(sgn * img), pars)
+        assert np.array_equal(fftshift(r.label_map), sgn * exp)
+        assert len(r.islands) == case["expected_total_islands"]
+
+
+def _field(rows, cols, seed, n_src=12, amp=(3.0, 30.0)):
+    rng = np.random.default_rng(seed)
+    img = noise(rows, cols, sigma=1.0, seed=seed)
+    for _ in range(n_src):
+        a = rng.uniform(*amp) * rng.choice([-1.0, 1.0])
+        img += gaussian2d(rows, cols, rng.uniform(0, cols), rng.uniform(0, rows), a, rng.uniform(0.8, 3.0),
+                          rng.uniform(0.8, 3.0), rng.uniform(0, np.pi))
+    return np.asfortranarray(img)
+
+
+MEDIANS = [capi.MEDIAN_BINMEDIAN, capi.MEDIAN_BINAPPROX, capi.MEDIAN_NTHELEMENT, capi.MEDIAN_ZEROMEDIAN]
+
+
+@pytest.mark.parametrize("median,conn4,find_neg,gen_map", list(itertools.product(
+    MEDIANS, [False, True], [True, False], [True, False]))[::3])
+def test_gpu_sourcefind_modes(gpu, orc, median, conn4, find_neg, gen_map):
+    img = _field(256, 192, seed=median * 10 + conn4)
+    pars = capi.sourcefind_pars(5.0, 3.0, median_method=median, ccl_4connectivity=conn4,
+                                find_negative_sources=find_neg, generate_labelmap=gen_map)
+    compare(gpu.source_find(img, pars), orc.source_find(img, pars))
+
+
+@pytest.mark.parametrize("rows,cols,seed", [(64, 64, 1), (130, 258, 2), (512, 512, 3), (1024, 1024, 4)])
+def test_gpu_sourcefind_sizes(gpu, orc, rows, cols, seed):
+    img = _field(rows, cols, seed=seed, n_src=max(3, rows * cols // 20000))
+    pars = capi.sourcefind_pars(5.0, 3.0, median_method=capi.MEDIAN_BINMEDIAN)
+    compare(gpu.source_find(img, pars), orc.source_find(img, pars))
+
+
+def test_gpu_sourcefind_given_rms_and_iters(gpu, orc):
+    img = _field(200, 200, seed=11)
+    for rms, iters in ((0.9, 5), (0.0, 1), (0.0, 0), (0.0, 12)):
+        pars = capi.sourcefind_pars(5.0, 3.0, rms_est=rms, sigma_clip_iters=iters, median_method=capi.MEDIAN_BINAPPROX)
+        compare(gpu.source_find(img, pars), orc.source_find(img, pars))
+
+
+def test_gpu_sourcefind_degenerate(gpu, orc):
+    """Constant image (sigma 0: exact-median fallback), sparse image (most pixels exactly 0:
+    radix selection), all-source image (every pixel above threshold)."""
+    pars = capi.sourcefind_pars(5.0, 3.0, median_method=capi.MEDIAN_BINMEDIAN, source_min_area=1)
+    c = np.full((64, 64), 2.5, order="F")
+    compare(gpu.source_find(c, pars), orc.source_find(c, pars))
+    s = np.zeros((128, 128), order="F")
+    rng = np.random.default_rng(5)
+    idx = rng.choice(s.size, 700, replace=False)
+    s.ravel(order="F")[idx] = rng.normal(size=700) * 10
+    s = np.asfortranarray(s)
+    for m in (capi.MEDIAN_BINMEDIAN, capi.MEDIAN_BINAPPROX, capi.MEDIAN_NTHELEMENT):
+        pars.median_method = m
+        compare(gpu.source_find(s, pars), orc.source_find(s, pars))
+    pars = capi.sourcefind_pars(1.0, 0.5, rms_est=1.0, median_method=capi.MEDIAN_ZEROMEDIAN, source_min_area=1)
+    a = np.full((32, 32), 10.0, order="F")
+    a[5, 5] = 11.0
+    compare(gpu.source_find(a, pars), orc.source_find(a, pars))
+
+
+def test_gpu_source_detection_kats(gpu):
+    """sourcefind_test_BasicSourceDetection / NegativeSourceDetection through the GPU."""
+    for sign, neg in ((1.0, False), (-1.0, True)):
+        ydim, xdim = 128, 64
+        bright = gaussian2d(ydim, xdim, 48.24, 52.66, sign * 10.0, 1.5, 1.2, 1.0)
+        faint = gaussian2d(ydim, xdim, 32, 64, sign * 3.5, 1.5, 1.2, 1.0)
+        p = capi.sourcefind_pars(4, 3, rms_est=1.0, find_negative_sources=neg)
+        r0 = gpu.source_find(fftshift(bright), p)
+        r1 = gpu.source_find(fftshift(bright + faint), p)
+        r2 = gpu.source_find(fftshift(bright + 2 * faint), p)
+        assert len(r0.islands) == 1 and len(r1.islands) == 1 and len(r2.islands) == 2
+        assert abs(r0.islands[0]["extremum_x_idx"] - 48.24) < 0.5
+        assert abs(r0.islands[0]["extremum_y_idx"] - 52.66) < 0.5
+
+
+def test_gpu_pipeline_c2_full_size(gpu, orc):
+    """Workload c2 at full size: the oracle image through both source finders is bit-exact,
+    and the GPU image gives the same islands as the oracle image."""
+    snap = simdata.workload_snapshot("c2")
+    pars = capi.imager_pars(4096, 1.0)
+    img, _ = gpu.image_visibilities(snap["vis"], snap["weights"], snap["uvw"], pars)
+    oimg, _, _, _ = orc.image_visibilities(pars, snap["vis"], snap["weights"], snap["uvw"])
+    sfp = capi.sourcefind_pars(5.0, 3.0, median_method=capi.MEDIAN_BINMEDIAN)
+    ores = orc.source_find(oimg, sfp)
+    compare(gpu.source_find(oimg, sfp), ores)
+    g2 = gpu.source_find(img, sfp)
+    assert [(i["label_idx"], i["extremum_x_idx"], i["extremum_y_idx"]) for i in g2.islands] == \
+           [(i["label_idx"], i["extremum_x_idx"], i["extremum_y_idx"]) for i in ores[1]]
+    assert len(ores[1]) >= 40
