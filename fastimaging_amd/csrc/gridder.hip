@@ -275,71 +275,204 @@ __global__ void zero_u64_kernel(unsigned long long* p)
     p[0] = 0ull;
 }
 
-template <typename T>
-__global__ __launch_bounds__(GWG) void grid_tiles_kernel(GridGeom g, const Entry* entries, const uint32_t* offsets,
-    uint32_t cap, const double* ktab, const double* uv, cplx<T>* vgrid, cplx<T>* bgrid)
+// ---------------------------------------------------------------------------
+// Binning by per-block LDS counting sort (no global atomics): block b takes
+// visibilities [4096 b, 4096 b + 4096); counts[tile * nblk + b] holds its
+// entries per tile, so one exclusive scan of the flat array yields both the
+// per-tile entry ranges and each block's write cursor inside every tile.
+constexpr int BWG = 1024;
+constexpr int VPB = CANON_BLOCK; // visibilities per binning block (also the canonical sum block)
+constexpr int MAX_LDS_TILES = 24576;
+
+__global__ __launch_bounds__(BWG) void grid_bin_count_kernel(GridGeom g, const double* uv, const double* vis, const double* wts,
+    uint32_t* counts, int nblk, double* wpart, unsigned long long* ngood)
 {
-    __shared__ double acc_r[TILE_C * TILE_R];
-    __shared__ double acc_i[TILE_C * TILE_R];
-    __shared__ double acc_b[TILE_C * TILE_R];
+#pragma clang fp contract(off)
+    extern __shared__ uint32_t hist[];
+    __shared__ double wl[VPB];
+    const int ntiles = g.tiles_r * g.tiles_c;
+    for (int t = threadIdx.x; t < ntiles; t += BWG)
+        hist[t] = 0;
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * VPB;
+    unsigned long long ng = 0;
+    for (int j = 0; j < VPB / BWG; j++) {
+        const int local = threadIdx.x + j * BWG;
+        const long long i = base + local;
+        double wgood = 0.0;
+        if (i < g.n) {
+            const Prep p = prep_vis(g, uv, vis, wts, i);
+            if (p.good) {
+                wgood = p.w;
+                ng++;
+                for (int pass = 0; pass <= p.dup; pass++) {
+                    int gx, gy, cpx, cpy;
+                    pass_geom(g, p, pass, gx, gy, cpx, cpy);
+                    int rt[MAXT], ct[MAXT], nrt, nct;
+                    footprint_tiles(g, gx, gy, rt, nrt, ct, nct);
+                    for (int a = 0; a < nct; a++)
+                        for (int b = 0; b < nrt; b++)
+                            atomicAdd(&hist[ct[a] * g.tiles_r + rt[b]], 1u);
+                }
+            }
+        }
+        wl[local] = wgood;
+    }
+    __syncthreads();
+    // canonical sum of the good weights: lane t adds elements t, t+256, ..., t+15*256
+    double a = 0.0;
+    if (threadIdx.x < CANON_LANES)
+        for (int j = 0; j < CANON_PER_LANE; j++)
+            a = a + wl[threadIdx.x + j * CANON_LANES];
+    __syncthreads();
+    if (threadIdx.x < CANON_LANES)
+        wl[threadIdx.x] = a;
+    for (int off = CANON_LANES / 2; off >= 1; off >>= 1) {
+        __syncthreads();
+        if (threadIdx.x < off)
+            wl[threadIdx.x] = wl[threadIdx.x] + wl[threadIdx.x + off];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        wpart[2 * blockIdx.x] = wl[0];
+        wpart[2 * blockIdx.x + 1] = 0.0;
+    }
+    for (int t = threadIdx.x; t < ntiles; t += BWG)
+        counts[size_t(t) * nblk + blockIdx.x] = hist[t];
+    if (ngood) {
+        for (int off = 32; off >= 1; off >>= 1)
+            ng += __shfl_down(ng, off, 64);
+        if ((threadIdx.x & 63) == 0 && ng)
+            atomicAdd(ngood, ng);
+    }
+}
+
+__global__ __launch_bounds__(BWG) void grid_bin_scatter_kernel(GridGeom g, const double* uv, const double* vis, const double* wts,
+    const uint32_t* flat_off, int nblk, Entry* entries, uint32_t cap)
+{
+    extern __shared__ uint32_t cur[];
+    const int ntiles = g.tiles_r * g.tiles_c;
+    for (int t = threadIdx.x; t < ntiles; t += BWG)
+        cur[t] = flat_off[size_t(t) * nblk + blockIdx.x];
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * VPB;
+    for (int j = 0; j < VPB / BWG; j++) {
+        const long long i = base + threadIdx.x + j * BWG;
+        if (i >= g.n)
+            break;
+        const Prep p = prep_vis(g, uv, vis, wts, i);
+        if (!p.good)
+            continue;
+        for (int pass = 0; pass <= p.dup; pass++) {
+            int gx, gy, cpx, cpy;
+            pass_geom(g, p, pass, gx, gy, cpx, cpy);
+            int rt[MAXT], ct[MAXT], nrt, nct;
+            footprint_tiles(g, gx, gy, rt, nrt, ct, nct);
+            Entry e;
+            e.gx = gx;
+            e.gy = gy;
+            e.cp = cpx | (cpy << 16);
+            e.idx = int((i << 1) | pass);
+            e.vr = p.vr;
+            e.vi = pass ? -p.vi : p.vi;
+            e.w = p.w;
+            for (int a = 0; a < nct; a++)
+                for (int b = 0; b < nrt; b++) {
+                    const uint32_t slot = atomicAdd(&cur[ct[a] * g.tiles_r + rt[b]], 1u);
+                    if (slot < cap)
+                        entries[slot] = e;
+                }
+        }
+    }
+}
+
+// Accumulates every entry binned to this tile.  One wave per entry, lane = kernel tap
+// (row fastest); LDS column stride TILE_R + ks keeps the taps of one entry on distinct
+// banks; entries are staged through LDS 256 at a time.
+constexpr int EBUF = GWG;
+
+template <typename T>
+__global__ __launch_bounds__(GWG) void grid_tiles_kernel(GridGeom g, const Entry* entries, const uint32_t* offsets, int ostride,
+    uint32_t cap, const double* ktab, int ktab_lds, const double* uv, cplx<T>* vgrid, cplx<T>* bgrid)
+{
+    extern __shared__ double sm[];
+    __shared__ Entry ebuf[EBUF];
+    const int S = TILE_R + (g.ks <= 15 ? g.ks : 1);
+    double* acc_r = sm;
+    double* acc_i = sm + TILE_C * S;
+    double* acc_b = sm + 2 * TILE_C * S;
+    double* kt = sm + 3 * TILE_C * S;
+    const int cs = (g.os / 2) * 2 + 1;
+    const double* K1 = ktab;
+    if (ktab_lds) {
+        for (int k = threadIdx.x; k < cs * g.ks; k += GWG)
+            kt[k] = ktab[k];
+        K1 = kt;
+    }
     const int t = blockIdx.x;
     const int rt = t % g.tiles_r, ct = t / g.tiles_r;
     const int r0 = rt * TILE_R, c0 = ct * TILE_C;
-    for (int k = threadIdx.x; k < TILE_C * TILE_R; k += GWG) {
+    for (int k = threadIdx.x; k < TILE_C * S; k += GWG) {
         acc_r[k] = 0.0;
         acc_i[k] = 0.0;
         acc_b[k] = 0.0;
     }
-    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t eb = min(offsets[t], cap), ee = min(offsets[t + 1], cap);
+    const uint32_t eb = min(offsets[size_t(t) * ostride], cap), ee = min(offsets[size_t(t + 1) * ostride], cap);
     const int ntaps = g.ks * g.ks;
-    for (uint32_t e = eb + wave; e < ee; e += GWG / 64) {
-        const Entry E = entries[e];
-        const int cpx = E.cp & 0xffff, cpy = E.cp >> 16;
-        double fx = 0.0, fy = 0.0, ksum = 1.0;
-        if (g.exact) {
-            // exact kernel for this visibility (make_kernel_array, conv_func.h:279-303)
-            const long long vi = E.idx >> 1;
-            const double u = uv[vi] * g.scale, v = uv[g.n + vi] * g.scale;
-            double uu = u, vv = v;
-            if (g.halfplane && v < 0.0) {
-                uu = -u;
-                vv = -v;
+    for (uint32_t e0 = eb; e0 < ee; e0 += EBUF) {
+        const int cnt = int(min(uint32_t(EBUF), ee - e0));
+        __syncthreads();
+        if (int(threadIdx.x) < cnt)
+            ebuf[threadIdx.x] = entries[e0 + threadIdx.x];
+        __syncthreads();
+        for (int q = wave; q < cnt; q += GWG / 64) {
+            const Entry E = ebuf[q];
+            const int cpx = E.cp & 0xffff, cpy = E.cp >> 16;
+            double fx = 0.0, fy = 0.0, ksum = 1.0;
+            if (g.exact) {
+                // exact kernel for this visibility (make_kernel_array, conv_func.h:279-303)
+                const long long vi = E.idx >> 1;
+                const double u = uv[vi] * g.scale, v = uv[g.n + vi] * g.scale;
+                double uu = u, vv = v;
+                if (g.halfplane && v < 0.0) {
+                    uu = -u;
+                    vv = -v;
+                }
+                fx = uu - int(rint(uu));
+                fy = vv - int(rint(vv));
+                if (E.idx & 1) {
+                    fx = -fx;
+                    fy = -fy;
+                }
+                double sx = 0.0, sy = 0.0;
+                for (int qq = 0; qq < g.ks; qq++) {
+                    sx += dev_kernel(g, double(qq - g.s) - fx);
+                    sy += dev_kernel(g, double(qq - g.s) - fy);
+                }
+                ksum = sx * sy;
             }
-            fx = uu - int(rint(uu));
-            fy = vv - int(rint(vv));
-            if (E.idx & 1) {
-                fx = -fx;
-                fy = -fy;
+            for (int tap = lane; tap < ntaps; tap += 64) {
+                const int i = tap % g.ks, j = tap / g.ks;
+                const int r = tap_row(g, E.gy, i);
+                if (r < 0)
+                    continue;
+                const int c = tap_col(g, E.gx, j);
+                const int lr = r - r0, lc = c - c0;
+                if (lr < 0 || lr >= TILE_R || lc < 0 || lc >= TILE_C)
+                    continue;
+                double K;
+                if (g.exact)
+                    K = (dev_kernel(g, double(i - g.s) - fy) * dev_kernel(g, double(j - g.s) - fx)) / ksum;
+                else
+                    K = K1[cpy * g.ks + i] * K1[cpx * g.ks + j];
+                const double kw = K * E.w;
+                const int a = lc * S + lr;
+                unsafeAtomicAdd(&acc_r[a], E.vr * kw);
+                unsafeAtomicAdd(&acc_i[a], E.vi * kw);
+                if (g.beam)
+                    unsafeAtomicAdd(&acc_b[a], kw);
             }
-            double sx = 0.0, sy = 0.0;
-            for (int q = 0; q < g.ks; q++) {
-                sx += dev_kernel(g, double(q - g.s) - fx);
-                sy += dev_kernel(g, double(q - g.s) - fy);
-            }
-            ksum = sx * sy;
-        }
-        for (int tap = lane; tap < ntaps; tap += 64) {
-            const int i = tap % g.ks, j = tap / g.ks;
-            const int r = tap_row(g, E.gy, i);
-            if (r < 0)
-                continue;
-            const int c = tap_col(g, E.gx, j);
-            const int lr = r - r0, lc = c - c0;
-            if (lr < 0 || lr >= TILE_R || lc < 0 || lc >= TILE_C)
-                continue;
-            double K;
-            if (g.exact)
-                K = (dev_kernel(g, double(i - g.s) - fy) * dev_kernel(g, double(j - g.s) - fx)) / ksum;
-            else
-                K = ktab[cpy * g.ks + i] * ktab[cpx * g.ks + j];
-            const double kw = K * E.w;
-            const int a = lc * TILE_R + lr;
-            unsafeAtomicAdd(&acc_r[a], E.vr * kw);
-            unsafeAtomicAdd(&acc_i[a], E.vi * kw);
-            if (g.beam)
-                unsafeAtomicAdd(&acc_b[a], kw);
         }
     }
     __syncthreads();
@@ -348,9 +481,10 @@ __global__ __launch_bounds__(GWG) void grid_tiles_kernel(GridGeom g, const Entry
         const int r = r0 + lr, c = c0 + lc;
         if (r < g.rows && c < g.N) {
             const size_t o = size_t(r) + size_t(c) * g.rows;
-            vgrid[o] = { T(acc_r[k]), T(acc_i[k]) };
+            const int a = lc * S + lr;
+            vgrid[o] = { T(acc_r[a]), T(acc_i[a]) };
             if (g.beam)
-                bgrid[o] = { T(acc_b[k]), T(0) };
+                bgrid[o] = { T(acc_b[a]), T(0) };
         }
     }
 }
@@ -401,6 +535,7 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     const int ntiles = g.tiles_r * g.tiles_c;
     // kernel table (oversampled 1-D kernels), computed on the host once per configuration
     const double* d_tab = nullptr;
+    const int cs = (os / 2) * 2 + 1;
     if (!a.kernel_exact) {
         char key[160];
         snprintf(key, sizeof(key), "k1d:%d:%.17g:%d:%d", kf.kind, kf.param, a.support, os);
@@ -409,10 +544,7 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         d_tab = dev_table(ctx, "k1d:none", std::vector<double>(1, 0.0));
     }
 
-    uint32_t* counts = (uint32_t*)ws_get(ctx, "grid_counts", sizeof(uint32_t) * ntiles);
-    uint32_t* offsets = (uint32_t*)ws_get(ctx, "grid_offsets", sizeof(uint32_t) * (ntiles + 1));
-    uint32_t* cursor = (uint32_t*)ws_get(ctx, "grid_cursor", sizeof(uint32_t) * ntiles);
-    const size_t nblk = (n + CANON_BLOCK - 1) / CANON_BLOCK;
+    const size_t nblk = std::max<size_t>(1, (n + VPB - 1) / VPB);
     double* wpart = (double*)ws_get(ctx, "grid_wpart", sizeof(double) * (2 * nblk + 2));
     double* wscr = (double*)ws_get(ctx, "grid_wscr", sizeof(double) * canon_scratch_doubles(nblk));
     double* wres = (double*)ws_get(ctx, "grid_wres", sizeof(double) * 2);
@@ -422,19 +554,47 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     STP_REQUIRE(cap64 < (size_t(1) << 32), "too many gridding entries for one call");
     const uint32_t cap = uint32_t(cap64);
     Entry* entries = (Entry*)ws_get(ctx, "grid_entries", sizeof(Entry) * cap64);
-
-    fill_u32(ctx, counts, 0u, ntiles);
     if (d_ngood)
         launch(ctx, "fill", zero_u64_kernel, dim3(1), dim3(1), 0, d_ngood);
-    launch(ctx, "grid_count", grid_count_kernel, dim3(unsigned(std::max<size_t>(nblk, 1))), dim3(GWG), 0, g, d_uv, d_vis,
-        d_w, counts, wpart, d_ngood);
-    exclusive_scan_u32(ctx, counts, ntiles, offsets, cursor);
-    launch(ctx, "grid_scatter", grid_scatter_kernel, dim3(div_up(n, GWG)), dim3(GWG), 0, g, d_uv, d_vis, d_w, cursor,
-        entries, cap);
-    launch(ctx, "grid_tiles", grid_tiles_kernel<T>, dim3(ntiles), dim3(GWG), 0, g, (const Entry*)entries,
-        (const uint32_t*)offsets, cap, d_tab, d_uv, d_vis_grid, d_sampling_grid);
+
+    const uint32_t* offsets;
+    int ostride;
+    if (ntiles <= MAX_LDS_TILES) {
+        const size_t m = size_t(ntiles) * nblk;
+        uint32_t* counts = (uint32_t*)ws_get(ctx, "grid_bcounts", sizeof(uint32_t) * m);
+        uint32_t* flat = (uint32_t*)ws_get(ctx, "grid_boffsets", sizeof(uint32_t) * (m + 1));
+        const size_t hist_bytes = sizeof(uint32_t) * size_t(ntiles);
+        HIP_CHECK(hipFuncSetAttribute((const void*)grid_bin_count_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(hist_bytes)));
+        HIP_CHECK(hipFuncSetAttribute((const void*)grid_bin_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(hist_bytes)));
+        launch(ctx, "grid_count", grid_bin_count_kernel, dim3(unsigned(nblk)), dim3(BWG), hist_bytes, g, d_uv, d_vis, d_w, counts,
+            int(nblk), wpart, d_ngood);
+        scan_u32(ctx, counts, m, flat);
+        launch(ctx, "grid_scatter", grid_bin_scatter_kernel, dim3(unsigned(nblk)), dim3(BWG), hist_bytes, g, d_uv, d_vis, d_w,
+            (const uint32_t*)flat, int(nblk), entries, cap);
+        offsets = flat;
+        ostride = int(nblk);
+    } else {
+        // very large grids: global per-tile counters
+        uint32_t* counts = (uint32_t*)ws_get(ctx, "grid_counts", sizeof(uint32_t) * ntiles);
+        uint32_t* offs = (uint32_t*)ws_get(ctx, "grid_offsets", sizeof(uint32_t) * (ntiles + 1));
+        uint32_t* cursor = (uint32_t*)ws_get(ctx, "grid_cursor", sizeof(uint32_t) * ntiles);
+        fill_u32(ctx, counts, 0u, ntiles);
+        launch(ctx, "grid_count", grid_count_kernel, dim3(unsigned(nblk)), dim3(GWG), 0, g, d_uv, d_vis, d_w, counts, wpart,
+            d_ngood);
+        exclusive_scan_u32(ctx, counts, ntiles, offs, cursor);
+        launch(ctx, "grid_scatter", grid_scatter_kernel, dim3(div_up(n, GWG)), dim3(GWG), 0, g, d_uv, d_vis, d_w, cursor,
+            entries, cap);
+        offsets = offs;
+        ostride = 1;
+    }
+    const int S = TILE_R + (g.ks <= 15 ? g.ks : 1);
+    const int ktab_lds = (!a.kernel_exact && cs * g.ks <= 1024) ? 1 : 0;
+    const size_t tile_bytes = sizeof(double) * (3 * size_t(TILE_C) * S + (ktab_lds ? size_t(cs) * g.ks : 0));
+    HIP_CHECK(hipFuncSetAttribute((const void*)grid_tiles_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, int(tile_bytes)));
+    launch(ctx, "grid_tiles", grid_tiles_kernel<T>, dim3(ntiles), dim3(GWG), tile_bytes, g, (const Entry*)entries, offsets,
+        ostride, cap, d_tab, ktab_lds, d_uv, d_vis_grid, d_sampling_grid);
     // sample_grid_total = 2 * sum(good weights) (gridder.h:415-423)
-    canon_reduce_pairs(ctx, wpart, std::max<size_t>(nblk, 1), wscr, wres);
+    canon_reduce_pairs(ctx, wpart, nblk, wscr, wres);
     launch(ctx, "grid_total", scale_total_kernel, dim3(1), dim3(1), 0, (const double*)wres, d_total);
 }
 

@@ -49,4 +49,8 @@ void fill_u32(stp_ctx* ctx, uint32_t* p, uint32_t v, size_t n);
 // Exclusive scan of m uint32 counts into offsets[0..m] and cursor[0..m).
 void exclusive_scan_u32(stp_ctx* ctx, const uint32_t* counts, int m, uint32_t* offsets, uint32_t* cursor);
 
+// Device-wide exclusive scan of m uint32 values: out[i] = sum(in[0..i)), out[m] = total.
+// Three passes (chunk sums, scan of the sums, chunk rescan) over chunks of 4096.
+void scan_u32(stp_ctx* ctx, const uint32_t* in, size_t m, uint32_t* out);
+
 } // namespace stp_hip

@@ -1,4 +1,6 @@
 // Canonical-order pair reduction and exclusive scan kernels (see reduce.h).
+#include <algorithm>
+
 #include "reduce.h"
 
 #pragma clang fp contract(off)
@@ -112,6 +114,75 @@ void fill_u32(stp_ctx* ctx, uint32_t* p, uint32_t v, size_t n)
 void exclusive_scan_u32(stp_ctx* ctx, const uint32_t* counts, int m, uint32_t* offsets, uint32_t* cursor)
 {
     launch(ctx, "tile_scan", exclusive_scan_kernel, dim3(1), dim3(SCAN_WG), 0, counts, m, offsets, cursor);
+}
+
+} // namespace stp_hip
+
+namespace {
+
+constexpr int SCAN_CHUNK = 4096; // 256 threads x 16 consecutive values
+
+__global__ __launch_bounds__(256) void chunk_sum_kernel(const uint32_t* in, size_t m, uint32_t* sums)
+{
+    __shared__ uint32_t s[256];
+    const size_t base = size_t(blockIdx.x) * SCAN_CHUNK + size_t(threadIdx.x) * 16;
+    uint32_t a = 0;
+    for (int j = 0; j < 16; j++)
+        if (base + j < m)
+            a += in[base + j];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = 128; off >= 1; off >>= 1) {
+        if (threadIdx.x < off)
+            s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        sums[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(256) void chunk_rescan_kernel(const uint32_t* in, size_t m, const uint32_t* chunk_off, uint32_t* out)
+{
+    __shared__ uint32_t s[256];
+    const size_t base = size_t(blockIdx.x) * SCAN_CHUNK + size_t(threadIdx.x) * 16;
+    uint32_t v[16];
+    uint32_t a = 0;
+    for (int j = 0; j < 16; j++) {
+        v[j] = (base + j < m) ? in[base + j] : 0u;
+        a += v[j];
+    }
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t t = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = chunk_off[blockIdx.x] + s[threadIdx.x] - a;
+    for (int j = 0; j < 16; j++) {
+        if (base + j < m)
+            out[base + j] = run;
+        run += v[j];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255)
+        out[m] = run;
+}
+
+} // namespace
+
+namespace stp_hip {
+
+void scan_u32(stp_ctx* ctx, const uint32_t* in, size_t m, uint32_t* out)
+{
+    const size_t nb = std::max<size_t>(1, (m + SCAN_CHUNK - 1) / SCAN_CHUNK);
+    STP_REQUIRE(nb <= (1u << 24), "scan too large");
+    uint32_t* sums = (uint32_t*)ws_get(ctx, "scan_sums", sizeof(uint32_t) * nb);
+    uint32_t* offs = (uint32_t*)ws_get(ctx, "scan_offs", sizeof(uint32_t) * (nb + 1));
+    uint32_t* cur = (uint32_t*)ws_get(ctx, "scan_cur", sizeof(uint32_t) * nb);
+    launch(ctx, "scan_sums", chunk_sum_kernel, dim3(unsigned(nb)), dim3(256), 0, in, m, sums);
+    exclusive_scan_u32(ctx, sums, int(nb), offs, cur);
+    launch(ctx, "scan_rescan", chunk_rescan_kernel, dim3(unsigned(nb)), dim3(256), 0, in, m, (const uint32_t*)offs, out);
 }
 
 } // namespace stp_hip

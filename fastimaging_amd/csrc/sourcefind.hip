@@ -246,7 +246,8 @@ __global__ __launch_bounds__(SWG) void sf_gather(const T* x, size_t n, GatherSpe
                 b = LIN_BINS + 1;
             take = b >= gs.b_lo && b <= gs.b_hi;
         } else {
-            take = (okey(v) >> gs.shift) == gs.prefix;
+            // shift 64: no prefix selected yet, every element is a candidate
+            take = (gs.shift >= 64) || ((okey(v) >> gs.shift) == gs.prefix);
         }
         if (take) {
             const unsigned slot = atomicAdd(counter, 1u);

@@ -1,6 +1,7 @@
 """GPU parity of the source finder through the C-ABI.  Bar: label maps, label
 counts, island ids/positions/areas and the statistics (mean, sigma, median,
-RMS) bit-exact against the oracle on the same image; moments within 1e-9."""
+RMS) bit-exact against the oracle on the same image; the moments fit within the
+tolerances of fits_agree()."""
 import itertools
 import json
 import os
@@ -18,7 +19,31 @@ LAB = json.load(open(os.path.join(GOLDEN, "labeling_kat.json")))
 ISLAND_EXACT = ("label_idx", "sign", "extremum_y_idx", "extremum_x_idx", "num_samples", "extremum_val")
 
 
-def compare(gpu_res, orc_res, moments_tol=1e-9):
+def fits_agree(a, b, tol=1e-9, axes_tol=1e-6):
+    """Moments fit (sourcefind.cpp:691-717).  Amplitude and barycentre within tol.
+    The axes come from xx_bar - x_bar^2 (sourcefind.cpp:597-599), which cancels sums of
+    order (N/2)^2 * flux, so the per-label summation order (unspecified in the reference,
+    TBB) shows up there: squared axes are compared relative to semimajor^2 (a squared
+    semiminor that rounds negative gives NaN on either side and counts as 0), and the
+    angle only for clearly elongated islands, modulo pi."""
+    a, b = np.array(a, dtype=float), np.array(b, dtype=float)
+    if not np.allclose(a[:3], b[:3], rtol=tol, atol=tol):
+        return False
+    A2a, A2b = a[3] ** 2, b[3] ** 2
+    B2a, B2b = np.nan_to_num(a[4] ** 2), np.nan_to_num(b[4] ** 2)
+    if abs(A2a - A2b) > axes_tol * A2b or abs(B2a - B2b) > axes_tol * A2b:
+        return False
+    # line-shaped islands (semiminor ~ 0 or NaN): xy_bar is at rounding level and the
+    # angle branch (sourcefind.cpp:707-711) is decided by its sign, on either side
+    line = np.isnan(a[4]) or np.isnan(b[4]) or B2a <= 1e-6 * A2a or B2b <= 1e-6 * A2b
+    if not line and (A2b - B2b) > 1e-3 * A2b:
+        d = (a[5] - b[5] + np.pi / 2) % np.pi - np.pi / 2
+        if abs(d) > axes_tol:
+            return False
+    return True
+
+
+def compare(gpu_res, orc_res):
     lm, isl, st = orc_res
     assert np.array_equal(gpu_res.label_map, lm)
     g = gpu_res.stats
@@ -31,8 +56,7 @@ def compare(gpu_res, orc_res, moments_tol=1e-9):
     for a, b in zip(gpu_res.islands, isl):
         for f in ISLAND_EXACT:
             assert a[f] == b[f], f
-        ma, mb = np.array(a["moments_fit"]), np.array(b["moments_fit"])
-        assert np.allclose(ma, mb, rtol=moments_tol, atol=moments_tol)
+        assert fits_agree(a["moments_fit"], b["moments_fit"]), (a, b)
 
 
 @pytest.mark.parametrize("case", LAB["cases"], ids=[c["name"] for c in LAB["cases"]])
