@@ -94,6 +94,20 @@ typedef struct stp_imager_pars {
     int32_t r_fft;           /* stp_fft_routine; accepted and ignored (no FFTW) */
 } stp_imager_pars;
 
+/* Convolution-kernel functor (src/stp/convolution/conv_func.h:23-262):
+ *   TopHat(half_base_width), Triangle(half_base_width), Sinc(trunc, width),
+ *   Gaussian(trunc, width), GaussianSinc(trunc, width_gaussian, width_sinc),
+ *   PSWF(trunc).  `param` is half_base_width or trunc; the widths apply to the
+ *   functors that take them (Gaussian uses width_normalization_gaussian, Sinc
+ *   width_normalization_sinc).  stp_kernel_spec_init() fills the reference's
+ *   default widths (1.0, 1.0; GaussianSinc 2.52 and 1.55, conv_func.h:216-217). */
+typedef struct stp_kernel_spec {
+    int32_t kernel_function; /* stp_kernel_function */
+    double param;
+    double width_normalization_gaussian;
+    double width_normalization_sinc;
+} stp_kernel_spec;
+
 /* stp::W_ProjectionPars (src/stp/types.h:178-249).  W-projection is on iff
  * num_wplanes > 0 (types.h:233-239). */
 typedef struct stp_wproj_pars {
@@ -193,6 +207,7 @@ size_t stp_ctx_workspace_bytes(const stp_ctx* ctx);
 /* Fills the defaults of the reference's ImagerPars full constructor
  * (types.h:132-158) for the given size and cell. */
 void stp_imager_pars_init(stp_imager_pars* p, uint32_t image_size, double cell_size);
+void stp_kernel_spec_init(stp_kernel_spec* k, int32_t kernel_function, double param);
 void stp_wproj_pars_init(stp_wproj_pars* p);
 void stp_sourcefind_pars_init(stp_sourcefind_pars* p, double detection_n_sigma, double analysis_n_sigma);
 
@@ -203,6 +218,9 @@ void stp_sourcefind_pars_init(stp_sourcefind_pars* p, double detection_n_sigma, 
 /* stp::image_visibilities<T>(kernel_creator, vis, vis_weights, uvw_lambda,
  *                            img_pars, w_proj, a_proj)
  *   src/stp/imager/imager.h:233-376 (dispatch: src/stp/imager/imager.cpp:13-69)
+ * kernel: the kernel_creator functor; NULL selects the functor the way the
+ * ImageVisibilities constructor does (img_pars->kernel_function with
+ * trunc / half_base_width = kernel_support, imager.cpp:25-61).
  * Host buffers: vis = n_vis complex (2*n_vis doubles), vis_weights = n_vis,
  * uvw_lambda = n_vis x 3 column-major (u column, v column, w column).
  * image and beam: image_size x image_size column-major, caller-allocated; beam
@@ -211,6 +229,7 @@ stp_status stp_image_visibilities(stp_ctx* ctx,
     const stp_imager_pars* img_pars,
     const stp_wproj_pars* w_proj,   /* NULL = disabled */
     const stp_aproj_pars* a_proj,   /* NULL = disabled */
+    const stp_kernel_spec* kernel,  /* NULL = dispatch on img_pars */
     size_t n_vis,
     const double* vis,
     const double* vis_weights,
@@ -229,6 +248,7 @@ stp_status stp_image_visibilities_device(stp_ctx* ctx,
     const stp_imager_pars* img_pars,
     const stp_wproj_pars* w_proj,
     const stp_aproj_pars* a_proj,
+    const stp_kernel_spec* kernel,
     size_t n_vis,
     const double* d_vis,
     const double* d_vis_weights,
@@ -247,8 +267,7 @@ stp_status stp_image_visibilities_device(stp_ctx* ctx,
  * rows = image_size/2+1 if halfplane_gridding else image_size.  sampling_grid
  * may be NULL when generate_beam == 0. */
 stp_status stp_convolve_to_grid(stp_ctx* ctx,
-    int32_t kernel_function,
-    double kernel_param,            /* functor argument: half_base_width (TopHat/Triangle) or trunc (others) */
+    const stp_kernel_spec* kernel,  /* the kernel_creator functor */
     uint32_t support,
     int32_t image_size,
     size_t n_vis,
