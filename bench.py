@@ -90,13 +90,14 @@ def main():
     ctx.htod(d_uvw, np.asfortranarray(snap["uvw"]))
 
     do_sf = not args.no_sourcefind
+    island_buf = (capi.Island * (1 << 16))()
 
     def step():
         diag = ctx.image_visibilities_device(pars, n, d_vis, d_w, d_uvw, d_img, d_beam, precision=prec,
                                              synchronize=not do_sf)
         isl = None
         if do_sf:
-            isl, _ = ctx.source_find_device(d_img, N, N, sfp, d_label_map=d_lab, precision=prec)
+            isl, _ = ctx.source_find_device(d_img, N, N, sfp, d_label_map=d_lab, precision=prec, out=island_buf)
         return diag, isl
 
     for _ in range(args.warmup):

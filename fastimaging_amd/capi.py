@@ -39,6 +39,17 @@ class ImagerPars(ctypes.Structure):
     ]
 
 
+class KernelSpec(ctypes.Structure):
+    """The kernel_creator functor of image_visibilities / convolve_to_grid
+    (src/stp/convolution/conv_func.h:23-262)."""
+    _fields_ = [
+        ("kernel_function", ctypes.c_int32),
+        ("param", ctypes.c_double),
+        ("width_normalization_gaussian", ctypes.c_double),
+        ("width_normalization_sinc", ctypes.c_double),
+    ]
+
+
 class WProjPars(ctypes.Structure):
     """stp::W_ProjectionPars (src/stp/types.h:178-249)."""
     _fields_ = [
@@ -144,6 +155,22 @@ def imager_pars(image_size, cell_size, kernel_function=KERNEL_GAUSSIANSINC, kern
     return p
 
 
+def kernel_spec(kernel_function, param, width_normalization_gaussian=None, width_normalization_sinc=None):
+    """Functor spec with the reference's default widths (conv_func.h:106-170;
+    GaussianSinc 2.52 / 1.55, conv_func.h:216-217) unless given."""
+    if isinstance(kernel_function, KernelSpec):
+        return kernel_function
+    gs = kernel_function == KERNEL_GAUSSIANSINC
+    k = KernelSpec()
+    k.kernel_function = kernel_function
+    k.param = param
+    k.width_normalization_gaussian = (2.52 if gs else 1.0) if width_normalization_gaussian is None \
+        else width_normalization_gaussian
+    k.width_normalization_sinc = (1.55 if gs else 1.0) if width_normalization_sinc is None \
+        else width_normalization_sinc
+    return k
+
+
 def wproj_pars(num_wplanes=0, max_wpconv_support=0, undersampling_opt=1, kernel_trunc_perc=0.0,
                wplanes_median=False):
     p = WProjPars()
@@ -198,16 +225,19 @@ _SIG = {
     "stp_ctx_release_workspace": (ctypes.c_int, [_P]),
     "stp_ctx_workspace_bytes": (ctypes.c_size_t, [_P]),
     "stp_imager_pars_init": (None, [ctypes.POINTER(ImagerPars), ctypes.c_uint32, ctypes.c_double]),
+    "stp_kernel_spec_init": (None, [ctypes.POINTER(KernelSpec), ctypes.c_int32, ctypes.c_double]),
     "stp_wproj_pars_init": (None, [ctypes.POINTER(WProjPars)]),
     "stp_sourcefind_pars_init": (None, [ctypes.POINTER(SourceFindPars), ctypes.c_double, ctypes.c_double]),
     "stp_image_visibilities": (ctypes.c_int, [_P, ctypes.POINTER(ImagerPars), ctypes.POINTER(WProjPars),
-                                              ctypes.POINTER(AProjPars), ctypes.c_size_t, _P, _P, _P,
+                                              ctypes.POINTER(AProjPars), ctypes.POINTER(KernelSpec),
+                                              ctypes.c_size_t, _P, _P, _P,
                                               ctypes.c_int32, _P, _P, ctypes.POINTER(ImagerDiag)]),
     "stp_image_visibilities_device": (ctypes.c_int, [_P, ctypes.POINTER(ImagerPars), ctypes.POINTER(WProjPars),
-                                                     ctypes.POINTER(AProjPars), ctypes.c_size_t, _P, _P, _P,
+                                                     ctypes.POINTER(AProjPars), ctypes.POINTER(KernelSpec),
+                                                     ctypes.c_size_t, _P, _P, _P,
                                                      ctypes.c_int32, _P, _P, ctypes.c_int32,
                                                      ctypes.POINTER(ImagerDiag)]),
-    "stp_convolve_to_grid": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double, ctypes.c_uint32, ctypes.c_int32,
+    "stp_convolve_to_grid": (ctypes.c_int, [_P, ctypes.POINTER(KernelSpec), ctypes.c_uint32, ctypes.c_int32,
                                             ctypes.c_size_t, _P, _P, _P, ctypes.c_int32, ctypes.c_uint32,
                                             ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, _P, _D]),
     "stp_fft_c2r": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, _P]),

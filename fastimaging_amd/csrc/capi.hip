@@ -70,10 +70,34 @@ struct ImagingPlan {
 };
 
 // Full imaging pipeline on device buffers: grid -> c2r (image, beam) -> normalise.
-template <typename T, typename OT>
-void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp, size_t n, const double* d_vis,
-    const double* d_w, const double* d_uvw, OT* d_image, OT* d_beam, double* d_total, unsigned long long* d_ngood)
+// The functor the ImageVisibilities constructor builds from the parameters
+// (imager.cpp:25-61): trunc / half_base_width = kernel_support, default widths.
+stp_kernel_spec dispatch_kernel(const stp_imager_pars* p)
 {
+    stp_kernel_spec k;
+    stp_kernel_spec_init(&k, p->kernel_function, double(p->kernel_support));
+    return k;
+}
+
+void check_kernel_spec(const stp_kernel_spec* k)
+{
+    STP_REQUIRE(k->kernel_function >= STP_KERNEL_TOPHAT && k->kernel_function <= STP_KERNEL_PSWF, "unknown kernel function");
+    STP_REQUIRE(std::isfinite(k->param) && std::isfinite(k->width_normalization_gaussian) &&
+            std::isfinite(k->width_normalization_sinc) && k->width_normalization_gaussian != 0.0 &&
+            k->width_normalization_sinc != 0.0,
+        "kernel parameters must be finite and widths non-zero");
+    if (k->kernel_function == STP_KERNEL_TOPHAT || k->kernel_function == STP_KERNEL_TRIANGLE ||
+        k->kernel_function == STP_KERNEL_PSWF)
+        STP_REQUIRE(k->param > 0.0, "half_base_width / trunc must be > 0");
+}
+
+template <typename T, typename OT>
+void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp, const stp_kernel_spec* kspec, size_t n,
+    const double* d_vis, const double* d_w, const double* d_uvw, OT* d_image, OT* d_beam, double* d_total,
+    unsigned long long* d_ngood)
+{
+    const stp_kernel_spec kern = kspec ? *kspec : dispatch_kernel(p);
+    check_kernel_spec(&kern);
     const int N = int(p->padded_image_size);
     const bool beam = p->generate_beam != 0;
     STP_REQUIRE(!beam || d_beam != nullptr, "beam buffer is NULL with generate_beam");
@@ -88,8 +112,7 @@ void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp
     a.halfplane = true;
     a.generate_beam = beam;
     a.uv_scale = ((p->cell_size / 3600.0) * (M_PI / 180.0)) * double(N); // imager.h:293
-    a.kernel_function = p->kernel_function;
-    a.kernel_param = double(p->kernel_support); // functor trunc = kernel_support (imager.cpp:26-60)
+    a.kernel = kern;
     const int rows = N / 2 + 1;
     const size_t ge = size_t(rows) * N;
     cplx<T>* grids = (cplx<T>*)ws_get(ctx, "imager_grids", sizeof(cplx<T>) * ge * (beam ? 2 : 1));
@@ -98,9 +121,10 @@ void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp
     ns.image_size = int(p->image_size);
     ns.total_ptr = d_total;
     if (p->gridding_correction) {
-        const KernelFn kf = kernel_fn(p->kernel_function, double(p->kernel_support));
-        char key[160];
-        snprintf(key, sizeof(key), "gcf:%d:%.17g:%d:%d", kf.kind, kf.param, N, p->analytic_gcf);
+        const KernelFn kf = kernel_fn(kern);
+        char key[192];
+        snprintf(key, sizeof(key), "gcf:%d:%.17g:%.17g:%.17g:%d:%d", kf.kind, kf.param, kf.width_g, kf.width_s, N,
+            p->analytic_gcf);
         ns.gcf = has_table(ctx, key) ? dev_table(ctx, key, {})
                                      : dev_table(ctx, key, img_dom_kernel(kf, size_t(N), false, p->analytic_gcf != 0));
     }
@@ -213,6 +237,17 @@ void stp_imager_pars_init(stp_imager_pars* p, uint32_t image_size, double cell_s
     p->r_fft = STP_FFT_ESTIMATE;
 }
 
+void stp_kernel_spec_init(stp_kernel_spec* k, int32_t kernel_function, double param)
+{
+    if (!k)
+        return;
+    k->kernel_function = kernel_function;
+    k->param = param;
+    // default width normalisations (conv_func.h:106-170, GaussianSinc :216-217)
+    k->width_normalization_gaussian = (kernel_function == STP_KERNEL_GAUSSIANSINC) ? 2.52 : 1.0;
+    k->width_normalization_sinc = (kernel_function == STP_KERNEL_GAUSSIANSINC) ? 1.55 : 1.0;
+}
+
 void stp_wproj_pars_init(stp_wproj_pars* p)
 {
     if (!p)
@@ -239,7 +274,7 @@ void stp_sourcefind_pars_init(stp_sourcefind_pars* p, double det, double ana)
 }
 
 stp_status stp_image_visibilities_device(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp,
-    const stp_aproj_pars* ap, size_t n, const double* d_vis, const double* d_w, const double* d_uvw, int32_t precision,
+    const stp_aproj_pars* ap, const stp_kernel_spec* kernel, size_t n, const double* d_vis, const double* d_w, const double* d_uvw, int32_t precision,
     void* d_image, void* d_beam, int32_t synchronize, stp_imager_diag* diag)
 {
     return guarded(ctx, [&] {
@@ -250,7 +285,8 @@ stp_status stp_image_visibilities_device(stp_ctx* ctx, const stp_imager_pars* p,
         unsigned long long* d_ngood = (unsigned long long*)(d_total + 1);
         LaunchTimer whole(ctx, "image_visibilities");
         if (precision == STP_PRECISION_FP64) {
-            run_imager<double, double>(ctx, p, wp, n, d_vis, d_w, d_uvw, (double*)d_image, (double*)d_beam, d_total, d_ngood);
+            run_imager<double, double>(ctx, p, wp, kernel, n, d_vis, d_w, d_uvw, (double*)d_image, (double*)d_beam, d_total,
+                d_ngood);
         } else {
             throw Error(STP_ERR_UNSUPPORTED, "FP32 compensated path not available in this build yet");
         }
@@ -272,7 +308,7 @@ stp_status stp_image_visibilities_device(stp_ctx* ctx, const stp_imager_pars* p,
 }
 
 stp_status stp_image_visibilities(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp,
-    const stp_aproj_pars* ap, size_t n, const double* vis, const double* w, const double* uvw, int32_t precision,
+    const stp_aproj_pars* ap, const stp_kernel_spec* kernel, size_t n, const double* vis, const double* w, const double* uvw, int32_t precision,
     double* image, double* beam, stp_imager_diag* diag)
 {
     return guarded(ctx, [&] {
@@ -290,7 +326,7 @@ stp_status stp_image_visibilities(stp_ctx* ctx, const stp_imager_pars* p, const 
             HIP_CHECK(hipMemcpyAsync(d_in + 3 * n, uvw, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
         }
         stp_imager_diag dg{};
-        const stp_status st = stp_image_visibilities_device(ctx, p, wp, ap, n, d_in, d_in + 2 * n, d_in + 3 * n,
+        const stp_status st = stp_image_visibilities_device(ctx, p, wp, ap, kernel, n, d_in, d_in + 2 * n, d_in + 3 * n,
             precision, d_out, p->generate_beam ? d_out + osz * isz : nullptr, 1, &dg);
         if (st != STP_OK)
             throw Error(st, ctx->last_error);
@@ -317,7 +353,7 @@ stp_status stp_image_visibilities(stp_ctx* ctx, const stp_imager_pars* p, const 
     });
 }
 
-stp_status stp_convolve_to_grid(stp_ctx* ctx, int32_t kernel_function, double kernel_param, uint32_t support,
+stp_status stp_convolve_to_grid(stp_ctx* ctx, const stp_kernel_spec* kernel, uint32_t support,
     int32_t image_size, size_t n, const double* uv, const double* vis, const double* w, int32_t kernel_exact,
     uint32_t oversampling, int32_t shift_uv, int32_t halfplane, int32_t generate_beam, double* vis_grid,
     double* sampling_grid, double* sample_grid_total)
@@ -325,7 +361,8 @@ stp_status stp_convolve_to_grid(stp_ctx* ctx, int32_t kernel_function, double ke
     return guarded(ctx, [&] {
         STP_REQUIRE(vis_grid != nullptr, "vis_grid is NULL");
         STP_REQUIRE(!generate_beam || sampling_grid != nullptr, "sampling_grid is NULL with generate_beam");
-        STP_REQUIRE(kernel_function >= STP_KERNEL_TOPHAT && kernel_function <= STP_KERNEL_PSWF, "unknown kernel");
+        STP_REQUIRE(kernel != nullptr, "kernel spec is NULL");
+        check_kernel_spec(kernel);
         STP_REQUIRE(kernel_exact || oversampling >= 1, "oversampling must be >= 1");
         STP_REQUIRE(n == 0 || (uv && vis && w), "visibility buffers are NULL");
         const int N = image_size;
@@ -348,8 +385,7 @@ stp_status stp_convolve_to_grid(stp_ctx* ctx, int32_t kernel_function, double ke
         a.halfplane = halfplane != 0;
         a.generate_beam = generate_beam != 0;
         a.uv_scale = 1.0;
-        a.kernel_function = kernel_function;
-        a.kernel_param = kernel_param;
+        a.kernel = *kernel;
         grid_visibilities<double>(ctx, a, n, d_in, d_in + 2 * n, d_in + 4 * n, grids, generate_beam ? grids + ge : nullptr,
             d_total, nullptr);
         HIP_CHECK(hipMemcpyAsync(vis_grid, grids, sizeof(cplx<double>) * ge, hipMemcpyDeviceToHost, ctx->stream));

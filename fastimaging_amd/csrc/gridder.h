@@ -14,8 +14,7 @@ struct GridderArgs {
     bool halfplane;
     bool generate_beam;
     double uv_scale;     // applied to u and v (imager: cell_rad * N; unit gridder: 1)
-    int kernel_function;
-    double kernel_param;
+    stp_kernel_spec kernel; // the kernel_creator functor
 };
 
 // Grids n visibilities into vis_grid (and sampling_grid), each rows x N complex

@@ -243,14 +243,14 @@ __device__ double dev_kernel(const GridGeom& g, double r)
     case STP_KERNEL_TRIANGLE:
         return fmax(1.0 - fabs(r) / g.kparam, 0.0);
     case STP_KERNEL_SINC: {
-        double x = r * (M_PI / 1.0);
+        double x = r * (M_PI / g.kws);
         if (x == 0.0)
             x = 1.0e-20;
         v = sin(x) / x;
         break;
     }
     case STP_KERNEL_GAUSSIAN:
-        v = exp((-1.0 / (1.0 * 1.0)) * (r * r));
+        v = exp((-1.0 / (g.kwg * g.kwg)) * (r * r));
         break;
     default: { // GaussianSinc
         double x = r * (M_PI / g.kws);
@@ -524,7 +524,7 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     g.tiles_c = int(div_up(N, TILE_C));
     g.scale = a.uv_scale;
     g.n = (long long)n;
-    const KernelFn kf = kernel_fn(a.kernel_function, a.kernel_param);
+    const KernelFn kf = kernel_fn(a.kernel);
     g.kfunc = kf.kind;
     g.kparam = kf.param;
     g.kwg = kf.width_g;
@@ -538,7 +538,7 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     const int cs = (os / 2) * 2 + 1;
     if (!a.kernel_exact) {
         char key[160];
-        snprintf(key, sizeof(key), "k1d:%d:%.17g:%d:%d", kf.kind, kf.param, a.support, os);
+        snprintf(key, sizeof(key), "k1d:%d:%.17g:%.17g:%.17g:%d:%d", kf.kind, kf.param, kf.width_g, kf.width_s, a.support, os);
         d_tab = has_table(ctx, key) ? dev_table(ctx, key, {}) : dev_table(ctx, key, kernel_1d_table(kf, a.support, os));
     } else {
         d_tab = dev_table(ctx, "k1d:none", std::vector<double>(1, 0.0));

@@ -19,6 +19,12 @@ KernelFn kernel_fn(int kind, double param)
     return k;
 }
 
+KernelFn kernel_fn(const stp_kernel_spec& s)
+{
+    KernelFn k{ s.kernel_function, s.param, s.width_normalization_gaussian, s.width_normalization_sinc };
+    return k;
+}
+
 static inline double gauss(double x, double w)
 {
     const double c = -1.0 / (w * w);
@@ -66,10 +72,10 @@ void eval_kernel(const KernelFn& k, const double* x, size_t n, double* out)
             v = std::max(1.0 - std::fabs(r) / k.param, 0.0);
             break;
         case STP_KERNEL_SINC:
-            v = sinc(r, 1.0);
+            v = sinc(r, k.width_s);
             break;
         case STP_KERNEL_GAUSSIAN:
-            v = gauss(r, 1.0);
+            v = gauss(r, k.width_g);
             break;
         case STP_KERNEL_GAUSSIANSINC:
             v = gauss(r, k.width_g) * sinc(r, k.width_s);
@@ -92,10 +98,10 @@ bool eval_gcf(const KernelFn& k, const double* x, size_t n, double* out)
 {
     for (size_t i = 0; i < n; i++) {
         if (k.kind == STP_KERNEL_GAUSSIAN) {
-            const double a = x[i] * M_PI / (1.0 * 2);
+            const double a = x[i] * M_PI / (k.width_g * 2);
             out[i] = std::exp(-1.0 * (a * a));
         } else if (k.kind == STP_KERNEL_SINC) {
-            out[i] = (std::fabs(x[i]) < 1.0) ? 1.0 : 0.0;
+            out[i] = (std::fabs(x[i]) < (1.0 * k.width_s)) ? 1.0 : 0.0;
         } else if (k.kind == STP_KERNEL_PSWF) {
             out[i] = spheroidal(x[i]);
         } else {

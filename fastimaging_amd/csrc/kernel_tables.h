@@ -9,6 +9,8 @@
 #include <cstddef>
 #include <vector>
 
+#include "../../include/stp_capi.h"
+
 namespace stp_hip {
 
 struct KernelFn {
@@ -18,6 +20,8 @@ struct KernelFn {
 };
 
 KernelFn kernel_fn(int kind, double param);
+// From a C-ABI kernel spec (widths as given).
+KernelFn kernel_fn(const struct stp_kernel_spec& spec);
 void eval_kernel(const KernelFn& k, const double* x, size_t n, double* out);
 bool eval_gcf(const KernelFn& k, const double* x, size_t n, double* out);
 

@@ -27,6 +27,34 @@ def _ptr(a):
     return a.ctypes.data_as(_P) if a is not None else None
 
 
+class IslandList:
+    """Sequence view over the first n records of a capi.Island array; records
+    are converted to dicts (capi.island_tuple) only when indexed."""
+
+    def __init__(self, arr, n):
+        self._arr = arr
+        self._n = n
+
+    def __len__(self):
+        return self._n
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[j] for j in range(*i.indices(self._n))]
+        if i < 0:
+            i += self._n
+        if not 0 <= i < self._n:
+            raise IndexError(i)
+        return capi.island_tuple(self._arr[i])
+
+    def __iter__(self):
+        return (capi.island_tuple(self._arr[i]) for i in range(self._n))
+
+    def raw(self):
+        """The underlying ctypes records (first len(self) are valid)."""
+        return self._arr
+
+
 class SourceFindResult:
     """Members of stp::SourceFindImage (sourcefind.h:146-164) that the C-ABI returns."""
 
@@ -64,8 +92,11 @@ class Context:
 
     # ----------------------------------------------------------------- imaging
     def image_visibilities(self, vis, vis_weights, uvw_lambda, img_pars, w_proj=None, a_proj=None,
-                           precision=capi.PRECISION_FP64, return_diag=False):
-        """stp::image_visibilities (imager.h:233-376); returns (image, beam)."""
+                           precision=capi.PRECISION_FP64, return_diag=False, kernel=None):
+        """stp::image_visibilities (imager.h:233-376); returns (image, beam).
+
+        kernel: capi.KernelSpec (the kernel_creator functor) or None for the
+        ImageVisibilities dispatch on img_pars (imager.cpp:26-60)."""
         uvw = np.asfortranarray(uvw_lambda, dtype=np.float64)
         n = uvw.shape[0]
         vis = np.ascontiguousarray(vis, dtype=np.complex128).reshape(n)
@@ -76,15 +107,23 @@ class Context:
         diag = capi.ImagerDiag()
         self._check(self.lib.stp_image_visibilities(
             self.h, ctypes.byref(img_pars), ctypes.byref(w_proj) if w_proj is not None else None,
-            ctypes.byref(a_proj) if a_proj is not None else None, n, _ptr(vis), _ptr(w), _ptr(uvw),
+            ctypes.byref(a_proj) if a_proj is not None else None, ctypes.byref(kernel) if kernel is not None else None,
+            n, _ptr(vis), _ptr(w), _ptr(uvw),
             precision, _ptr(image), _ptr(beam), ctypes.byref(diag)))
         if diag.image_rows == 0:
             image, beam = np.empty((0, 0)), (np.empty((0, 0)) if img_pars.generate_beam else None)
         return (image, beam, diag) if return_diag else (image, beam)
 
     def convolve_to_grid(self, kernel_function, kernel_param, support, image_size, uv, vis, weights,
-                         kernel_exact=True, oversampling=1, shift_uv=True, halfplane=True, generate_beam=True):
-        """stp::convolve_to_grid (gridder.h:240-997); returns (vis_grid, sampling_grid, total)."""
+                         kernel_exact=True, oversampling=1, shift_uv=True, halfplane=True, generate_beam=True,
+                         widths=None):
+        """stp::convolve_to_grid (gridder.h:240-997); returns (vis_grid, sampling_grid, total).
+
+        kernel_function / kernel_param / widths describe the kernel_creator
+        functor (widths = (gaussian, sinc) width normalisations, None for the
+        functor's defaults)."""
+        wg, ws = widths if widths is not None else (None, None)
+        spec = capi.kernel_spec(kernel_function, float(kernel_param), wg, ws)
         uv = np.asfortranarray(uv, dtype=np.float64)
         n = uv.shape[0]
         vis = np.ascontiguousarray(vis, dtype=np.complex128).reshape(n)
@@ -94,7 +133,7 @@ class Context:
         sg = np.empty((rows, image_size), dtype=np.complex128, order="F") if generate_beam else None
         total = ctypes.c_double()
         self._check(self.lib.stp_convolve_to_grid(
-            self.h, kernel_function, float(kernel_param), support, image_size, n, _ptr(uv), _ptr(vis), _ptr(w),
+            self.h, ctypes.byref(spec), support, image_size, n, _ptr(uv), _ptr(vis), _ptr(w),
             int(kernel_exact), oversampling, int(shift_uv), int(halfplane), int(generate_beam), _ptr(vg), _ptr(sg),
             ctypes.byref(total)))
         return vg, sg, total.value
@@ -145,21 +184,29 @@ class Context:
         self._check(self.lib.stp_memcpy_dtoh(self.h, _ptr(arr), dptr, arr.nbytes))
 
     def image_visibilities_device(self, img_pars, n, d_vis, d_w, d_uvw, d_image, d_beam, precision=0,
-                                  synchronize=False, w_proj=None):
+                                  synchronize=False, w_proj=None, kernel=None):
         diag = capi.ImagerDiag()
         self._check(self.lib.stp_image_visibilities_device(
-            self.h, ctypes.byref(img_pars), ctypes.byref(w_proj) if w_proj is not None else None, None, n,
+            self.h, ctypes.byref(img_pars), ctypes.byref(w_proj) if w_proj is not None else None, None,
+            ctypes.byref(kernel) if kernel is not None else None, n,
             d_vis, d_w, d_uvw, precision, d_image, d_beam, int(synchronize),
             ctypes.byref(diag) if synchronize else None))
         return diag
 
-    def source_find_device(self, d_image, rows, cols, pars, d_label_map=None, precision=0, capacity=1 << 16):
-        isl = (capi.Island * capacity)()
+    def source_find_device(self, d_image, rows, cols, pars, d_label_map=None, precision=0, capacity=1 << 16,
+                           out=None):
+        """Source finding on a device-resident image.  Returns (IslandList, stats).
+        out: a reusable (capi.Island * capacity) array; the returned list then
+        views it and is overwritten by the next call that reuses it."""
+        if out is not None:
+            isl, capacity = out, len(out)
+        else:
+            isl = (capi.Island * capacity)()
         n = ctypes.c_size_t()
         st = capi.SourceFindStats()
         self._check(self.lib.stp_source_find_device(self.h, d_image, precision, rows, cols, ctypes.byref(pars),
                                                     d_label_map, isl, capacity, ctypes.byref(n), ctypes.byref(st)))
-        return [capi.island_tuple(isl[i]) for i in range(n.value)], st
+        return IslandList(isl, n.value), st
 
     def synchronize(self):
         self._check(self.lib.stp_ctx_synchronize(self.h))

@@ -45,7 +45,7 @@ static void normalise(const double* img, size_t padded, size_t image_size, doubl
     }
 }
 
-ImagerResult image_visibilities(const stp_imager_pars& p, const stp_wproj_pars* wpp, size_t n,
+ImagerResult image_visibilities(const stp_imager_pars& p, const stp_wproj_pars* wpp, const stp_kernel_spec* kernel, size_t n,
     const cx* vis, const double* weights, const double* uvw, int nthreads)
 {
     const int padded = int(p.padded_image_size);
@@ -64,8 +64,9 @@ ImagerResult image_visibilities(const stp_imager_pars& p, const stp_wproj_pars* 
         if (p.kernel_exact)
             throw std::invalid_argument("W-projection requires kernel_exact == false (imager.h:287)");
     }
-    // GaussianSinc etc. receive trunc = kernel_support (imager.cpp:26-60)
-    const KernelFunc kf = make_kernel_func(p.kernel_function, double(p.kernel_support));
+    // kernel_creator: given, or built by the ImageVisibilities dispatch with
+    // trunc / half_base_width = kernel_support (imager.cpp:26-60)
+    const KernelFunc kf = kernel ? make_kernel_func(*kernel) : make_kernel_func(p.kernel_function, double(p.kernel_support));
 
     // uv -> pixels (imager.h:292-300)
     const double inv_px = ((p.cell_size / 3600.0) * (M_PI / 180.0)) * double(padded);

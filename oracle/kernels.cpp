@@ -21,6 +21,11 @@ KernelFunc make_kernel_func(int kind, double param)
     return k;
 }
 
+KernelFunc make_kernel_func(const stp_kernel_spec& s)
+{
+    return KernelFunc{ s.kernel_function, s.param, s.width_normalization_gaussian, s.width_normalization_sinc };
+}
+
 // Gaussian::operator() (conv_func.cpp:71-82) without truncation.
 static inline double gaussian_value(double x, double w)
 {
@@ -81,12 +86,12 @@ void eval_kernel(const KernelFunc& k, const double* x, size_t n, double* out)
             v = std::max(1.0 - std::fabs(r) / k.param, 0.0);
             break;
         case STP_KERNEL_SINC:
-            v = sinc_value(r, 1.0);
+            v = sinc_value(r, k.width_s);
             if (k.param > 0.0 && std::fabs(r) > k.param)
                 v = 0.0;
             break;
         case STP_KERNEL_GAUSSIAN:
-            v = gaussian_value(r, 1.0);
+            v = gaussian_value(r, k.width_g);
             if (k.param > 0.0 && std::fabs(r) > k.param)
                 v = 0.0;
             break;
@@ -114,12 +119,12 @@ bool eval_gcf(const KernelFunc& k, const double* x, size_t n, double* out)
     for (size_t i = 0; i < n; i++) {
         switch (k.kind) {
         case STP_KERNEL_GAUSSIAN: { // conv_func.cpp:85-89
-            const double a = x[i] * M_PI / (1.0 * 2);
+            const double a = x[i] * M_PI / (k.width_g * 2);
             out[i] = std::exp(-1.0 * (a * a));
             break;
         }
         case STP_KERNEL_SINC: // conv_func.cpp:62-67
-            out[i] = (std::fabs(x[i]) < 1.0) ? 1.0 : 0.0;
+            out[i] = (std::fabs(x[i]) < (1.0 * k.width_s)) ? 1.0 : 0.0;
             break;
         case STP_KERNEL_PSWF: // conv_func.cpp:199-202
             out[i] = grdsf(x[i]);

@@ -42,6 +42,8 @@ struct KernelFunc {
     double width_s;  // Sinc width normalisation
 };
 KernelFunc make_kernel_func(int kind, double param);
+// From a kernel spec (functor arguments as given, conv_func.h:23-262).
+KernelFunc make_kernel_func(const stp_kernel_spec& spec);
 // Applies the functor element-wise (operator() of the functor classes).
 void eval_kernel(const KernelFunc& k, const double* x, size_t n, double* out);
 // gcf() of the functor (analytic grid-correction); false if the reference asserts.
@@ -102,7 +104,7 @@ struct ImagerResult {
     double sample_grid_total = 0.0;
     size_t num_good_vis = 0;
 };
-ImagerResult image_visibilities(const stp_imager_pars& p, const stp_wproj_pars* wp, size_t n,
+ImagerResult image_visibilities(const stp_imager_pars& p, const stp_wproj_pars* wp, const stp_kernel_spec* kernel, size_t n,
     const cx* vis, const double* weights, const double* uvw, int nthreads);
 
 /* ---------------- statistics: src/stp/common/matrix_math.cpp ---------------- */

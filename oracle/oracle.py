@@ -41,39 +41,46 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def _spec(kind, param, widths):
+    """KernelSpec of a functor; widths = (width_normalization_gaussian,
+    width_normalization_sinc) or None for the reference's defaults."""
+    wg, ws = widths if widths is not None else (None, None)
+    return ctypes.byref(capi.kernel_spec(kind, param, wg, ws))
+
+
 def threads():
     return lib().oracle_default_threads()
 
 
-def kernel_eval(kind, param, x):
+def kernel_eval(kind, param, x, widths=None):
     x = _f64(x)
     out = np.empty_like(x)
-    _check(lib().oracle_kernel_eval(ctypes.c_int(kind), ctypes.c_double(param), _ptr(x), ctypes.c_size_t(x.size), _ptr(out)))
+    _check(lib().oracle_kernel_eval(_spec(kind, param, widths), _ptr(x), ctypes.c_size_t(x.size), _ptr(out)))
     return out
 
 
-def make_1d_kernel(kind, param, support, offset, oversampling=1, pad=False, normalize=True):
+def make_1d_kernel(kind, param, support, offset, oversampling=1, pad=False, normalize=True, widths=None):
     n = 2 * (support + (1 if pad else 0)) * oversampling + 1
     out = np.empty(n)
-    _check(lib().oracle_make_1d_kernel(ctypes.c_int(kind), ctypes.c_double(param), ctypes.c_int(support),
+    _check(lib().oracle_make_1d_kernel(_spec(kind, param, widths), ctypes.c_int(support),
                                        ctypes.c_double(offset), ctypes.c_int(oversampling), ctypes.c_int(int(pad)),
                                        ctypes.c_int(int(normalize)), _ptr(out)))
     return out
 
 
-def make_kernel_array(kind, param, support, offset, oversampling=1, pad=False, normalize=True):
+def make_kernel_array(kind, param, support, offset, oversampling=1, pad=False, normalize=True, widths=None):
     n = 2 * (support + (1 if pad else 0)) * oversampling + 1
     out = np.empty((n, n), order="F")
-    _check(lib().oracle_make_kernel_array(ctypes.c_int(kind), ctypes.c_double(param), ctypes.c_int(support),
+    _check(lib().oracle_make_kernel_array(_spec(kind, param, widths), ctypes.c_int(support),
                                           ctypes.c_double(offset[0]), ctypes.c_double(offset[1]),
                                           ctypes.c_int(oversampling), ctypes.c_int(int(pad)),
                                           ctypes.c_int(int(normalize)), _ptr(out)))
     return out
 
 
-def img_dom_kernel(kind, param, size, normalize=False, analytic=False):
+def img_dom_kernel(kind, param, size, normalize=False, analytic=False, widths=None):
     out = np.empty(size)
-    _check(lib().oracle_img_dom_kernel(ctypes.c_int(kind), ctypes.c_double(param), ctypes.c_size_t(size),
+    _check(lib().oracle_img_dom_kernel(_spec(kind, param, widths), ctypes.c_size_t(size),
                                        ctypes.c_int(int(normalize)), ctypes.c_int(int(analytic)), _ptr(out)))
     return out
 
@@ -87,7 +94,7 @@ def oversampled_indices(frac, oversampling):
 
 def convolve_to_grid(kind, param, support, image_size, uv, vis, weights, kernel_exact=True, oversampling=1,
                      shift_uv=True, halfplane=True, generate_beam=True, wproj=None, w_lambda=None,
-                     cell_size=0.0, analytic_gcf=True, nthreads=None):
+                     cell_size=0.0, analytic_gcf=True, nthreads=None, widths=None):
     """convolve_to_grid (src/stp/gridder/gridder.h:240-997). uv: (n, 2) pixel units."""
     uv = np.asfortranarray(uv, dtype=np.float64)
     n = uv.shape[0]
@@ -101,7 +108,7 @@ def convolve_to_grid(kind, param, support, image_size, uv, vis, weights, kernel_
     wl = _f64(w_lambda) if w_lambda is not None else None
     wp = wproj if wproj is not None else None
     _check(lib().oracle_convolve_to_grid(
-        ctypes.c_int(kind), ctypes.c_double(param), ctypes.c_uint(support), ctypes.c_int(image_size),
+        _spec(kind, param, widths), ctypes.c_uint(support), ctypes.c_int(image_size),
         ctypes.c_size_t(n), _ptr(uv), _ptr(vis), _ptr(weights), ctypes.c_int(int(kernel_exact)),
         ctypes.c_uint(oversampling), ctypes.c_int(int(shift_uv)), ctypes.c_int(int(halfplane)),
         ctypes.c_int(int(generate_beam)), ctypes.byref(wp) if wp is not None else None, _ptr(wl),
@@ -141,7 +148,7 @@ def wproj_kernel_cache(w_value, aa, workarea, cell_rad, oversampling, scaling, s
     return np.ascontiguousarray(np.swapaxes(out, 2, 3))
 
 
-def image_visibilities(pars, vis, weights, uvw, wproj=None, nthreads=None):
+def image_visibilities(pars, vis, weights, uvw, wproj=None, nthreads=None, kernel=None):
     """image_visibilities (src/stp/imager/imager.h:233-376). Returns (image, beam, total, ngood)."""
     uvw = np.asfortranarray(uvw, dtype=np.float64)
     n = uvw.shape[0]
@@ -154,6 +161,7 @@ def image_visibilities(pars, vis, weights, uvw, wproj=None, nthreads=None):
     ngood = ctypes.c_uint64()
     empty = ctypes.c_int()
     _check(lib().oracle_image_visibilities(ctypes.byref(pars), ctypes.byref(wproj) if wproj is not None else None,
+                                           ctypes.byref(kernel) if kernel is not None else None,
                                            ctypes.c_size_t(n), _ptr(vis), _ptr(weights), _ptr(uvw),
                                            ctypes.c_int(nthreads or threads()), _ptr(image), _ptr(beam),
                                            ctypes.byref(total), ctypes.byref(ngood), ctypes.byref(empty)))
@@ -242,11 +250,11 @@ def source_find(img, pars, capacity=100000):
     return lm, [capi.island_tuple(isl[i]) for i in range(n.value)], st
 
 
-def populate_kernel_cache(kind, param, support, oversampling):
+def populate_kernel_cache(kind, param, support, oversampling, widths=None):
     """populate_kernel_cache (gridder.h:73-105): returns [a, b] -> (2s+1, 2s+1) real matrices."""
     cs = (oversampling // 2) * 2 + 1
     ks = 2 * support + 1
     out = np.empty((cs, cs, ks, ks), dtype=np.complex128)
-    _check(lib().oracle_populate_kernel_cache(ctypes.c_int(kind), ctypes.c_double(param), ctypes.c_uint(support),
+    _check(lib().oracle_populate_kernel_cache(_spec(kind, param, widths), ctypes.c_uint(support),
                                               ctypes.c_uint(oversampling), _ptr(out)))
     return np.ascontiguousarray(np.swapaxes(out, 2, 3))

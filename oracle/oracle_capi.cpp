@@ -29,40 +29,40 @@ extern "C" {
 const char* oracle_last_error() { return g_err.c_str(); }
 int oracle_default_threads() { return default_threads(); }
 
-int oracle_kernel_eval(int kind, double param, const double* x, size_t n, double* out)
+int oracle_kernel_eval(const stp_kernel_spec* k, const double* x, size_t n, double* out)
 {
-    return guard([&] { eval_kernel(make_kernel_func(kind, param), x, n, out); });
+    return guard([&] { eval_kernel(make_kernel_func(*k), x, n, out); });
 }
 
-int oracle_make_1d_kernel(int kind, double param, int support, double offset, int os, int pad, int normalize, double* out)
+int oracle_make_1d_kernel(const stp_kernel_spec* k, int support, double offset, int os, int pad, int normalize, double* out)
 {
     return guard([&] {
-        auto v = make_1d_kernel(make_kernel_func(kind, param), support, offset, os, pad != 0, normalize != 0);
+        auto v = make_1d_kernel(make_kernel_func(*k), support, offset, os, pad != 0, normalize != 0);
         std::memcpy(out, v.data(), v.size() * sizeof(double));
     });
 }
 
-int oracle_make_kernel_array(int kind, double param, int support, double off_x, double off_y, int os, int pad, int normalize, double* out)
+int oracle_make_kernel_array(const stp_kernel_spec* k, int support, double off_x, double off_y, int os, int pad, int normalize, double* out)
 {
     return guard([&] {
-        auto v = make_kernel_array(make_kernel_func(kind, param), support, off_x, off_y, os, pad != 0, normalize != 0);
+        auto v = make_kernel_array(make_kernel_func(*k), support, off_x, off_y, os, pad != 0, normalize != 0);
         std::memcpy(out, v.data(), v.size() * sizeof(double));
     });
 }
 
-int oracle_img_dom_kernel(int kind, double param, size_t size, int normalize, int analytic, double* out)
+int oracle_img_dom_kernel(const stp_kernel_spec* k, size_t size, int normalize, int analytic, double* out)
 {
     return guard([&] {
-        auto v = img_dom_kernel(make_kernel_func(kind, param), size, normalize != 0, analytic != 0);
+        auto v = img_dom_kernel(make_kernel_func(*k), size, normalize != 0, analytic != 0);
         std::memcpy(out, v.data(), v.size() * sizeof(double));
     });
 }
 
 // out: cs x cs blocks of (2s+1)^2 complex, block (a, b) at offset (a*cs + b)
-int oracle_populate_kernel_cache(int kind, double param, unsigned support, unsigned os, double* out)
+int oracle_populate_kernel_cache(const stp_kernel_spec* k, unsigned support, unsigned os, double* out)
 {
     return guard([&] {
-        auto c = populate_kernel_cache(make_kernel_func(kind, param), support, os);
+        auto c = populate_kernel_cache(make_kernel_func(*k), support, os);
         size_t off = 0;
         for (auto& k : c) {
             std::memcpy(out + off, k.data(), k.size() * sizeof(cx));
@@ -76,7 +76,7 @@ int oracle_oversampled_indices(const double* frac, size_t n, unsigned os, int* o
     return guard([&] { calculate_oversampled_kernel_indices(frac, n, os, out); });
 }
 
-int oracle_convolve_to_grid(int kind, double param, unsigned support, int image_size, size_t n,
+int oracle_convolve_to_grid(const stp_kernel_spec* k, unsigned support, int image_size, size_t n,
     const double* uv, const double* vis, const double* w, int exact, unsigned os, int shift_uv, int halfplane,
     int beam, const stp_wproj_pars* wpp, const double* w_lambda, double cell_size, int analytic_gcf, int nthreads,
     double* vis_grid, double* sampling_grid, double* total, uint64_t* ngood)
@@ -91,7 +91,7 @@ int oracle_convolve_to_grid(int kind, double param, unsigned support, int image_
             wp.hankel = wpp->hankel_opt != 0;
             wp.wplanes_median = wpp->wplanes_median != 0;
         }
-        GridderOutput g = convolve_to_grid(make_kernel_func(kind, param), support, image_size, uv,
+        GridderOutput g = convolve_to_grid(make_kernel_func(*k), support, image_size, uv,
             reinterpret_cast<const cx*>(vis), w, n, exact != 0, os, shift_uv != 0, halfplane != 0, beam != 0, wp, w_lambda,
             cell_size, analytic_gcf != 0, nthreads);
         std::memcpy(vis_grid, g.vis_grid.data(), g.vis_grid.size() * sizeof(cx));
@@ -128,11 +128,11 @@ int oracle_wproj_kernel_cache(double w_value, const double* aa, unsigned workare
 }
 
 // Returns 0 with *empty = 1 when sample_grid_total <= 0 (outputs untouched).
-int oracle_image_visibilities(const stp_imager_pars* p, const stp_wproj_pars* wp, size_t n, const double* vis,
+int oracle_image_visibilities(const stp_imager_pars* p, const stp_wproj_pars* wp, const stp_kernel_spec* k, size_t n, const double* vis,
     const double* weights, const double* uvw, int nthreads, double* image, double* beam, double* total, uint64_t* ngood, int* empty)
 {
     return guard([&] {
-        ImagerResult r = image_visibilities(*p, wp, n, reinterpret_cast<const cx*>(vis), weights, uvw, nthreads);
+        ImagerResult r = image_visibilities(*p, wp, k, n, reinterpret_cast<const cx*>(vis), weights, uvw, nthreads);
         *total = r.sample_grid_total;
         if (ngood)
             *ngood = r.num_good_vis;
