@@ -254,6 +254,7 @@ _SIG = {
     "stp_memcpy_htod": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "stp_memcpy_dtoh": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "stp_ctx_set_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "stp_ctx_set_timing_filter": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "stp_ctx_reset_timings": (ctypes.c_int, [_P]),
     "stp_ctx_get_timings": (ctypes.c_int, [_P, ctypes.c_char_p, _D, ctypes.POINTER(ctypes.c_uint64),
                                            ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),

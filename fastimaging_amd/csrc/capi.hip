@@ -172,6 +172,9 @@ stp_status stp_ctx_destroy(stp_ctx* ctx)
     for (auto& kv : ctx->ws)
         if (kv.second.ptr)
             (void)hipFree(kv.second.ptr);
+    for (auto& kv : ctx->host_ws)
+        if (kv.second.ptr)
+            (void)hipHostFree(kv.second.ptr);
     for (auto& kv : ctx->tables)
         if (kv.second.ptr)
             (void)hipFree(kv.second.ptr);
@@ -497,6 +500,27 @@ stp_status stp_memcpy_dtoh(stp_ctx* ctx, void* dst, const void* src, size_t byte
 stp_status stp_ctx_set_timing(stp_ctx* ctx, int32_t enable)
 {
     return guarded(ctx, [&] { ctx->timing = enable != 0; });
+}
+
+stp_status stp_ctx_set_timing_filter(stp_ctx* ctx, const char* names)
+{
+    return guarded(ctx, [&] {
+        ctx->timing_filter.clear();
+        if (!names)
+            return;
+        std::string cur;
+        for (const char* c = names;; c++) {
+            if (*c == ',' || *c == 0) {
+                if (!cur.empty())
+                    ctx->timing_filter.push_back(cur);
+                cur.clear();
+                if (*c == 0)
+                    break;
+            } else {
+                cur.push_back(*c);
+            }
+        }
+    });
 }
 
 stp_status stp_ctx_reset_timings(stp_ctx* ctx)

@@ -69,9 +69,11 @@ struct stp_ctx {
     hipStream_t stream = nullptr;
     std::string last_error;
     std::map<std::string, stp_hip::DevBuf> ws;
+    std::map<std::string, stp_hip::DevBuf> host_ws; // pinned host staging (hipHostMalloc)
     std::map<int, stp_hip::Twiddles> twiddles;
     std::map<std::string, stp_hip::DevBuf> tables;
     bool timing = false;
+    std::vector<std::string> timing_filter; // empty: every launch
     std::vector<stp_hip::PendingTiming> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, stp_hip::TimingAcc> timings;
@@ -82,6 +84,8 @@ namespace stp_hip {
 
 // Returns a device buffer of at least `bytes` for slot `name` (contents undefined).
 void* ws_get(stp_ctx* ctx, const std::string& name, size_t bytes);
+// Returns a pinned host buffer of at least `bytes` for slot `name` (contents undefined).
+void* host_stage(stp_ctx* ctx, const std::string& name, size_t bytes);
 // Device copy of a host table, uploaded once per key and kept for the context's lifetime.
 const double* dev_table(stp_ctx* ctx, const std::string& key, const std::vector<double>& host);
 bool has_table(stp_ctx* ctx, const std::string& key);

@@ -26,6 +26,26 @@ void* ws_get(stp_ctx* ctx, const std::string& name, size_t bytes)
     return b.ptr;
 }
 
+void* host_stage(stp_ctx* ctx, const std::string& name, size_t bytes)
+{
+    if (bytes == 0)
+        bytes = 256;
+    DevBuf& b = ctx->host_ws[name];
+    if (b.bytes >= bytes)
+        return b.ptr;
+    if (b.ptr) {
+        // queued copies may still target the old buffer
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        HIP_CHECK(hipHostFree(b.ptr));
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    const size_t alloc = (bytes + 4095) & ~size_t(4095);
+    HIP_CHECK(hipHostMalloc(&b.ptr, alloc, hipHostMallocDefault));
+    b.bytes = alloc;
+    return b.ptr;
+}
+
 bool has_table(stp_ctx* ctx, const std::string& key) { return ctx->tables.count(key) != 0; }
 
 const double* dev_table(stp_ctx* ctx, const std::string& key, const std::vector<double>& host)
@@ -85,7 +105,16 @@ LaunchTimer::LaunchTimer(stp_ctx* c, const char* n)
     : ctx(c)
     , name(n)
 {
-    if (ctx->timing) {
+    bool on = ctx->timing;
+    if (on && !ctx->timing_filter.empty()) {
+        on = false;
+        for (const auto& f : ctx->timing_filter)
+            if (f == n) {
+                on = true;
+                break;
+            }
+    }
+    if (on) {
         a = take_event(ctx);
         b = take_event(ctx);
         HIP_CHECK(hipEventRecord(a, ctx->stream));

@@ -215,6 +215,11 @@ class Context:
     def set_timing(self, enable=True):
         self._check(self.lib.stp_ctx_set_timing(self.h, int(enable)))
 
+    def set_timing_filter(self, names=None):
+        """Times only the named kernels (iterable of names); None times every launch."""
+        arg = ",".join(names).encode() if names else None
+        self._check(self.lib.stp_ctx_set_timing_filter(self.h, arg))
+
     def reset_timings(self):
         self._check(self.lib.stp_ctx_reset_timings(self.h))
 

@@ -347,6 +347,9 @@ stp_status stp_memcpy_dtoh(stp_ctx* ctx, void* dst, const void* src, size_t byte
 /* When enabled, every kernel launch of the context is bracketed by HIP events
  * recorded on the context stream, and durations accumulate per kernel name. */
 stp_status stp_ctx_set_timing(stp_ctx* ctx, int32_t enable);
+/* Restricts the timing to the kernels named in `names` (comma-separated, e.g.
+ * "grid_tiles,fft_u1"); NULL or "" times every launch. */
+stp_status stp_ctx_set_timing_filter(stp_ctx* ctx, const char* names);
 stp_status stp_ctx_reset_timings(stp_ctx* ctx);
 /* Copies up to `cap` entries: names (NUL-terminated, 64 bytes each), total
  * milliseconds and launch counts.  *n receives the number of distinct kernels. */

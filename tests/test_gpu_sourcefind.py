@@ -129,6 +129,23 @@ def test_gpu_sourcefind_degenerate(gpu, orc):
     compare(gpu.source_find(a, pars), orc.source_find(a, pars))
 
 
+@pytest.mark.parametrize("median", [capi.MEDIAN_BINMEDIAN, capi.MEDIAN_NTHELEMENT])
+def test_gpu_exact_median_two_level(gpu, orc, median):
+    """A peaked distribution (99% of pixels with sigma 1e-3, 1% with sigma 100) puts
+    ~all pixels in the first-level bin of the median, so the selection refines
+    through the second level; the median must still be the exact order statistic."""
+    rng = np.random.default_rng(42)
+    n = 1024
+    img = rng.normal(scale=1e-3, size=(n, n))
+    wide = rng.random(size=(n, n)) < 0.01
+    img[wide] = rng.normal(scale=100.0, size=int(wide.sum()))
+    img = np.asfortranarray(img)
+    pars = capi.sourcefind_pars(5.0, 3.0, median_method=median)
+    res = gpu.source_find(img, pars)
+    assert res.stats.bg_level == np.median(img)
+    compare(res, orc.source_find(img, pars))
+
+
 def test_gpu_source_detection_kats(gpu):
     """sourcefind_test_BasicSourceDetection / NegativeSourceDetection through the GPU."""
     for sign, neg in ((1.0, False), (-1.0, True)):
