@@ -2,20 +2,25 @@
 
 One step = one snapshot through stp_image_visibilities_device (gridding, c2r of
 image and beam, normalisation) followed by stp_source_find_device, with the
-visibilities already resident in HBM.  Workload: N=4096, cell 1", 1M
+visibilities already resident in HBM.  Workload c2: N=4096, cell 1", 1M
 visibilities, 50 point sources (log-uniform 0.5-20 Jy), noise 0.1 Jy,
 GaussianSinc support 3, oversampling 9, beam on; source finding at 5 sigma
 detection / 3 sigma analysis, BINMEDIAN, 8-connectivity.
 
-Prints one JSON line (rank 0).  A short untimed pass brackets every launch with
-HIP events on the library's stream for the per-kernel table; the timed region
-then carries events around the dominant kernel only (the kernel with the
-largest total time), whose average duration and algorithmic bytes per launch
-give `roofline`.
+Three arithmetic paths run: the FP32 path with FP64 tile sums ("fp32c", the
+headline `value`), the FP32 path with two-float tile sums ("fp32tf") and the
+plain FP64 path ("fp64").  For each, a short
+untimed pass brackets every launch with HIP events on the library's stream for
+the per-kernel table; the timed region then carries events around the dominant
+kernel only, whose average duration and algorithmic bytes per launch give
+`roofline`.  The CPU baseline leg times the FP64 CPU oracle on the host and
+reports each path's image error, and the source position and flux error,
+against it.
+
+Prints one JSON line.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -26,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+METRIC = "snapshots/s + p50/p99 latency, FP32-comp vs FP64 GPU at <=1e-4 peak err"
 
 
 def parse():
@@ -34,28 +40,111 @@ def parse():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32c"])
+    ap.add_argument("--precision", default="all", choices=["all", "both", "fp64", "fp32c", "fp32tf"],
+                    help="paths to run (all: fp64, fp32c and fp32tf; both: fp64 and fp32c); "
+                         "the headline is fp32c when it runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sourcefind", action="store_true")
     ap.add_argument("--cpu-snapshots", type=int, default=2)
     return ap.parse_args()
 
 
-def kernel_bytes(name, N, rows, nvis, n_entries, nb):
-    """Algorithmic HBM bytes per launch of each kernel (S = 8 bytes per real)."""
-    grid = rows * N * 16  # one complex FP64 half-plane grid
-    img = N * N * 8
+def kernel_bytes(name, N, rows, nvis, n_entries, nb, S):
+    """Algorithmic HBM bytes per launch (S = bytes per real of the grid and image)."""
+    grid = rows * N * 2 * S  # one complex half-plane grid
+    img = N * N * S
     if name in ("fft_u1", "fft_u2"):
         return 2 * grid * nb            # read + write every grid once
     if name == "fft_v_c2r":
         return grid * nb + img * nb     # read grids, write images
     if name == "grid_tiles":
-        return grid * nb                # write every grid element once
+        return grid * nb + 40 * n_entries  # write every grid element once, read the binned entries
     if name == "grid_count":
         return 48 * nvis                # read u, v, w(unused), vis, weight
     if name == "grid_scatter":
         return 48 * nvis + 40 * n_entries
     return None
+
+
+def measure(ctx, capi, args, pars, sfp, prec, n, N, bufs, do_sf):
+    d_vis, d_w, d_uvw, d_img, d_beam, d_lab = bufs
+    island_buf = (capi.Island * (1 << 16))()
+    S = 8 if prec == capi.PRECISION_FP64 else 4
+    rows = N // 2 + 1
+
+    def step():
+        ctx.image_visibilities_device(pars, n, d_vis, d_w, d_uvw, d_img, d_beam, precision=prec,
+                                      synchronize=not do_sf)
+        isl = None
+        if do_sf:
+            isl, _ = ctx.source_find_device(d_img, N, N, sfp, d_label_map=d_lab, precision=prec, out=island_buf)
+        return isl
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    # entries written by the gridder (for the byte counts): estimate from the tile geometry
+    n_entries = int(1.2 * n)
+
+    # per-kernel breakdown: every launch bracketed by HIP events (untimed pass)
+    ctx.set_timing(True)
+    ctx.set_timing_filter(None)
+    ctx.reset_timings()
+    nb_steps = max(1, min(5, args.steps))
+    for _ in range(nb_steps):
+        step()
+    ctx.synchronize()
+    kernels = {}
+    for name, (ms, cnt) in ctx.timings().items():
+        if name in ("image_visibilities", "fill"):
+            continue
+        avg_ms = ms / max(cnt, 1)
+        kb = kernel_bytes(name, N, rows, n, n_entries, 2, S)
+        kernels[name] = {"avg_us": round(avg_ms * 1000.0, 2), "per_step": round(cnt / nb_steps, 2),
+                         "GBs": (round(kb / (avg_ms * 1e-3) / 1e9, 1) if kb else None)}
+    dominant = max((k for k in kernels if kernel_bytes(k, N, rows, n, n_entries, 2, S)),
+                   key=lambda k: kernels[k]["avg_us"] * kernels[k]["per_step"])
+
+    # timed region: only the dominant kernel carries events (its live duration for the roofline)
+    ctx.set_timing_filter([dominant])
+    ctx.reset_timings()
+    lat = []
+    islands = None
+    ctx.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        islands = step()
+        ctx.synchronize()
+        lat.append(time.perf_counter() - s0)
+    ctx.synchronize()
+    t_total = time.perf_counter() - t_start
+    dom_ms, dom_cnt = ctx.timings()[dominant]
+    ctx.set_timing(False)
+    ctx.set_timing_filter(None)
+
+    dom_bytes = kernel_bytes(dominant, N, rows, n, n_entries, 2, S)
+    dom_avg_s = dom_ms * 1e-3 / max(dom_cnt, 1)
+    achieved = dom_bytes / dom_avg_s / 1e9
+    ms_per_step = 1000.0 * t_total / args.steps
+    # whole-snapshot compulsory bytes (beam on, 6 source-find passes over the image)
+    B = 48 * n + 2 * (2 * rows * N * 2 * S) + 2 * N * N * S + (6 * N * N * S + 4 * N * N if do_sf else 0)
+    image = np.empty((N, N), order="F", dtype=np.float64 if S == 8 else np.float32)
+    ctx.dtoh(image, d_img)
+    return {
+        "value": args.steps / t_total,
+        "ms_per_step": ms_per_step,
+        "p50_ms": float(np.percentile(lat, 50) * 1000.0),
+        "p99_ms": float(np.percentile(lat, 99) * 1000.0),
+        "roofline": {"bound": "hbm", "kernel": dominant, "avg_us": round(dom_avg_s * 1e6, 2),
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "path_roofline": {"compulsory_bytes": B, "achieved_GBs": round(B / (ms_per_step * 1e-3) / 1e9, 1),
+                          "frac": round(B / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "kernels": kernels,
+        "islands": [dict(i) for i in islands] if islands is not None else None,
+        "image": image,
+    }
 
 
 def main():
@@ -77,136 +166,96 @@ def main():
     pars = capi.imager_pars(N, wl["cell_arcsec"], kernel_function=capi.KERNEL_GAUSSIANSINC, kernel_support=3,
                             kernel_exact=False, oversampling=9, generate_beam=True)
     sfp = capi.sourcefind_pars(5.0, 3.0, median_method=capi.MEDIAN_BINMEDIAN)
-    prec = capi.PRECISION_FP64 if args.precision == "fp64" else capi.PRECISION_FP32C
-    osz = 8 if prec == capi.PRECISION_FP64 else 4
+    do_sf = not args.no_sourcefind
 
     ctx = Context(0)
-    d_vis = ctx.malloc(16 * n)
-    d_w = ctx.malloc(8 * n)
-    d_uvw = ctx.malloc(24 * n)
-    d_img = ctx.malloc(osz * N * N)
-    d_beam = ctx.malloc(osz * N * N)
-    d_lab = ctx.malloc(4 * N * N)
-    ctx.htod(d_vis, np.ascontiguousarray(snap["vis"]))
-    ctx.htod(d_w, np.ascontiguousarray(snap["weights"]))
-    ctx.htod(d_uvw, np.asfortranarray(snap["uvw"]))
+    bufs = [ctx.malloc(16 * n), ctx.malloc(8 * n), ctx.malloc(24 * n), ctx.malloc(8 * N * N), ctx.malloc(8 * N * N),
+            ctx.malloc(4 * N * N)]
+    ctx.htod(bufs[0], np.ascontiguousarray(snap["vis"]))
+    ctx.htod(bufs[1], np.ascontiguousarray(snap["weights"]))
+    ctx.htod(bufs[2], np.asfortranarray(snap["uvw"]))
 
-    do_sf = not args.no_sourcefind
-    island_buf = (capi.Island * (1 << 16))()
-
-    def step():
-        diag = ctx.image_visibilities_device(pars, n, d_vis, d_w, d_uvw, d_img, d_beam, precision=prec,
-                                             synchronize=not do_sf)
-        isl = None
-        if do_sf:
-            isl, _ = ctx.source_find_device(d_img, N, N, sfp, d_label_map=d_lab, precision=prec, out=island_buf)
-        return diag, isl
-
-    for _ in range(args.warmup):
-        step()
-    ctx.synchronize()
-    rows = N // 2 + 1
-    # entries written by the gridder (for the scatter kernel's byte count): estimate from geometry
-    n_entries = int(1.3 * n)
-
-    # per-kernel breakdown: every launch bracketed by HIP events (untimed pass)
-    ctx.set_timing(True)
-    ctx.set_timing_filter(None)
-    ctx.reset_timings()
-    for _ in range(max(1, min(5, args.steps))):
-        step()
-    ctx.synchronize()
-    timings = ctx.timings()
-    kernels = {}
-    for name, (ms, cnt) in timings.items():
-        if name in ("image_visibilities", "fill"):
-            continue
-        avg_ms = ms / max(cnt, 1)
-        kb = kernel_bytes(name, N, rows, n, n_entries, 2)
-        kernels[name] = {"avg_us": round(avg_ms * 1000.0, 2), "launches": int(cnt),
-                         "total_ms": round(ms, 3),
-                         "GBs": (round(kb / (avg_ms * 1e-3) / 1e9, 1) if kb else None)}
-    dominant = max((k for k in kernels if kernel_bytes(k, N, rows, n, n_entries, 2)),
-                   key=lambda k: kernels[k]["total_ms"])
-
-    # timed region: only the dominant kernel carries events (its live duration for the roofline)
-    ctx.set_timing_filter([dominant])
-    ctx.reset_timings()
-    lat = []
-    ctx.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        s0 = time.perf_counter()
-        _, islands = step()
-        ctx.synchronize()
-        lat.append(time.perf_counter() - s0)
-    ctx.synchronize()
-    t_total = time.perf_counter() - t_start
-    dom_ms, dom_cnt = ctx.timings()[dominant]
-    ctx.set_timing(False)
-    ctx.set_timing_filter(None)
-
-    # correctness of the measured output against the oracle (also the CPU baseline)
-    image = np.empty((N, N), order="F", dtype=np.float64 if osz == 8 else np.float32)
-    ctx.dtoh(image, d_img)
-    ms_per_step = 1000.0 * t_total / args.steps
-    snaps_per_s = args.steps / t_total
-    p50 = float(np.percentile(lat, 50) * 1000.0)
-    p99 = float(np.percentile(lat, 99) * 1000.0)
-
-    dom_bytes = kernel_bytes(dominant, N, rows, n, n_entries, 2)
-    dom_avg_s = dom_ms * 1e-3 / max(dom_cnt, 1)
-    achieved = dom_bytes / dom_avg_s / 1e9
-    # whole-snapshot compulsory bytes (FP64, beam on, 6 source-find passes)
-    B = 48 * n + 2 * (2 * (N // 2 + 1) * N * 16) + 2 * N * N * 8 + (6 * N * N * 8 + 4 * N * N if do_sf else 0)
+    paths = {"all": ["fp64", "fp32c", "fp32tf"], "both": ["fp64", "fp32c"], "fp64": ["fp64"], "fp32c": ["fp32c"],
+             "fp32tf": ["fp32tf"]}[args.precision]
+    precs = {"fp64": capi.PRECISION_FP64, "fp32c": capi.PRECISION_FP32C, "fp32tf": capi.PRECISION_FP32_TWOFLOAT}
+    res = {}
+    for p in paths:
+        res[p] = measure(ctx, capi, args, pars, sfp, precs[p], n, N, bufs, do_sf)
+    head_name = "fp32c" if "fp32c" in res else paths[0]
+    head = res[head_name]
 
     result = {
-        "metric": "snapshots/s + p50/p99 latency, FP32-comp vs FP64 GPU at <=1e-4 peak err",
-        "value": round(snaps_per_s, 3),
+        "metric": METRIC,
+        "value": round(head["value"], 3),
         "unit": "snapshots/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "p50_ms": round(p50, 4),
-        "p99_ms": round(p99, 4),
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "p50_ms": round(head["p50_ms"], 4),
+        "p99_ms": round(head["p99_ms"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == capi.PRECISION_FP64 else "f32c",
+        "dtype": "f64" if head_name == "fp64" else "f32",
         "data": "synthetic (deterministic SplitMix64 point-source snapshot, inputs resident in HBM)",
         "config": {"workload": f"{args.workload}: N={N}, cell {wl['cell_arcsec']}\", {n} vis, "
                                f"{wl['n_sources']} sources, GaussianSinc s3 os9, beam, "
-                               + ("source find 5/3 sigma BINMEDIAN 8-conn" if do_sf else "imaging only"),
-                   "image_size": N, "n_vis": n, "precision": args.precision},
-        "roofline": {"bound": "hbm", "kernel": dominant, "avg_us": round(dom_avg_s * 1e6, 2),
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
-        "path_roofline": {"compulsory_bytes": B, "achieved_GBs": round(B / (ms_per_step * 1e-3) / 1e9, 1),
-                          "frac": round(B / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "kernels": kernels,
-        "islands": len(islands) if islands is not None else None,
+                               + ("source find 5/3 sigma BINMEDIAN 8-conn" if do_sf else "imaging only")
+                               + {"fp32c": ", FP32 taps with FP64 tile sums, FP32 grids and FFT",
+                                  "fp32tf": ", FP32 taps with two-float tile sums, FP32 grids and FFT",
+                                  "fp64": ", FP64"}[head_name],
+                   "image_size": N, "n_vis": n, "path": head_name},
+        "roofline": head["roofline"],
+        "path_roofline": head["path_roofline"],
+        "kernels": head["kernels"],
     }
+    for p, r in res.items():
+        if p != head_name:
+            result[p] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()
+                         if k in ("value", "ms_per_step", "p50_ms", "p99_ms", "roofline", "path_roofline", "kernels")}
+    for p in ("fp32c", "fp32tf"):
+        if "fp64" in res and p in res:
+            result[f"speedup_{p}_vs_fp64"] = round(res[p]["value"] / res["fp64"]["value"], 3)
+    result["islands"] = len(head["islands"]) if head["islands"] is not None else None
 
     if not args.no_cpu_baseline:
         from oracle import oracle
         nthr = oracle.threads()
         t0 = time.perf_counter()
         k = max(1, args.cpu_snapshots)
-        oimg = None
+        oimg = oisl = None
         for _ in range(k):
-            oimg, obeam, _, _ = oracle.image_visibilities(pars, snap["vis"], snap["weights"], snap["uvw"])
+            oimg, _, _, _ = oracle.image_visibilities(pars, snap["vis"], snap["weights"], snap["uvw"])
             if do_sf:
-                oracle.source_find(oimg, sfp)
+                _, oisl, _ = oracle.source_find(oimg, sfp)
         cpu_s = (time.perf_counter() - t0) / k
         result["cpu_baseline"] = {"value": round(1.0 / cpu_s, 4), "unit": "snapshots/s", "cores": nthr,
                                   "kind": "port",
                                   "sample": f"{k} full snapshot(s) of the same workload through the FP64 CPU oracle "
                                             f"(threaded gridder/FFT, own radix-2 FFT: FFTW absent)"}
-        err = float(np.max(np.abs(image.astype(np.float64) - oimg)) / np.max(np.abs(oimg)))
-        result["image_err_vs_oracle"] = err
+        peak = float(np.max(np.abs(oimg)))
+        acc = {}
+        for p, r in res.items():
+            a = {"image_err": float(np.max(np.abs(r["image"].astype(np.float64) - oimg)) / peak)}
+            if do_sf and r["islands"] is not None:
+                key = lambda i: (i["sign"], i["extremum_x_idx"], i["extremum_y_idx"])
+                same = [key(i) for i in r["islands"]] == [key(i) for i in oisl]
+                a["islands_match_oracle"] = same
+                a["n_islands"] = len(r["islands"])
+                if same and oisl:
+                    a["flux_err"] = max(abs(x["extremum_val"] - y["extremum_val"])
+                                        for x, y in zip(r["islands"], oisl)) / peak
+                    a["position_err_px"] = max(
+                        float(np.hypot(x["moments_fit"][1] - y["moments_fit"][1],
+                                       x["moments_fit"][2] - y["moments_fit"][2]))
+                        for x, y in zip(r["islands"], oisl))
+            acc[p] = a
+        result["accuracy_vs_oracle"] = acc
     result["gen_s"] = round(gen_s, 2)
     print(json.dumps(result), flush=True)
+    for b in bufs:
+        ctx.free(b)
     ctx.close()
 
 

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libstp_hip.so")
 # enums (src/stp/types.h:32-88)
 KERNEL_TOPHAT, KERNEL_TRIANGLE, KERNEL_SINC, KERNEL_GAUSSIAN, KERNEL_GAUSSIANSINC, KERNEL_PSWF = range(6)
 MEDIAN_ZEROMEDIAN, MEDIAN_BINMEDIAN, MEDIAN_BINAPPROX, MEDIAN_NTHELEMENT = range(4)
-PRECISION_FP64, PRECISION_FP32C = 0, 1
+PRECISION_FP64, PRECISION_FP32C, PRECISION_FP32_TWOFLOAT = 0, 1, 2
 
 STATUS_NAMES = {
     0: "STP_OK", 1: "STP_ERR_INVALID_ARGUMENT", 2: "STP_ERR_UNSUPPORTED", 3: "STP_ERR_DEVICE",

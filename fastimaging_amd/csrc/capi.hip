@@ -94,7 +94,7 @@ void check_kernel_spec(const stp_kernel_spec* k)
 template <typename T, typename OT>
 void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp, const stp_kernel_spec* kspec, size_t n,
     const double* d_vis, const double* d_w, const double* d_uvw, OT* d_image, OT* d_beam, double* d_total,
-    unsigned long long* d_ngood)
+    unsigned long long* d_ngood, bool twofloat)
 {
     const stp_kernel_spec kern = kspec ? *kspec : dispatch_kernel(p);
     check_kernel_spec(&kern);
@@ -113,6 +113,7 @@ void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp
     a.generate_beam = beam;
     a.uv_scale = ((p->cell_size / 3600.0) * (M_PI / 180.0)) * double(N); // imager.h:293
     a.kernel = kern;
+    a.twofloat = twofloat;
     const int rows = N / 2 + 1;
     const size_t ge = size_t(rows) * N;
     cplx<T>* grids = (cplx<T>*)ws_get(ctx, "imager_grids", sizeof(cplx<T>) * ge * (beam ? 2 : 1));
@@ -287,12 +288,13 @@ stp_status stp_image_visibilities_device(stp_ctx* ctx, const stp_imager_pars* p,
         double* d_total = (double*)ws_get(ctx, "imager_total", sizeof(double) * 2);
         unsigned long long* d_ngood = (unsigned long long*)(d_total + 1);
         LaunchTimer whole(ctx, "image_visibilities");
-        if (precision == STP_PRECISION_FP64) {
+        STP_REQUIRE(precision >= STP_PRECISION_FP64 && precision <= STP_PRECISION_FP32_TWOFLOAT, "unknown precision");
+        if (precision == STP_PRECISION_FP64)
             run_imager<double, double>(ctx, p, wp, kernel, n, d_vis, d_w, d_uvw, (double*)d_image, (double*)d_beam, d_total,
-                d_ngood);
-        } else {
-            throw Error(STP_ERR_UNSUPPORTED, "FP32 compensated path not available in this build yet");
-        }
+                d_ngood, false);
+        else
+            run_imager<float, float>(ctx, p, wp, kernel, n, d_vis, d_w, d_uvw, (float*)d_image, (float*)d_beam, d_total,
+                d_ngood, precision == STP_PRECISION_FP32_TWOFLOAT);
         if (synchronize || diag) {
             double tot = 0.0;
             unsigned long long ng = 0;
@@ -320,6 +322,7 @@ stp_status stp_image_visibilities(stp_ctx* ctx, const stp_imager_pars* p, const 
         STP_REQUIRE(!p->generate_beam || beam != nullptr, "beam buffer is NULL with generate_beam");
         STP_REQUIRE(n == 0 || (vis && w && uvw), "visibility buffers are NULL");
         const size_t isz = size_t(p->image_size) * p->image_size;
+        STP_REQUIRE(precision >= STP_PRECISION_FP64 && precision <= STP_PRECISION_FP32_TWOFLOAT, "unknown precision");
         const size_t osz = (precision == STP_PRECISION_FP64) ? sizeof(double) : sizeof(float);
         double* d_in = (double*)ws_get(ctx, "host_in", sizeof(double) * (6 * n + 1));
         char* d_out = (char*)ws_get(ctx, "host_out", osz * isz * 2);
@@ -341,6 +344,7 @@ stp_status stp_image_visibilities(stp_ctx* ctx, const stp_imager_pars* p, const 
                 HIP_CHECK(hipStreamSynchronize(ctx->stream));
             } else {
                 std::vector<float> tmp(isz);
+                HIP_CHECK(hipStreamSynchronize(ctx->stream));
                 HIP_CHECK(hipMemcpy(tmp.data(), d_out, sizeof(float) * isz, hipMemcpyDeviceToHost));
                 for (size_t i = 0; i < isz; i++)
                     image[i] = tmp[i];
@@ -461,6 +465,8 @@ stp_status stp_source_find_device(stp_ctx* ctx, const void* d_image, int32_t pre
 {
     return guarded(ctx, [&] {
         STP_REQUIRE(d_image && p && n_islands, "NULL argument");
+        STP_REQUIRE(prec >= STP_PRECISION_FP64 && prec <= STP_PRECISION_FP32_TWOFLOAT, "unknown precision");
+        STP_REQUIRE(cap == 0 || islands != nullptr, "island buffer is NULL with a non-zero capacity");
         source_find_device(ctx, d_image, prec, rows, cols, *p, d_label_map, islands, cap, n_islands, stats);
     });
 }

@@ -15,6 +15,7 @@ struct GridderArgs {
     bool generate_beam;
     double uv_scale;     // applied to u and v (imager: cell_rad * N; unit gridder: 1)
     stp_kernel_spec kernel; // the kernel_creator functor
+    bool twofloat = false;  // FP32 grids: two-float LDS sums instead of FP64 LDS sums
 };
 
 // Grids n visibilities into vis_grid (and sampling_grid), each rows x N complex

@@ -16,6 +16,7 @@
 // beam += kval (gridder.h:854-858); the oversampled kernel is the separable
 // product ky[row] * kx[col] of normalised 1-D kernels (gridder.h:84-102).
 #include <cmath>
+#include <type_traits>
 
 #include "gridder.h"
 #include "kernel_tables.h"
@@ -386,105 +387,215 @@ __global__ __launch_bounds__(BWG) void grid_bin_scatter_kernel(GridGeom g, const
     }
 }
 
-// Accumulates every entry binned to this tile.  One wave per entry, lane = kernel tap
-// (row fastest); LDS column stride TILE_R + ks keeps the taps of one entry on distinct
-// banks; entries are staged through LDS 256 at a time.
-constexpr int EBUF = GWG;
+// ---------------------------------------------------------------------------
+// Tile accumulation: one workgroup per TILE_R x TILE_C tile accumulates the taps
+// of every entry binned to it in LDS, then writes the tile once (no global
+// atomics, no memset).  Lane mapping for supports up to 31: a wave takes
+// 64 / ks entries at a time, lane (entry e, kernel column j) walks the ks kernel
+// rows, so one load of kx[j] serves ks taps and the ks LDS atomics of a lane are
+// independent.  Wider kernels take one entry per wave, lanes striding columns.
+//
+// FP64: ds_add_f64 per channel.  FP32 (compensated): each channel keeps an
+// unevaluated pair hi + lo.  The returning atomic add on hi yields the old value,
+// so the rounding error of the sum it stored is known exactly (Knuth's TwoSum on
+// old and x) and is itself added to lo.
+constexpr int TWG = 512;
+constexpr int TEB = 256;                  // entries staged per batch
+constexpr int KV_DOUBLES = 3584;          // exact mode: per-entry kernel vectors (28 KB)
 
-template <typename T>
-__global__ __launch_bounds__(GWG) void grid_tiles_kernel(GridGeom g, const Entry* entries, const uint32_t* offsets, int ostride,
-    uint32_t cap, const double* ktab, int ktab_lds, const double* uv, cplx<T>* vgrid, cplx<T>* bgrid)
-{
-    extern __shared__ double sm[];
-    __shared__ Entry ebuf[EBUF];
-    const int S = TILE_R + (g.ks <= 15 ? g.ks : 1);
-    double* acc_r = sm;
-    double* acc_i = sm + TILE_C * S;
-    double* acc_b = sm + 2 * TILE_C * S;
-    double* kt = sm + 3 * TILE_C * S;
-    const int cs = (g.os / 2) * 2 + 1;
-    const double* K1 = ktab;
-    if (ktab_lds) {
-        for (int k = threadIdx.x; k < cs * g.ks; k += GWG)
-            kt[k] = ktab[k];
-        K1 = kt;
+constexpr int RG = 4;                     // kernel rows per group of LDS atomics
+
+// FP64 accumulation (ds_add_f64) of FP64 or FP32 taps.
+struct AccF64 {
+    using S = double;
+    static constexpr int arrays = 3;      // re, im, beam
+    template <int R, typename T>
+    __device__ static void add_rows(double* base, int plane, const int* aa, const T* xr, const T* xi,
+        const T* xb, bool beam)
+    {
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+            if (aa[u] < 0)
+                continue;
+            unsafeAtomicAdd(&base[aa[u]], double(xr[u]));
+            unsafeAtomicAdd(&base[plane + aa[u]], double(xi[u]));
+            if (beam)
+                unsafeAtomicAdd(&base[2 * plane + aa[u]], double(xb[u]));
+        }
     }
+    __device__ static double value(const double* base, int plane, int ch, int a) { return base[ch * plane + a]; }
+};
+// Error of the rounded sum s = fl(old + x) (TwoSum).
+__device__ __forceinline__ float two_sum_err(float old, float x)
+{
+    const float sum = old + x;
+    const float bp = sum - old;
+    return (old - (sum - bp)) + (x - bp);
+}
+// Two-float accumulation of FP32 taps: hi planes (re, im, beam), then lo planes.
+struct AccTwoFloat {
+    using S = float;
+    static constexpr int arrays = 6;
+    template <int R, typename T>
+    __device__ static void add_rows(float* base, int plane, const int* aa, const T* xr, const T* xi,
+        const T* xb, bool beam)
+    {
+        float orr[R], oi[R], ob[R];
+        // returning adds on hi: each yields the old value, so the stored sum is fl(old + x)
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+            orr[u] = oi[u] = ob[u] = 0.f;
+            if (aa[u] < 0)
+                continue;
+            orr[u] = atomicAdd(&base[aa[u]], xr[u]);
+            oi[u] = atomicAdd(&base[plane + aa[u]], xi[u]);
+            if (beam)
+                ob[u] = atomicAdd(&base[2 * plane + aa[u]], xb[u]);
+        }
+        // the exact rounding errors go to the lo planes
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+            if (aa[u] < 0)
+                continue;
+            const float er = two_sum_err(orr[u], xr[u]), ei = two_sum_err(oi[u], xi[u]);
+            if (er != 0.f)
+                atomicAdd(&base[3 * plane + aa[u]], er);
+            if (ei != 0.f)
+                atomicAdd(&base[4 * plane + aa[u]], ei);
+            if (beam) {
+                const float eb = two_sum_err(ob[u], xb[u]);
+                if (eb != 0.f)
+                    atomicAdd(&base[5 * plane + aa[u]], eb);
+            }
+        }
+    }
+    __device__ static float value(const float* base, int plane, int ch, int a)
+    {
+        return base[ch * plane + a] + base[(3 + ch) * plane + a];
+    }
+};
+
+template <typename T, typename Acc>
+__global__ __launch_bounds__(TWG) void grid_tiles_kernel(GridGeom g, const Entry* entries, const uint32_t* offsets, int ostride,
+    uint32_t cap, const double* ktab, const double* uv, cplx<T>* vgrid, cplx<T>* bgrid)
+{
+#pragma clang fp contract(off)
+    using AS = typename Acc::S;
+    extern __shared__ double dyn[];
+    __shared__ Entry ebuf[TEB];
+    const int ks = g.ks, s = g.s;
+    const int S = TILE_R + (ks <= 15 ? ks : 1);     // column stride of the LDS tile
+    const int plane = TILE_C * S;
+    AS* acc = reinterpret_cast<AS*>(dyn);
+    const int acc_bytes = Acc::arrays * plane * int(sizeof(AS));
+    double* kv = dyn + (acc_bytes + 7) / 8;          // kernel table, or exact-mode vectors then sums
+    const int cs = (g.os / 2) * 2 + 1;
+    T* kt = reinterpret_cast<T*>(kv);
+    double* ksum = kv + KV_DOUBLES;
+    if (!g.exact)
+        for (int k = threadIdx.x; k < cs * ks; k += TWG)
+            kt[k] = T(ktab[k]);
+    for (int k = threadIdx.x; k < Acc::arrays * plane; k += TWG)
+        acc[k] = AS(0);
     const int t = blockIdx.x;
     const int rt = t % g.tiles_r, ct = t / g.tiles_r;
     const int r0 = rt * TILE_R, c0 = ct * TILE_C;
-    for (int k = threadIdx.x; k < TILE_C * S; k += GWG) {
-        acc_r[k] = 0.0;
-        acc_i[k] = 0.0;
-        acc_b[k] = 0.0;
-    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int epw = ks <= 64 ? 64 / ks : 1;          // entries per wave pass
+    const int slot = ks <= 64 ? lane / ks : 0;
+    const int jl = ks <= 64 ? lane - slot * ks : lane;
+    const bool lane_on = slot < epw;
     const uint32_t eb = min(offsets[size_t(t) * ostride], cap), ee = min(offsets[size_t(t + 1) * ostride], cap);
-    const int ntaps = g.ks * g.ks;
-    for (uint32_t e0 = eb; e0 < ee; e0 += EBUF) {
-        const int cnt = int(min(uint32_t(EBUF), ee - e0));
+    const int batch = g.exact ? min(TEB, KV_DOUBLES / (2 * ks)) : TEB;
+    for (uint32_t e0 = eb; e0 < ee; e0 += uint32_t(batch)) {
+        const int cnt = int(min(uint32_t(batch), ee - e0));
         __syncthreads();
-        if (int(threadIdx.x) < cnt)
-            ebuf[threadIdx.x] = entries[e0 + threadIdx.x];
-        __syncthreads();
-        for (int q = wave; q < cnt; q += GWG / 64) {
-            const Entry E = ebuf[q];
-            const int cpx = E.cp & 0xffff, cpy = E.cp >> 16;
-            double fx = 0.0, fy = 0.0, ksum = 1.0;
+        if (int(threadIdx.x) < cnt) {
+            const Entry E = entries[e0 + threadIdx.x];
+            ebuf[threadIdx.x] = E;
             if (g.exact) {
-                // exact kernel for this visibility (make_kernel_array, conv_func.h:279-303)
+                // exact kernel of this visibility pass (make_kernel_array, conv_func.h:279-303)
                 const long long vi = E.idx >> 1;
-                const double u = uv[vi] * g.scale, v = uv[g.n + vi] * g.scale;
-                double uu = u, vv = v;
-                if (g.halfplane && v < 0.0) {
-                    uu = -u;
-                    vv = -v;
+                double uu = uv[vi] * g.scale, vv = uv[g.n + vi] * g.scale;
+                if (g.halfplane && vv < 0.0) {
+                    uu = -uu;
+                    vv = -vv;
                 }
-                fx = uu - int(rint(uu));
-                fy = vv - int(rint(vv));
+                double fx = uu - int(rint(uu)), fy = vv - int(rint(vv));
                 if (E.idx & 1) {
                     fx = -fx;
                     fy = -fy;
                 }
+                double* ky = &kv[2 * ks * threadIdx.x];
+                double* kx = ky + ks;
                 double sx = 0.0, sy = 0.0;
-                for (int qq = 0; qq < g.ks; qq++) {
-                    sx += dev_kernel(g, double(qq - g.s) - fx);
-                    sy += dev_kernel(g, double(qq - g.s) - fy);
+                for (int q = 0; q < ks; q++) {
+                    ky[q] = dev_kernel(g, double(q - s) - fy);
+                    kx[q] = dev_kernel(g, double(q - s) - fx);
+                    sy += ky[q];
+                    sx += kx[q];
                 }
-                ksum = sx * sy;
+                ksum[threadIdx.x] = sx * sy;
             }
-            for (int tap = lane; tap < ntaps; tap += 64) {
-                const int i = tap % g.ks, j = tap / g.ks;
-                const int r = tap_row(g, E.gy, i);
-                if (r < 0)
-                    continue;
+        }
+        __syncthreads();
+        for (int q0 = wave * epw; q0 < cnt; q0 += (TWG / 64) * epw) {
+            const int q = q0 + slot;
+            if (!lane_on || q >= cnt)
+                continue;
+            const Entry& E = ebuf[q];
+            const int cpx = E.cp & 0xffff, cpy = E.cp >> 16;
+            const T w = T(E.w), vr = T(E.vr), vim = T(E.vi);
+            const double* kyd = g.exact ? &kv[2 * ks * q] : nullptr;
+            const double inv = g.exact ? ksum[q] : 1.0;
+            for (int j = jl; j < ks; j += (ks <= 64 ? ks : 64)) {
                 const int c = tap_col(g, E.gx, j);
-                const int lr = r - r0, lc = c - c0;
-                if (lr < 0 || lr >= TILE_R || lc < 0 || lc >= TILE_C)
+                const int lc = c - c0;
+                if (lc < 0 || lc >= TILE_C)
                     continue;
-                double K;
-                if (g.exact)
-                    K = (dev_kernel(g, double(i - g.s) - fy) * dev_kernel(g, double(j - g.s) - fx)) / ksum;
-                else
-                    K = K1[cpy * g.ks + i] * K1[cpx * g.ks + j];
-                const double kw = K * E.w;
-                const int a = lc * S + lr;
-                unsafeAtomicAdd(&acc_r[a], E.vr * kw);
-                unsafeAtomicAdd(&acc_i[a], E.vi * kw);
-                if (g.beam)
-                    unsafeAtomicAdd(&acc_b[a], kw);
+                const double kxd = g.exact ? kyd[ks + j] : 0.0;
+                const T kx = g.exact ? T(0) : kt[cpx * ks + j];
+                // RG kernel rows at a time: their atomics are issued back to back
+                for (int i0 = 0; i0 < ks; i0 += RG) {
+                    int aa[RG];
+                    T xr[RG], xi[RG], xb[RG];
+#pragma unroll
+                    for (int u = 0; u < RG; u++) {
+                        const int i = i0 + u;
+                        aa[u] = -1;
+                        xr[u] = xi[u] = xb[u] = T(0);
+                        if (i >= ks)
+                            continue;
+                        const int r = tap_row(g, E.gy, i);
+                        const int lr = r - r0;
+                        if (r < 0 || lr < 0 || lr >= TILE_R)
+                            continue;
+                        T K;
+                        if (g.exact)
+                            K = T((kyd[i] * kxd) / inv);
+                        else
+                            K = kt[cpy * ks + i] * kx;
+                        const T kw = K * w; // gridder.h:854-858
+                        aa[u] = lc * S + lr;
+                        xr[u] = vr * kw;
+                        xi[u] = vim * kw;
+                        xb[u] = kw;
+                    }
+                    Acc::template add_rows<RG, T>(acc, plane, aa, xr, xi, xb, g.beam != 0);
+                }
             }
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < TILE_C * TILE_R; k += GWG) {
+    for (int k = threadIdx.x; k < TILE_C * TILE_R; k += TWG) {
         const int lr = k % TILE_R, lc = k / TILE_R;
         const int r = r0 + lr, c = c0 + lc;
         if (r < g.rows && c < g.N) {
             const size_t o = size_t(r) + size_t(c) * g.rows;
             const int a = lc * S + lr;
-            vgrid[o] = { T(acc_r[a]), T(acc_i[a]) };
+            vgrid[o] = { T(Acc::value(acc, plane, 0, a)), T(Acc::value(acc, plane, 1, a)) };
             if (g.beam)
-                bgrid[o] = { T(acc_b[a]), T(0) };
+                bgrid[o] = { T(Acc::value(acc, plane, 2, a)), T(0) };
         }
     }
 }
@@ -587,12 +698,18 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         offsets = offs;
         ostride = 1;
     }
-    const int S = TILE_R + (g.ks <= 15 ? g.ks : 1);
-    const int ktab_lds = (!a.kernel_exact && cs * g.ks <= 1024) ? 1 : 0;
-    const size_t tile_bytes = sizeof(double) * (3 * size_t(TILE_C) * S + (ktab_lds ? size_t(cs) * g.ks : 0));
-    HIP_CHECK(hipFuncSetAttribute((const void*)grid_tiles_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, int(tile_bytes)));
-    launch(ctx, "grid_tiles", grid_tiles_kernel<T>, dim3(ntiles), dim3(GWG), tile_bytes, g, (const Entry*)entries, offsets,
-        ostride, cap, d_tab, ktab_lds, d_uv, d_vis_grid, d_sampling_grid);
+    STP_REQUIRE(a.kernel_exact || cs * g.ks <= KV_DOUBLES, "oversampled kernel table too large");
+    {
+        const bool twofloat = std::is_same<T, float>::value && a.twofloat;
+        const int S = TILE_R + (g.ks <= 15 ? g.ks : 1);
+        const size_t acc_bytes = twofloat ? 6 * size_t(TILE_C) * S * sizeof(float) : 3 * size_t(TILE_C) * S * sizeof(double);
+        const size_t kv_bytes = a.kernel_exact ? sizeof(double) * (KV_DOUBLES + TEB) : sizeof(double) * size_t(cs) * g.ks;
+        const size_t dyn_bytes = ((acc_bytes + 7) / 8) * 8 + kv_bytes;
+        auto kern = twofloat ? grid_tiles_kernel<T, AccTwoFloat> : grid_tiles_kernel<T, AccF64>;
+        HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(dyn_bytes)));
+        launch(ctx, "grid_tiles", kern, dim3(ntiles), dim3(TWG), dyn_bytes, g, (const Entry*)entries, offsets, ostride, cap,
+            d_tab, d_uv, d_vis_grid, d_sampling_grid);
+    }
     // sample_grid_total = 2 * sum(good weights) (gridder.h:415-423)
     canon_reduce_pairs(ctx, wpart, nblk, wscr, wres);
     launch(ctx, "grid_total", scale_total_kernel, dim3(1), dim3(1), 0, (const double*)wres, d_total);
@@ -600,5 +717,7 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
 
 template void grid_visibilities<double>(stp_ctx*, const GridderArgs&, size_t, const double*, const double*,
     const double*, cplx<double>*, cplx<double>*, double*, unsigned long long*);
+template void grid_visibilities<float>(stp_ctx*, const GridderArgs&, size_t, const double*, const double*,
+    const double*, cplx<float>*, cplx<float>*, double*, unsigned long long*);
 
 } // namespace stp_hip

@@ -65,11 +65,18 @@ typedef enum stp_median_method {
 } stp_median_method;
 
 /* Arithmetic of the gridder + FFT.  FP64 is the reference's USE_FLOAT=OFF build.
- * FP32C grids in FP32 with two-float compensated accumulation and FP64 index,
- * phase and normalisation math; the FFT then runs in FP32. */
+ * The FP32 forms keep the uv-to-pixel index and phase math, the kernel-sum
+ * normalisation and the weight total in FP64, compute the taps in FP32, store
+ * FP32 grids and images and run the FFT in FP32 (twiddles rounded from long
+ * double).  They differ in how each grid cell's taps are summed in the LDS tile:
+ *   FP32C           FP64 sums (ds_add_f64), rounded to FP32 once per cell;
+ *   FP32_TWOFLOAT   two-float hi + lo sums: TwoSum on the old value returned by
+ *                   the FP32 LDS atomic, the exact rounding error added to lo.
+ * Source finding on an FP32 image reads FP32 and accumulates in FP64. */
 typedef enum stp_precision {
     STP_PRECISION_FP64 = 0,
-    STP_PRECISION_FP32C = 1
+    STP_PRECISION_FP32C = 1,
+    STP_PRECISION_FP32_TWOFLOAT = 2
 } stp_precision;
 
 /* ------------------------------------------------------------------------- */

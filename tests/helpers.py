@@ -28,3 +28,46 @@ def noise(rows, cols, sigma=1.0, mean=0.0, seed=0):
 def peak_rel_err(a, b):
     """max |a - b| / max |b|."""
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+ISLAND_EXACT = ("label_idx", "sign", "extremum_y_idx", "extremum_x_idx", "num_samples", "extremum_val")
+
+
+def fits_agree(a, b, tol=1e-9, axes_tol=1e-6):
+    """Moments fit (sourcefind.cpp:691-717).  Amplitude and barycentre within tol.
+    The axes come from xx_bar - x_bar^2 (sourcefind.cpp:597-599), which cancels sums of
+    order (N/2)^2 * flux, so the per-label summation order (unspecified in the reference,
+    TBB) shows up there: squared axes are compared relative to semimajor^2 (a squared
+    semiminor that rounds negative gives NaN on either side and counts as 0), and the
+    angle only for clearly elongated islands, modulo pi."""
+    a, b = np.array(a, dtype=float), np.array(b, dtype=float)
+    if not np.allclose(a[:3], b[:3], rtol=tol, atol=tol):
+        return False
+    A2a, A2b = a[3] ** 2, b[3] ** 2
+    B2a, B2b = np.nan_to_num(a[4] ** 2), np.nan_to_num(b[4] ** 2)
+    if abs(A2a - A2b) > axes_tol * A2b or abs(B2a - B2b) > axes_tol * A2b:
+        return False
+    # line-shaped islands (semiminor ~ 0 or NaN): xy_bar is at rounding level and the
+    # angle branch (sourcefind.cpp:707-711) is decided by its sign, on either side
+    line = np.isnan(a[4]) or np.isnan(b[4]) or B2a <= 1e-6 * A2a or B2b <= 1e-6 * A2b
+    if not line and (A2b - B2b) > 1e-3 * A2b:
+        d = (a[5] - b[5] + np.pi / 2) % np.pi - np.pi / 2
+        if abs(d) > axes_tol:
+            return False
+    return True
+
+
+def compare(gpu_res, orc_res):
+    lm, isl, st = orc_res
+    assert np.array_equal(gpu_res.label_map, lm)
+    g = gpu_res.stats
+    for f in ("bg_level", "rms_est", "mean", "sigma"):
+        a, b = getattr(g, f), getattr(st, f)
+        assert a == b or (np.isnan(a) and np.isnan(b)), (f, a, b)  # bit-exact (NaN where the algebra gives NaN)
+    for f in ("num_labels_pos", "num_labels_neg", "num_valid_labels", "num_islands"):
+        assert getattr(g, f) == getattr(st, f), f
+    assert len(gpu_res.islands) == len(isl)
+    for a, b in zip(gpu_res.islands, isl):
+        for f in ISLAND_EXACT:
+            assert a[f] == b[f], f
+        assert fits_agree(a["moments_fit"], b["moments_fit"]), (a, b)
