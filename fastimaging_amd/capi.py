@@ -19,6 +19,7 @@ STATUS_NAMES = {
     0: "STP_OK", 1: "STP_ERR_INVALID_ARGUMENT", 2: "STP_ERR_UNSUPPORTED", 3: "STP_ERR_DEVICE",
     4: "STP_ERR_OUT_OF_MEMORY", 5: "STP_ERR_INTERNAL", 6: "STP_ERR_CAPACITY",
 }
+STATUS_CODES = {v: k for k, v in STATUS_NAMES.items()}
 
 
 class ImagerPars(ctypes.Structure):

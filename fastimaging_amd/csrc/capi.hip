@@ -101,8 +101,6 @@ void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp
     const int N = int(p->padded_image_size);
     const bool beam = p->generate_beam != 0;
     STP_REQUIRE(!beam || d_beam != nullptr, "beam buffer is NULL with generate_beam");
-    if (wp && wp->num_wplanes > 0)
-        throw Error(STP_ERR_UNSUPPORTED, "W-projection is not available in this build yet");
     GridderArgs a;
     a.image_size = N;
     a.support = int(p->kernel_support);
@@ -114,6 +112,10 @@ void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp
     a.uv_scale = ((p->cell_size / 3600.0) * (M_PI / 180.0)) * double(N); // imager.h:293
     a.kernel = kern;
     a.twofloat = twofloat;
+    if (wp && wp->num_wplanes > 0)
+        a.wproj = *wp;
+    a.cell_rad = (p->cell_size / 3600.0) * (M_PI / 180.0);
+    a.analytic_gcf = p->analytic_gcf != 0;
     const int rows = N / 2 + 1;
     const size_t ge = size_t(rows) * N;
     cplx<T>* grids = (cplx<T>*)ws_get(ctx, "imager_grids", sizeof(cplx<T>) * ge * (beam ? 2 : 1));

@@ -16,6 +16,10 @@ struct GridderArgs {
     double uv_scale;     // applied to u and v (imager: cell_rad * N; unit gridder: 1)
     stp_kernel_spec kernel; // the kernel_creator functor
     bool twofloat = false;  // FP32 grids: two-float LDS sums instead of FP64 LDS sums
+    // W-projection (enabled when wproj.num_wplanes > 0): the uv input carries a w column
+    stp_wproj_pars wproj{};
+    double cell_rad = 0.0;  // cell size in radians (W-kernel image-domain sampling)
+    bool analytic_gcf = false; // image-domain AA kernel of the W-kernels from gcf() (conv_func.h:351-412)
 };
 
 // Grids n visibilities into vis_grid (and sampling_grid), each rows x N complex

@@ -19,6 +19,7 @@
 #include <type_traits>
 
 #include "gridder.h"
+#include "fft.h"
 #include "kernel_tables.h"
 #include "reduce.h"
 
@@ -38,12 +39,16 @@ struct GridGeom {
     long long n;
     int kfunc;
     double kparam, kwg, kws;
+    // W-projection: visibility k of the pass is perm[k] (|w| order) and lies in plane[k]
+    int wproj = 0;
+    const uint32_t* perm = nullptr;
+    const uint16_t* plane = nullptr;
 };
 
 struct Entry {
     int gx, gy;   // kernel centre on the (shifted) grid for this pass
-    int cp;       // cp_x | cp_y << 16 (oversampled kernel indices)
-    int idx;      // visibility index << 1 | pass
+    int cp;       // cp_x | cp_y << 16 (oversampled kernel indices); W-projection: bit 31 = conjugate kernel
+    int idx;      // visibility index << 1 | pass; W-projection: the w-plane
     double vr, vi, w;
 };
 static_assert(sizeof(Entry) == 40, "entry layout");
@@ -53,17 +58,20 @@ struct Prep {
     int gx, gy, cpx, cpy;
     double fx, fy;
     double vr, vi, w;
+    double wl;   // w (wavelengths) after the half-plane conversion (W-projection)
 };
 
-__device__ __forceinline__ Prep prep_vis(const GridGeom& g, const double* uv, const double* vis, const double* wts, long long i)
+__device__ __forceinline__ Prep prep_vis(const GridGeom& g, const double* uv, const double* vis, const double* wts, long long k)
 {
     Prep p;
+    const long long i = g.perm ? (long long)g.perm[k] : k;
     double u = uv[i] * g.scale;
     double v = uv[g.n + i] * g.scale;
     p.vr = vis[2 * i];
     p.vi = vis[2 * i + 1];
     p.w = wts[i];
     p.dup = 0;
+    p.wl = g.wproj ? uv[2 * g.n + i] : 0.0;
     p.good = (fabs(u) < 1.0e9 && fabs(v) < 1.0e9) ? 1 : 0; // also rejects NaN
     if (!p.good)
         u = v = 0.0;
@@ -73,6 +81,7 @@ __device__ __forceinline__ Prep prep_vis(const GridGeom& g, const double* uv, co
             u = -u;
             v = -v;
             p.vi = -p.vi;
+            p.wl = -p.wl; // gridder.cpp:39-66
         } else {
             p.dup = (v < (g.s + 1)) ? 1 : 0;
         }
@@ -113,6 +122,31 @@ __device__ __forceinline__ void pass_geom(const GridGeom& g, const Prep& p, int 
             cpy = g.os - p.cpy;
         }
     }
+}
+
+// Entry of visibility position k, pass `pass`.  The conjugate pass conjugates the
+// visibility and negates w (gridder.h:794-805); a negative w selects the
+// conjugate W-kernel (gridder.h:826-843).
+__device__ __forceinline__ Entry make_entry(const GridGeom& g, const Prep& p, int pass, long long k)
+{
+    int gx, gy, cpx, cpy;
+    pass_geom(g, p, pass, gx, gy, cpx, cpy);
+    Entry e;
+    e.gx = gx;
+    e.gy = gy;
+    e.cp = cpx | (cpy << 16);
+    e.vr = p.vr;
+    e.vi = pass ? -p.vi : p.vi;
+    e.w = p.w;
+    if (g.wproj) {
+        const double wl = pass ? -p.wl : p.wl;
+        if (wl < 0.0)
+            e.cp |= int(0x80000000u);
+        e.idx = int(g.plane[k]);
+    } else {
+        e.idx = int((k << 1) | pass);
+    }
+    return e;
 }
 
 // Grid row of kernel row i, or -1 if dropped.  Oversampled path: rows < 0 skipped,
@@ -213,18 +247,9 @@ __global__ __launch_bounds__(GWG) void grid_scatter_kernel(GridGeom g, const dou
     if (!p.good)
         return;
     for (int pass = 0; pass <= p.dup; pass++) {
-        int gx, gy, cpx, cpy;
-        pass_geom(g, p, pass, gx, gy, cpx, cpy);
+        const Entry e = make_entry(g, p, pass, i);
         int rt[MAXT], ct[MAXT], nrt, nct;
-        footprint_tiles(g, gx, gy, rt, nrt, ct, nct);
-        Entry e;
-        e.gx = gx;
-        e.gy = gy;
-        e.cp = cpx | (cpy << 16);
-        e.idx = int((i << 1) | pass);
-        e.vr = p.vr;
-        e.vi = pass ? -p.vi : p.vi;
-        e.w = p.w;
+        footprint_tiles(g, e.gx, e.gy, rt, nrt, ct, nct);
         for (int a = 0; a < nct; a++)
             for (int b = 0; b < nrt; b++) {
                 const uint32_t slot = atomicAdd(&cursor[ct[a] * g.tiles_r + rt[b]], 1u);
@@ -365,18 +390,9 @@ __global__ __launch_bounds__(BWG) void grid_bin_scatter_kernel(GridGeom g, const
         if (!p.good)
             continue;
         for (int pass = 0; pass <= p.dup; pass++) {
-            int gx, gy, cpx, cpy;
-            pass_geom(g, p, pass, gx, gy, cpx, cpy);
+            const Entry e = make_entry(g, p, pass, i);
             int rt[MAXT], ct[MAXT], nrt, nct;
-            footprint_tiles(g, gx, gy, rt, nrt, ct, nct);
-            Entry e;
-            e.gx = gx;
-            e.gy = gy;
-            e.cp = cpx | (cpy << 16);
-            e.idx = int((i << 1) | pass);
-            e.vr = p.vr;
-            e.vi = pass ? -p.vi : p.vi;
-            e.w = p.w;
+            footprint_tiles(g, e.gx, e.gy, rt, nrt, ct, nct);
             for (int a = 0; a < nct; a++)
                 for (int b = 0; b < nrt; b++) {
                     const uint32_t slot = atomicAdd(&cur[ct[a] * g.tiles_r + rt[b]], 1u);
@@ -600,6 +616,337 @@ __global__ __launch_bounds__(TWG) void grid_tiles_kernel(GridGeom g, const Entry
     }
 }
 
+// ---------------------------------------------------------------------------
+// W-projection (gridder.h:311-367,467-486,767-842; aw_projection.cpp:19-210,366-474)
+//
+//  1. |w| order: stable LSD radix sort of the 64-bit keys bits(|w|) with the
+//     visibility index as payload (the reference's arma::sort_index, gridder.h:324,
+//     with ties kept in index order).
+//  2. planes: good flags in |w| order, their prefix sum, then the equal-count plane
+//     boundaries and plane |w| (mean or median) of average_w_planes
+//     (gridder.cpp:68-146) on the device.
+//  3. W-kernels per plane: the image-domain kernel aa(x) aa(y) exp(-2 pi i w (n-1)) / n
+//     on an (workarea * os)^2 array, a forward c2c FFT, and the (os+1)^2 oversampled
+//     (2s+1)^2 kernels cut from it, each normalised by its real sum.
+//  4. gridding: the entries of every plane binned by tile together; the tile
+//     kernel reads each entry's complex kernel from its plane's cache, so the grid
+//     is written once for all planes.
+constexpr int RS_KEYS = 1024;          // keys per radix-sort block
+
+__global__ void wsort_keys_kernel(const double* uvw, long long n, unsigned long long* key, uint32_t* idx)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    key[i] = (unsigned long long)__double_as_longlong(fabs(uvw[2 * n + i]));
+    idx[i] = uint32_t(i);
+}
+
+__global__ __launch_bounds__(256) void rs_hist_kernel(const unsigned long long* key, long long n, int shift, uint32_t* counts,
+    int nblk)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * RS_KEYS;
+    for (int t = threadIdx.x; t < RS_KEYS; t += 256) {
+        const long long i = base + t;
+        if (i < n)
+            atomicAdd(&h[(key[i] >> shift) & 0xff], 1u);
+    }
+    __syncthreads();
+    counts[size_t(threadIdx.x) * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// One wave per block of RS_KEYS keys, in order: lanes with equal digits find each
+// other by ballots on the 8 digit bits, rank among themselves by lane, and take
+// consecutive slots after the digit's running cursor, so the scatter is stable.
+__global__ __launch_bounds__(64) void rs_scatter_kernel(const unsigned long long* kin, const uint32_t* iin, long long n, int shift,
+    const uint32_t* offs, int nblk, unsigned long long* kout, uint32_t* iout)
+{
+    __shared__ uint32_t run[256];
+    const int lane = threadIdx.x;
+    for (int d = lane; d < 256; d += 64)
+        run[d] = offs[size_t(d) * nblk + blockIdx.x];
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * RS_KEYS;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int c = 0; c < RS_KEYS; c += 64) {
+        const long long i = base + c + lane;
+        const bool valid = i < n;
+        const unsigned long long k = valid ? kin[i] : 0ull;
+        const uint32_t d = uint32_t(k >> shift) & 0xffu;
+        unsigned long long peers = __ballot(valid);
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long m = __ballot(valid && bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = uint32_t(__popcll(peers & below));
+        const uint32_t cnt = uint32_t(__popcll(peers));
+        uint32_t slot = 0;
+        if (valid)
+            slot = run[d];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0)
+            run[d] = slot + cnt;
+        __builtin_amdgcn_wave_barrier();
+        if (valid) {
+            kout[slot + rank] = k;
+            iout[slot + rank] = iin[i];
+        }
+    }
+}
+
+__global__ void wgood_kernel(GridGeom g, const double* uvw, const double* vis, const double* wts, uint32_t* flag)
+{
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= g.n)
+        return;
+    flag[k] = prep_vis(g, uvw, vis, wts, k).good ? 1u : 0u;
+}
+
+// Smallest k in [lo, hi] with G[k] >= target (G non-decreasing), or hi.
+__device__ unsigned long long lower_bound_u32(const uint32_t* G, unsigned long long lo, unsigned long long hi,
+    unsigned long long target)
+{
+    while (lo < hi) {
+        const unsigned long long mid = lo + (hi - lo) / 2;
+        if (G[mid] >= target)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+// average_w_planes (gridder.cpp:68-146) over the |w|-sorted good flags: G is their
+// exclusive prefix sum (G[n] = number of good visibilities).  Writes first[0..P]
+// (first[P] = n), the good count of each plane, and for the median variant the
+// plane |w|.
+__global__ void wplane_bounds_kernel(const uint32_t* G, long long n, int P, int median, const unsigned long long* skey,
+    uint32_t* first, uint32_t* count, double* w_plane)
+{
+    const unsigned long long num_gvis = G[n];
+    unsigned long long begin = 0;
+    auto wv = [&](unsigned long long k) { return __longlong_as_double((long long)skey[k]); };
+    for (int p = 0; p < P; p++) {
+        const unsigned long long plane_size = (unsigned long long)ceil(double(num_gvis - begin) / double(P - p));
+        unsigned long long k = begin, counter = 0;
+        if (plane_size > 0 && begin < (unsigned long long)n) {
+            const unsigned long long target = (unsigned long long)G[begin] + plane_size;
+            k = lower_bound_u32(G, begin + 1, (unsigned long long)n, target);
+            counter = (unsigned long long)G[k] - G[begin];
+        }
+        first[p] = uint32_t(begin);
+        count[p] = uint32_t(counter);
+        if (median) {
+            unsigned long long middle = plane_size / 2;
+            if ((begin + plane_size) > num_gvis)
+                middle = (num_gvis - begin) / 2;
+            double v = 0.0;
+            if (counter > 0) {
+                // the good element with counter == middle: the first position whose
+                // inclusive good count reaches middle + 1
+                const unsigned long long mid = lower_bound_u32(G, begin + 1, k, (unsigned long long)G[begin] + middle + 1) - 1;
+                v = (plane_size % 2 == 0 && mid > 0) ? (wv(mid - 1) + wv(mid)) / 2.0 : wv(mid);
+            }
+            w_plane[p] = v;
+        }
+        begin = k;
+    }
+    first[P] = uint32_t(n);
+}
+
+// Mean |w| of each plane's good visibilities (one block per plane).
+__global__ __launch_bounds__(256) void wplane_mean_kernel(const unsigned long long* skey, const uint32_t* flag,
+    const uint32_t* first, const uint32_t* count, double* w_plane)
+{
+    __shared__ double red[256];
+    const int p = blockIdx.x;
+    double sum = 0.0;
+    for (unsigned long long k = first[p] + threadIdx.x; k < first[p + 1]; k += 256)
+        if (flag[k])
+            sum += __longlong_as_double((long long)skey[k]);
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        w_plane[p] = count[p] > 0 ? red[0] / double(count[p]) : 0.0;
+}
+
+__global__ void wplane_ids_kernel(const uint32_t* first, int P, long long n, uint16_t* plane)
+{
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n)
+        return;
+    int lo = 0, hi = P - 1; // last plane p with first[p] <= k
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if ((long long)first[mid] <= k)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    plane[k] = uint16_t(lo);
+}
+
+// generate_image_domain_convolution_kernel (aw_projection.cpp:50-111): the
+// array x array image-domain kernel, zero outside the +-workarea/2 corners.
+__global__ void wk_fill_kernel(cplx<double>* comb, int arr, int half, const double* aa, const double* w_plane, int p,
+    double scaled_cell)
+{
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long long)arr * arr)
+        return;
+    const int i = int(e % arr), j = int(e / arr);
+    // distance of the sample from the origin, folded: i and arr - i map to px = i
+    const int pi = (i < half) ? i : ((i > arr - half) ? arr - i : -1);
+    const int pj = (j < half) ? j : ((j > arr - half) ? arr - j : -1);
+    cplx<double> v{ 0.0, 0.0 };
+    if (pi >= 0 && pj >= 0) {
+        // combine_2d_kernel_value (aw_projection.cpp:455-474)
+        const double a = aa[half + pi] * aa[half + pj];
+        if (pi == 0 && pj == 0) {
+            v.x = a;
+        } else {
+            const double dx = double(pi) * scaled_cell, dy = double(pj) * scaled_cell;
+            const double r2 = dx * dx + dy * dy;
+            if (r2 >= 1.0) {
+                v.x = a;
+            } else {
+                const double nn = sqrt(1.0 - r2);
+                const double ph = (-2.0 * M_PI) * w_plane[p] * (nn - 1.0);
+                double sn, cs;
+                sincos(ph, &sn, &cs);
+                v.x = ((cs / nn) * aa[half + pi]) * aa[half + pj];
+                v.y = ((sn / nn) * aa[half + pi]) * aa[half + pj];
+            }
+        }
+    }
+    comb[e] = v;
+}
+
+// generate_kernel_cache, non-Hankel branch (aw_projection.cpp:415-443): kernel
+// (cp_x = x, cp_y = y), element (row i, column j) = conv[cxi(x, j) + cyi(y, i) * arr],
+// normalised by the sum of its real parts.  One block per (x, y).
+template <typename KT>
+__global__ __launch_bounds__(256) void wk_extract_kernel(const cplx<double>* conv, int arr, int os, int support,
+    cplx<KT>* cache)
+{
+    __shared__ double red[256];
+    const int cs = (os / 2) * 2 + 1;
+    const int x = blockIdx.x % cs, y = blockIdx.x / cs;
+    const int ks = 2 * support + 1;
+    const long long ctr = arr / 2, op = os / 2;
+    const long long oc = ctr - (long long)support * os + op;
+    double sum = 0.0;
+    for (int e = threadIdx.x; e < ks * ks; e += 256) {
+        const int i = e % ks, j = e / ks;
+        const long long cxi = (oc - x + (long long)j * os + ctr) % arr;
+        const long long cyi = (oc - y + (long long)i * os + ctr) % arr;
+        sum += conv[cxi + cyi * arr].x;
+    }
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    const double inv = 1.0 / red[0];
+    cplx<KT>* out = cache + size_t(blockIdx.x) * ks * ks;
+    for (int e = threadIdx.x; e < ks * ks; e += 256) {
+        const int i = e % ks, j = e / ks;
+        const long long cxi = (oc - x + (long long)j * os + ctr) % arr;
+        const long long cyi = (oc - y + (long long)i * os + ctr) % arr;
+        const cplx<double> v = conv[cxi + cyi * arr];
+        out[e] = { KT(v.x * inv), KT(v.y * inv) };
+    }
+}
+
+// W-projection tile accumulation: one wave per entry, lane = tile row, the lane
+// walks the tile's columns.  Per tap (gridder.h:826-861): kval = K * w with K the
+// plane's complex kernel (conjugated for w < 0), grid += vis * kval, beam += kval.
+// LDS: FP64 sums of grid (re, im) and beam (re, im).
+constexpr int WTWG = 512;
+constexpr int WEB = 256;
+
+template <typename T, typename KT>
+__global__ __launch_bounds__(WTWG) void grid_wtiles_kernel(GridGeom g, const Entry* entries, const uint32_t* offsets,
+    int ostride, uint32_t cap, const cplx<KT>* cache, cplx<T>* vgrid, cplx<T>* bgrid)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double acc[];
+    __shared__ Entry ebuf[WEB];
+    const int ks = g.ks, s = g.s, N = g.N;
+    const int S = TILE_R + 1;
+    const int plane = TILE_C * S;
+    const int cs = (g.os / 2) * 2 + 1;
+    const size_t kk = size_t(ks) * ks;
+    for (int k = threadIdx.x; k < 4 * plane; k += WTWG)
+        acc[k] = 0.0;
+    const int t = blockIdx.x;
+    const int rt = t % g.tiles_r, ct = t / g.tiles_r;
+    const int r0 = rt * TILE_R, c0 = ct * TILE_C;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = r0 + lane;
+    const uint32_t eb = min(offsets[size_t(t) * ostride], cap), ee = min(offsets[size_t(t + 1) * ostride], cap);
+    for (uint32_t e0 = eb; e0 < ee; e0 += WEB) {
+        const int cnt = int(min(uint32_t(WEB), ee - e0));
+        __syncthreads();
+        if (int(threadIdx.x) < cnt)
+            ebuf[threadIdx.x] = entries[e0 + threadIdx.x];
+        __syncthreads();
+        for (int q = wave; q < cnt; q += WTWG / 64) {
+            const Entry& E = ebuf[q];
+            // kernel row of my grid row: direct, or wrapped from >= N (rows < 0 are skipped)
+            int i = r - (E.gy - s);
+            if (i < 0 || i >= ks)
+                i += N;
+            if (i < 0 || i >= ks || r >= g.rows)
+                continue;
+            const bool conj = E.cp < 0;
+            const int cpx = E.cp & 0x7ff, cpy = (E.cp >> 16) & 0x7ff;
+            const cplx<KT>* K = cache + (size_t(E.idx) * cs * cs + size_t(cpy) * cs + cpx) * kk;
+            const double w = E.w, vr = E.vr, vi = E.vi;
+            int j = c0 - (E.gx - s); // kernel column of the tile's first column, mod N
+            j = j < 0 ? j + N : (j >= N ? j - N : j);
+            const int a0 = lane;
+            for (int lc = 0; lc < TILE_C; lc++, j = (j + 1 == N) ? 0 : j + 1) {
+                if (j >= ks || c0 + lc >= N)
+                    continue;
+                const cplx<KT> kv = K[size_t(j) * ks + i];
+                const double kr = double(kv.x) * w;
+                const double ki = (conj ? -double(kv.y) : double(kv.y)) * w;
+                const int a = lc * S + a0;
+                unsafeAtomicAdd(&acc[a], vr * kr - vi * ki);
+                unsafeAtomicAdd(&acc[plane + a], vr * ki + vi * kr);
+                if (g.beam) {
+                    unsafeAtomicAdd(&acc[2 * plane + a], kr);
+                    unsafeAtomicAdd(&acc[3 * plane + a], ki);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < TILE_C * TILE_R; k += WTWG) {
+        const int lr = k % TILE_R, lc = k / TILE_R;
+        const int rr = r0 + lr, c = c0 + lc;
+        if (rr < g.rows && c < N) {
+            const size_t o = size_t(rr) + size_t(c) * g.rows;
+            const int a = lc * S + lr;
+            vgrid[o] = { T(acc[a]), T(acc[plane + a]) };
+            if (g.beam)
+                bgrid[o] = { T(acc[2 * plane + a]), T(acc[3 * plane + a]) };
+        }
+    }
+}
+
 } // namespace
 
 template <typename T>
@@ -607,10 +954,19 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     const double* d_w, cplx<T>* d_vis_grid, cplx<T>* d_sampling_grid, double* d_total, unsigned long long* d_ngood)
 {
     const int N = a.image_size;
+    const bool wproj = a.wproj.num_wplanes > 0;
+    // W-projection grids with max_wpconv_support (gridder.h:260-268)
+    const int support = wproj ? int(a.wproj.max_wpconv_support) : a.support;
     STP_REQUIRE(N >= 4 && (N % 2) == 0, "image_size must be even and >= 4");
-    STP_REQUIRE(a.support >= 1 && a.support <= 64, "kernel support must be in [1, 64]");
-    STP_REQUIRE(2 * a.support + 1 <= N, "kernel wider than the grid");
+    STP_REQUIRE(support >= 1 && support <= 64, "kernel support must be in [1, 64]");
+    STP_REQUIRE(2 * support + 1 <= N, "kernel wider than the grid");
     STP_REQUIRE(n < (size_t(1) << 30), "at most 2^30 visibilities per call");
+    if (wproj) {
+        STP_REQUIRE(!a.kernel_exact, "W-projection requires kernel_exact == false");
+        STP_REQUIRE(a.wproj.hankel_opt == 0, "Hankel W-kernels are not implemented");
+        STP_REQUIRE(a.wproj.num_wplanes <= 65535, "at most 65535 w-planes");
+        STP_REQUIRE(n < (size_t(1) << 31), "too many visibilities for W-projection");
+    }
     int os = a.oversampling;
     if (a.kernel_exact) {
         os = 1;
@@ -624,8 +980,8 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     g.N = N;
     g.half = N / 2;
     g.rows = a.halfplane ? g.half + 1 : N;
-    g.s = a.support;
-    g.ks = 2 * a.support + 1;
+    g.s = support;
+    g.ks = 2 * support + 1;
     g.os = os;
     g.halfplane = a.halfplane;
     g.shift = a.shift_uv;
@@ -647,7 +1003,9 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     // kernel table (oversampled 1-D kernels), computed on the host once per configuration
     const double* d_tab = nullptr;
     const int cs = (os / 2) * 2 + 1;
-    if (!a.kernel_exact) {
+    if (wproj) {
+        d_tab = dev_table(ctx, "k1d:none", std::vector<double>(1, 0.0));
+    } else if (!a.kernel_exact) {
         char key[160];
         snprintf(key, sizeof(key), "k1d:%d:%.17g:%.17g:%.17g:%d:%d", kf.kind, kf.param, kf.width_g, kf.width_s, a.support, os);
         d_tab = has_table(ctx, key) ? dev_table(ctx, key, {}) : dev_table(ctx, key, kernel_1d_table(kf, a.support, os));
@@ -659,14 +1017,88 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     double* wpart = (double*)ws_get(ctx, "grid_wpart", sizeof(double) * (2 * nblk + 2));
     double* wscr = (double*)ws_get(ctx, "grid_wscr", sizeof(double) * canon_scratch_doubles(nblk));
     double* wres = (double*)ws_get(ctx, "grid_wres", sizeof(double) * 2);
-    // worst-case entries: every footprint spans at most (ceil(ks/T)+2) tiles per axis, two passes
-    const size_t per_pass = size_t(std::min(MAXT, int(div_up(g.ks, TILE_R)) + 2)) * size_t(std::min(MAXT, int(div_up(g.ks, TILE_C)) + 2));
+    // worst-case entries: a footprint of ks cells spans at most ceil((ks-1)/T)+1 tiles per
+    // axis, one more where it wraps across the grid edge; two passes
+    const size_t per_pass = size_t(std::min(MAXT, int(div_up(g.ks - 1, TILE_R)) + 2)) *
+        size_t(std::min(MAXT, int(div_up(g.ks - 1, TILE_C)) + 2));
     const size_t cap64 = std::max<size_t>(1, 2 * per_pass * n);
     STP_REQUIRE(cap64 < (size_t(1) << 32), "too many gridding entries for one call");
     const uint32_t cap = uint32_t(cap64);
     Entry* entries = (Entry*)ws_get(ctx, "grid_entries", sizeof(Entry) * cap64);
     if (d_ngood)
         launch(ctx, "fill", zero_u64_kernel, dim3(1), dim3(1), 0, d_ngood);
+
+    using KT = T; // W-kernel cache precision follows the grid
+    cplx<KT>* wcache = nullptr;
+    if (wproj) {
+        const int P = int(a.wproj.num_wplanes);
+        // 1. |w| order
+        unsigned long long* k0 = (unsigned long long*)ws_get(ctx, "w_key0", sizeof(unsigned long long) * n);
+        unsigned long long* k1 = (unsigned long long*)ws_get(ctx, "w_key1", sizeof(unsigned long long) * n);
+        uint32_t* i0 = (uint32_t*)ws_get(ctx, "w_idx0", sizeof(uint32_t) * n);
+        uint32_t* i1 = (uint32_t*)ws_get(ctx, "w_idx1", sizeof(uint32_t) * n);
+        const size_t nrb = std::max<size_t>(1, div_up(n, RS_KEYS));
+        uint32_t* rcounts = (uint32_t*)ws_get(ctx, "w_rcounts", sizeof(uint32_t) * 256 * nrb);
+        uint32_t* roffs = (uint32_t*)ws_get(ctx, "w_roffs", sizeof(uint32_t) * (256 * nrb + 1));
+        launch(ctx, "w_keys", wsort_keys_kernel, dim3(unsigned(div_up(n, 256))), dim3(256), 0, d_uv, (long long)n, k0, i0);
+        for (int shift = 0; shift < 64; shift += 8) {
+            launch(ctx, "w_sort_hist", rs_hist_kernel, dim3(unsigned(nrb)), dim3(256), 0, (const unsigned long long*)k0,
+                (long long)n, shift, rcounts, int(nrb));
+            scan_u32(ctx, rcounts, 256 * nrb, roffs);
+            launch(ctx, "w_sort_scatter", rs_scatter_kernel, dim3(unsigned(nrb)), dim3(64), 0, (const unsigned long long*)k0,
+                (const uint32_t*)i0, (long long)n, shift, (const uint32_t*)roffs, int(nrb), k1, i1);
+            std::swap(k0, k1);
+            std::swap(i0, i1);
+        }
+        g.perm = i0;
+        // 2. planes
+        uint32_t* flag = (uint32_t*)ws_get(ctx, "w_flag", sizeof(uint32_t) * n);
+        uint32_t* G = (uint32_t*)ws_get(ctx, "w_gscan", sizeof(uint32_t) * (n + 1));
+        uint32_t* first = (uint32_t*)ws_get(ctx, "w_first", sizeof(uint32_t) * (P + 1));
+        uint32_t* pcount = (uint32_t*)ws_get(ctx, "w_pcount", sizeof(uint32_t) * P);
+        double* w_plane = (double*)ws_get(ctx, "w_plane", sizeof(double) * P);
+        uint16_t* plane_of = (uint16_t*)ws_get(ctx, "w_planeof", sizeof(uint16_t) * n);
+        launch(ctx, "w_good", wgood_kernel, dim3(unsigned(div_up(n, 256))), dim3(256), 0, g, d_uv, d_vis, d_w, flag);
+        scan_u32(ctx, flag, n, G);
+        launch(ctx, "w_planes", wplane_bounds_kernel, dim3(1), dim3(1), 0, (const uint32_t*)G, (long long)n, P,
+            int(a.wproj.wplanes_median != 0), (const unsigned long long*)k0, first, pcount, w_plane);
+        if (!a.wproj.wplanes_median)
+            launch(ctx, "w_plane_mean", wplane_mean_kernel, dim3(unsigned(P)), dim3(256), 0, (const unsigned long long*)k0,
+                (const uint32_t*)flag, (const uint32_t*)first, (const uint32_t*)pcount, w_plane);
+        launch(ctx, "w_plane_ids", wplane_ids_kernel, dim3(unsigned(div_up(n, 256))), dim3(256), 0, (const uint32_t*)first, P,
+            (long long)n, plane_of);
+        g.plane = plane_of;
+        g.wproj = 1;
+        // 3. W-kernels: workarea (gridder.h:467-486), image-domain AA kernel, one FFT per plane
+        const int us = a.wproj.undersampling_opt > 0 ? (1 << (a.wproj.undersampling_opt - 1)) : 0;
+        int workarea = 2;
+        if (us > 0)
+            while (workarea / us < g.ks)
+                workarea *= 2;
+        else
+            while (workarea < N)
+                workarea *= 2;
+        const int arr = workarea * os;
+        STP_REQUIRE(arr <= 4096, "W-kernel work array (workarea * oversampling) must be <= 4096");
+        STP_REQUIRE(arr >= 2 * (g.ks * os), "W-kernel work array too small for the support");
+        const double scaling = double(N) / double(workarea);
+        char key[192];
+        snprintf(key, sizeof(key), "waa:%d:%.17g:%.17g:%.17g:%d:%d", kf.kind, kf.param, kf.width_g, kf.width_s, workarea,
+            a.analytic_gcf ? 1 : 0);
+        const double* d_aa = has_table(ctx, key) ? dev_table(ctx, key, {})
+                                                 : dev_table(ctx, key, img_dom_kernel(kf, size_t(workarea), false, a.analytic_gcf));
+        cplx<double>* comb = (cplx<double>*)ws_get(ctx, "w_comb", sizeof(cplx<double>) * size_t(arr) * arr);
+        cplx<double>* conv = (cplx<double>*)ws_get(ctx, "w_conv", sizeof(cplx<double>) * size_t(arr) * arr);
+        const size_t per_plane = size_t(cs) * cs * size_t(g.ks) * g.ks;
+        wcache = (cplx<KT>*)ws_get(ctx, "w_cache", sizeof(cplx<KT>) * per_plane * P);
+        for (int p = 0; p < P; p++) {
+            launch(ctx, "w_kernel_fill", wk_fill_kernel, dim3(unsigned(div_up(size_t(arr) * arr, 256))), dim3(256), 0, comb, arr,
+                workarea / 2, d_aa, (const double*)w_plane, p, a.cell_rad * scaling);
+            fft_c2c_2d<double>(ctx, comb, conv, arr, true);
+            launch(ctx, "w_kernel_extract", wk_extract_kernel<KT>, dim3(unsigned(cs * cs)), dim3(256), 0,
+                (const cplx<double>*)conv, arr, os, g.s, wcache + size_t(p) * per_plane);
+        }
+    }
 
     const uint32_t* offsets;
     int ostride;
@@ -698,8 +1130,14 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         offsets = offs;
         ostride = 1;
     }
-    STP_REQUIRE(a.kernel_exact || cs * g.ks <= KV_DOUBLES, "oversampled kernel table too large");
-    {
+    if (wproj) {
+        const size_t dyn_bytes = sizeof(double) * 4 * size_t(TILE_C) * (TILE_R + 1);
+        HIP_CHECK(hipFuncSetAttribute((const void*)grid_wtiles_kernel<T, KT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+            int(dyn_bytes)));
+        launch(ctx, "grid_wtiles", grid_wtiles_kernel<T, KT>, dim3(ntiles), dim3(WTWG), dyn_bytes, g, (const Entry*)entries,
+            offsets, ostride, cap, (const cplx<KT>*)wcache, d_vis_grid, d_sampling_grid);
+    } else {
+        STP_REQUIRE(a.kernel_exact || cs * g.ks <= KV_DOUBLES, "oversampled kernel table too large");
         const bool twofloat = std::is_same<T, float>::value && a.twofloat;
         const int S = TILE_R + (g.ks <= 15 ? g.ks : 1);
         const size_t acc_bytes = twofloat ? 6 * size_t(TILE_C) * S * sizeof(float) : 3 * size_t(TILE_C) * S * sizeof(double);
