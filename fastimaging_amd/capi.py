@@ -137,6 +137,46 @@ class ImagerDiag(ctypes.Structure):
     ]
 
 
+class ServerCfg(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("num_slots", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64)]
+
+
+class Job(ctypes.Structure):
+    """stp_job: one snapshot imaging (+ source finding) request."""
+    _fields_ = [
+        ("img_pars", ImagerPars),
+        ("w_proj", WProjPars),
+        ("use_kernel_spec", ctypes.c_int32),
+        ("kernel", KernelSpec),
+        ("precision", ctypes.c_int32),
+        ("n_vis", ctypes.c_size_t),
+        ("vis", ctypes.c_void_p),
+        ("vis_weights", ctypes.c_void_p),
+        ("uvw_lambda", ctypes.c_void_p),
+        ("image", ctypes.c_void_p),
+        ("beam", ctypes.c_void_p),
+        ("source_find", ctypes.c_int32),
+        ("sf_pars", SourceFindPars),
+        ("label_map", ctypes.c_void_p),
+        ("islands", ctypes.c_void_p),
+        ("island_cap", ctypes.c_size_t),
+    ]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("slot", ctypes.c_int32),
+        ("n_islands", ctypes.c_size_t),
+        ("stats", SourceFindStats),
+        ("diag", ImagerDiag),
+        ("t_submit", ctypes.c_double),
+        ("t_start", ctypes.c_double),
+        ("t_end", ctypes.c_double),
+        ("error", ctypes.c_char * 256),
+    ]
+
+
 def imager_pars(image_size, cell_size, kernel_function=KERNEL_GAUSSIANSINC, kernel_support=3,
                 kernel_exact=False, oversampling=9, generate_beam=True, gridding_correction=False,
                 analytic_gcf=False, padding_factor=1.0):
@@ -254,6 +294,15 @@ _SIG = {
     "stp_device_free": (ctypes.c_int, [_P, _P]),
     "stp_memcpy_htod": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "stp_memcpy_dtoh": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "stp_server_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(ServerCfg)]),
+    "stp_server_destroy": (ctypes.c_int, [_P]),
+    "stp_server_submit": (ctypes.c_int, [_P, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_uint64)]),
+    "stp_server_poll": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32)]),
+    "stp_server_wait": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(JobResult)]),
+    "stp_job_device_bytes": (ctypes.c_uint64, [ctypes.POINTER(Job)]),
+    "stp_server_load": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                       ctypes.POINTER(ctypes.c_uint64)]),
+    "stp_server_last_error": (ctypes.c_char_p, [_P]),
     "stp_ctx_set_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
     "stp_ctx_set_timing_filter": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "stp_ctx_reset_timings": (ctypes.c_int, [_P]),

@@ -1080,7 +1080,6 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
                 workarea *= 2;
         const int arr = workarea * os;
         STP_REQUIRE(arr <= 4096, "W-kernel work array (workarea * oversampling) must be <= 4096");
-        STP_REQUIRE(arr >= 2 * (g.ks * os), "W-kernel work array too small for the support");
         const double scaling = double(N) / double(workarea);
         char key[192];
         snprintf(key, sizeof(key), "waa:%d:%.17g:%.17g:%.17g:%d:%d", kf.kind, kf.param, kf.width_g, kf.width_s, workarea,

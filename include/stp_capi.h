@@ -349,6 +349,69 @@ stp_status stp_memcpy_htod(stp_ctx* ctx, void* dst, const void* src, size_t byte
 stp_status stp_memcpy_dtoh(stp_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 /* ------------------------------------------------------------------------- */
+/* Serving: a request loop with continuous batching                           */
+/* ------------------------------------------------------------------------- */
+/* No reference counterpart: the reference runs one snapshot per process call
+ * (src/reduce/reduce.cpp:113-143).  A server owns `num_slots` contexts on one
+ * device, each with its own stream and resident kernel tables, FFT twiddles and
+ * grid workspace.  Submitted jobs queue in arrival order; a free slot takes the
+ * next job as soon as the job's estimated device footprint fits in what is left
+ * of `hbm_budget_bytes`, so jobs that arrive while others run are admitted as
+ * slots and memory free up, and their kernels overlap on the device.
+ * Host buffers of a job must stay valid until stp_server_wait returns for it. */
+typedef struct stp_server stp_server;
+
+typedef struct stp_server_cfg {
+    int32_t device;
+    uint32_t num_slots;         /* concurrent jobs (>= 1) */
+    uint64_t hbm_budget_bytes;  /* admission limit; 0 = 80 % of the device's free memory */
+} stp_server_cfg;
+
+typedef struct stp_job {
+    stp_imager_pars img_pars;
+    stp_wproj_pars w_proj;      /* num_wplanes = 0: no W-projection */
+    int32_t use_kernel_spec;    /* 0: kernel selected from img_pars like the ImageVisibilities ctor */
+    stp_kernel_spec kernel;
+    int32_t precision;          /* stp_precision */
+    size_t n_vis;
+    const double* vis;          /* host: n complex (re, im) */
+    const double* vis_weights;  /* host: n */
+    const double* uvw_lambda;   /* host: n x 3 column-major */
+    double* image;              /* host out N x N, or NULL */
+    double* beam;               /* host out N x N (generate_beam), or NULL */
+    int32_t source_find;        /* run SourceFindImage on the image */
+    stp_sourcefind_pars sf_pars;
+    int32_t* label_map;         /* host out N x N, or NULL */
+    stp_island* islands;        /* host out, capacity island_cap */
+    size_t island_cap;
+} stp_job;
+
+typedef struct stp_job_result {
+    int32_t status;             /* stp_status of the job */
+    int32_t slot;               /* slot that ran it */
+    size_t n_islands;
+    stp_sourcefind_stats stats;
+    stp_imager_diag diag;
+    double t_submit, t_start, t_end;   /* seconds on the server's steady clock */
+    char error[256];
+} stp_job_result;
+
+stp_status stp_server_create(stp_server** server, const stp_server_cfg* cfg);
+/* Waits for every submitted job, then frees the slots. */
+stp_status stp_server_destroy(stp_server* server);
+/* Queues a copy of *job; *job_id identifies it for poll / wait. */
+stp_status stp_server_submit(stp_server* server, const stp_job* job, uint64_t* job_id);
+/* *done = 1 once the job has finished (successfully or not). */
+stp_status stp_server_poll(stp_server* server, uint64_t job_id, int32_t* done);
+/* Blocks until the job has finished and fills *result; each id can be waited once. */
+stp_status stp_server_wait(stp_server* server, uint64_t job_id, stp_job_result* result);
+/* Estimated device bytes of a job (what admission reserves). */
+uint64_t stp_job_device_bytes(const stp_job* job);
+/* Jobs queued, running, and bytes reserved right now. */
+stp_status stp_server_load(stp_server* server, uint32_t* queued, uint32_t* running, uint64_t* reserved_bytes);
+const char* stp_server_last_error(const stp_server* server);
+
+/* ------------------------------------------------------------------------- */
 /* Kernel timing (HIP events on the context stream)                           */
 /* ------------------------------------------------------------------------- */
 /* When enabled, every kernel launch of the context is bracketed by HIP events

@@ -30,14 +30,36 @@ def test_library_exports_all_declared_symbols():
     assert sorted(capi.EXPORTED_SYMBOLS) == names
 
 
-def test_struct_layouts_match_header():
-    # sizes implied by the C declarations (x86-64 alignment rules)
-    assert ctypes.sizeof(capi.ImagerPars) == 56
-    assert ctypes.sizeof(capi.WProjPars) == 40
-    assert ctypes.sizeof(capi.SourceFindPars) == 56
-    assert ctypes.sizeof(capi.Island) == 24 + 8 + 16 + 48
-    assert ctypes.sizeof(capi.SourceFindStats) == 48
-    assert ctypes.sizeof(capi.ImagerDiag) == 32
+STRUCTS = {
+    "stp_imager_pars": "ImagerPars", "stp_kernel_spec": "KernelSpec", "stp_wproj_pars": "WProjPars",
+    "stp_aproj_pars": "AProjPars", "stp_sourcefind_pars": "SourceFindPars", "stp_bounding_box": "BoundingBox",
+    "stp_gaussian2d": "Gaussian2d", "stp_island": "Island", "stp_sourcefind_stats": "SourceFindStats",
+    "stp_imager_diag": "ImagerDiag", "stp_server_cfg": "ServerCfg", "stp_job": "Job", "stp_job_result": "JobResult",
+}
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """sizeof and every field offset of the C structs (compiled from the header with gcc)
+    equal the ctypes mirrors."""
+    import subprocess
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "stp_capi.h"', "int main(void) {"]
+    for cname, pyname in STRUCTS.items():
+        cls = getattr(capi, pyname)
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = dict(line.split() for line in out if line.strip())
+    for cname, pyname in STRUCTS.items():
+        cls = getattr(capi, pyname)
+        assert int(got[cname]) == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(cls, f).offset, (cname, f)
 
 
 def test_ctx_create_without_device_fails_cleanly():
