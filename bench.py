@@ -66,6 +66,22 @@ def kernel_bytes(name, N, rows, nvis, n_entries, nb, S):
     return None
 
 
+def pmc_traffic(workload, path, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (profiles/traffic_<workload>_<path>.txt: FETCH_SIZE + WRITE_SIZE, calibrated with
+    a copy kernel), or None when no summary covers it."""
+    fn = os.path.join(ROOT, "profiles", f"traffic_{workload}_{path}.txt")
+    if not os.path.exists(fn):
+        return None
+    for line in open(fn):
+        f = line.split()
+        if line.startswith("#") or len(f) != 4:
+            continue
+        if f[0] == kernel:
+            return int(f[2]) + int(f[3])
+    return None
+
+
 def measure(ctx, capi, args, pars, sfp, prec, n, N, bufs, do_sf):
     d_vis, d_w, d_uvw, d_img, d_beam, d_lab = bufs
     island_buf = (capi.Island * (1 << 16))()
@@ -184,6 +200,8 @@ def main():
     head_name = "fp32c" if "fp32c" in res else paths[0]
     head = res[head_name]
 
+    for p, r in res.items():
+        r["roofline"]["traffic"] = pmc_traffic(args.workload, p, r["roofline"]["kernel"])
     result = {
         "metric": METRIC,
         "value": round(head["value"], 3),
