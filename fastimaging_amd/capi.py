@@ -137,6 +137,23 @@ class ImagerDiag(ctypes.Structure):
     ]
 
 
+class BatchItem(ctypes.Structure):
+    """stp_batch_item: one snapshot of a batch, device buffers."""
+    _fields_ = [
+        ("img_pars", ImagerPars),
+        ("w_proj", WProjPars),
+        ("n_vis", ctypes.c_size_t),
+        ("d_vis", ctypes.c_void_p),
+        ("d_vis_weights", ctypes.c_void_p),
+        ("d_uvw_lambda", ctypes.c_void_p),
+        ("d_image", ctypes.c_void_p),
+        ("d_beam", ctypes.c_void_p),
+    ]
+
+
+BATCH_GRAPH, BATCH_AUTOTUNE = 1, 2
+
+
 class ServerCfg(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("num_slots", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64)]
 
@@ -294,6 +311,12 @@ _SIG = {
     "stp_device_free": (ctypes.c_int, [_P, _P]),
     "stp_memcpy_htod": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "stp_memcpy_dtoh": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "stp_ctx_set_autotune": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p]),
+    "stp_ctx_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(ImagerPars), ctypes.POINTER(WProjPars), ctypes.c_size_t,
+                                          ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                          _D]),
+    "stp_image_batch": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(BatchItem), ctypes.POINTER(KernelSpec),
+                                       ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, _D]),
     "stp_server_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(ServerCfg)]),
     "stp_server_destroy": (ctypes.c_int, [_P]),
     "stp_server_submit": (ctypes.c_int, [_P, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_uint64)]),

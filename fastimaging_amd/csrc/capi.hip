@@ -1,7 +1,11 @@
 // extern "C" entry points of the gfx950 STP library (include/stp_capi.h).
 // Every entry point converts exceptions into a status code and a message.
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <string>
+#include <vector>
 #include <exception>
 
 #include "common.h"
@@ -91,6 +95,19 @@ void check_kernel_spec(const stp_kernel_spec* k)
         STP_REQUIRE(k->param > 0.0, "half_base_width / trunc must be > 0");
 }
 
+// Imaging shape of the autotuner: padded size, support, oversampling, kernel
+// mode, precision, W-projection planes and the visibility count's octave.
+std::string tune_key(const stp_imager_pars* p, const stp_wproj_pars* wp, size_t n, int precision)
+{
+    int oct = 0;
+    while ((size_t(1) << oct) <= n)
+        oct++;
+    char k[160];
+    snprintf(k, sizeof(k), "N%u:s%u:os%u:x%d:p%d:w%u:ws%u:n%d", p->padded_image_size, p->kernel_support, p->oversampling,
+        p->kernel_exact, precision, wp ? wp->num_wplanes : 0u, wp ? wp->max_wpconv_support : 0u, oct);
+    return k;
+}
+
 template <typename T, typename OT>
 void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp, const stp_kernel_spec* kspec, size_t n,
     const double* d_vis, const double* d_w, const double* d_uvw, OT* d_image, OT* d_beam, double* d_total,
@@ -116,6 +133,12 @@ void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp
         a.wproj = *wp;
     a.cell_rad = (p->cell_size / 3600.0) * (M_PI / 180.0);
     a.analytic_gcf = p->analytic_gcf != 0;
+    int fft_l1 = 0;
+    auto tc = ctx->tune.find(tune_key(p, wp, n, std::is_same<T, double>::value ? 0 : (twofloat ? 2 : 1)));
+    if (tc != ctx->tune.end()) {
+        a.tile_cols = tc->second.tile_cols;
+        fft_l1 = tc->second.fft_l1;
+    }
     const int rows = N / 2 + 1;
     const size_t ge = size_t(rows) * N;
     cplx<T>* grids = (cplx<T>*)ws_get(ctx, "imager_grids", sizeof(cplx<T>) * ge * (beam ? 2 : 1));
@@ -123,6 +146,7 @@ void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp
     NormSpec ns;
     ns.image_size = int(p->image_size);
     ns.total_ptr = d_total;
+    ns.u_l1 = fft_l1;
     if (p->gridding_correction) {
         const KernelFn kf = kernel_fn(kern);
         char key[192];
@@ -135,6 +159,108 @@ void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp
     outs.p[0] = d_image;
     outs.p[1] = d_beam;
     fft_c2r_2d<T, OT>(ctx, grids, beam ? 2 : 1, N, N, outs, ns);
+}
+
+// Imaging on device buffers in the requested precision.
+void run_imaging(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp, const stp_kernel_spec* kernel, size_t n,
+    const double* d_vis, const double* d_w, const double* d_uvw, int32_t precision, void* d_image, void* d_beam,
+    double* d_total, unsigned long long* d_ngood)
+{
+    STP_REQUIRE(precision >= STP_PRECISION_FP64 && precision <= STP_PRECISION_FP32_TWOFLOAT, "unknown precision");
+    if (precision == STP_PRECISION_FP64)
+        run_imager<double, double>(ctx, p, wp, kernel, n, d_vis, d_w, d_uvw, (double*)d_image, (double*)d_beam, d_total, d_ngood,
+            false);
+    else
+        run_imager<float, float>(ctx, p, wp, kernel, n, d_vis, d_w, d_uvw, (float*)d_image, (float*)d_beam, d_total, d_ngood,
+            precision == STP_PRECISION_FP32_TWOFLOAT);
+}
+
+// Autotuner: times the imaging of this very input with each candidate launch
+// configuration (gridder tile width 16 / 32 / 64 columns, then the u-axis FFT
+// split around the balanced one) and keeps the fastest for the shape.
+void autotune_shape(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp, const stp_kernel_spec* kernel, size_t n,
+    const double* d_vis, const double* d_w, const double* d_uvw, int32_t precision, void* d_image, void* d_beam,
+    double* d_total, unsigned long long* d_ngood)
+{
+    const std::string key = tune_key(p, wp, n, precision);
+    if (ctx->tune.count(key))
+        return;
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    auto time_one = [&](const TuneChoice& c) {
+        ctx->tune[key] = c;
+        run_imaging(ctx, p, wp, kernel, n, d_vis, d_w, d_uvw, precision, d_image, d_beam, d_total, d_ngood); // warm
+        HIP_CHECK(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < 2; r++)
+            run_imaging(ctx, p, wp, kernel, n, d_vis, d_w, d_uvw, precision, d_image, d_beam, d_total, d_ngood);
+        HIP_CHECK(hipEventRecord(e1, ctx->stream));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        return double(ms) * 500.0; // us per run
+    };
+    TuneChoice best;
+    best.us = 1e300;
+    try {
+        for (int tcw : { 16, 32, 64 }) {
+            TuneChoice c;
+            c.tile_cols = tcw;
+            c.us = time_one(c);
+            if (c.us < best.us)
+                best = c;
+        }
+        const int N = int(p->padded_image_size);
+        int lg = 0;
+        while ((1 << lg) < N)
+            lg++;
+        const int bal = 1 << ((lg + 1) / 2);
+        for (int l1 : { bal / 2, bal * 2 }) {
+            if (l1 < 2 || l1 > 128 || N / l1 < 2 || N / l1 > 128)
+                continue;
+            TuneChoice c = best;
+            c.fft_l1 = l1;
+            c.us = time_one(c);
+            if (c.us < best.us)
+                best = c;
+        }
+    } catch (...) {
+        ctx->tune.erase(key);
+        ctx->timing = timing;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    ctx->timing = timing;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->tune[key] = best;
+    if (!ctx->tune_file.empty()) {
+        if (FILE* f = std::fopen(ctx->tune_file.c_str(), "a")) {
+            std::fprintf(f, "%s %d %d %.3f\n", key.c_str(), best.tile_cols, best.fft_l1, best.us);
+            std::fclose(f);
+        }
+    }
+}
+
+void load_tune_file(stp_ctx* ctx, const std::string& fn)
+{
+    FILE* f = std::fopen(fn.c_str(), "r");
+    if (!f)
+        return;
+    char key[160];
+    int tcw, l1;
+    double us;
+    while (std::fscanf(f, "%159s %d %d %lf", key, &tcw, &l1, &us) == 4) {
+        TuneChoice c;
+        c.tile_cols = tcw;
+        c.fft_l1 = l1;
+        c.us = us;
+        ctx->tune[key] = c;
+    }
+    std::fclose(f);
 }
 
 } // namespace
@@ -162,6 +288,10 @@ stp_status stp_ctx_create(stp_ctx** out, int device)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess)
         ctx->num_cus = prop.multiProcessorCount;
+    if (const char* tf = std::getenv("STP_TUNE_FILE")) {
+        ctx->tune_file = tf;
+        load_tune_file(ctx, ctx->tune_file);
+    }
     *out = ctx;
     return STP_OK;
 }
@@ -172,6 +302,13 @@ stp_status stp_ctx_destroy(stp_ctx* ctx)
         return STP_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    invalidate_graphs(ctx);
+    for (auto s : ctx->branch_streams) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+    for (auto e : ctx->branch_events)
+        (void)hipEventDestroy(e);
     for (auto& kv : ctx->ws)
         if (kv.second.ptr)
             (void)hipFree(kv.second.ptr);
@@ -206,7 +343,8 @@ stp_status stp_ctx_synchronize(stp_ctx* ctx)
 stp_status stp_ctx_release_workspace(stp_ctx* ctx)
 {
     return guarded(ctx, [&] {
-        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        HIP_CHECK(hipDeviceSynchronize());
+        invalidate_graphs(ctx);
         for (auto& kv : ctx->ws)
             if (kv.second.ptr)
                 HIP_CHECK(hipFree(kv.second.ptr));
@@ -289,8 +427,11 @@ stp_status stp_image_visibilities_device(stp_ctx* ctx, const stp_imager_pars* p,
         STP_REQUIRE(n == 0 || (d_vis && d_w && d_uvw), "visibility buffers are NULL");
         double* d_total = (double*)ws_get(ctx, "imager_total", sizeof(double) * 2);
         unsigned long long* d_ngood = (unsigned long long*)(d_total + 1);
-        LaunchTimer whole(ctx, "image_visibilities");
         STP_REQUIRE(precision >= STP_PRECISION_FP64 && precision <= STP_PRECISION_FP32_TWOFLOAT, "unknown precision");
+        if (ctx->autotune)
+            autotune_shape(ctx, p, wp && wp->num_wplanes > 0 ? wp : nullptr, kernel, n, d_vis, d_w, d_uvw, precision, d_image,
+                d_beam, d_total, d_ngood);
+        LaunchTimer whole(ctx, "image_visibilities");
         if (precision == STP_PRECISION_FP64)
             run_imager<double, double>(ctx, p, wp, kernel, n, d_vis, d_w, d_uvw, (double*)d_image, (double*)d_beam, d_total,
                 d_ngood, false);
@@ -508,6 +649,160 @@ stp_status stp_memcpy_dtoh(stp_ctx* ctx, void* dst, const void* src, size_t byte
 stp_status stp_ctx_set_timing(stp_ctx* ctx, int32_t enable)
 {
     return guarded(ctx, [&] { ctx->timing = enable != 0; });
+}
+
+stp_status stp_ctx_set_autotune(stp_ctx* ctx, int32_t mode, const char* tune_file)
+{
+    return guarded(ctx, [&] {
+        STP_REQUIRE(mode == 0 || mode == 1, "autotune mode must be 0 or 1");
+        ctx->autotune = mode;
+        ctx->tune_file = tune_file ? tune_file : "";
+        if (!ctx->tune_file.empty())
+            load_tune_file(ctx, ctx->tune_file);
+    });
+}
+
+stp_status stp_ctx_get_tuning(stp_ctx* ctx, const stp_imager_pars* img_pars, const stp_wproj_pars* w_proj, size_t n_vis,
+    int32_t precision, int32_t* tile_cols, int32_t* fft_l1, double* us)
+{
+    return guarded(ctx, [&] {
+        STP_REQUIRE(img_pars != nullptr, "img_pars is NULL");
+        auto it = ctx->tune.find(tune_key(img_pars, w_proj && w_proj->num_wplanes > 0 ? w_proj : nullptr, n_vis, precision));
+        const TuneChoice c = it != ctx->tune.end() ? it->second : TuneChoice{};
+        if (tile_cols)
+            *tile_cols = c.tile_cols;
+        if (fft_l1)
+            *fft_l1 = c.fft_l1;
+        if (us)
+            *us = c.us;
+    });
+}
+
+stp_status stp_image_batch(stp_ctx* ctx, size_t n_items, const stp_batch_item* items, const stp_kernel_spec* kernel,
+    int32_t precision, uint32_t max_concurrent, uint32_t flags, double* sample_grid_totals)
+{
+    return guarded(ctx, [&] {
+        STP_REQUIRE(n_items == 0 || items != nullptr, "items is NULL");
+        STP_REQUIRE(precision >= STP_PRECISION_FP64 && precision <= STP_PRECISION_FP32_TWOFLOAT, "unknown precision");
+        for (size_t i = 0; i < n_items; i++) {
+            const stp_batch_item& it = items[i];
+            check_imager_pars(&it.img_pars, &it.w_proj, nullptr);
+            STP_REQUIRE(it.d_image != nullptr, "batch item image buffer is NULL");
+            STP_REQUIRE(!it.img_pars.generate_beam || it.d_beam != nullptr, "batch item beam buffer is NULL");
+            STP_REQUIRE(it.n_vis == 0 || (it.d_vis && it.d_vis_weights && it.d_uvw_lambda), "batch item visibilities are NULL");
+        }
+        if (n_items == 0)
+            return;
+        const int K = int(std::min<size_t>(n_items, max_concurrent ? std::min(max_concurrent, 16u) : 4u));
+        while (int(ctx->branch_streams.size()) < K) {
+            hipStream_t st;
+            HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            ctx->branch_streams.push_back(st);
+        }
+        while (ctx->branch_events.size() < size_t(K) + 1) {
+            hipEvent_t e;
+            HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            ctx->branch_events.push_back(e);
+        }
+        const stp_wproj_pars* none = nullptr;
+        auto wp_of = [&](const stp_batch_item& it) { return it.w_proj.num_wplanes > 0 ? &it.w_proj : none; };
+        // per-snapshot totals and good counts live in the base workspace
+        double* totals = (double*)ws_get(ctx, "batch_totals", sizeof(double) * 2 * n_items);
+        if ((flags & STP_BATCH_AUTOTUNE) || ctx->autotune)
+            for (size_t i = 0; i < n_items; i++) {
+                const stp_batch_item& it = items[i];
+                autotune_shape(ctx, &it.img_pars, wp_of(it), kernel, it.n_vis, it.d_vis, it.d_vis_weights, it.d_uvw_lambda,
+                    precision, it.d_image, it.d_beam, totals + 2 * i, (unsigned long long*)(totals + 2 * i + 1));
+            }
+        hipStream_t main = ctx->stream;
+        auto enqueue = [&]() {
+            HIP_CHECK(hipEventRecord(ctx->branch_events[K], main));
+            for (int b = 0; b < K; b++)
+                HIP_CHECK(hipStreamWaitEvent(ctx->branch_streams[b], ctx->branch_events[K], 0));
+            try {
+                for (size_t i = 0; i < n_items; i++) {
+                    const int b = int(i % size_t(K));
+                    ctx->stream = ctx->branch_streams[b];
+                    ctx->ws_slot = b + 1;
+                    const stp_batch_item& it = items[i];
+                    run_imaging(ctx, &it.img_pars, wp_of(it), kernel, it.n_vis, it.d_vis, it.d_vis_weights, it.d_uvw_lambda,
+                        precision, it.d_image, it.d_beam, totals + 2 * i, (unsigned long long*)(totals + 2 * i + 1));
+                }
+            } catch (...) {
+                ctx->stream = main;
+                ctx->ws_slot = 0;
+                throw;
+            }
+            ctx->stream = main;
+            ctx->ws_slot = 0;
+            for (int b = 0; b < K; b++) {
+                HIP_CHECK(hipEventRecord(ctx->branch_events[b], ctx->branch_streams[b]));
+                HIP_CHECK(hipStreamWaitEvent(main, ctx->branch_events[b], 0));
+            }
+        };
+        if (flags & STP_BATCH_GRAPH) {
+            // signature: everything the captured launches depend on
+            std::string sig = "K" + std::to_string(K) + ":p" + std::to_string(precision);
+            char buf[512];
+            if (kernel) {
+                snprintf(buf, sizeof(buf), ":k%d,%.17g,%.17g,%.17g", kernel->kernel_function, kernel->param,
+                    kernel->width_normalization_gaussian, kernel->width_normalization_sinc);
+                sig += buf;
+            }
+            for (size_t i = 0; i < n_items; i++) {
+                const stp_batch_item& it = items[i];
+                const stp_imager_pars& q = it.img_pars;
+                snprintf(buf, sizeof(buf), "|%u,%u,%.17g,%.17g,%d,%u,%d,%u,%d,%d,%d,%u,%u,%u,%.17g,%d,%zu,%p,%p,%p,%p,%p", q.image_size,
+                    q.padded_image_size, q.cell_size, q.padding_factor, q.kernel_function, q.kernel_support, q.kernel_exact,
+                    q.oversampling, q.generate_beam, q.gridding_correction, q.analytic_gcf, it.w_proj.num_wplanes,
+                    it.w_proj.max_wpconv_support, it.w_proj.undersampling_opt, it.w_proj.kernel_trunc_perc,
+                    it.w_proj.wplanes_median, it.n_vis, (const void*)it.d_vis, (const void*)it.d_vis_weights,
+                    (const void*)it.d_uvw_lambda, it.d_image, it.d_beam);
+                sig += buf;
+            }
+            auto g = ctx->graphs.find(sig);
+            if (g != ctx->graphs.end()) {
+                HIP_CHECK(hipGraphLaunch(g->second, main));
+            } else {
+                // eager run: sizes every workspace and uploads every table, so the
+                // capture below records kernel launches only
+                enqueue();
+                HIP_CHECK(hipStreamSynchronize(main));
+                const bool timing = ctx->timing;
+                ctx->timing = false;
+                hipGraph_t graph = nullptr;
+                HIP_CHECK(hipStreamBeginCapture(main, hipStreamCaptureModeRelaxed));
+                try {
+                    enqueue();
+                } catch (...) {
+                    hipGraph_t dropped = nullptr;
+                    (void)hipStreamEndCapture(main, &dropped);
+                    if (dropped)
+                        (void)hipGraphDestroy(dropped);
+                    ctx->timing = timing;
+                    throw;
+                }
+                ctx->timing = timing;
+                HIP_CHECK(hipStreamEndCapture(main, &graph));
+                hipGraphExec_t exec = nullptr;
+                const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                HIP_CHECK(ie);
+                ctx->graphs[sig] = exec;
+            }
+        } else {
+            enqueue();
+        }
+        if (sample_grid_totals) {
+            std::vector<double> t(2 * n_items);
+            HIP_CHECK(hipMemcpyAsync(t.data(), totals, sizeof(double) * 2 * n_items, hipMemcpyDeviceToHost, main));
+            HIP_CHECK(hipStreamSynchronize(main));
+            for (size_t i = 0; i < n_items; i++)
+                sample_grid_totals[i] = t[2 * i];
+        } else {
+            HIP_CHECK(hipStreamSynchronize(main));
+        }
+    });
 }
 
 stp_status stp_ctx_set_timing_filter(stp_ctx* ctx, const char* names)

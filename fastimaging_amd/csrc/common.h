@@ -50,6 +50,13 @@ struct TimingAcc {
     uint64_t count = 0;
 };
 
+// Launch configuration picked by the autotuner for one imaging shape.
+struct TuneChoice {
+    int tile_cols = 0; // gridder tile width
+    int fft_l1 = 0;    // four-step split of the u-axis FFT
+    double us = 0.0;   // measured imaging time with this choice
+};
+
 struct PendingTiming {
     std::string name;
     hipEvent_t start, stop;
@@ -74,6 +81,16 @@ struct stp_ctx {
     std::map<std::string, stp_hip::DevBuf> tables;
     bool timing = false;
     std::vector<std::string> timing_filter; // empty: every launch
+    // autotuner: chosen launch configuration per imaging shape; mode 0 off,
+    // 1 tune shapes on first use; results appended to tune_file when set
+    std::map<std::string, stp_hip::TuneChoice> tune;
+    int autotune = 0;
+    std::string tune_file;
+    // batches: workspace namespace of the running branch and the branch streams
+    int ws_slot = 0;
+    std::vector<hipStream_t> branch_streams;
+    std::vector<hipEvent_t> branch_events;
+    std::map<std::string, hipGraphExec_t> graphs;
     std::vector<stp_hip::PendingTiming> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, stp_hip::TimingAcc> timings;
@@ -84,6 +101,8 @@ namespace stp_hip {
 
 // Returns a device buffer of at least `bytes` for slot `name` (contents undefined).
 void* ws_get(stp_ctx* ctx, const std::string& name, size_t bytes);
+// Destroys the context's captured batch graphs (their buffers are about to change).
+void invalidate_graphs(stp_ctx* ctx);
 // Returns a pinned host buffer of at least `bytes` for slot `name` (contents undefined).
 void* host_stage(stp_ctx* ctx, const std::string& name, size_t bytes);
 // Device copy of a host table, uploaded once per key and kept for the context's lifetime.

@@ -10,20 +10,29 @@ void* ws_get(stp_ctx* ctx, const std::string& name, size_t bytes)
 {
     if (bytes == 0)
         bytes = 256;
-    DevBuf& b = ctx->ws[name];
+    DevBuf& b = ctx->ws[ctx->ws_slot ? name + "#" + std::to_string(ctx->ws_slot) : name];
     if (b.bytes >= bytes)
         return b.ptr;
     if (b.ptr) {
-        // the old buffer may still be read by queued work
-        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        // the old buffer may still be read by queued work, and captured batch
+        // graphs refer to it
+        HIP_CHECK(hipDeviceSynchronize());
         HIP_CHECK(hipFree(b.ptr));
         b.ptr = nullptr;
         b.bytes = 0;
+        invalidate_graphs(ctx);
     }
     const size_t alloc = (bytes + 4095) & ~size_t(4095);
     HIP_CHECK(hipMalloc(&b.ptr, alloc));
     b.bytes = alloc;
     return b.ptr;
+}
+
+void invalidate_graphs(stp_ctx* ctx)
+{
+    for (auto& kv : ctx->graphs)
+        (void)hipGraphExecDestroy(kv.second);
+    ctx->graphs.clear();
 }
 
 void* host_stage(stp_ctx* ctx, const std::string& name, size_t bytes)

@@ -12,6 +12,7 @@ struct NormSpec {
     int image_size = 0;
     const double* total_ptr = nullptr; // device scalar; NULL = no scaling
     const double* gcf = nullptr;       // device array of the padded size; NULL = none
+    int u_l1 = 0;                      // four-step split of the u axis, N = L1 * (N / L1); 0 = balanced
 };
 
 template <typename OT>

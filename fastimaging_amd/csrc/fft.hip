@@ -301,9 +301,16 @@ __global__ __launch_bounds__(WG) void fft_cols_c2c(const cplx<T>* in, int rows, 
             out[size_t(x) * rows + r] = b[r];
 }
 
+// N = L1 * L2 for the four-step u transform: `req` when it is a valid split
+// (both factors in [2, 128]), otherwise the balanced one.
 template <typename T>
-void split_len(int N, int& L1, int& L2)
+void split_len(int N, int& L1, int& L2, int req = 0)
 {
+    if (req >= 2 && req <= 128 && (req & (req - 1)) == 0 && N % req == 0 && N / req >= 2 && N / req <= 128) {
+        L1 = req;
+        L2 = N / req;
+        return;
+    }
     int lg = 0;
     while ((1 << lg) < N)
         lg++;
@@ -314,10 +321,11 @@ void split_len(int N, int& L1, int& L2)
 #define STP_POW2_CASES(X) X(2) X(4) X(8) X(16) X(32) X(64) X(128)
 
 template <typename T>
-void launch_u_passes(stp_ctx* ctx, cplx<T>* grids, int nbatch, int rows, int N, size_t grid_elems, const T* tw, bool inverse)
+void launch_u_passes(stp_ctx* ctx, cplx<T>* grids, int nbatch, int rows, int N, size_t grid_elems, const T* tw, bool inverse,
+    int req_l1 = 0)
 {
     int L1, L2;
-    split_len<T>(N, L1, L2);
+    split_len<T>(N, L1, L2, req_l1);
     using namespace stp_hip;
     switch (L1) {
 #define U1CASE(LL)                                                                                          \
@@ -356,9 +364,9 @@ void fft_c2r_2d(stp_ctx* ctx, cplx<T>* grids, int nbatch, int n_rows, int N, con
     const size_t grid_elems = size_t(rows) * N;
     const Twiddles& twu = twiddles_for(ctx, N);
     const T* tw = std::is_same<T, double>::value ? (const T*)twu.d : (const T*)twu.f;
-    launch_u_passes<T>(ctx, grids, nbatch, rows, N, grid_elems, tw, true);
+    launch_u_passes<T>(ctx, grids, nbatch, rows, N, grid_elems, tw, true, ns.u_l1);
     int L1, L2;
-    split_len<T>(N, L1, L2);
+    split_len<T>(N, L1, L2, ns.u_l1);
     const Twiddles& twv = twiddles_for(ctx, n_rows);
     const T* tvn = std::is_same<T, double>::value ? (const T*)twv.d : (const T*)twv.f;
     STP_REQUIRE(nbatch >= 1 && nbatch <= 4, "1 to 4 grids per FFT launch");

@@ -20,6 +20,7 @@ struct GridderArgs {
     stp_wproj_pars wproj{};
     double cell_rad = 0.0;  // cell size in radians (W-kernel image-domain sampling)
     bool analytic_gcf = false; // image-domain AA kernel of the W-kernels from gcf() (conv_func.h:351-412)
+    int tile_cols = 0;      // tile width (16, 32 or 64 grid columns); 0 = default
 };
 
 // Grids n visibilities into vis_grid (and sampling_grid), each rows x N complex

@@ -27,7 +27,7 @@ namespace stp_hip {
 namespace {
 
 constexpr int TILE_R = 64;  // grid rows (v) per tile: 64 contiguous complex per column
-constexpr int TILE_C = 32;  // grid columns (u) per tile
+constexpr int TILE_C = 32;  // default grid columns (u) per tile; GridderArgs::tile_cols overrides
 constexpr int GWG = 256;
 constexpr int MAXT = 8;     // max distinct tiles per dimension of one footprint
 
@@ -35,6 +35,7 @@ struct GridGeom {
     int N, rows, half, s, ks, os;
     int halfplane, shift, exact, beam;
     int tiles_r, tiles_c;
+    int tc;       // tile width in grid columns (TILE_R rows per tile)
     double scale;
     long long n;
     int kfunc;
@@ -193,7 +194,7 @@ __device__ __forceinline__ void footprint_tiles(const GridGeom& g, int gx, int g
             add_unique(rt, nrt, r / TILE_R);
     }
     for (int j = 0; j < g.ks; j++)
-        add_unique(ct, nct, tap_col(g, gx, j) / TILE_C);
+        add_unique(ct, nct, tap_col(g, gx, j) / g.tc);
 }
 
 __global__ __launch_bounds__(GWG) void grid_count_kernel(GridGeom g, const double* uv, const double* vis,
@@ -501,7 +502,7 @@ __global__ __launch_bounds__(TWG) void grid_tiles_kernel(GridGeom g, const Entry
     __shared__ Entry ebuf[TEB];
     const int ks = g.ks, s = g.s;
     const int S = TILE_R + (ks <= 15 ? ks : 1);     // column stride of the LDS tile
-    const int plane = TILE_C * S;
+    const int plane = g.tc * S;
     AS* acc = reinterpret_cast<AS*>(dyn);
     const int acc_bytes = Acc::arrays * plane * int(sizeof(AS));
     double* kv = dyn + (acc_bytes + 7) / 8;          // kernel table, or exact-mode vectors then sums
@@ -515,7 +516,7 @@ __global__ __launch_bounds__(TWG) void grid_tiles_kernel(GridGeom g, const Entry
         acc[k] = AS(0);
     const int t = blockIdx.x;
     const int rt = t % g.tiles_r, ct = t / g.tiles_r;
-    const int r0 = rt * TILE_R, c0 = ct * TILE_C;
+    const int r0 = rt * TILE_R, c0 = ct * g.tc;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int epw = ks <= 64 ? 64 / ks : 1;          // entries per wave pass
     const int slot = ks <= 64 ? lane / ks : 0;
@@ -567,7 +568,7 @@ __global__ __launch_bounds__(TWG) void grid_tiles_kernel(GridGeom g, const Entry
             for (int j = jl; j < ks; j += (ks <= 64 ? ks : 64)) {
                 const int c = tap_col(g, E.gx, j);
                 const int lc = c - c0;
-                if (lc < 0 || lc >= TILE_C)
+                if (lc < 0 || lc >= g.tc)
                     continue;
                 const double kxd = g.exact ? kyd[ks + j] : 0.0;
                 const T kx = g.exact ? T(0) : kt[cpx * ks + j];
@@ -603,7 +604,7 @@ __global__ __launch_bounds__(TWG) void grid_tiles_kernel(GridGeom g, const Entry
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < TILE_C * TILE_R; k += TWG) {
+    for (int k = threadIdx.x; k < g.tc * TILE_R; k += TWG) {
         const int lr = k % TILE_R, lc = k / TILE_R;
         const int r = r0 + lr, c = c0 + lc;
         if (r < g.rows && c < g.N) {
@@ -885,14 +886,14 @@ __global__ __launch_bounds__(WTWG) void grid_wtiles_kernel(GridGeom g, const Ent
     __shared__ Entry ebuf[WEB];
     const int ks = g.ks, s = g.s, N = g.N;
     const int S = TILE_R + 1;
-    const int plane = TILE_C * S;
+    const int plane = g.tc * S;
     const int cs = (g.os / 2) * 2 + 1;
     const size_t kk = size_t(ks) * ks;
     for (int k = threadIdx.x; k < 4 * plane; k += WTWG)
         acc[k] = 0.0;
     const int t = blockIdx.x;
     const int rt = t % g.tiles_r, ct = t / g.tiles_r;
-    const int r0 = rt * TILE_R, c0 = ct * TILE_C;
+    const int r0 = rt * TILE_R, c0 = ct * g.tc;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = r0 + lane;
     const uint32_t eb = min(offsets[size_t(t) * ostride], cap), ee = min(offsets[size_t(t + 1) * ostride], cap);
@@ -917,7 +918,7 @@ __global__ __launch_bounds__(WTWG) void grid_wtiles_kernel(GridGeom g, const Ent
             int j = c0 - (E.gx - s); // kernel column of the tile's first column, mod N
             j = j < 0 ? j + N : (j >= N ? j - N : j);
             const int a0 = lane;
-            for (int lc = 0; lc < TILE_C; lc++, j = (j + 1 == N) ? 0 : j + 1) {
+            for (int lc = 0; lc < g.tc; lc++, j = (j + 1 == N) ? 0 : j + 1) {
                 if (j >= ks || c0 + lc >= N)
                     continue;
                 const cplx<KT> kv = K[size_t(j) * ks + i];
@@ -934,7 +935,7 @@ __global__ __launch_bounds__(WTWG) void grid_wtiles_kernel(GridGeom g, const Ent
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < TILE_C * TILE_R; k += WTWG) {
+    for (int k = threadIdx.x; k < g.tc * TILE_R; k += WTWG) {
         const int lr = k % TILE_R, lc = k / TILE_R;
         const int rr = r0 + lr, c = c0 + lc;
         if (rr < g.rows && c < N) {
@@ -988,7 +989,9 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     g.exact = a.kernel_exact;
     g.beam = a.generate_beam && d_sampling_grid != nullptr;
     g.tiles_r = int(div_up(g.rows, TILE_R));
-    g.tiles_c = int(div_up(N, TILE_C));
+    g.tc = a.tile_cols > 0 ? a.tile_cols : TILE_C;
+    STP_REQUIRE(g.tc == 16 || g.tc == 32 || g.tc == 64, "tile width must be 16, 32 or 64 columns");
+    g.tiles_c = int(div_up(N, g.tc));
     g.scale = a.uv_scale;
     g.n = (long long)n;
     const KernelFn kf = kernel_fn(a.kernel);
@@ -1020,7 +1023,7 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     // worst-case entries: a footprint of ks cells spans at most ceil((ks-1)/T)+1 tiles per
     // axis, one more where it wraps across the grid edge; two passes
     const size_t per_pass = size_t(std::min(MAXT, int(div_up(g.ks - 1, TILE_R)) + 2)) *
-        size_t(std::min(MAXT, int(div_up(g.ks - 1, TILE_C)) + 2));
+        size_t(std::min(MAXT, int(div_up(g.ks - 1, g.tc)) + 2));
     const size_t cap64 = std::max<size_t>(1, 2 * per_pass * n);
     STP_REQUIRE(cap64 < (size_t(1) << 32), "too many gridding entries for one call");
     const uint32_t cap = uint32_t(cap64);
@@ -1130,7 +1133,7 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         ostride = 1;
     }
     if (wproj) {
-        const size_t dyn_bytes = sizeof(double) * 4 * size_t(TILE_C) * (TILE_R + 1);
+        const size_t dyn_bytes = sizeof(double) * 4 * size_t(g.tc) * (TILE_R + 1);
         HIP_CHECK(hipFuncSetAttribute((const void*)grid_wtiles_kernel<T, KT>, hipFuncAttributeMaxDynamicSharedMemorySize,
             int(dyn_bytes)));
         launch(ctx, "grid_wtiles", grid_wtiles_kernel<T, KT>, dim3(ntiles), dim3(WTWG), dyn_bytes, g, (const Entry*)entries,
@@ -1139,7 +1142,7 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         STP_REQUIRE(a.kernel_exact || cs * g.ks <= KV_DOUBLES, "oversampled kernel table too large");
         const bool twofloat = std::is_same<T, float>::value && a.twofloat;
         const int S = TILE_R + (g.ks <= 15 ? g.ks : 1);
-        const size_t acc_bytes = twofloat ? 6 * size_t(TILE_C) * S * sizeof(float) : 3 * size_t(TILE_C) * S * sizeof(double);
+        const size_t acc_bytes = twofloat ? 6 * size_t(g.tc) * S * sizeof(float) : 3 * size_t(g.tc) * S * sizeof(double);
         const size_t kv_bytes = a.kernel_exact ? sizeof(double) * (KV_DOUBLES + TEB) : sizeof(double) * size_t(cs) * g.ks;
         const size_t dyn_bytes = ((acc_bytes + 7) / 8) * 8 + kv_bytes;
         auto kern = twofloat ? grid_tiles_kernel<T, AccTwoFloat> : grid_tiles_kernel<T, AccF64>;

@@ -208,6 +208,26 @@ class Context:
                                                     d_label_map, isl, capacity, ctypes.byref(n), ctypes.byref(st)))
         return IslandList(isl, n.value), st
 
+    # ------------------------------------------------------- batches / tuning
+    def set_autotune(self, mode=1, tune_file=None):
+        self._check(self.lib.stp_ctx_set_autotune(self.h, mode, tune_file.encode() if tune_file else None))
+
+    def get_tuning(self, img_pars, n_vis, precision=capi.PRECISION_FP64, w_proj=None):
+        tc, l1, us = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        self._check(self.lib.stp_ctx_get_tuning(self.h, ctypes.byref(img_pars),
+                                                ctypes.byref(w_proj) if w_proj is not None else None, n_vis, precision,
+                                                ctypes.byref(tc), ctypes.byref(l1), ctypes.byref(us)))
+        return tc.value, l1.value, us.value
+
+    def image_batch(self, items, precision=capi.PRECISION_FP64, max_concurrent=0, flags=0, kernel=None):
+        """stp_image_batch over capi.BatchItem records; returns the per-snapshot sample_grid_total."""
+        arr = (capi.BatchItem * len(items))(*items)
+        totals = np.zeros(len(items))
+        self._check(self.lib.stp_image_batch(self.h, len(items), arr,
+                                             ctypes.byref(kernel) if kernel is not None else None, precision,
+                                             max_concurrent, flags, totals.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return totals
+
     def synchronize(self):
         self._check(self.lib.stp_ctx_synchronize(self.h))
 

@@ -349,6 +349,45 @@ stp_status stp_memcpy_htod(stp_ctx* ctx, void* dst, const void* src, size_t byte
 stp_status stp_memcpy_dtoh(stp_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 /* ------------------------------------------------------------------------- */
+/* Autotuner and batches                                                      */
+/* ------------------------------------------------------------------------- */
+/* No reference counterpart.  With autotuning on, the first imaging call of a
+ * shape (padded size, support, oversampling, kernel mode, precision, W planes,
+ * visibility-count octave) times the gridder tile widths 16 / 32 / 64 columns
+ * and the u-axis FFT splits next to the balanced one on that call's input and
+ * keeps the fastest for the context; `tune_file` (NULL: none) is read at once
+ * and appended with every new choice.  mode: 0 off, 1 tune new shapes. */
+stp_status stp_ctx_set_autotune(stp_ctx* ctx, int32_t mode, const char* tune_file);
+/* The configuration chosen for a shape (tile_cols 0: none yet). */
+stp_status stp_ctx_get_tuning(stp_ctx* ctx, const stp_imager_pars* img_pars, const stp_wproj_pars* w_proj, size_t n_vis,
+    int32_t precision, int32_t* tile_cols, int32_t* fft_l1, double* us);
+
+/* One snapshot of a batch; every buffer is in device memory. */
+typedef struct stp_batch_item {
+    stp_imager_pars img_pars;
+    stp_wproj_pars w_proj;       /* num_wplanes = 0: no W-projection */
+    size_t n_vis;
+    const double* d_vis;
+    const double* d_vis_weights;
+    const double* d_uvw_lambda;
+    void* d_image;               /* image_size^2 of the precision's type */
+    void* d_beam;                /* NULL unless generate_beam */
+} stp_batch_item;
+
+#define STP_BATCH_GRAPH 1u       /* capture the batch once, replay the hipGraph on later calls */
+#define STP_BATCH_AUTOTUNE 2u    /* tune every new shape of the batch before running it */
+
+/* Images n_items snapshots.  They run on up to max_concurrent streams of the
+ * context (0 = 4), each stream with its own workspace, so small snapshots fill
+ * the device together.  With STP_BATCH_GRAPH the first call with a given batch
+ * (parameters, sizes and buffer addresses) runs eagerly and records the whole
+ * batch into one hipGraph; later calls with the same batch launch that graph.
+ * sample_grid_totals (host, n_items, or NULL) receives 2 * sum of good weights
+ * per snapshot; the call returns after the batch has finished. */
+stp_status stp_image_batch(stp_ctx* ctx, size_t n_items, const stp_batch_item* items, const stp_kernel_spec* kernel,
+    int32_t precision, uint32_t max_concurrent, uint32_t flags, double* sample_grid_totals);
+
+/* ------------------------------------------------------------------------- */
 /* Serving: a request loop with continuous batching                           */
 /* ------------------------------------------------------------------------- */
 /* No reference counterpart: the reference runs one snapshot per process call

@@ -35,6 +35,7 @@ STRUCTS = {
     "stp_aproj_pars": "AProjPars", "stp_sourcefind_pars": "SourceFindPars", "stp_bounding_box": "BoundingBox",
     "stp_gaussian2d": "Gaussian2d", "stp_island": "Island", "stp_sourcefind_stats": "SourceFindStats",
     "stp_imager_diag": "ImagerDiag", "stp_server_cfg": "ServerCfg", "stp_job": "Job", "stp_job_result": "JobResult",
+    "stp_batch_item": "BatchItem",
 }
 
 

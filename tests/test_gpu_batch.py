@@ -1,0 +1,139 @@
+"""GPU tests of batches (stp_image_batch) and the autotuner: a batch of snapshots
+with uneven sizes gives the images each snapshot gives alone (FP64: within 1e-12 of
+the peak, the order of the LDS atomics being the only difference), every replay of
+the captured hipGraph matches the eager run, and autotuned launch configurations
+keep the oracle parity."""
+import numpy as np
+import pytest
+
+from fastimaging_amd import capi, simdata
+from tests.helpers import peak_rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+class _Batch:
+    """Snapshots with their inputs and outputs in device memory."""
+
+    def __init__(self, gpu, specs, precision):
+        self.gpu, self.prec = gpu, precision
+        self.snaps, self.items, self.bufs = [], [], []
+        osz = 8 if precision == capi.PRECISION_FP64 else 4
+        for N, cell, nv, ns, seed, wp, wmax in specs:
+            snap = simdata.snapshot(N, cell, nv, ns, seed=seed, field_frac=0.5 if wp is not None else 0.25, w_max=wmax)
+            b = [gpu.malloc(16 * nv), gpu.malloc(8 * nv), gpu.malloc(24 * nv), gpu.malloc(osz * N * N),
+                 gpu.malloc(osz * N * N)]
+            gpu.htod(b[0], np.ascontiguousarray(snap["vis"]))
+            gpu.htod(b[1], np.ascontiguousarray(snap["weights"]))
+            gpu.htod(b[2], np.asfortranarray(snap["uvw"]))
+            it = capi.BatchItem()
+            it.img_pars = capi.imager_pars(N, cell)
+            if wp is not None:
+                it.w_proj = wp
+            it.n_vis = nv
+            it.d_vis, it.d_vis_weights, it.d_uvw_lambda, it.d_image, it.d_beam = [x.value for x in b]
+            self.snaps.append(snap)
+            self.items.append(it)
+            self.bufs.append(b)
+
+    def images(self):
+        out = []
+        dt = np.float64 if self.prec == capi.PRECISION_FP64 else np.float32
+        for it in self.items:
+            N = it.img_pars.image_size
+            img = np.empty((N, N), order="F", dtype=dt)
+            beam = np.empty((N, N), order="F", dtype=dt)
+            self.gpu.dtoh(img, it.d_image)
+            self.gpu.dtoh(beam, it.d_beam)
+            out.append((img.astype(np.float64), beam.astype(np.float64)))
+        return out
+
+    def clear(self):
+        for it in self.items:
+            N = it.img_pars.image_size
+            z = np.zeros(N * N * (8 if self.prec == capi.PRECISION_FP64 else 4), dtype=np.uint8)
+            self.gpu.htod(it.d_image, z)
+            self.gpu.htod(it.d_beam, z)
+
+    def free(self):
+        for b in self.bufs:
+            for x in b:
+                self.gpu.free(x)
+
+
+SPECS = [
+    (512, 2.0, 20_000, 5, 701, None, 0.0),
+    (1024, 1.0, 120_000, 12, 702, None, 0.0),
+    (256, 4.0, 5_000, 3, 703, None, 0.0),
+    (1024, 1.0, 60_000, 30, 704, None, 0.0),
+    (256, 20.0, 4_000, 6, 705, "w", 300.0),
+    (2048, 1.0, 300_000, 40, 706, None, 0.0),
+]
+
+
+def _specs():
+    return [(N, c, nv, ns, sd, capi.wproj_pars(4, 8) if wp else None, wm) for N, c, nv, ns, sd, wp, wm in SPECS]
+
+
+@pytest.mark.parametrize("precision", [capi.PRECISION_FP64, capi.PRECISION_FP32C])
+def test_batch_matches_single_and_graph_replays(gpu, orc, precision):
+    b = _Batch(gpu, _specs(), precision)
+    try:
+        tol = 1e-12 if precision == capi.PRECISION_FP64 else 1e-6
+        # each snapshot alone through the single-snapshot entry point
+        single = []
+        for it in b.items:
+            gpu.image_visibilities_device(it.img_pars, it.n_vis, it.d_vis, it.d_vis_weights, it.d_uvw_lambda,
+                                          it.d_image, it.d_beam, precision=precision, synchronize=True,
+                                          w_proj=it.w_proj if it.w_proj.num_wplanes else None)
+        single = b.images()
+        # the oracle for two of them
+        for k in (1, 4):
+            it, s = b.items[k], b.snaps[k]
+            oimg, obeam, _, _ = orc.image_visibilities(it.img_pars, s["vis"], s["weights"], s["uvw"],
+                                                       wproj=it.w_proj if it.w_proj.num_wplanes else None)
+            assert peak_rel_err(single[k][0], oimg) < (1e-10 if precision == capi.PRECISION_FP64 else 1e-4)
+        # eager batch on 3 concurrent streams
+        b.clear()
+        totals = gpu.image_batch(b.items, precision=precision, max_concurrent=3)
+        eager = b.images()
+        for (a, ab), (c, cb) in zip(single, eager):
+            assert peak_rel_err(c, a) <= tol and peak_rel_err(cb, ab) <= tol
+        assert np.all(totals > 0)
+        # graph: first call records, later calls replay; host work and syncs in between
+        for rep in range(4):
+            b.clear()
+            gpu.image_batch(b.items, precision=precision, max_concurrent=3, flags=capi.BATCH_GRAPH)
+            got = b.images()
+            for (a, ab), (c, cb) in zip(eager, got):
+                assert peak_rel_err(c, a) <= tol and peak_rel_err(cb, ab) <= tol, rep
+    finally:
+        b.free()
+
+
+def test_autotune(gpu, orc, tmp_path):
+    """Autotuning picks a tile width and an FFT split per shape, records them in the
+    tune file, and the tuned configuration still matches the oracle."""
+    tf = str(tmp_path / "tune.txt")
+    gpu.set_autotune(1, tf)
+    try:
+        snap = simdata.workload_snapshot("c1")
+        pars = capi.imager_pars(1024, 1.0)
+        img, beam = gpu.image_visibilities(snap["vis"], snap["weights"], snap["uvw"], pars)
+        tcw, l1, us = gpu.get_tuning(pars, snap["vis"].size)
+        assert tcw in (16, 32, 64) and us > 0
+        assert open(tf).read().split()[1] == str(tcw)
+        oimg, obeam, _, _ = orc.image_visibilities(pars, snap["vis"], snap["weights"], snap["uvw"])
+        assert peak_rel_err(img, oimg) < 1e-10 and peak_rel_err(beam, obeam) < 1e-10
+        # every candidate configuration keeps the parity: force each through a tune file
+        key = open(tf).read().split()[0]
+        for t, split in ((16, 0), (32, 32), (64, 128)):
+            forced = str(tmp_path / f"forced_{t}.txt")
+            with open(forced, "w") as f:
+                f.write(f"{key} {t} {split} 1.0\n")
+            gpu.set_autotune(0, forced)
+            assert gpu.get_tuning(pars, snap["vis"].size)[:2] == (t, split)
+            img, beam = gpu.image_visibilities(snap["vis"], snap["weights"], snap["uvw"], pars)
+            assert peak_rel_err(img, oimg) < 1e-10 and peak_rel_err(beam, obeam) < 1e-10, t
+    finally:
+        gpu.set_autotune(0, None)
