@@ -39,7 +39,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "c5"],
+                    help="c1/c2: one snapshot per step (headline: c2); c4: 48-snapshot graph-replayed batches; "
+                         "c5: a served Poisson request stream")
+    ap.add_argument("--snapshots", type=int, default=48, help="c4: snapshots per batch")
+    ap.add_argument("--requests", type=int, default=60, help="c5: requests in the stream")
+    ap.add_argument("--slots", type=int, default=4, help="c5: server slots")
     ap.add_argument("--precision", default="all", choices=["all", "both", "fp64", "fp32c", "fp32tf"],
                     help="paths to run (all: fp64, fp32c and fp32tf; both: fp64 and fp32c); "
                          "the headline is fp32c when it runs")
@@ -163,12 +168,156 @@ def measure(ctx, capi, args, pars, sfp, prec, n, N, bufs, do_sf):
     }
 
 
+def bench_batch(args):
+    """Workload c4: 48 snapshots (N in {2048, 4096}, 2e4-2e6 visibilities) imaged per
+    call of stp_image_batch, FP32 path, autotuned shapes, captured once as a hipGraph
+    and replayed; compared with the same batch on the FP64 path run eagerly."""
+    from fastimaging_amd import capi, simdata
+    from fastimaging_amd.stp import Context
+    specs = simdata.batch_specs(args.snapshots)
+    t0 = time.time()
+    snaps = [simdata.snapshot(**sp) for sp in specs]
+    gen_s = time.time() - t0
+    ctx = Context(0)
+    items, bufs = [], []
+    for sp, sn in zip(specs, snaps):
+        N, nv = sp["image_size"], sp["n_vis"]
+        b = [ctx.malloc(16 * nv), ctx.malloc(8 * nv), ctx.malloc(24 * nv), ctx.malloc(8 * N * N), ctx.malloc(8 * N * N)]
+        ctx.htod(b[0], np.ascontiguousarray(sn["vis"]))
+        ctx.htod(b[1], np.ascontiguousarray(sn["weights"]))
+        ctx.htod(b[2], np.asfortranarray(sn["uvw"]))
+        it = capi.BatchItem()
+        it.img_pars = capi.imager_pars(N, sp["cell_arcsec"])
+        it.n_vis = nv
+        it.d_vis, it.d_vis_weights, it.d_uvw_lambda, it.d_image, it.d_beam = [x.value for x in b]
+        items.append(it)
+        bufs.append(b)
+    out = {}
+    for name, prec, flags in (("fp32c_graph", capi.PRECISION_FP32C, capi.BATCH_GRAPH | capi.BATCH_AUTOTUNE),
+                              ("fp64_eager", capi.PRECISION_FP64, 0)):
+        t1 = time.perf_counter()
+        ctx.image_batch(items, precision=prec, flags=flags)  # tunes and records on the first call
+        first = time.perf_counter() - t1
+        ctx.image_batch(items, precision=prec, flags=flags)
+        reps = max(1, args.steps // 10)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            ctx.image_batch(items, precision=prec, flags=flags)
+        dt = (time.perf_counter() - t1) / reps
+        out[name] = {"snapshots_per_s": round(len(items) / dt, 2), "ms_per_batch": round(dt * 1000, 3),
+                     "first_call_s": round(first, 3), "batches_timed": reps}
+    # accuracy of the FP32 batch against the FP64 oracle on the smallest snapshots
+    acc = []
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+        ctx.image_batch(items, precision=capi.PRECISION_FP32C, flags=capi.BATCH_GRAPH)
+        order = sorted(range(len(items)), key=lambda k: specs[k]["image_size"] ** 2 + 50 * specs[k]["n_vis"])[:3]
+        for k in order:
+            N = specs[k]["image_size"]
+            img = np.empty((N, N), order="F", dtype=np.float32)
+            ctx.dtoh(img, items[k].d_image)
+            oimg, _, _, _ = oracle.image_visibilities(items[k].img_pars, snaps[k]["vis"], snaps[k]["weights"],
+                                                      snaps[k]["uvw"])
+            acc.append({"snapshot": k, "N": N, "n_vis": specs[k]["n_vis"],
+                        "image_err": float(np.max(np.abs(img - oimg)) / np.max(np.abs(oimg)))})
+    res = {"metric": "snapshots/s (48-snapshot batch, hipGraph replay)", "value": out["fp32c_graph"]["snapshots_per_s"],
+           "unit": "snapshots/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+           "data": "synthetic (deterministic SplitMix64 snapshots, inputs resident in HBM)",
+           "config": {"workload": f"c4: {len(items)} snapshots, N in {{2048, 4096}}, 2e4-2e6 vis (total "
+                                  f"{sum(sp['n_vis'] for sp in specs)}), GaussianSinc s3 os9, beam, imaging only",
+                      "n_snapshots": len(items)},
+           "paths": out,
+           "speedup_fp32c_graph_vs_fp64_eager": round(out["fp32c_graph"]["snapshots_per_s"] /
+                                                      out["fp64_eager"]["snapshots_per_s"], 3),
+           "accuracy_vs_oracle": acc, "gen_s": round(gen_s, 2)}
+    print(json.dumps(res), flush=True)
+    for b in bufs:
+        for x in b:
+            ctx.free(x)
+    ctx.close()
+
+
+def bench_serve(args):
+    """Workload c5: a served stream of imaging + source-finding requests (N in
+    {1024..8192}, 1e4-4e6 visibilities, 30 % W-projection) arriving as a Poisson
+    process at ~75 % of the server's measured capacity; reports latency
+    percentiles (submit to result, host buffers in and out) and throughput."""
+    import threading
+    from fastimaging_amd import capi, simdata
+    from fastimaging_amd.serve import Server
+    specs = simdata.serve_specs()
+    t0 = time.time()
+    pool = []
+    for sp in specs:
+        kw = {k: v for k, v in sp.items() if k != "wproj"}
+        sn = simdata.snapshot(**kw)
+        pars = capi.imager_pars(sp["image_size"], sp["cell_arcsec"])
+        wp = capi.wproj_pars(32, 32) if sp["wproj"] else None
+        pool.append((sn, pars, wp))
+    gen_s = time.time() - t0
+    sfp = capi.sourcefind_pars(5.0, 3.0, median_method=capi.MEDIAN_BINMEDIAN)
+    prec = capi.PRECISION_FP32C
+
+    def submit(srv, k):
+        sn, pars, wp = pool[k % len(pool)]
+        return srv.submit(sn["vis"], sn["weights"], sn["uvw"], pars, w_proj=wp, precision=prec, sourcefind=sfp,
+                          want_image=False, want_beam=False, island_cap=1 << 18)
+
+    with Server(num_slots=args.slots) as srv:
+        # warm every slot on every request shape, then measure capacity with a closed loop
+        for r in [srv.wait(submit(srv, k)) for k in range(len(pool) * 2)]:
+            pass
+        t1 = time.perf_counter()
+        ids = [submit(srv, k) for k in range(len(pool) * 2)]
+        for i in ids:
+            srv.wait(i)
+        capacity = len(ids) / (time.perf_counter() - t1)
+        lam = 0.75 * capacity
+        rng = np.random.default_rng(20260)
+        gaps = rng.exponential(1.0 / lam, args.requests)
+        ids = [None] * args.requests
+
+        def producer():
+            t = time.perf_counter()
+            for k in range(args.requests):
+                t += gaps[k]
+                d = t - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids[k] = submit(srv, k)
+
+        th = threading.Thread(target=producer)
+        t1 = time.perf_counter()
+        th.start()
+        th.join()
+        res = [srv.wait(i) for i in ids]
+        wall = time.perf_counter() - t1
+    lat = np.array([(r["t_end"] - r["t_submit"]) * 1000.0 for r in res])
+    svc = np.array([(r["t_end"] - r["t_start"]) * 1000.0 for r in res])
+    out = {"metric": "served snapshots/s + p50/p99 latency (Poisson arrivals)", "value": round(len(res) / wall, 3),
+           "unit": "snapshots/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+           "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+           "service_p50_ms": round(float(np.percentile(svc, 50)), 3),
+           "arrival_rate_per_s": round(lam, 3), "capacity_per_s": round(capacity, 3),
+           "data": "synthetic (deterministic SplitMix64 request pool, host buffers in and out)",
+           "config": {"workload": f"c5: {len(pool)} request shapes cycled, N in {{1024..8192}}, 1e4-4e6 vis, 30 % "
+                                  f"W-projection (32 planes, support 32), FP32 path, source finding, "
+                                  f"{args.slots} slots", "requests": args.requests, "slots": args.slots},
+           "gen_s": round(gen_s, 2)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "c4":
+        return bench_batch(args)
+    if args.workload == "c5":
+        return bench_serve(args)
 
     from fastimaging_amd import capi, simdata
     from fastimaging_amd.stp import Context

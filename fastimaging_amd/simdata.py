@@ -107,3 +107,37 @@ def workload_snapshot(name, snapshot_index=0, n_vis=None, image_size=None):
     if image_size is not None:
         wl["image_size"] = image_size
     return snapshot(seed=seed_for(cfg, snapshot_index), **wl)
+
+
+def batch_specs(n_snapshots=48, config=3):
+    """Workload c4: per snapshot N in {2048, 4096}, cell 1", n_vis log-uniform in
+    [2e4, 2e6], 1-40 point sources, w = 0.  Returns a list of snapshot() kwargs."""
+    st = Stream(seed_for(config, 999))
+    u = st.uniform(3 * n_snapshots).reshape(n_snapshots, 3)
+    out = []
+    for k in range(n_snapshots):
+        N = 2048 if u[k, 0] < 0.5 else 4096
+        nv = int(round(np.exp(np.log(2e4) + u[k, 1] * (np.log(2e6) - np.log(2e4)))))
+        ns = 1 + int(u[k, 2] * 40) if u[k, 2] < 1.0 else 40
+        out.append(dict(image_size=N, cell_arcsec=1.0, n_vis=nv, n_sources=min(ns, 40), seed=seed_for(config, k),
+                        amp_range=(1.0, 10.0), log_amp=False, field_frac=0.25, w_max=0.0))
+    return out
+
+
+def serve_specs(n_requests=12, config=4, max_vis=4_000_000):
+    """Workload c5: N in {1024, 2048, 4096, 8192}, n_vis log-uniform in [1e4, max_vis],
+    30 % W-projection requests (|w| <= 1500, 32 planes, support <= 32, cell 2"),
+    the others cell 1" and w = 0.  Returns snapshot() kwargs plus 'wproj' (bool)."""
+    st = Stream(seed_for(config, 999))
+    u = st.uniform(4 * n_requests).reshape(n_requests, 4)
+    out = []
+    for k in range(n_requests):
+        N = (1024, 2048, 4096, 8192)[min(3, int(u[k, 0] * 4))]
+        nv = int(round(np.exp(np.log(1e4) + u[k, 1] * (np.log(max_vis) - np.log(1e4)))))
+        w = u[k, 2] < 0.3
+        ns = 1 + min(39, int(u[k, 3] * 40))
+        out.append(dict(image_size=N, cell_arcsec=2.0 if w else 1.0, n_vis=nv, n_sources=ns, seed=seed_for(config, k),
+                        amp_range=(1.0, 10.0), log_amp=False, field_frac=0.5 if w else 0.25,
+                        w_max=1500.0 if w else 0.0, wproj=w))
+    return out
+
