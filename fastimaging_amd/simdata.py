@@ -49,8 +49,13 @@ class Stream:
 
 
 def snapshot(image_size, cell_arcsec, n_vis, n_sources, seed, amp_range=(1.0, 10.0), log_amp=False,
-             field_frac=0.25, w_max=0.0, noise_sigma=0.1, chunk=1 << 18):
-    """Returns dict(uvw (n,3) F-order, vis complex128 (n,), weights (n,), sources list of (A, l, m))."""
+             field_frac=0.25, w_max=0.0, noise_sigma=0.1, chunk=1 << 18, reference_model=False):
+    """Returns dict(uvw (n,3) F-order, vis complex128 (n,), weights (n,), sources list of (A, l, m)).
+
+    Visibilities follow the benchmark recipe V = sum A exp(-2 pi i (u l + v m + w (n - 1))).
+    reference_model=True uses instead the reference's own point-source model
+    (src/stp/visibility/visibility.cpp:15-28), V = sum A n exp(+2 pi i (u l + v m + w (n - 1))),
+    the sign convention its W-kernels (aw_projection.cpp:455-474) undo."""
     st = Stream(seed)
     cell = cell_arcsec * ARCSEC
     # sources
@@ -79,7 +84,10 @@ def snapshot(image_size, cell_arcsec, n_vis, n_sources, seed, amp_range=(1.0, 10
         ph = u * l[None, :] + v * m[None, :]
         if w_max > 0.0:
             ph = ph + uvw[b:e, 2][:, None] * nterm[None, :]
-        vis[b:e] = (a[None, :] * np.exp(-2j * math.pi * ph)).sum(axis=1)
+        if reference_model:
+            vis[b:e] = ((a * (nterm + 1.0))[None, :] * np.exp(2j * math.pi * ph)).sum(axis=1)
+        else:
+            vis[b:e] = (a[None, :] * np.exp(-2j * math.pi * ph)).sum(axis=1)
     vis += noise_sigma * (st.normal(n_vis) + 1j * st.normal(n_vis))
     return dict(uvw=uvw, vis=vis, weights=np.ones(n_vis), sources=list(zip(a, l, m)),
                 image_size=image_size, cell_arcsec=cell_arcsec)

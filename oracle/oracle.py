@@ -148,6 +148,17 @@ def wproj_kernel_cache(w_value, aa, workarea, cell_rad, oversampling, scaling, s
     return np.ascontiguousarray(np.swapaxes(out, 2, 3))
 
 
+def average_w_planes(abs_w_sorted, good, planes, median=False):
+    """average_w_planes (gridder.cpp:68-146) on |w| already in ascending order."""
+    w = _f64(abs_w_sorted)
+    g = np.ascontiguousarray(good, dtype=np.uint32)
+    avg = np.empty(planes)
+    first = np.empty(planes, dtype=np.uint64)
+    _check(lib().oracle_average_w_planes(_ptr(w), _ptr(g), ctypes.c_size_t(w.size), ctypes.c_uint(planes),
+                                         ctypes.c_int(int(median)), _ptr(avg), _ptr(first)))
+    return avg, first
+
+
 def image_visibilities(pars, vis, weights, uvw, wproj=None, nthreads=None, kernel=None):
     """image_visibilities (src/stp/imager/imager.h:233-376). Returns (image, beam, total, ngood)."""
     uvw = np.asfortranarray(uvw, dtype=np.float64)

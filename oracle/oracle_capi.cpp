@@ -127,6 +127,22 @@ int oracle_wproj_kernel_cache(double w_value, const double* aa, unsigned workare
     });
 }
 
+int oracle_average_w_planes(const double* abs_w, const uint32_t* good, size_t n, unsigned planes, int median, double* w_avg,
+    uint64_t* first_idx)
+{
+    return guard([&] {
+        std::vector<double> w(abs_w, abs_w + n);
+        std::vector<uint32_t> g(good, good + n);
+        std::vector<double> avg;
+        std::vector<size_t> first;
+        average_w_planes(w, g, planes, avg, first, median != 0);
+        for (unsigned p = 0; p < planes; p++) {
+            w_avg[p] = avg[p];
+            first_idx[p] = first[p];
+        }
+    });
+}
+
 // Returns 0 with *empty = 1 when sample_grid_total <= 0 (outputs untouched).
 int oracle_image_visibilities(const stp_imager_pars* p, const stp_wproj_pars* wp, const stp_kernel_spec* k, size_t n, const double* vis,
     const double* weights, const double* uvw, int nthreads, double* image, double* beam, double* total, uint64_t* ngood, int* empty)
