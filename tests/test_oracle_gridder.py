@@ -128,6 +128,39 @@ def test_gaussiansinc_closed_form(orc):
     assert np.allclose(got, exp, atol=TOL, rtol=0)
 
 
+def test_gaussian_sinc_functors_with_widths(orc):
+    """Gaussian(trunc, w) = exp(-(x/w)^2) and Sinc(trunc, w) = sin(pi x / w)/(pi x / w)
+    (conv_func.cpp:44-82), zero beyond trunc, with non-default widths."""
+    x = np.linspace(-3.0, 3.0, 25)
+    g = orc.kernel_eval(capi.KERNEL_GAUSSIAN, 2.5, x, widths=(1.7, 1.0))
+    assert np.allclose(g, np.where(np.abs(x) > 2.5, 0.0, np.exp(-(x / 1.7) ** 2)), atol=1e-15, rtol=0)
+    sn = orc.kernel_eval(capi.KERNEL_SINC, 2.5, x, widths=(1.0, 1.3))
+    assert np.allclose(sn, np.where(np.abs(x) > 2.5, 0.0, np.sinc(x / 1.3)), atol=1e-15, rtol=0)
+    # GaussianSinc is their product with its own default widths (conv_func.h:216-217)
+    gs = orc.kernel_eval(capi.KERNEL_GAUSSIANSINC, 3.0, x)
+    assert np.allclose(gs, np.where(np.abs(x) > 3.0, 0.0, np.exp(-(x / 2.52) ** 2) * np.sinc(x / 1.55)),
+                       atol=1e-15, rtol=0)
+
+
+def test_pswf_properties(orc):
+    """PSWF(trunc) = psi(x/trunc) (1 - (x/trunc)^2) with psi Schwab's rational spheroidal
+    approximation (conv_func.cpp:112-196): even, psi(0) = 1 (the published
+    normalisation, to the approximation's 1e-6), continuous where the two rational pieces
+    meet (|nu| = 0.75), decreasing on [0, 1], zero at and beyond trunc."""
+    trunc = 3.0
+    nu = np.linspace(0.0, 1.0, 401)
+    k = orc.kernel_eval(capi.KERNEL_PSWF, trunc, nu * trunc)
+    km = orc.kernel_eval(capi.KERNEL_PSWF, trunc, -nu * trunc)
+    assert np.array_equal(k, km)
+    assert abs(k[0] - 1.0) < 1e-6
+    assert np.all(np.diff(k) <= 1e-12)
+    assert abs(k[-1]) < 1e-12
+    eps = 1e-9
+    a, b = orc.kernel_eval(capi.KERNEL_PSWF, trunc, np.array([(0.75 - eps) * trunc, (0.75 + eps) * trunc]))
+    assert abs(a - b) < 1e-6
+    assert np.all(orc.kernel_eval(capi.KERNEL_PSWF, trunc, np.array([3.1, 4.0, -5.0])) == 0.0)
+
+
 def test_tophat_triangle_closed_forms(orc):
     """conv_test_tophat.cpp / conv_test_triangle.cpp: piecewise definitions at break points."""
     x = np.array([-2.0, -1.5, -1.0, -0.5, 0.0, 0.5, 1.0, 1.5, 2.0])
