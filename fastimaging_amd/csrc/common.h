@@ -101,6 +101,8 @@ namespace stp_hip {
 
 // Returns a device buffer of at least `bytes` for slot `name` (contents undefined).
 void* ws_get(stp_ctx* ctx, const std::string& name, size_t bytes);
+// Bytes held by workspace slot `name` (0 when it has none).
+size_t ws_bytes(stp_ctx* ctx, const std::string& name);
 // Destroys the context's captured batch graphs (their buffers are about to change).
 void invalidate_graphs(stp_ctx* ctx);
 // Returns a pinned host buffer of at least `bytes` for slot `name` (contents undefined).

@@ -28,6 +28,12 @@ void* ws_get(stp_ctx* ctx, const std::string& name, size_t bytes)
     return b.ptr;
 }
 
+size_t ws_bytes(stp_ctx* ctx, const std::string& name)
+{
+    auto it = ctx->ws.find(ctx->ws_slot ? name + "#" + std::to_string(ctx->ws_slot) : name);
+    return it == ctx->ws.end() ? 0 : it->second.bytes;
+}
+
 void invalidate_graphs(stp_ctx* ctx)
 {
     for (auto& kv : ctx->graphs)

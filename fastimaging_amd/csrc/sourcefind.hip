@@ -9,13 +9,15 @@
 //              radix-select fallback for degenerate data;
 //  RMS         estimate_rms sigma clipping (sourcefind.cpp:22-154), device-side
 //              iteration state, no host round trips;
-//  labelling   union-find over the visual (fft-shifted) image with the
+//  labelling   union-find over the visual (fft-shifted) image by 64 x 64 tiles
+//              (LDS inside a tile, global across tile borders) with the
 //              reference's ids: rank of each component's first pixel in the
 //              scan (visual column, memory row) order (ccl.h:107-484,505-773);
-//  islands     extrema (strict compare from the detection threshold, ties to the
-//              smallest memory index), detection validity, label-map cleaning,
-//              moments and counts (sourcefind.cpp:342-628); the island list and
-//              the moments fit (:235-278, :691-717) are assembled on the host.
+//  islands     counts, extrema (ties to the smallest memory index) and moments
+//              summed per tile component in LDS and combined across tiles,
+//              detection validity, label-map cleaning (sourcefind.cpp:342-628);
+//              the island list and the moments fit (:235-278, :691-717) are
+//              assembled on the device.
 #include <algorithm>
 #include <cfloat>
 #include <climits>
@@ -55,6 +57,8 @@ struct SFState {
     unsigned int num_valid, num_islands;     // detections; islands after the area cut
     int fallback;                            // exact median needs the radix selection
     int pad2;
+    unsigned int num_records;                // tile components of the labelling (record capacity check)
+    unsigned int pad3;
 };
 
 template <typename T>
@@ -626,52 +630,133 @@ __global__ void sf_thresholds(SFState* st, int estimate, double rms_in, double d
 }
 
 // ------------------------------------------------------------------ labelling
-// key of memory pixel (r, mc): visual column cv = (mc + cols/2) % cols; key = cv * rows + r
-template <typename T>
-__global__ __launch_bounds__(SWG) void ccl_classify(const T* x, int rows, int cols, const SFState* st, int find_neg,
-    uint8_t* cls, uint32_t* P)
+// Connected components (ccl.h:107-484) by tiles.  The visual (fft-shifted)
+// image is cut into tiles of CT_W visual columns x CT_H visual rows; tile rows
+// are aligned so that no tile straddles visual row R/2, where the memory row
+// wraps, so inside a tile the key (visual column * R + memory row) grows with
+// the local index c * CT_H + rr.  A component's root is its smallest key, i.e.
+// its first pixel in the reference's scan order (visual column, memory row).
+//
+//  ccl_tile        classification; vertical runs from wave ballots (a wave holds
+//                  one tile column); union-find of the runs in LDS, one union per
+//                  pair of touching runs; the island sums of every component of
+//                  the tile (count, extremum, its smallest memory index, moments)
+//                  in LDS; a component with a same-class neighbour outside the
+//                  tile is "open";
+//  ccl_border      union-find across tile borders, on the open components only;
+//  ccl_open_roots  flattens open components, marks their global roots;
+//  ccl_records     one record per tile component: closed ones are final, open
+//                  ones are combined into their global root with atomics.
+// Label ids are ranks of roots in key order: a 64-bit root mask per segment (one
+// tile's rows of one column) and a scan of the segment counts give every root's
+// id without a pass over the image.
+constexpr int CT_W = 64, CT_H = 64, CT_PIX = CT_W * CT_H, CT_THREADS = 1024, CT_WAVES = CT_THREADS / 64;
+constexpr int CT_PPT = CT_PIX / CT_THREADS;  // pixels per thread: columns w + CT_WAVES j, row = lane
+constexpr int CT_RCAP = 480;  // components of a tile summed in LDS; more go to global atomics
+constexpr uint32_t P_BG = 0xFFFFFFFFu, P_OPEN = 0x80000000u, P_NEG = 0x40000000u, P_KEY = 0x3FFFFFFFu;
+constexpr uint32_t L_KEY = 0xFFFu, L_OPEN = 0x1000u;
+
+struct TileGeom {
+    int R, C, half_r, half_c;
+    int base;    // visual row where tile row 0 would start (<= 0): tile rows are aligned to R/2
+    int nty, ntx;
+    int ty_mid;  // tile row starting at visual row R/2: key order starts there
+    unsigned nseg;
+};
+
+TileGeom tile_geom(int rows, int cols)
 {
-    const size_t n = size_t(rows) * cols;
-    const size_t m = size_t(blockIdx.x) * SWG + threadIdx.x;
-    if (m >= n)
-        return;
-    const int r = int(m % rows), mc = int(m / rows);
-    const int cv = (mc + cols / 2) % cols;
-    const size_t k = size_t(cv) * rows + r;
-    const double v = ldv(x, m);
-    uint8_t c = 0;
-    if (v > st->a_pos)
-        c = 1;
-    else if (find_neg && v < st->a_neg)
-        c = 2;
-    cls[k] = c;
-    if (c)
-        P[k] = uint32_t(k);
+    TileGeom g{};
+    g.R = rows;
+    g.C = cols;
+    g.half_r = rows / 2;
+    g.half_c = cols / 2;
+    const int off = g.half_r % CT_H;
+    g.base = off ? off - CT_H : 0;
+    g.nty = (rows - g.base + CT_H - 1) / CT_H;
+    g.ntx = (cols + CT_W - 1) / CT_W;
+    g.ty_mid = (g.half_r - g.base) / CT_H;
+    g.nseg = unsigned(cols) * unsigned(g.nty);
+    return g;
 }
 
-__device__ __forceinline__ uint32_t uf_find(const uint32_t* P, uint32_t x)
+__device__ __forceinline__ int tile_vr0(const TileGeom& g, int ty) { return max(0, g.base + CT_H * ty); }
+__device__ __forceinline__ int tile_vr1(const TileGeom& g, int ty) { return min(g.R, g.base + CT_H * (ty + 1)); }
+// visual row <-> memory row (the same map both ways, R even)
+__device__ __forceinline__ int flip_row(const TileGeom& g, int r) { return r >= g.half_r ? r - g.half_r : r + g.half_r; }
+__device__ __forceinline__ int mem_col(const TileGeom& g, int cv) { return cv >= g.half_c ? cv - g.half_c : cv + g.half_c; }
+// segment (visual column cv, tile row ty) in key order
+__device__ __forceinline__ unsigned seg_index(const TileGeom& g, int cv, int ty)
 {
-    uint32_t p = P[x];
-    while (p != x) {
-        x = p;
-        p = P[x];
+    const int pos = ty >= g.ty_mid ? ty - g.ty_mid : ty + (g.nty - g.ty_mid);
+    return unsigned(cv) * unsigned(g.nty) + unsigned(pos);
+}
+
+// Sums of one island (or of one tile component of it).
+struct IslandSum {
+    unsigned long long ext;  // order key of the extremum
+    uint32_t idx;            // smallest memory index holding the extremum
+    int cnt;
+    double mom[6];           // sum x v, y v, x x v, y y v, x y v, v
+};
+
+struct SegIds {
+    TileGeom g;
+    const unsigned long long* mask;  // roots per segment: [0, nseg) positive, [nseg, 2 nseg) negative
+    const uint32_t* offs;            // exclusive scan of the popcounts of mask, 2 nseg + 1 entries
+};
+
+// Position of a root in the island list: positive ids 0..kp-1, then negative ones.
+__device__ __forceinline__ uint32_t gid_at(const SegIds& s, unsigned sseg, int rr)
+{
+    return s.offs[sseg] + uint32_t(__popcll(s.mask[sseg] & ((1ull << rr) - 1ull)));
+}
+__device__ __forceinline__ unsigned seg_of_key(const TileGeom& g, uint32_t key, bool neg, int& rr)
+{
+    const int cv = int(key / unsigned(g.R)), vr = flip_row(g, int(key % unsigned(g.R)));
+    const int ty = (vr - g.base) / CT_H;
+    rr = vr - tile_vr0(g, ty);
+    return seg_index(g, cv, ty) + (neg ? g.nseg : 0u);
+}
+__device__ __forceinline__ uint32_t root_gid(const SegIds& s, uint32_t key, bool neg)
+{
+    int rr;
+    const unsigned sseg = seg_of_key(s.g, key, neg, rr);
+    return gid_at(s, sseg, rr);
+}
+
+// Per-component records written by ccl_tile.
+struct CompRecords {
+    uint32_t* key;   // local root key | P_NEG | P_OPEN
+    uint32_t* pos;   // (signed segment << 6) | row of the local root in it
+    IslandSum* sum;
+    uint32_t cap;
+};
+
+__device__ __forceinline__ uint32_t lds_find(const uint32_t* par, uint32_t a)
+{
+    const volatile uint32_t* vp = par;
+    uint32_t p = vp[a];
+    while (p != a) {
+        a = p;
+        p = vp[a];
     }
-    return x;
+    return a;
 }
 
-// Links the trees of a and b; the smaller key becomes the root (lock-free, atomicMin).
-__device__ void uf_union(uint32_t* P, uint32_t a, uint32_t b)
+// Links the trees of a and b in LDS; the smaller local index (smaller key) becomes the root.
+__device__ void lds_union(uint32_t* par, uint32_t a, uint32_t b)
 {
     bool done;
     do {
-        a = uf_find(P, a);
-        b = uf_find(P, b);
+        a = lds_find(par, a);
+        b = lds_find(par, b);
         if (a < b) {
-            const uint32_t old = atomicMin(&P[b], a);
+            const uint32_t old = atomicMin(&par[b], a);
             done = (old == b);
             b = old;
         } else if (b < a) {
-            const uint32_t old = atomicMin(&P[a], b);
+            const uint32_t old = atomicMin(&par[a], b);
             done = (old == a);
             a = old;
         } else {
@@ -680,219 +765,508 @@ __device__ void uf_union(uint32_t* P, uint32_t a, uint32_t b)
     } while (!done);
 }
 
-__global__ __launch_bounds__(SWG) void ccl_merge(const uint8_t* cls, uint32_t* P, int rows, int cols, int conn4)
+__device__ __forceinline__ uint32_t g_find(const uint32_t* P, uint32_t a)
 {
-    const size_t n = size_t(rows) * cols;
-    const size_t k = size_t(blockIdx.x) * SWG + threadIdx.x;
-    if (k >= n)
-        return;
-    const uint8_t c = cls[k];
-    if (!c)
-        return;
-    const int r = int(k % rows), cv = int(k / rows);
-    const int rshift = rows / 2;
-    const int vr = (r + rshift) % rows;
-    auto key_of = [&](int ncv, int nvr) -> uint32_t { return uint32_t(size_t(ncv) * rows + (nvr + rows - rshift) % rows); };
-    // backward neighbours in the visual image, no wrap: left, up, up-left, down-left
-    if (cv > 0) {
-        const uint32_t nk = key_of(cv - 1, vr);
-        if (cls[nk] == c)
-            uf_union(P, uint32_t(k), nk);
+    const volatile uint32_t* vp = P;
+    uint32_t p = vp[a] & P_KEY;
+    while (p != a) {
+        a = p;
+        p = vp[a] & P_KEY;
     }
-    if (vr > 0) {
-        const uint32_t nk = key_of(cv, vr - 1);
-        if (cls[nk] == c)
-            uf_union(P, uint32_t(k), nk);
-    }
-    if (!conn4 && cv > 0) {
-        if (vr > 0) {
-            const uint32_t nk = key_of(cv - 1, vr - 1);
-            if (cls[nk] == c)
-                uf_union(P, uint32_t(k), nk);
-        }
-        if (vr < rows - 1) {
-            const uint32_t nk = key_of(cv - 1, vr + 1);
-            if (cls[nk] == c)
-                uf_union(P, uint32_t(k), nk);
-        }
-    }
+    return a;
 }
 
-__global__ __launch_bounds__(SWG) void ccl_flatten(const uint8_t* cls, uint32_t* P, size_t n)
+// Global union of two open components of one class (flags: P_OPEN and the class bit).
+__device__ void g_union(uint32_t* P, uint32_t a, uint32_t b, uint32_t flags)
 {
-    const size_t k = size_t(blockIdx.x) * SWG + threadIdx.x;
-    if (k < n && cls[k])
-        P[k] = uf_find(P, uint32_t(k));
+    bool done;
+    do {
+        a = g_find(P, a);
+        b = g_find(P, b);
+        if (a < b) {
+            const uint32_t old = atomicMin(&P[b], a | flags);
+            done = (old == (b | flags));
+            b = old & P_KEY;
+        } else if (b < a) {
+            const uint32_t old = atomicMin(&P[a], b | flags);
+            done = (old == (a | flags));
+            a = old & P_KEY;
+        } else {
+            done = true;
+        }
+    } while (!done);
 }
 
-// roots per block of CANON_BLOCK keys (thread t owns keys [t*16, t*16+16) of the block)
-__global__ __launch_bounds__(SWG) void ccl_root_count(const uint8_t* cls, const uint32_t* P, size_t n, uint32_t* cpos, uint32_t* cneg)
+template <typename T>
+__global__ __launch_bounds__(CT_THREADS) void ccl_tile(const T* __restrict__ x, TileGeom g, const SFState* st, int find_neg,
+    int conn4, uint32_t* __restrict__ P, unsigned long long* __restrict__ rmask, CompRecords rec, uint32_t* rec_count)
 {
-    __shared__ uint32_t sp[SWG], sn[SWG];
-    const size_t base = size_t(blockIdx.x) * CANON_BLOCK + size_t(threadIdx.x) * CANON_PER_LANE;
-    uint32_t p = 0, q = 0;
-    for (int j = 0; j < CANON_PER_LANE; j++) {
-        const size_t k = base + j;
-        if (k < n && cls[k] && P[k] == k) {
-            if (cls[k] == 1)
-                p++;
-            else
-                q++;
+    __shared__ uint32_t s_par[CT_PIX];
+    __shared__ uint8_t s_cls[CT_PIX];
+    __shared__ uint8_t s_fl[CT_H + 2], s_fr[CT_H + 2], s_ft[CT_W], s_fb[CT_W];
+    __shared__ unsigned long long s_mask[CT_W];
+    __shared__ uint32_t s_pre[CT_W + 1];
+    __shared__ uint32_t s_base;
+    __shared__ unsigned long long r_ext[CT_RCAP];
+    __shared__ uint32_t r_idx[CT_RCAP];
+    __shared__ int r_cnt[CT_RCAP];
+    __shared__ double r_mom[6][CT_RCAP];
+
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int tx = int(blockIdx.x) % g.ntx, ty = int(blockIdx.x) / g.ntx;
+    const int cv0 = tx * CT_W, nc = min(CT_W, g.C - cv0);
+    const int vr0 = tile_vr0(g, ty), nr = tile_vr1(g, ty) - vr0;
+    const int rbase = flip_row(g, vr0);  // memory row of tile row 0; tile row rr is memory row rbase + rr
+    const double a_pos = st->a_pos, a_neg = st->a_neg;
+    auto classify = [&](double v) -> uint32_t { return v > a_pos ? 1u : ((find_neg && v < a_neg) ? 2u : 0u); };
+    auto pix = [&](int cv, int vr) -> double { return ldv(x, size_t(mem_col(g, cv)) * size_t(g.R) + size_t(flip_row(g, vr))); };
+    const unsigned long long lt = (1ull << lane) - 1ull;
+
+    // 1. classification and vertical runs: info = class | run start << 2 | run end << 8
+    T vals[CT_PPT];
+    uint32_t info[CT_PPT];
+#pragma unroll
+    for (int j = 0; j < CT_PPT; j++) {
+        const int c = w + CT_WAVES * j, li = c * CT_H + lane;
+        uint32_t cl = 0;
+        T v = T(0);
+        if (c < nc && lane < nr) {
+            v = x[size_t(mem_col(g, cv0 + c)) * size_t(g.R) + size_t(rbase + lane)];
+            cl = classify(double(v));
+        }
+        const unsigned long long m1 = __ballot(cl == 1u), m2 = __ballot(cl == 2u);
+        const unsigned long long M = cl == 2u ? m2 : m1;
+        const unsigned long long zb = ~M & lt, za = ~M & ~(lt | (1ull << lane));
+        const uint32_t rs = zb ? uint32_t(64 - __builtin_clzll(zb)) : 0u;
+        const uint32_t re = za ? uint32_t(__builtin_ctzll(za) - 1) : 63u;
+        vals[j] = v;
+        info[j] = cl | (rs << 2) | (re << 8);
+        s_cls[li] = uint8_t(cl);
+        s_par[li] = uint32_t(c * CT_H) + (cl ? rs : uint32_t(lane));
+    }
+    // classes of the frame around the tile (0 outside the image)
+    for (int q = t; q < 2 * (CT_H + 2) + 2 * CT_W; q += CT_THREADS) {
+        uint32_t cl = 0;
+        if (q < 2 * (CT_H + 2)) {
+            const bool right = q >= CT_H + 2;
+            const int fr = (right ? q - (CT_H + 2) : q) - 1;  // -1 .. CT_H
+            const int cv = right ? cv0 + nc : cv0 - 1, vr = vr0 + fr;
+            if (fr <= nr && cv >= 0 && cv < g.C && vr >= 0 && vr < g.R)
+                cl = classify(pix(cv, vr));
+            (right ? s_fr : s_fl)[fr + 1] = uint8_t(cl);
+        } else {
+            const bool bottom = q >= 2 * (CT_H + 2) + CT_W;
+            const int c = q - 2 * (CT_H + 2) - (bottom ? CT_W : 0);
+            const int vr = bottom ? vr0 + nr : vr0 - 1;
+            if (c < nc && vr >= 0 && vr < g.R)
+                cl = classify(pix(cv0 + c, vr));
+            (bottom ? s_fb : s_ft)[c] = uint8_t(cl);
         }
     }
-    sp[threadIdx.x] = p;
-    sn[threadIdx.x] = q;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t a = 0, b = 0;
-        for (int t = 0; t < SWG; t++) {
-            a += sp[t];
-            b += sn[t];
-        }
-        cpos[blockIdx.x] = a;
-        cneg[blockIdx.x] = b;
-    }
-}
 
-__global__ __launch_bounds__(SWG) void ccl_root_ids(const uint8_t* cls, const uint32_t* P, size_t n, const uint32_t* opos,
-    const uint32_t* oneg, int32_t* ID)
-{
-    __shared__ uint32_t sp[SWG], sn[SWG];
-    const size_t base = size_t(blockIdx.x) * CANON_BLOCK + size_t(threadIdx.x) * CANON_PER_LANE;
-    uint32_t p = 0, q = 0;
-    for (int j = 0; j < CANON_PER_LANE; j++) {
-        const size_t k = base + j;
-        if (k < n && cls[k] && P[k] == k) {
-            if (cls[k] == 1)
-                p++;
-            else
-                q++;
+    // 2. one union per pair of touching runs of neighbouring columns: a run with
+    // rows [s, e] meets the left column's rows [s - 1, e + 1] (8-connectivity) or
+    // [s, e]; each maximal same-class stretch there is linked once, at its first row
+#pragma unroll
+    for (int j = 0; j < CT_PPT; j++) {
+        const uint32_t cl = info[j] & 3u;
+        const int c = w + CT_WAVES * j;
+        if (!cl || c == 0)
+            continue;
+        const int rs = int((info[j] >> 2) & 63u), re = int((info[j] >> 8) & 63u);
+        const uint32_t a = uint32_t(c * CT_H + rs), lb = uint32_t((c - 1) * CT_H);
+        auto lsame = [&](int r) { return r >= 0 && r < nr && s_cls[lb + r] == cl; };
+        if (conn4) {
+            if (lsame(lane) && (lane == rs || !lsame(lane - 1)))
+                lds_union(s_par, a, lb + lane);
+        } else {
+            if (lsame(lane) && !lsame(lane - 1))
+                lds_union(s_par, a, lb + lane);
+            if (lane == rs && lsame(rs - 1))
+                lds_union(s_par, a, lb + rs - 1);
+            if (lane == re && lsame(re + 1) && !lsame(re))
+                lds_union(s_par, a, lb + re + 1);
         }
     }
-    sp[threadIdx.x] = p;
-    sn[threadIdx.x] = q;
     __syncthreads();
-    // exclusive prefix over the threads of the block (serial in LDS by thread 0 would be slow; Hillis-Steele)
-    for (int off = 1; off < SWG; off <<= 1) {
-        const uint32_t a = threadIdx.x >= off ? sp[threadIdx.x - off] : 0u;
-        const uint32_t b = threadIdx.x >= off ? sn[threadIdx.x - off] : 0u;
-        __syncthreads();
-        sp[threadIdx.x] += a;
-        sn[threadIdx.x] += b;
-        __syncthreads();
+    // 3. roots of the run starts (a component's root, its smallest index, starts a run)
+#pragma unroll
+    for (int j = 0; j < CT_PPT; j++) {
+        const int c = w + CT_WAVES * j;
+        if ((info[j] & 3u) && uint32_t(lane) == ((info[j] >> 2) & 63u))
+            s_par[c * CT_H + lane] = lds_find(s_par, c * CT_H + lane);
     }
-    uint32_t rp = opos[blockIdx.x] + sp[threadIdx.x] - p;
-    uint32_t rn = oneg[blockIdx.x] + sn[threadIdx.x] - q;
-    for (int j = 0; j < CANON_PER_LANE; j++) {
-        const size_t k = base + j;
-        if (k < n && cls[k] && P[k] == k) {
-            if (cls[k] == 1)
-                ID[k] = int32_t(++rp);
-            else
-                ID[k] = -int32_t(++rn);
+    __syncthreads();
+    // every pixel points at its root; components with a same-class neighbour across
+    // the tile border are open
+#pragma unroll
+    for (int j = 0; j < CT_PPT; j++) {
+        const uint32_t cl = info[j] & 3u;
+        const int c = w + CT_WAVES * j, li = c * CT_H + lane;
+        if (!cl)
+            continue;
+        const int rs = int((info[j] >> 2) & 63u);
+        const uint32_t root = s_par[c * CT_H + rs] & L_KEY;
+        if (lane != rs)
+            s_par[li] = root;
+        if (!(c == 0 || c == nc - 1 || lane == 0 || lane == nr - 1))
+            continue;
+        bool open = false;
+        if (c == 0)
+            open = open || s_fl[lane + 1] == cl || (!conn4 && (s_fl[lane] == cl || s_fl[lane + 2] == cl));
+        if (c == nc - 1)
+            open = open || s_fr[lane + 1] == cl || (!conn4 && (s_fr[lane] == cl || s_fr[lane + 2] == cl));
+        if (lane == 0)
+            open = open || s_ft[c] == cl || (!conn4 && ((c > 0 && s_ft[c - 1] == cl) || (c + 1 < nc && s_ft[c + 1] == cl)));
+        if (lane == nr - 1)
+            open = open || s_fb[c] == cl || (!conn4 && ((c > 0 && s_fb[c - 1] == cl) || (c + 1 < nc && s_fb[c + 1] == cl)));
+        if (open)
+            atomicOr(&s_par[root], L_OPEN);
+    }
+    __syncthreads();
+
+    // 4. roots, their slots (rank in key order) and the closed roots of every segment
+#pragma unroll
+    for (int j = 0; j < CT_PPT; j++) {
+        const uint32_t cl = info[j] & 3u;
+        const int c = w + CT_WAVES * j, li = c * CT_H + lane;
+        const uint32_t pv = s_par[li];
+        const bool is_root = cl && (pv & L_KEY) == uint32_t(li);
+        const bool closed = is_root && !(pv & L_OPEN);
+        const unsigned long long m_all = __ballot(is_root);
+        const unsigned long long m_pos = __ballot(closed && cl == 1u);
+        const unsigned long long m_neg = __ballot(closed && cl == 2u);
+        if (lane == 0) {
+            s_mask[c] = m_all;
+            if (c < nc) {
+                const unsigned seg = seg_index(g, cv0 + c, ty);
+                rmask[seg] = m_pos;
+                rmask[g.nseg + seg] = m_neg;
+            }
         }
+        info[j] |= uint32_t(is_root) << 14;
+    }
+    __syncthreads();
+    if (w == 0) {
+        const uint32_t cnt = uint32_t(__popcll(s_mask[lane]));
+        uint32_t incl = cnt;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t o = __shfl_up(incl, off);
+            if (lane >= off)
+                incl += o;
+        }
+        s_pre[lane] = incl - cnt;
+        if (lane == 63) {
+            s_pre[64] = incl;
+            s_base = atomicAdd(rec_count, incl);
+        }
+    }
+    __syncthreads();
+    const uint32_t total = s_pre[64], rbase_rec = s_base;
+    const bool ok = size_t(rbase_rec) + size_t(total) <= size_t(rec.cap);
+#pragma unroll
+    for (int j = 0; j < CT_PPT; j++) {
+        if (!((info[j] >> 14) & 1u))
+            continue;
+        const uint32_t cl = info[j] & 3u;
+        const int c = w + CT_WAVES * j, li = c * CT_H + lane;
+        const uint32_t slot = s_pre[c] + uint32_t(__popcll(s_mask[c] & lt));
+        const uint32_t pv = s_par[li];
+        s_par[li] = pv | (slot << 16);
+        const unsigned long long e0 = cl == 1u ? 0ull : ~0ull;
+        if (ok) {
+            const uint32_t k = rbase_rec + slot;
+            const uint32_t gkey = uint32_t(cv0 + c) * uint32_t(g.R) + uint32_t(rbase + lane);
+            rec.key[k] = gkey | (cl == 2u ? P_NEG : 0u) | ((pv & L_OPEN) ? P_OPEN : 0u);
+            rec.pos[k] = ((seg_index(g, cv0 + c, ty) + (cl == 2u ? g.nseg : 0u)) << 6) | uint32_t(lane);
+        }
+        if (slot < uint32_t(CT_RCAP)) {
+            r_ext[slot] = e0;
+            r_cnt[slot] = 0;
+            r_idx[slot] = ~0u;
+#pragma unroll
+            for (int q = 0; q < 6; q++)
+                r_mom[q][slot] = 0.0;
+        } else if (ok) {
+            IslandSum z{};
+            z.ext = e0;
+            z.idx = ~0u;
+            rec.sum[rbase_rec + slot] = z;
+        }
+    }
+    // the barrier's workgroup-scope release orders these stores before the atomics
+    // below: every access of the block meets in the L2 of its XCD
+    __syncthreads();
+
+    // 5. island sums: count, extremum, moments (sourcefind.cpp:337-628)
+#pragma unroll
+    for (int j = 0; j < CT_PPT; j++) {
+        const uint32_t cl = info[j] & 3u;
+        if (!cl)
+            continue;
+        const int c = w + CT_WAVES * j, li = c * CT_H + lane;
+        const uint32_t slot = (s_par[s_par[li] & L_KEY] >> 16) & 0xFFFu;
+        const double val = double(vals[j]);
+        const unsigned long long kv = okey(val);
+        const double x_idx = double(cv0 + c), y_idx = double(vr0 + lane);
+        const double xb = x_idx * val, yb = y_idx * val;
+        const double xx = xb * x_idx, yy = yb * y_idx, xy = xb * y_idx;
+        if (slot < uint32_t(CT_RCAP)) {
+            atomicAdd(&r_cnt[slot], 1);
+            if (cl == 1u)
+                atomicMax(&r_ext[slot], kv);
+            else
+                atomicMin(&r_ext[slot], kv);
+            unsafeAtomicAdd(&r_mom[0][slot], xb);
+            unsafeAtomicAdd(&r_mom[1][slot], yb);
+            unsafeAtomicAdd(&r_mom[2][slot], xx);
+            unsafeAtomicAdd(&r_mom[3][slot], yy);
+            unsafeAtomicAdd(&r_mom[4][slot], xy);
+            unsafeAtomicAdd(&r_mom[5][slot], val);
+        } else if (ok) {
+            IslandSum& s = rec.sum[rbase_rec + slot];
+            atomicAdd(&s.cnt, 1);
+            if (cl == 1u)
+                atomicMax(&s.ext, kv);
+            else
+                atomicMin(&s.ext, kv);
+            unsafeAtomicAdd(&s.mom[0], xb);
+            unsafeAtomicAdd(&s.mom[1], yb);
+            unsafeAtomicAdd(&s.mom[2], xx);
+            unsafeAtomicAdd(&s.mom[3], yy);
+            unsafeAtomicAdd(&s.mom[4], xy);
+            unsafeAtomicAdd(&s.mom[5], val);
+        }
+    }
+    __syncthreads();
+    // smallest memory index holding the extremum
+#pragma unroll
+    for (int j = 0; j < CT_PPT; j++) {
+        const uint32_t cl = info[j] & 3u;
+        if (!cl)
+            continue;
+        const int c = w + CT_WAVES * j, li = c * CT_H + lane;
+        const uint32_t slot = (s_par[s_par[li] & L_KEY] >> 16) & 0xFFFu;
+        const unsigned long long kv = okey(double(vals[j]));
+        const uint32_t m = uint32_t(mem_col(g, cv0 + c)) * uint32_t(g.R) + uint32_t(rbase + lane);
+        if (slot < uint32_t(CT_RCAP)) {
+            if (kv == r_ext[slot])
+                atomicMin(&r_idx[slot], m);
+        } else if (ok) {
+            IslandSum& s = rec.sum[rbase_rec + slot];
+            if (kv == __hip_atomic_load(&s.ext, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMin(&s.idx, m);
+        }
+    }
+    __syncthreads();
+
+    // 6. records of the LDS-summed components; parents of the tile's pixels
+    if (ok) {
+        for (uint32_t s = t; s < min(total, uint32_t(CT_RCAP)); s += CT_THREADS) {
+            IslandSum o;
+            o.ext = r_ext[s];
+            o.idx = r_idx[s];
+            o.cnt = r_cnt[s];
+#pragma unroll
+            for (int q = 0; q < 6; q++)
+                o.mom[q] = r_mom[q][s];
+            rec.sum[rbase_rec + s] = o;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < CT_PPT; j++) {
+        const int c = w + CT_WAVES * j;
+        if (c >= nc || lane >= nr)
+            continue;
+        const uint32_t cl = info[j] & 3u;
+        const uint32_t gk = uint32_t(cv0 + c) * uint32_t(g.R) + uint32_t(rbase + lane);
+        uint32_t out = P_BG;
+        if (cl) {
+            const uint32_t root = s_par[c * CT_H + lane] & L_KEY;
+            const uint32_t rk = uint32_t(cv0 + int(root / CT_H)) * uint32_t(g.R) + uint32_t(rbase + int(root % CT_H));
+            out = rk | (cl == 2u ? P_NEG : 0u) | ((s_par[root] & L_OPEN) ? P_OPEN : 0u);
+        }
+        P[gk] = out;
     }
 }
 
-__global__ __launch_bounds__(SWG) void ccl_label_out(const uint8_t* cls, const uint32_t* P, const int32_t* ID, int rows, int cols,
-    int32_t* lab)
+// Unions across the tile borders: the tile's first column with the column on its
+// left (threads 0..63) and its first row with the row above (threads 64..127).
+__global__ __launch_bounds__(128) void ccl_border(uint32_t* P, TileGeom g, int conn4)
 {
-    const size_t n = size_t(rows) * cols;
-    const size_t m = size_t(blockIdx.x) * SWG + threadIdx.x;
-    if (m >= n)
+    const int t = threadIdx.x;
+    const int tx = int(blockIdx.x) % g.ntx, ty = int(blockIdx.x) / g.ntx;
+    const int cv0 = tx * CT_W, nc = min(CT_W, g.C - cv0);
+    const int vr0 = tile_vr0(g, ty), nr = tile_vr1(g, ty) - vr0;
+    int cv, vr;
+    if (t < 64) {
+        if (t >= nr || cv0 == 0)
+            return;
+        cv = cv0;
+        vr = vr0 + t;
+    } else {
+        if (t - 64 >= nc || vr0 == 0)
+            return;
+        cv = cv0 + (t - 64);
+        vr = vr0;
+    }
+    const uint32_t a = uint32_t(cv) * uint32_t(g.R) + uint32_t(flip_row(g, vr));
+    const uint32_t pa = P[a];
+    if (pa == P_BG)
         return;
-    const int r = int(m % rows), mc = int(m / rows);
-    const size_t k = size_t((mc + cols / 2) % cols) * rows + r;
-    lab[m] = cls[k] ? ID[P[k]] : 0;
+    const uint32_t flags = pa & (P_OPEN | P_NEG);
+    for (int d = -1; d <= 1; d++) {
+        if (conn4 && d != 0)
+            continue;
+        int ncv, nvr;
+        if (t < 64) {
+            ncv = cv0 - 1;
+            nvr = vr + d;
+        } else {
+            ncv = cv + d;
+            nvr = vr0 - 1;
+        }
+        if (ncv < 0 || ncv >= g.C || nvr < 0 || nvr >= g.R)
+            continue;
+        const uint32_t b = uint32_t(ncv) * uint32_t(g.R) + uint32_t(flip_row(g, nvr));
+        const uint32_t pb = P[b];
+        if (pb == P_BG || (pb & P_NEG) != (pa & P_NEG))
+            continue;
+        g_union(P, a, b, flags);
+    }
+}
+
+// Open components: local root -> global root; global roots join the root masks.
+__global__ __launch_bounds__(256) void ccl_open_roots(uint32_t* P, TileGeom g, CompRecords rec, const uint32_t* rec_count,
+    unsigned long long* rmask)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x, nrec = *rec_count;
+    if (nrec > rec.cap || i >= nrec)  // overflowed records are incomplete: the host labels again
+        return;
+    const uint32_t k = rec.key[i];
+    if (!(k & P_OPEN))
+        return;
+    const uint32_t lr = k & P_KEY;
+    const uint32_t gr = g_find(P, lr);
+    P[lr] = gr | (k & (P_OPEN | P_NEG));
+    if (gr == lr) {
+        int rr;
+        const unsigned sseg = seg_of_key(g, lr, k & P_NEG, rr);
+        atomicOr(&rmask[sseg], 1ull << rr);
+    }
+}
+
+__global__ __launch_bounds__(256) void ccl_seg_counts(const unsigned long long* rmask, size_t m, uint32_t* cnt)
+{
+    const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i < m)
+        cnt[i] = uint32_t(__popcll(rmask[i]));
+}
+
+// Open components whose local root is the global root start their island's sums.
+__global__ __launch_bounds__(256) void ccl_open_init(const uint32_t* P, SegIds s, CompRecords rec, const uint32_t* rec_count,
+    IslandSum* isum)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x, nrec = *rec_count;
+    if (nrec > rec.cap || i >= nrec)
+        return;
+    const uint32_t k = rec.key[i];
+    const uint32_t lr = k & P_KEY;
+    if (!(k & P_OPEN) || (P[lr] & P_KEY) != lr)
+        return;
+    const bool neg = k & P_NEG;
+    IslandSum z{};
+    z.ext = neg ? ~0ull : 0ull;
+    z.idx = ~0u;
+    isum[root_gid(s, lr, neg)] = z;
+}
+
+// Island sums by list position: closed components are stored, open ones combined.
+__global__ __launch_bounds__(256) void ccl_records(const uint32_t* P, SegIds s, CompRecords rec, const uint32_t* rec_count,
+    IslandSum* isum)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x, nrec = *rec_count;
+    if (nrec > rec.cap || i >= nrec)
+        return;
+    const uint32_t k = rec.key[i];
+    if (!(k & P_OPEN)) {
+        const uint32_t pos = rec.pos[i];
+        isum[gid_at(s, pos >> 6, int(pos & 63u))] = rec.sum[i];
+        return;
+    }
+    const bool neg = k & P_NEG;
+    IslandSum& d = isum[root_gid(s, P[k & P_KEY] & P_KEY, neg)];
+    const IslandSum& r = rec.sum[i];
+    if (neg)
+        atomicMin(&d.ext, r.ext);
+    else
+        atomicMax(&d.ext, r.ext);
+    atomicAdd(&d.cnt, r.cnt);
+    for (int q = 0; q < 6; q++)
+        unsafeAtomicAdd(&d.mom[q], r.mom[q]);
+}
+
+// Open islands: smallest memory index among the components holding the extremum.
+__global__ __launch_bounds__(256) void ccl_open_idx(const uint32_t* P, SegIds s, CompRecords rec, const uint32_t* rec_count,
+    IslandSum* isum)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x, nrec = *rec_count;
+    if (nrec > rec.cap || i >= nrec)
+        return;
+    const uint32_t k = rec.key[i];
+    if (!(k & P_OPEN))
+        return;
+    IslandSum& d = isum[root_gid(s, P[k & P_KEY] & P_KEY, k & P_NEG)];
+    if (rec.sum[i].ext == d.ext)
+        atomicMin(&d.idx, rec.sum[i].idx);
+}
+
+// Label map in memory order: +-(id + 1), 0 for background and, with gen_map,
+// for islands whose extremum misses the detection threshold (sourcefind.cpp:462-492).
+__global__ __launch_bounds__(CT_THREADS) void ccl_label_map(const uint32_t* __restrict__ P, SegIds s, const SFState* st,
+    const IslandSum* __restrict__ isum, int gen_map, int32_t* __restrict__ lab)
+{
+    const TileGeom& g = s.g;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int tx = int(blockIdx.x) % g.ntx, ty = int(blockIdx.x) / g.ntx;
+    const int cv0 = tx * CT_W, nc = min(CT_W, g.C - cv0);
+    const int vr0 = tile_vr0(g, ty), nr = tile_vr1(g, ty) - vr0;
+    const int rbase = flip_row(g, vr0);
+    if (lane >= nr)
+        return;
+    const double d_pos = st->d_pos, d_neg = st->d_neg;
+    const uint32_t kp = s.offs[g.nseg];
+    uint32_t pv[CT_PPT], root[CT_PPT];
+#pragma unroll
+    for (int j = 0; j < CT_PPT; j++) {
+        const int c = w + CT_WAVES * j;
+        pv[j] = c < nc ? P[uint32_t(cv0 + c) * uint32_t(g.R) + uint32_t(rbase + lane)] : P_BG;
+    }
+#pragma unroll
+    for (int j = 0; j < CT_PPT; j++)
+        root[j] = (pv[j] != P_BG && (pv[j] & P_OPEN)) ? (P[pv[j] & P_KEY] & P_KEY) : (pv[j] & P_KEY);
+#pragma unroll
+    for (int j = 0; j < CT_PPT; j++) {
+        const int c = w + CT_WAVES * j;
+        if (c >= nc)
+            continue;
+        int32_t L = 0;
+        if (pv[j] != P_BG) {
+            const bool neg = pv[j] & P_NEG;
+            const uint32_t gid = root_gid(s, root[j], neg);
+            bool keep = true;
+            if (gen_map) {
+                const double e = okey_inv(isum[gid].ext);
+                keep = neg ? (e < d_neg) : (e > d_pos);
+            }
+            L = keep ? (neg ? -int32_t(gid - kp + 1) : int32_t(gid + 1)) : 0;
+        }
+        lab[size_t(mem_col(g, cv0 + c)) * size_t(g.R) + size_t(rbase + lane)] = L;
+    }
 }
 
 // ------------------------------------------------------------------ islands
-template <typename T>
-__global__ __launch_bounds__(SWG) void sf_extrema(const T* x, const int32_t* lab, size_t n, unsigned long long* ep, unsigned long long* en)
-{
-    const size_t m = size_t(blockIdx.x) * SWG + threadIdx.x;
-    if (m >= n)
-        return;
-    const int32_t l = lab[m];
-    if (l > 0)
-        atomicMax(&ep[l - 1], okey(ldv(x, m)));
-    else if (l < 0)
-        atomicMin(&en[-l - 1], okey(ldv(x, m)));
-}
-
-// Memory index of each island's extremum: the smallest index holding the extremal key.
-template <typename T>
-__global__ __launch_bounds__(SWG) void sf_extrema_idx(const T* x, const int32_t* lab, size_t n, const unsigned long long* ep,
-    const unsigned long long* en, unsigned long long* ip, unsigned long long* in_)
-{
-    const size_t m = size_t(blockIdx.x) * SWG + threadIdx.x;
-    if (m >= n)
-        return;
-    const int32_t l = lab[m];
-    if (l > 0) {
-        if (okey(ldv(x, m)) == ep[l - 1])
-            atomicMin(&ip[l - 1], (unsigned long long)m);
-    } else if (l < 0) {
-        if (okey(ldv(x, m)) == en[-l - 1])
-            atomicMin(&in_[-l - 1], (unsigned long long)m);
-    }
-}
-
-__global__ void sf_init_labels(unsigned long long* ep, unsigned long long* en, unsigned long long* ip, unsigned long long* in_,
-    int* cp, int* cn, double* mp, double* mn, uint32_t kp, uint32_t kn, const SFState* st)
-{
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < kp) {
-        ep[i] = okey(st->d_pos);
-        ip[i] = ~0ull;
-        cp[i] = 0;
-        for (int q = 0; q < 6; q++)
-            mp[6 * i + q] = 0.0;
-    }
-    if (i < kn) {
-        en[i] = okey(st->d_neg);
-        in_[i] = ~0ull;
-        cn[i] = 0;
-        for (int q = 0; q < 6; q++)
-            mn[6 * i + q] = 0.0;
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(SWG) void sf_moments(const T* x, int32_t* lab, int rows, int cols, const SFState* st,
-    const unsigned long long* ep, const unsigned long long* en, int gen_map, int* cp, int* cn, double* mp, double* mn)
-{
-    const size_t n = size_t(rows) * cols;
-    const size_t m = size_t(blockIdx.x) * SWG + threadIdx.x;
-    if (m >= n)
-        return;
-    const int32_t l = lab[m];
-    if (!l)
-        return;
-    // detection validity: extremum strictly beyond the detection threshold (sourcefind.cpp:417-436)
-    const bool valid = (l > 0) ? (okey_inv(ep[l - 1]) > st->d_pos) : (okey_inv(en[-l - 1]) < st->d_neg);
-    if (!valid) {
-        if (gen_map)
-            lab[m] = 0;
-        return;
-    }
-    const int i = int(m / rows), j = int(m % rows);
-    const int h_shift = cols / 2, v_shift = rows / 2;
-    const double val = ldv(x, m);
-    const double y_idx = j < v_shift ? double(j + v_shift) : double(j - v_shift);
-    const double x_idx = i < h_shift ? double(i + h_shift) : double(i - h_shift);
-    const double xb = x_idx * val, yb = y_idx * val;
-    const double xx = xb * x_idx, yy = yb * y_idx, xy = xb * y_idx;
-    double* mm = (l > 0) ? &mp[6 * size_t(l - 1)] : &mn[6 * size_t(-l - 1)];
-    atomicAdd((l > 0) ? &cp[l - 1] : &cn[-l - 1], 1);
-    unsafeAtomicAdd(&mm[0], xb);
-    unsafeAtomicAdd(&mm[1], yb);
-    unsafeAtomicAdd(&mm[2], xx);
-    unsafeAtomicAdd(&mm[3], yy);
-    unsafeAtomicAdd(&mm[4], xy);
-    unsafeAtomicAdd(&mm[5], val);
-}
-
 // IslandParams::estimate_moments_fit (sourcefind.cpp:691-717) and
 // Gaussian2dParams::convert_to_constrained_parameters (fitting.cpp:32-57).
 __device__ void moments_fit(stp_island& isl, double x_bar, double y_bar, double xx_bar, double yy_bar, double xy_bar,
@@ -934,33 +1308,30 @@ __device__ void moments_fit(stp_island& isl, double x_bar, double y_bar, double 
 // labels -1..-kn (the island order of sourcefind.cpp:235-278).
 // valid: extremum beyond the detection threshold (sourcefind.cpp:417-436);
 // keep: valid and at least min_area samples (sourcefind.cpp:240,263).
-__global__ __launch_bounds__(256) void sf_island_flags(const unsigned long long* ep, const unsigned long long* en,
-    const int* cp, const int* cn, uint32_t kp, uint32_t kn, const SFState* st, int min_area, uint32_t* keep, uint32_t* valid)
+__global__ __launch_bounds__(256) void sf_island_flags(const IslandSum* isum, uint32_t kp, uint32_t kn, const SFState* st,
+    int min_area, uint32_t* keep, uint32_t* valid)
 {
     const uint32_t g = blockIdx.x * 256 + threadIdx.x;
     if (g >= kp + kn)
         return;
     const bool pos = g < kp;
-    const uint32_t i = pos ? g : g - kp;
-    const double ext = okey_inv(pos ? ep[i] : en[i]);
+    const double ext = okey_inv(isum[g].ext);
     const bool v = pos ? (ext > st->d_pos) : (ext < st->d_neg);
-    const int c = pos ? cp[i] : cn[i];
     valid[g] = v ? 1u : 0u;
-    keep[g] = (v && c >= min_area) ? 1u : 0u;
+    keep[g] = (v && isum[g].cnt >= min_area) ? 1u : 0u;
 }
 
 // One island record per kept label at its rank among the kept labels.
-__global__ __launch_bounds__(256) void sf_island_emit(const unsigned long long* ep, const unsigned long long* en,
-    const unsigned long long* ip, const unsigned long long* in_, const int* cp, const int* cn, const double* mp,
-    const double* mn, uint32_t kp, uint32_t kn, const uint32_t* keep, const uint32_t* offs, const SFState* st, int rows,
-    int cols, double analysis_n_sigma, stp_island* out)
+__global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, uint32_t kp, uint32_t kn, const uint32_t* keep,
+    const uint32_t* offs, const SFState* st, int rows, int cols, double analysis_n_sigma, stp_island* out)
 {
     const uint32_t g = blockIdx.x * 256 + threadIdx.x;
     if (g >= kp + kn || !keep[g])
         return;
     const bool pos = g < kp;
     const uint32_t i = pos ? g : g - kp;
-    const size_t li = size_t(pos ? ip[i] : in_[i]);
+    const IslandSum& s = isum[g];
+    const size_t li = size_t(s.idx);
     const int r = int(li % size_t(rows)), c = int(li / size_t(rows));
     const int h_shift = cols / 2, v_shift = rows / 2;
     stp_island isl{};
@@ -968,20 +1339,21 @@ __global__ __launch_bounds__(256) void sf_island_emit(const unsigned long long* 
     isl.sign = pos ? 1 : -1;
     isl.extremum_y_idx = r < v_shift ? r + v_shift : r - v_shift;
     isl.extremum_x_idx = c < h_shift ? c + h_shift : c - h_shift;
-    isl.num_samples = pos ? cp[i] : cn[i];
-    isl.extremum_val = okey_inv(pos ? ep[i] : en[i]);
+    isl.num_samples = s.cnt;
+    isl.extremum_val = okey_inv(s.ext);
     // moments normalised by the flux sum (sourcefind.cpp:594-613)
-    const double* m = pos ? &mp[6 * size_t(i)] : &mn[6 * size_t(i)];
+    const double* m = s.mom;
     const double xb = m[0] / m[5], yb = m[1] / m[5];
     moments_fit(isl, xb, yb, m[2] / m[5] - xb * xb, m[3] / m[5] - yb * yb, m[4] / m[5] - xb * yb, st->rms,
         analysis_n_sigma);
     out[offs[g]] = isl;
 }
 
-__global__ void sf_label_counts(SFState* st, const uint32_t* pos_total, const uint32_t* neg_total, const SelState* ss)
+__global__ void sf_label_counts(SFState* st, const uint32_t* segoff, unsigned nseg, const uint32_t* nrec, const SelState* ss)
 {
-    st->num_pos = *pos_total;
-    st->num_neg = *neg_total;
+    st->num_pos = segoff[nseg];
+    st->num_neg = segoff[2 * nseg] - segoff[nseg];
+    st->num_records = *nrec;
     st->fallback = ss->fallback;
 }
 
@@ -1114,7 +1486,7 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
     STP_REQUIRE(rows >= 2 && cols >= 2 && rows % 2 == 0 && cols % 2 == 0, "rows and cols must be even (ccl.h:116-117)");
     STP_REQUIRE(rows <= 65536 && cols <= 65536, "rows and cols must be <= 65536 (ccl.h:118-119)");
     const size_t n = size_t(rows) * size_t(cols);
-    STP_REQUIRE(n < (size_t(1) << 31), "image too large for 32-bit labels");
+    STP_REQUIRE(n < (size_t(1) << 30), "image too large for 30-bit label keys");
     if (p.gaussian_fitting)
         throw Error(STP_ERR_UNSUPPORTED, "Gaussian least-squares fitting is not implemented");
     STP_REQUIRE(p.median_method >= STP_MEDIAN_ZEROMEDIAN && p.median_method <= STP_MEDIAN_NTHELEMENT, "unknown median method");
@@ -1182,30 +1554,40 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
             p.detection_n_sigma, p.analysis_n_sigma);
     };
 
-    // connected components (ccl.h), ids in the reference's scan order
-    uint8_t* cls = wsp<uint8_t>(ctx, "ccl_cls", n);
+    // connected components and island sums by tiles (ccl.h:107-773, sourcefind.cpp:337-628)
+    const TileGeom tg = tile_geom(rows, cols);
+    const unsigned ntiles = unsigned(tg.ntx) * unsigned(tg.nty);
+    const size_t nseg2 = 2 * size_t(tg.nseg);
     uint32_t* P = wsp<uint32_t>(ctx, "ccl_parent", n);
-    int32_t* ID = wsp<int32_t>(ctx, "ccl_id", n);
-    int32_t* lab = d_lab_out ? d_lab_out : wsp<int32_t>(ctx, "ccl_label", n);
-    uint32_t* cpos = wsp<uint32_t>(ctx, "ccl_cpos", nblk);
-    uint32_t* cneg = wsp<uint32_t>(ctx, "ccl_cneg", nblk);
-    uint32_t* opos = wsp<uint32_t>(ctx, "ccl_opos", nblk + 1);
-    uint32_t* oneg = wsp<uint32_t>(ctx, "ccl_oneg", nblk + 1);
-    uint32_t* cur = wsp<uint32_t>(ctx, "ccl_cur", nblk);
+    unsigned long long* rmask = wsp<unsigned long long>(ctx, "ccl_rmask", nseg2);
+    uint32_t* segcnt = wsp<uint32_t>(ctx, "ccl_segcnt", nseg2);
+    uint32_t* segoff = wsp<uint32_t>(ctx, "ccl_segoff", nseg2 + 1);
+    uint32_t* nrec = wsp<uint32_t>(ctx, "ccl_nrec", 1);
+    const int find_neg = int(p.find_negative_sources != 0), conn4 = int(p.ccl_4connectivity != 0);
+    // component records: a tile component needs one; at most one per foreground pixel
+    CompRecords rec{};
+    auto records = [&](size_t want) {
+        constexpr size_t per = sizeof(IslandSum) + 2 * sizeof(uint32_t);
+        const size_t c = std::min(n, std::max(want, ws_bytes(ctx, "ccl_records") / per));
+        char* b = (char*)ws_get(ctx, "ccl_records", c * per);
+        rec.sum = (IslandSum*)b;
+        rec.key = (uint32_t*)(b + sizeof(IslandSum) * c);
+        rec.pos = rec.key + c;
+        rec.cap = uint32_t(c);
+    };
+    records(std::max<size_t>(65536, n / 16));
     auto label = [&]() {
-    launch(ctx, "ccl_classify", ccl_classify<T>, dim3(div_up(n, SWG)), dim3(SWG), 0, x, rows, cols, (const SFState*)st,
-        int(p.find_negative_sources != 0), cls, P);
-    launch(ctx, "ccl_merge", ccl_merge, dim3(div_up(n, SWG)), dim3(SWG), 0, (const uint8_t*)cls, P, rows, cols,
-        int(p.ccl_4connectivity != 0));
-    launch(ctx, "ccl_flatten", ccl_flatten, dim3(div_up(n, SWG)), dim3(SWG), 0, (const uint8_t*)cls, P, n);
-    launch(ctx, "ccl_root_count", ccl_root_count, dim3(unsigned(nblk)), dim3(SWG), 0, (const uint8_t*)cls, (const uint32_t*)P,
-        n, cpos, cneg);
-    exclusive_scan_u32(ctx, cpos, int(nblk), opos, cur);
-    exclusive_scan_u32(ctx, cneg, int(nblk), oneg, cur);
-    launch(ctx, "ccl_root_ids", ccl_root_ids, dim3(unsigned(nblk)), dim3(SWG), 0, (const uint8_t*)cls, (const uint32_t*)P, n,
-        (const uint32_t*)opos, (const uint32_t*)oneg, ID);
-    launch(ctx, "ccl_label_out", ccl_label_out, dim3(div_up(n, SWG)), dim3(SWG), 0, (const uint8_t*)cls, (const uint32_t*)P,
-        (const int32_t*)ID, rows, cols, lab);
+        fill_u32(ctx, nrec, 0u, 1);
+        launch(ctx, "ccl_tile", ccl_tile<T>, dim3(ntiles), dim3(CT_THREADS), 0, x, tg, (const SFState*)st, find_neg, conn4, P,
+            rmask, rec, nrec);
+        launch(ctx, "ccl_border", ccl_border, dim3(ntiles), dim3(128), 0, P, tg, conn4);
+        launch(ctx, "ccl_open_roots", ccl_open_roots, dim3(div_up(rec.cap, 256)), dim3(256), 0, P, tg, rec,
+            (const uint32_t*)nrec, rmask);
+        launch(ctx, "ccl_seg_counts", ccl_seg_counts, dim3(div_up(nseg2, 256)), dim3(256), 0, (const unsigned long long*)rmask,
+            nseg2, segcnt);
+        scan_u32(ctx, segcnt, nseg2, segoff);
+        launch(ctx, "sf_counts", sf_label_counts, dim3(1), dim3(1), 0, st, (const uint32_t*)segoff, tg.nseg,
+            (const uint32_t*)nrec, (const SelState*)ss);
     };
     SFState* hs = (SFState*)host_stage(ctx, "sf_state", sizeof(SFState));
     auto read_state = [&]() {
@@ -1214,8 +1596,6 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
     };
     rms_and_thresholds();
     label();
-    launch(ctx, "sf_counts", sf_label_counts, dim3(1), dim3(1), 0, st, (const uint32_t*)(opos + nblk),
-        (const uint32_t*)(oneg + nblk), (const SelState*)ss);
     read_state();
     if (hs->fallback) {
         // data the linear bins cannot resolve (heavy ties): radix selection on
@@ -1224,46 +1604,46 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
         launch(ctx, "sf_fallback_clear", sf_fallback_clear, dim3(1), dim3(1), 0, ss);
         rms_and_thresholds();
         label();
-        launch(ctx, "sf_counts", sf_label_counts, dim3(1), dim3(1), 0, st, (const uint32_t*)(opos + nblk),
-            (const uint32_t*)(oneg + nblk), (const SelState*)ss);
         read_state();
+    }
+    if (hs->num_records > rec.cap) {
+        // more tile components than record slots: grow and label again (the
+        // capacity is kept for later calls)
+        records(size_t(hs->num_records) + size_t(hs->num_records) / 4);
+        label();
+        read_state();
+        if (hs->num_records > rec.cap)
+            throw Error(STP_ERR_INTERNAL, "component records overflow");
     }
     const uint32_t kp = hs->num_pos, kn = hs->num_neg;
 
-    // extrema, validity, moments (sourcefind.cpp:337-628)
-    unsigned long long* ep = wsp<unsigned long long>(ctx, "sf_ep", kp);
-    unsigned long long* en = wsp<unsigned long long>(ctx, "sf_en", kn);
-    unsigned long long* ip = wsp<unsigned long long>(ctx, "sf_ip", kp);
-    unsigned long long* in_ = wsp<unsigned long long>(ctx, "sf_in", kn);
-    int* cp = wsp<int>(ctx, "sf_cp", kp);
-    int* cn = wsp<int>(ctx, "sf_cn", kn);
-    double* mp = wsp<double>(ctx, "sf_mp", 6 * size_t(kp));
-    double* mn = wsp<double>(ctx, "sf_mn", 6 * size_t(kn));
-    launch(ctx, "sf_init_labels", sf_init_labels, dim3(div_up(std::max(kp, kn), 256)), dim3(256), 0, ep, en, ip, in_, cp, cn,
-        mp, mn, kp, kn, (const SFState*)st);
-    launch(ctx, "sf_extrema", sf_extrema<T>, dim3(div_up(n, SWG)), dim3(SWG), 0, x, (const int32_t*)lab, n, ep, en);
-    launch(ctx, "sf_extrema_idx", sf_extrema_idx<T>, dim3(div_up(n, SWG)), dim3(SWG), 0, x, (const int32_t*)lab, n,
-        (const unsigned long long*)ep, (const unsigned long long*)en, ip, in_);
-    launch(ctx, "sf_moments", sf_moments<T>, dim3(div_up(n, SWG)), dim3(SWG), 0, x, lab, rows, cols, (const SFState*)st,
-        (const unsigned long long*)ep, (const unsigned long long*)en, int(p.generate_labelmap != 0), cp, cn, mp, mn);
+    // island sums in list order (positive labels, then negative); label map (sourcefind.cpp:337-628)
+    const size_t nl = size_t(kp) + size_t(kn);
+    IslandSum* isum = wsp<IslandSum>(ctx, "sf_isum", nl);
+    const SegIds sid{ tg, rmask, segoff };
+    const unsigned rblocks = div_up(rec.cap, 256);
+    launch(ctx, "ccl_open_init", ccl_open_init, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, (const uint32_t*)nrec,
+        isum);
+    launch(ctx, "ccl_records", ccl_records, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, (const uint32_t*)nrec, isum);
+    launch(ctx, "ccl_open_idx", ccl_open_idx, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, (const uint32_t*)nrec,
+        isum);
+    if (d_lab_out)
+        launch(ctx, "ccl_label_map", ccl_label_map, dim3(ntiles), dim3(CT_THREADS), 0, (const uint32_t*)P, sid,
+            (const SFState*)st, (const IslandSum*)isum, int(p.generate_labelmap != 0), d_lab_out);
 
     // island list: validity, area cut, compaction and moments fit on the device
-    const size_t nl = size_t(kp) + size_t(kn);
     uint32_t* keep = wsp<uint32_t>(ctx, "sf_keep", nl);
     uint32_t* valid = wsp<uint32_t>(ctx, "sf_valid", nl);
     uint32_t* koff = wsp<uint32_t>(ctx, "sf_koff", nl + 1);
     uint32_t* voff = wsp<uint32_t>(ctx, "sf_voff", nl + 1);
     stp_island* disl = wsp<stp_island>(ctx, "sf_islands", nl);
     if (nl) {
-        launch(ctx, "sf_island_flags", sf_island_flags, dim3(unsigned(div_up(nl, 256))), dim3(256), 0,
-            (const unsigned long long*)ep, (const unsigned long long*)en, (const int*)cp, (const int*)cn, kp, kn,
-            (const SFState*)st, int(p.source_min_area), keep, valid);
+        launch(ctx, "sf_island_flags", sf_island_flags, dim3(unsigned(div_up(nl, 256))), dim3(256), 0, (const IslandSum*)isum,
+            kp, kn, (const SFState*)st, int(p.source_min_area), keep, valid);
         scan_u32(ctx, keep, nl, koff);
         scan_u32(ctx, valid, nl, voff);
-        launch(ctx, "sf_island_emit", sf_island_emit, dim3(unsigned(div_up(nl, 256))), dim3(256), 0,
-            (const unsigned long long*)ep, (const unsigned long long*)en, (const unsigned long long*)ip,
-            (const unsigned long long*)in_, (const int*)cp, (const int*)cn, (const double*)mp, (const double*)mn, kp, kn,
-            (const uint32_t*)keep, (const uint32_t*)koff, (const SFState*)st, rows, cols, p.analysis_n_sigma, disl);
+        launch(ctx, "sf_island_emit", sf_island_emit, dim3(unsigned(div_up(nl, 256))), dim3(256), 0, (const IslandSum*)isum, kp,
+            kn, (const uint32_t*)keep, (const uint32_t*)koff, (const SFState*)st, rows, cols, p.analysis_n_sigma, disl);
         launch(ctx, "sf_island_counts", sf_island_counts, dim3(1), dim3(1), 0, st, (const uint32_t*)(koff + nl),
             (const uint32_t*)(voff + nl));
     } else {

@@ -86,6 +86,28 @@ def test_gpu_sourcefind_degenerate(gpu, orc):
     compare(gpu.source_find(a, pars), orc.source_find(a, pars))
 
 
+@pytest.mark.parametrize("conn4", [False, True])
+def test_gpu_sourcefind_dense_components(gpu, orc, conn4):
+    """Noise labelled at 1 sigma: 500-600 components per 64 x 64 tile, more than a tile
+    sums in LDS, so part of every tile's islands take the global-atomic path; and
+    islands spread over many tiles (a 1-sigma threshold percolates)."""
+    img = noise(256, 320, seed=21)
+    pars = capi.sourcefind_pars(1.5, 1.0, rms_est=1.0, median_method=capi.MEDIAN_ZEROMEDIAN, ccl_4connectivity=conn4)
+    compare(gpu.source_find(img, pars, capacity=1 << 15), orc.source_find(img, pars, capacity=1 << 15))
+
+
+def test_gpu_sourcefind_component_records_grow(gpu, orc):
+    """A +-2 checkerboard with 4-connectivity makes every pixel its own island: 262144
+    components in a 512 x 512 image, more than the initial component-record capacity,
+    so the labelling grows it and runs again."""
+    cb = np.asfortranarray(2.0 * ((np.indices((512, 512)).sum(axis=0) % 2) * 2 - 1))
+    pars = capi.sourcefind_pars(1.5, 1.0, rms_est=1.0, median_method=capi.MEDIAN_ZEROMEDIAN, ccl_4connectivity=True)
+    for _ in range(2):  # the grown capacity is kept by the next call
+        res = gpu.source_find(cb, pars)
+        compare(res, orc.source_find(cb, pars))
+        assert res.stats.num_labels_pos == res.stats.num_labels_neg == 512 * 512 // 2
+
+
 @pytest.mark.parametrize("median", [capi.MEDIAN_BINMEDIAN, capi.MEDIAN_NTHELEMENT])
 def test_gpu_exact_median_two_level(gpu, orc, median):
     """A peaked distribution (99% of pixels with sigma 1e-3, 1% with sigma 100) puts
