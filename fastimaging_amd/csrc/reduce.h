@@ -41,6 +41,10 @@ __device__ __forceinline__ void canon_tree(double& a, double& b, double* lds)
 // canon_scratch_doubles(m) doubles.
 inline size_t canon_scratch_doubles(size_t m) { return 4 * ((m + CANON_BLOCK - 1) / CANON_BLOCK) + 4; }
 void canon_reduce_pairs(stp_ctx* ctx, const double* partials, size_t m, double* scratch, double* result);
+// Runs the canonical levels of canon_reduce_pairs while more than CANON_BLOCK pairs
+// remain; returns the remaining pairs (count in *m_out) for a last level done by
+// the caller with one block in the same order.
+const double* canon_reduce_to_block(stp_ctx* ctx, const double* partials, size_t m, double* scratch, size_t* m_out);
 
 // Fill kernels (used instead of hipMemsetAsync so the work stays capturable).
 void fill_f64(stp_ctx* ctx, double* p, double v, size_t n);
@@ -52,5 +56,7 @@ void exclusive_scan_u32(stp_ctx* ctx, const uint32_t* counts, int m, uint32_t* o
 // Device-wide exclusive scan of m uint32 values: out[i] = sum(in[0..i)), out[m] = total.
 // Three passes (chunk sums, scan of the sums, chunk rescan) over chunks of 4096.
 void scan_u32(stp_ctx* ctx, const uint32_t* in, size_t m, uint32_t* out);
+// The same scan over the population counts of m 64-bit masks.
+void scan_popc64(stp_ctx* ctx, const unsigned long long* in, size_t m, uint32_t* out);
 
 } // namespace stp_hip

@@ -101,6 +101,24 @@ void canon_reduce_pairs(stp_ctx* ctx, const double* partials, size_t m, double* 
     }
 }
 
+const double* canon_reduce_to_block(stp_ctx* ctx, const double* partials, size_t m, double* scratch, size_t* m_out)
+{
+    const double* src = partials;
+    size_t cur = m;
+    double* bufs[2] = { scratch, scratch + 2 * ((m + CANON_BLOCK - 1) / CANON_BLOCK) };
+    int which = 0;
+    while (cur > size_t(CANON_BLOCK)) {
+        const size_t nb = (cur + CANON_BLOCK - 1) / CANON_BLOCK;
+        double* dst = bufs[which];
+        launch(ctx, "canon_reduce", canon_pairs_kernel, dim3(unsigned(nb)), dim3(CANON_LANES), 0, src, cur, dst);
+        src = dst;
+        cur = nb;
+        which ^= 1;
+    }
+    *m_out = cur;
+    return src;
+}
+
 void fill_f64(stp_ctx* ctx, double* p, double v, size_t n)
 {
     launch(ctx, "fill", fill_f64_kernel, dim3(div_up(n, 256)), dim3(256), 0, p, v, n);
@@ -122,14 +140,24 @@ namespace {
 
 constexpr int SCAN_CHUNK = 4096; // 256 threads x 16 consecutive values
 
-__global__ __launch_bounds__(256) void chunk_sum_kernel(const uint32_t* in, size_t m, uint32_t* sums)
+struct LoadU32 {
+    const uint32_t* p;
+    __device__ uint32_t operator()(size_t i) const { return p[i]; }
+};
+struct LoadPopc64 {
+    const unsigned long long* p;
+    __device__ uint32_t operator()(size_t i) const { return uint32_t(__popcll(p[i])); }
+};
+
+template <typename L>
+__global__ __launch_bounds__(256) void chunk_sum_kernel(L in, size_t m, uint32_t* sums)
 {
     __shared__ uint32_t s[256];
     const size_t base = size_t(blockIdx.x) * SCAN_CHUNK + size_t(threadIdx.x) * 16;
     uint32_t a = 0;
     for (int j = 0; j < 16; j++)
         if (base + j < m)
-            a += in[base + j];
+            a += in(base + j);
     s[threadIdx.x] = a;
     __syncthreads();
     for (int off = 128; off >= 1; off >>= 1) {
@@ -141,14 +169,15 @@ __global__ __launch_bounds__(256) void chunk_sum_kernel(const uint32_t* in, size
         sums[blockIdx.x] = s[0];
 }
 
-__global__ __launch_bounds__(256) void chunk_rescan_kernel(const uint32_t* in, size_t m, const uint32_t* chunk_off, uint32_t* out)
+template <typename L>
+__global__ __launch_bounds__(256) void chunk_rescan_kernel(L in, size_t m, const uint32_t* chunk_off, uint32_t* out)
 {
     __shared__ uint32_t s[256];
     const size_t base = size_t(blockIdx.x) * SCAN_CHUNK + size_t(threadIdx.x) * 16;
     uint32_t v[16];
     uint32_t a = 0;
     for (int j = 0; j < 16; j++) {
-        v[j] = (base + j < m) ? in[base + j] : 0u;
+        v[j] = (base + j < m) ? in(base + j) : 0u;
         a += v[j];
     }
     s[threadIdx.x] = a;
@@ -173,16 +202,27 @@ __global__ __launch_bounds__(256) void chunk_rescan_kernel(const uint32_t* in, s
 
 namespace stp_hip {
 
-void scan_u32(stp_ctx* ctx, const uint32_t* in, size_t m, uint32_t* out)
+template <typename L>
+void scan_impl(stp_ctx* ctx, L in, size_t m, uint32_t* out)
 {
     const size_t nb = std::max<size_t>(1, (m + SCAN_CHUNK - 1) / SCAN_CHUNK);
     STP_REQUIRE(nb <= (1u << 24), "scan too large");
     uint32_t* sums = (uint32_t*)ws_get(ctx, "scan_sums", sizeof(uint32_t) * nb);
     uint32_t* offs = (uint32_t*)ws_get(ctx, "scan_offs", sizeof(uint32_t) * (nb + 1));
     uint32_t* cur = (uint32_t*)ws_get(ctx, "scan_cur", sizeof(uint32_t) * nb);
-    launch(ctx, "scan_sums", chunk_sum_kernel, dim3(unsigned(nb)), dim3(256), 0, in, m, sums);
+    launch(ctx, "scan_sums", chunk_sum_kernel<L>, dim3(unsigned(nb)), dim3(256), 0, in, m, sums);
     exclusive_scan_u32(ctx, sums, int(nb), offs, cur);
-    launch(ctx, "scan_rescan", chunk_rescan_kernel, dim3(unsigned(nb)), dim3(256), 0, in, m, (const uint32_t*)offs, out);
+    launch(ctx, "scan_rescan", chunk_rescan_kernel<L>, dim3(unsigned(nb)), dim3(256), 0, in, m, (const uint32_t*)offs, out);
+}
+
+void scan_u32(stp_ctx* ctx, const uint32_t* in, size_t m, uint32_t* out)
+{
+    scan_impl(ctx, LoadU32{ in }, m, out);
+}
+
+void scan_popc64(stp_ctx* ctx, const unsigned long long* in, size_t m, uint32_t* out)
+{
+    scan_impl(ctx, LoadPopc64{ in }, m, out);
 }
 
 } // namespace stp_hip

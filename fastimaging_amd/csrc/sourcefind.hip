@@ -57,7 +57,7 @@ struct SFState {
     unsigned int num_valid, num_islands;     // detections; islands after the area cut
     int fallback;                            // exact median needs the radix selection
     int pad2;
-    unsigned int num_records;                // tile components of the labelling (record capacity check)
+    unsigned int num_records;                // pool component records requested (capacity check)
     unsigned int pad3;
 };
 
@@ -78,6 +78,51 @@ __device__ __forceinline__ double okey_inv(unsigned long long k)
 {
     const unsigned long long b = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
     return __longlong_as_double((long long)b);
+}
+
+// Visits every element of x once, in no particular order (histograms, gathers):
+// 16-byte vector loads, four per thread in flight, grid-stride.  f(double value).
+constexpr int HWG = 1024;
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using V = float4; static constexpr int W = 4; };
+template <> struct Vec16<double> { using V = double2; static constexpr int W = 2; };
+
+template <typename T, typename F>
+__device__ __forceinline__ void stream_values(const T* x, size_t n, F&& f)
+{
+    using V = typename Vec16<T>::V;
+    constexpr int W = Vec16<T>::W, U = 4;
+    const size_t stride = size_t(gridDim.x) * blockDim.x, gt = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if ((reinterpret_cast<uintptr_t>(x) & 15u) != 0) {
+        for (size_t i = gt; i < n; i += stride)
+            f(double(x[i]));
+        return;
+    }
+    const size_t nv = n / W;
+    const V* xv = reinterpret_cast<const V*>(x);
+    size_t i = gt;
+    for (; i + (U - 1) * stride < nv; i += U * stride) {
+        V v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            v[u] = xv[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const T* e = reinterpret_cast<const T*>(&v[u]);
+#pragma unroll
+            for (int k = 0; k < W; k++)
+                f(double(e[k]));
+        }
+    }
+    for (; i < nv; i += stride) {
+        const V v = xv[i];
+        const T* e = reinterpret_cast<const T*>(&v);
+#pragma unroll
+        for (int k = 0; k < W; k++)
+            f(double(e[k]));
+    }
+    if (gt < n - nv * W)
+        f(double(x[nv * W + gt]));
 }
 
 // ------------------------------------------------------------------ sums
@@ -103,13 +148,30 @@ __global__ __launch_bounds__(SWG) void sf_sum_kernel(const T* x, size_t n, doubl
     }
 }
 
-__global__ void sf_mean_sigma_kernel(const double* sums, unsigned long long n, SFState* st)
+__device__ void set_mean_sigma(double s1, double s2, unsigned long long n, SFState* st)
 {
-    const double mean = sums[0] / double(n);
-    st->s1 = sums[0];
-    st->s2 = sums[1];
+    const double mean = s1 / double(n);
+    st->s1 = s1;
+    st->s2 = s2;
     st->mean = mean;
-    st->sigma = sqrt(sums[1] / double(n) - (mean * mean));
+    st->sigma = sqrt(s2 / double(n) - (mean * mean));
+}
+
+// Last level of the canonical reduction (m <= CANON_BLOCK pairs, one block of
+// CANON_LANES threads, the order of the canonical pair kernel); thread 0 gets the sums.
+__device__ __forceinline__ void canon_last_level(const double* in, unsigned m, double& a, double& b, double* lds)
+{
+    a = 0.0;
+    b = 0.0;
+#pragma unroll
+    for (int j = 0; j < CANON_PER_LANE; j++) {
+        const unsigned i = unsigned(j) * CANON_LANES + threadIdx.x;
+        if (i < m) {
+            a = a + in[2 * i];
+            b = b + in[2 * i + 1];
+        }
+    }
+    canon_tree(a, b, lds);
 }
 
 // ------------------------------------------------------------------ histograms
@@ -123,43 +185,44 @@ __device__ __forceinline__ int ba_bin(double x, double leftend, double scale)
 }
 
 template <typename T>
-__global__ __launch_bounds__(SWG) void sf_binapprox_hist(const T* x, size_t n, const SFState* st, uint32_t* bh)
+__global__ __launch_bounds__(HWG) void sf_binapprox_hist(const T* x, size_t n, const SFState* st, uint32_t* bh)
 {
     __shared__ uint32_t h[BA_BINS + 1];
     if (!st->ba_active)
         return;
-    for (int k = threadIdx.x; k < BA_BINS + 1; k += SWG)
+    for (int k = threadIdx.x; k < BA_BINS + 1; k += HWG)
         h[k] = 0;
     __syncthreads();
     const double scale = double(BA_BINS - 1) / (2 * st->sigma);
     const double left = st->mean - st->sigma;
-    for (size_t i = size_t(blockIdx.x) * SWG + threadIdx.x; i < n; i += size_t(gridDim.x) * SWG) {
-        const int b = ba_bin(ldv(x, i), left, scale);
+    stream_values(x, n, [&](double v) {
+        const int b = ba_bin(v, left, scale);
         if (b < 0)
             atomicAdd(&h[0], 1u);
         else if (b <= BA_BINS - 1)
             atomicAdd(&h[1 + b], 1u);
-    }
+    });
     __syncthreads();
-    for (int k = threadIdx.x; k < BA_BINS + 1; k += SWG)
+    for (int k = threadIdx.x; k < BA_BINS + 1; k += HWG)
         bh[size_t(blockIdx.x) * (BA_BINS + 1) + k] = h[k];
 }
 
 // radix digit histogram (8 bits at `shift`) among elements whose key >> (shift + 8) == prefix
 template <typename T>
-__global__ __launch_bounds__(SWG) void sf_radix_hist(const T* x, size_t n, int shift, unsigned long long prefix, int use_prefix, uint32_t* bh)
+__global__ __launch_bounds__(HWG) void sf_radix_hist(const T* x, size_t n, int shift, unsigned long long prefix, int use_prefix, uint32_t* bh)
 {
     __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    if (threadIdx.x < 256)
+        h[threadIdx.x] = 0;
     __syncthreads();
-    for (size_t i = size_t(blockIdx.x) * SWG + threadIdx.x; i < n; i += size_t(gridDim.x) * SWG) {
-        const unsigned long long k = okey(ldv(x, i));
-        if (use_prefix && (k >> (shift + 8)) != prefix)
-            continue;
-        atomicAdd(&h[(k >> shift) & 0xff], 1u);
-    }
+    stream_values(x, n, [&](double v) {
+        const unsigned long long k = okey(v);
+        if (!use_prefix || (k >> (shift + 8)) == prefix)
+            atomicAdd(&h[(k >> shift) & 0xff], 1u);
+    });
     __syncthreads();
-    bh[size_t(blockIdx.x) * 256 + threadIdx.x] = h[threadIdx.x];
+    if (threadIdx.x < 256)
+        bh[size_t(blockIdx.x) * 256 + threadIdx.x] = h[threadIdx.x];
 }
 
 // out[k] += sum over per-block histograms b of bh[b][k]; blockIdx.y takes a
@@ -237,17 +300,16 @@ struct GatherSpec {
 };
 
 template <typename T>
-__global__ __launch_bounds__(SWG) void sf_gather(const T* x, size_t n, GatherSpec gs, unsigned long long* cand, unsigned* counter, unsigned cap)
+__global__ __launch_bounds__(HWG) void sf_gather(const T* x, size_t n, GatherSpec gs, unsigned long long* cand, unsigned* counter, unsigned cap)
 {
-    for (size_t i = size_t(blockIdx.x) * SWG + threadIdx.x; i < n; i += size_t(gridDim.x) * SWG) {
-        const double v = ldv(x, i);
+    stream_values(x, n, [&](double v) {
         const bool take = (gs.shift >= 64) || ((okey(v) >> gs.shift) == gs.prefix);
         if (take) {
             const unsigned slot = atomicAdd(counter, 1u);
             if (slot < cap)
                 cand[slot] = okey(v);
         }
-    }
+    });
 }
 
 // Ascending bitonic sort of the first c (<= SORT_CAP) keys of cand into s
@@ -366,8 +428,27 @@ __device__ void rank_bins(const unsigned long long* h, int nb, unsigned long lon
     __syncthreads();
 }
 
-__global__ void sel_init(SFState* st, SelState* ss, unsigned long long k_lo, unsigned long long k_hi, int odd, int force)
+struct SelInit {
+    int enabled;                     // run the selection set-up in the statistics kernel
+    unsigned long long k_lo, k_hi;
+    int odd, force;
+    unsigned long long* hist1;       // LIN_BINS + 2 level-1 counts
+    unsigned long long* hist2;       // LIN_BINS level-2 counts
+    unsigned* counter;               // gathered candidates
+};
+
+// Selection set-up (thread 0) and cleared histograms and candidate counter (the block).
+__device__ void sel_init_block(SFState* st, SelState* ss, const SelInit& si)
 {
+    for (int k = threadIdx.x; k < LIN_BINS + 2; k += blockDim.x)
+        si.hist1[k] = 0;
+    for (int k = threadIdx.x; k < LIN_BINS; k += blockDim.x)
+        si.hist2[k] = 0;
+    if (threadIdx.x != 0)
+        return;
+    *si.counter = 0;
+    const unsigned long long k_lo = si.k_lo, k_hi = si.k_hi;
+    const int odd = si.odd, force = si.force;
     if (force)
         st->need_exact = 1;
     const bool active = st->need_exact != 0;
@@ -388,20 +469,38 @@ __global__ void sel_init(SFState* st, SelState* ss, unsigned long long k_lo, uns
     ss->scale = double(LIN_BINS) / (hi - lo);
 }
 
+__global__ __launch_bounds__(256) void sel_init(SFState* st, SelState* ss, SelInit si)
+{
+    sel_init_block(st, ss, si);
+}
+
+// Mean and sigma from the block partials of sf_sum (last canonical level), then
+// optionally the selection set-up.
+__global__ __launch_bounds__(CANON_LANES) void sf_stats(const double* part, unsigned m, unsigned long long n, SFState* st,
+    SelState* ss, SelInit si)
+{
+    __shared__ double lds[2 * CANON_LANES];
+    double a, b;
+    canon_last_level(part, m, a, b, lds);
+    if (threadIdx.x == 0)
+        set_mean_sigma(a, b, n, st);
+    if (si.enabled)
+        sel_init_block(st, ss, si);
+}
+
 template <typename T>
-__global__ __launch_bounds__(SWG) void sel_hist1(const T* x, size_t n, const SelState* ss, uint32_t* bh)
+__global__ __launch_bounds__(HWG) void sel_hist1(const T* x, size_t n, const SelState* ss, uint32_t* bh)
 {
     __shared__ uint32_t h[LIN_BINS + 2];
     if (!ss->run1)
         return;
     const double lo = ss->lo, scale = ss->scale;
-    for (int k = threadIdx.x; k < LIN_BINS + 2; k += SWG)
+    for (int k = threadIdx.x; k < LIN_BINS + 2; k += HWG)
         h[k] = 0;
     __syncthreads();
-    for (size_t i = size_t(blockIdx.x) * SWG + threadIdx.x; i < n; i += size_t(gridDim.x) * SWG)
-        atomicAdd(&h[lin_bin((ldv(x, i) - lo) * scale)], 1u);
+    stream_values(x, n, [&](double v) { atomicAdd(&h[lin_bin((v - lo) * scale)], 1u); });
     __syncthreads();
-    for (int k = threadIdx.x; k < LIN_BINS + 2; k += SWG)
+    for (int k = threadIdx.x; k < LIN_BINS + 2; k += HWG)
         bh[size_t(blockIdx.x) * (LIN_BINS + 2) + k] = h[k];
 }
 
@@ -434,24 +533,24 @@ __global__ __launch_bounds__(1024) void sel_pick1(SelState* ss, const unsigned l
 }
 
 template <typename T>
-__global__ __launch_bounds__(SWG) void sel_hist2(const T* x, size_t n, const SelState* ss, uint32_t* bh)
+__global__ __launch_bounds__(HWG) void sel_hist2(const T* x, size_t n, const SelState* ss, uint32_t* bh)
 {
     __shared__ uint32_t h[LIN_BINS];
     if (!ss->run2)
         return;
     const double lo = ss->lo, scale = ss->scale, base2 = ss->base2, scale2 = ss->scale2;
     const int b1lo = ss->b1_lo, b1hi = ss->b1_hi;
-    for (int k = threadIdx.x; k < LIN_BINS; k += SWG)
+    for (int k = threadIdx.x; k < LIN_BINS; k += HWG)
         h[k] = 0;
     __syncthreads();
-    for (size_t i = size_t(blockIdx.x) * SWG + threadIdx.x; i < n; i += size_t(gridDim.x) * SWG) {
-        const double t = (ldv(x, i) - lo) * scale;
+    stream_values(x, n, [&](double v) {
+        const double t = (v - lo) * scale;
         const int b = lin_bin(t);
         if (b >= b1lo && b <= b1hi)
             atomicAdd(&h[sub_bin(t, base2, scale2)], 1u);
-    }
+    });
     __syncthreads();
-    for (int k = threadIdx.x; k < LIN_BINS; k += SWG)
+    for (int k = threadIdx.x; k < LIN_BINS; k += HWG)
         bh[size_t(blockIdx.x) * LIN_BINS + k] = h[k];
 }
 
@@ -478,15 +577,14 @@ __global__ __launch_bounds__(1024) void sel_pick2(SelState* ss, const unsigned l
 }
 
 template <typename T>
-__global__ __launch_bounds__(SWG) void sel_gather(const T* x, size_t n, const SelState* ss, unsigned long long* cand,
+__global__ __launch_bounds__(HWG) void sel_gather(const T* x, size_t n, const SelState* ss, unsigned long long* cand,
     unsigned* counter)
 {
     if (!ss->gather)
         return;
     const double lo = ss->lo, scale = ss->scale, base2 = ss->base2, scale2 = ss->scale2;
     const int b1lo = ss->b1_lo, b1hi = ss->b1_hi, b2lo = ss->b2_lo, b2hi = ss->b2_hi, lvl = ss->level;
-    for (size_t i = size_t(blockIdx.x) * SWG + threadIdx.x; i < n; i += size_t(gridDim.x) * SWG) {
-        const double v = ldv(x, i);
+    stream_values(x, n, [&](double v) {
         const double t = (v - lo) * scale;
         const int b = lin_bin(t);
         bool take = b >= b1lo && b <= b1hi;
@@ -499,7 +597,7 @@ __global__ __launch_bounds__(SWG) void sel_gather(const T* x, size_t n, const Se
             if (slot < unsigned(SORT_CAP))
                 cand[slot] = okey(v);
         }
-    }
+    });
 }
 
 // Sorts the candidates and sets the median (mat_median_exact, matrix_math.cpp:33-45:
@@ -525,31 +623,24 @@ __global__ __launch_bounds__(1024) void sel_sort(SelState* ss, const unsigned lo
 }
 
 // ------------------------------------------------------------------ sigma clipping
-__global__ void sf_clip_init(SFState* st, unsigned long long n, int iters, double num_sigma)
-{
-    const double median = st->median;
-    const double mean = st->mean - median;
-    const double sigma = st->sigma;
-    st->d1 = double(mean) * double(n);
-    st->d2 = (double(sigma) * double(sigma) + double(mean) * double(mean)) * double(n);
-    st->valid_n = n;
-    st->clip_sigma = sigma;
-    st->num_sigma = num_sigma;
-    st->upper = num_sigma * sigma;
-    st->prev_upper = DBL_MAX;
-    st->iter = 0;
-    st->iters = iters;
-    st->done = iters <= 0;
-}
-
 template <typename T>
-__global__ __launch_bounds__(SWG) void sf_clip_kernel(const T* x, size_t n, const SFState* st, double* part, uint32_t* removed)
+__global__ __launch_bounds__(SWG) void sf_clip_kernel(const T* x, size_t n, const SFState* st, int iter, double num_sigma,
+    double* part, uint32_t* removed)
 {
     __shared__ double lds[2 * CANON_LANES];
     __shared__ uint32_t wc[SWG / 64];
-    if (st->done)
-        return;
-    const double median = st->median, upper = st->upper, prev = st->prev_upper;
+    double upper, prev;
+    if (iter == 0) {
+        // the state of sf_clip_init
+        upper = num_sigma * st->sigma;
+        prev = DBL_MAX;
+    } else {
+        if (st->done)
+            return;
+        upper = st->upper;
+        prev = st->prev_upper;
+    }
+    const double median = st->median;
     const size_t base = size_t(blockIdx.x) * CANON_BLOCK;
     double a = 0.0, b = 0.0;
     unsigned cnt = 0;
@@ -583,50 +674,103 @@ __global__ __launch_bounds__(SWG) void sf_clip_kernel(const T* x, size_t n, cons
     }
 }
 
-// One clipping step from the block partials (removed: one count per block).
-__global__ __launch_bounds__(256) void sf_clip_update(SFState* st, const double* sums, const uint32_t* removed, int nblk)
+__device__ void clip_init_state(SFState* st, unsigned long long n, int iters, double num_sigma)
 {
-    __shared__ unsigned long long red[256];
-    if (st->done)
-        return;
-    unsigned long long c = 0;
-    for (int b = threadIdx.x; b < nblk; b += 256)
-        c += removed[b];
-    red[threadIdx.x] = c;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off)
-            red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x != 0)
-        return;
-    const unsigned long long rem = red[0];
-    if (rem == 0) {
-        st->done = 1;
-        return;
-    }
-    st->valid_n -= rem;
-    st->d1 -= sums[0];
-    st->d2 -= sums[1];
-    const double vn = double(st->valid_n);
-    st->clip_sigma = sqrt(st->d2 / vn - (st->d1 / vn) * (st->d1 / vn));
-    st->prev_upper = st->upper;
-    st->upper = st->num_sigma * st->clip_sigma;
-    st->iter++;
-    if (st->iter >= st->iters)
-        st->done = 1;
+    const double median = st->median;
+    const double mean = st->mean - median;
+    const double sigma = st->sigma;
+    st->d1 = double(mean) * double(n);
+    st->d2 = (double(sigma) * double(sigma) + double(mean) * double(mean)) * double(n);
+    st->valid_n = n;
+    st->clip_sigma = sigma;
+    st->num_sigma = num_sigma;
+    st->upper = num_sigma * sigma;
+    st->prev_upper = DBL_MAX;
+    st->iter = 0;
+    st->iters = iters;
+    st->done = iters <= 0;
 }
 
-__global__ void sf_thresholds(SFState* st, int estimate, double rms_in, double det, double ana)
+__global__ void sf_clip_init(SFState* st, unsigned long long n, int iters, double num_sigma)
 {
-    const double rms = estimate ? st->clip_sigma : rms_in;
+    clip_init_state(st, n, iters, num_sigma);
+}
+
+// Detection and analysis thresholds (sourcefind.cpp:298-301).
+struct ThreshArgs {
+    int estimate;        // RMS from the clipping
+    double rms_in, det, ana;
+    uint32_t* reset;     // counter cleared for the labelling (may be null)
+};
+
+__device__ void set_thresholds(SFState* st, const ThreshArgs& ta)
+{
+    const double rms = ta.estimate ? st->clip_sigma : ta.rms_in;
     st->rms = rms;
     const double bg = st->median;
-    st->a_pos = bg + ana * rms;
-    st->a_neg = bg - ana * rms;
-    st->d_pos = bg + det * rms;
-    st->d_neg = bg - det * rms;
+    st->a_pos = bg + ta.ana * rms;
+    st->a_neg = bg - ta.ana * rms;
+    st->d_pos = bg + ta.det * rms;
+    st->d_neg = bg - ta.det * rms;
+    if (ta.reset)
+        *ta.reset = 0;
+}
+
+__global__ void sf_thresholds(SFState* st, ThreshArgs ta)
+{
+    set_thresholds(st, ta);
+}
+
+// One clipping step (sourcefind.cpp:39-151) from the block partials of sf_clip
+// (m <= CANON_BLOCK pairs after any earlier canonical levels; removed: one count
+// per block).  Iteration 0 first sets up the state; the last iteration also sets
+// the thresholds.
+__global__ __launch_bounds__(256) void sf_clip_update(SFState* st, const double* cpart, unsigned m, const uint32_t* removed,
+    int nblk, int iter, unsigned long long n, int iters, double num_sigma, int last, ThreshArgs ta)
+{
+    __shared__ double lds[2 * CANON_LANES];
+    __shared__ unsigned long long red[256];
+    __shared__ int s_done;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        if (iter == 0)
+            clip_init_state(st, n, iters, num_sigma);
+        s_done = st->done;
+    }
+    __syncthreads();
+    if (!s_done) {
+        double a, b;
+        canon_last_level(cpart, m, a, b, lds);
+        unsigned long long c = 0;
+        for (int k = t; k < nblk; k += 256)
+            c += removed[k];
+        red[t] = c;
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if (t < off)
+                red[t] += red[t + off];
+            __syncthreads();
+        }
+        if (t == 0) {
+            const unsigned long long rem = red[0];
+            if (rem == 0) {
+                st->done = 1;
+            } else {
+                st->valid_n -= rem;
+                st->d1 -= a;
+                st->d2 -= b;
+                const double vn = double(st->valid_n);
+                st->clip_sigma = sqrt(st->d2 / vn - (st->d1 / vn) * (st->d1 / vn));
+                st->prev_upper = st->upper;
+                st->upper = st->num_sigma * st->clip_sigma;
+                st->iter++;
+                if (st->iter >= st->iters)
+                    st->done = 1;
+            }
+        }
+    }
+    if (last && t == 0)
+        set_thresholds(st, ta);
 }
 
 // ------------------------------------------------------------------ labelling
@@ -638,11 +782,12 @@ __global__ void sf_thresholds(SFState* st, int estimate, double rms_in, double d
 // its first pixel in the reference's scan order (visual column, memory row).
 //
 //  ccl_tile        classification; vertical runs from wave ballots (a wave holds
-//                  one tile column); union-find of the runs in LDS, one union per
-//                  pair of touching runs; the island sums of every component of
-//                  the tile (count, extremum, its smallest memory index, moments)
-//                  in LDS; a component with a same-class neighbour outside the
-//                  tile is "open";
+//                  one tile column); a key-ordered list of the foreground pixels
+//                  in LDS, over which the rest runs: union-find of the runs, one
+//                  union per pair of touching runs; the island sums of every
+//                  component of the tile (count, extremum, its smallest memory
+//                  index, moments) in LDS; a component with a same-class
+//                  neighbour outside the tile is "open";
 //  ccl_border      union-find across tile borders, on the open components only;
 //  ccl_open_roots  flattens open components, marks their global roots;
 //  ccl_records     one record per tile component: closed ones are final, open
@@ -725,13 +870,30 @@ __device__ __forceinline__ uint32_t root_gid(const SegIds& s, uint32_t key, bool
     return gid_at(s, sseg, rr);
 }
 
-// Per-component records written by ccl_tile.
+// Per-component records written by ccl_tile: tile b owns records [b rt, b rt + rt);
+// components beyond rt take records of a shared pool after the tile regions (one
+// atomic per overflowing tile: a device-wide counter per tile would serialise).
 struct CompRecords {
-    uint32_t* key;   // local root key | P_NEG | P_OPEN
-    uint32_t* pos;   // (signed segment << 6) | row of the local root in it
+    uint32_t* key;          // local root key | P_NEG | P_OPEN
+    uint32_t* pos;          // (signed segment << 6) | row of the local root in it
     IslandSum* sum;
-    uint32_t cap;
+    uint32_t* tcount;       // components of each tile
+    uint32_t* pool_count;   // pool records requested
+    uint32_t rt, ntiles, pool_cap;
 };
+
+constexpr uint32_t CT_RT = 256;  // records of a tile's own region
+
+// Record i of the tile regions or the pool, when it holds a component.
+__device__ __forceinline__ bool rec_valid(const CompRecords& rec, uint32_t i)
+{
+    const uint32_t reg = rec.ntiles * rec.rt;
+    if (i < reg)
+        return (i % rec.rt) < min(rec.tcount[i / rec.rt], rec.rt);
+    return i - reg < *rec.pool_count;
+}
+// An overflowed pool leaves records unwritten: the host grows it and labels again.
+__device__ __forceinline__ bool pool_overflow(const CompRecords& rec) { return *rec.pool_count > rec.pool_cap; }
 
 __device__ __forceinline__ uint32_t lds_find(const uint32_t* par, uint32_t a)
 {
@@ -799,13 +961,16 @@ __device__ void g_union(uint32_t* P, uint32_t a, uint32_t b, uint32_t flags)
 
 template <typename T>
 __global__ __launch_bounds__(CT_THREADS) void ccl_tile(const T* __restrict__ x, TileGeom g, const SFState* st, int find_neg,
-    int conn4, uint32_t* __restrict__ P, unsigned long long* __restrict__ rmask, CompRecords rec, uint32_t* rec_count)
+    int conn4, uint32_t* __restrict__ P, unsigned long long* __restrict__ rmask, CompRecords rec)
 {
     __shared__ uint32_t s_par[CT_PIX];
+    __shared__ uint32_t s_list[CT_PIX];  // foreground pixels in key order: li | class << 12 | run start << 14 | run end << 20
     __shared__ uint8_t s_cls[CT_PIX];
     __shared__ uint8_t s_fl[CT_H + 2], s_fr[CT_H + 2], s_ft[CT_W], s_fb[CT_W];
-    __shared__ unsigned long long s_mask[CT_W];
+    __shared__ unsigned long long s_mask[CT_W];        // foreground of each column
+    __shared__ unsigned long long s_rmp[CT_W], s_rmn[CT_W];  // closed roots of each column
     __shared__ uint32_t s_pre[CT_W + 1];
+    __shared__ uint32_t s_rpre[CT_PIX / 64 + 1];
     __shared__ uint32_t s_base;
     __shared__ unsigned long long r_ext[CT_RCAP];
     __shared__ uint32_t r_idx[CT_RCAP];
@@ -820,29 +985,31 @@ __global__ __launch_bounds__(CT_THREADS) void ccl_tile(const T* __restrict__ x, 
     const double a_pos = st->a_pos, a_neg = st->a_neg;
     auto classify = [&](double v) -> uint32_t { return v > a_pos ? 1u : ((find_neg && v < a_neg) ? 2u : 0u); };
     auto pix = [&](int cv, int vr) -> double { return ldv(x, size_t(mem_col(g, cv)) * size_t(g.R) + size_t(flip_row(g, vr))); };
+    auto mem_index = [&](int c, int rr) -> uint32_t { return uint32_t(mem_col(g, cv0 + c)) * uint32_t(g.R) + uint32_t(rbase + rr); };
     const unsigned long long lt = (1ull << lane) - 1ull;
 
-    // 1. classification and vertical runs: info = class | run start << 2 | run end << 8
-    T vals[CT_PPT];
-    uint32_t info[CT_PPT];
+    // 1. classification and vertical runs (a wave holds one tile column per step)
+    uint32_t info[CT_PPT];  // class | run start << 2 | run end << 8
 #pragma unroll
     for (int j = 0; j < CT_PPT; j++) {
         const int c = w + CT_WAVES * j, li = c * CT_H + lane;
         uint32_t cl = 0;
-        T v = T(0);
-        if (c < nc && lane < nr) {
-            v = x[size_t(mem_col(g, cv0 + c)) * size_t(g.R) + size_t(rbase + lane)];
-            cl = classify(double(v));
-        }
+        if (c < nc && lane < nr)
+            cl = classify(ldv(x, size_t(mem_index(c, lane))));
         const unsigned long long m1 = __ballot(cl == 1u), m2 = __ballot(cl == 2u);
         const unsigned long long M = cl == 2u ? m2 : m1;
         const unsigned long long zb = ~M & lt, za = ~M & ~(lt | (1ull << lane));
         const uint32_t rs = zb ? uint32_t(64 - __builtin_clzll(zb)) : 0u;
         const uint32_t re = za ? uint32_t(__builtin_ctzll(za) - 1) : 63u;
-        vals[j] = v;
         info[j] = cl | (rs << 2) | (re << 8);
         s_cls[li] = uint8_t(cl);
         s_par[li] = uint32_t(c * CT_H) + (cl ? rs : uint32_t(lane));
+        if (lane == 0)
+            s_mask[c] = m1 | m2;
+    }
+    if (t < CT_W) {
+        s_rmp[t] = 0;
+        s_rmn[t] = 0;
     }
     // classes of the frame around the tile (0 outside the image)
     for (int q = t; q < 2 * (CT_H + 2) + 2 * CT_W; q += CT_THREADS) {
@@ -864,90 +1031,7 @@ __global__ __launch_bounds__(CT_THREADS) void ccl_tile(const T* __restrict__ x, 
         }
     }
     __syncthreads();
-
-    // 2. one union per pair of touching runs of neighbouring columns: a run with
-    // rows [s, e] meets the left column's rows [s - 1, e + 1] (8-connectivity) or
-    // [s, e]; each maximal same-class stretch there is linked once, at its first row
-#pragma unroll
-    for (int j = 0; j < CT_PPT; j++) {
-        const uint32_t cl = info[j] & 3u;
-        const int c = w + CT_WAVES * j;
-        if (!cl || c == 0)
-            continue;
-        const int rs = int((info[j] >> 2) & 63u), re = int((info[j] >> 8) & 63u);
-        const uint32_t a = uint32_t(c * CT_H + rs), lb = uint32_t((c - 1) * CT_H);
-        auto lsame = [&](int r) { return r >= 0 && r < nr && s_cls[lb + r] == cl; };
-        if (conn4) {
-            if (lsame(lane) && (lane == rs || !lsame(lane - 1)))
-                lds_union(s_par, a, lb + lane);
-        } else {
-            if (lsame(lane) && !lsame(lane - 1))
-                lds_union(s_par, a, lb + lane);
-            if (lane == rs && lsame(rs - 1))
-                lds_union(s_par, a, lb + rs - 1);
-            if (lane == re && lsame(re + 1) && !lsame(re))
-                lds_union(s_par, a, lb + re + 1);
-        }
-    }
-    __syncthreads();
-    // 3. roots of the run starts (a component's root, its smallest index, starts a run)
-#pragma unroll
-    for (int j = 0; j < CT_PPT; j++) {
-        const int c = w + CT_WAVES * j;
-        if ((info[j] & 3u) && uint32_t(lane) == ((info[j] >> 2) & 63u))
-            s_par[c * CT_H + lane] = lds_find(s_par, c * CT_H + lane);
-    }
-    __syncthreads();
-    // every pixel points at its root; components with a same-class neighbour across
-    // the tile border are open
-#pragma unroll
-    for (int j = 0; j < CT_PPT; j++) {
-        const uint32_t cl = info[j] & 3u;
-        const int c = w + CT_WAVES * j, li = c * CT_H + lane;
-        if (!cl)
-            continue;
-        const int rs = int((info[j] >> 2) & 63u);
-        const uint32_t root = s_par[c * CT_H + rs] & L_KEY;
-        if (lane != rs)
-            s_par[li] = root;
-        if (!(c == 0 || c == nc - 1 || lane == 0 || lane == nr - 1))
-            continue;
-        bool open = false;
-        if (c == 0)
-            open = open || s_fl[lane + 1] == cl || (!conn4 && (s_fl[lane] == cl || s_fl[lane + 2] == cl));
-        if (c == nc - 1)
-            open = open || s_fr[lane + 1] == cl || (!conn4 && (s_fr[lane] == cl || s_fr[lane + 2] == cl));
-        if (lane == 0)
-            open = open || s_ft[c] == cl || (!conn4 && ((c > 0 && s_ft[c - 1] == cl) || (c + 1 < nc && s_ft[c + 1] == cl)));
-        if (lane == nr - 1)
-            open = open || s_fb[c] == cl || (!conn4 && ((c > 0 && s_fb[c - 1] == cl) || (c + 1 < nc && s_fb[c + 1] == cl)));
-        if (open)
-            atomicOr(&s_par[root], L_OPEN);
-    }
-    __syncthreads();
-
-    // 4. roots, their slots (rank in key order) and the closed roots of every segment
-#pragma unroll
-    for (int j = 0; j < CT_PPT; j++) {
-        const uint32_t cl = info[j] & 3u;
-        const int c = w + CT_WAVES * j, li = c * CT_H + lane;
-        const uint32_t pv = s_par[li];
-        const bool is_root = cl && (pv & L_KEY) == uint32_t(li);
-        const bool closed = is_root && !(pv & L_OPEN);
-        const unsigned long long m_all = __ballot(is_root);
-        const unsigned long long m_pos = __ballot(closed && cl == 1u);
-        const unsigned long long m_neg = __ballot(closed && cl == 2u);
-        if (lane == 0) {
-            s_mask[c] = m_all;
-            if (c < nc) {
-                const unsigned seg = seg_index(g, cv0 + c, ty);
-                rmask[seg] = m_pos;
-                rmask[g.nseg + seg] = m_neg;
-            }
-        }
-        info[j] |= uint32_t(is_root) << 14;
-    }
-    __syncthreads();
+    // list of the foreground pixels (key order)
     if (w == 0) {
         const uint32_t cnt = uint32_t(__popcll(s_mask[lane]));
         uint32_t incl = cnt;
@@ -957,29 +1041,145 @@ __global__ __launch_bounds__(CT_THREADS) void ccl_tile(const T* __restrict__ x, 
                 incl += o;
         }
         s_pre[lane] = incl - cnt;
-        if (lane == 63) {
+        if (lane == 63)
             s_pre[64] = incl;
-            s_base = atomicAdd(rec_count, incl);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < CT_PPT; j++) {
+        const int c = w + CT_WAVES * j;
+        if (info[j] & 3u)
+            s_list[s_pre[c] + uint32_t(__popcll(s_mask[c] & lt))] =
+                uint32_t(c * CT_H + lane) | ((info[j] & 3u) << 12) | (((info[j] >> 2) & 63u) << 14) | (((info[j] >> 8) & 63u) << 20);
+    }
+    const int nfg = int(s_pre[64]);
+    __syncthreads();
+
+    // 2. one union per pair of touching runs of neighbouring columns: a run with
+    // rows [s, e] meets the left column's rows [s - 1, e + 1] (8-connectivity) or
+    // [s, e]; each maximal same-class stretch there is linked once, at its first row
+    for (int e = t; e < nfg; e += CT_THREADS) {
+        const uint32_t en = s_list[e];
+        const int li = int(en & 0xFFFu), c = li >> 6, rr = li & 63;
+        const uint32_t cl = (en >> 12) & 3u;
+        const int rs = int((en >> 14) & 63u), re = int((en >> 20) & 63u);
+        if (c == 0)
+            continue;
+        const uint32_t a = uint32_t(c * CT_H + rs), lb = uint32_t((c - 1) * CT_H);
+        auto lsame = [&](int r) { return r >= 0 && r < nr && s_cls[lb + r] == cl; };
+        if (conn4) {
+            if (lsame(rr) && (rr == rs || !lsame(rr - 1)))
+                lds_union(s_par, a, lb + rr);
+        } else {
+            if (lsame(rr) && !lsame(rr - 1))
+                lds_union(s_par, a, lb + rr);
+            if (rr == rs && lsame(rs - 1))
+                lds_union(s_par, a, lb + rs - 1);
+            if (rr == re && lsame(re + 1) && !lsame(re))
+                lds_union(s_par, a, lb + re + 1);
         }
     }
     __syncthreads();
-    const uint32_t total = s_pre[64], rbase_rec = s_base;
-    const bool ok = size_t(rbase_rec) + size_t(total) <= size_t(rec.cap);
-#pragma unroll
-    for (int j = 0; j < CT_PPT; j++) {
-        if (!((info[j] >> 14) & 1u))
+    // 3. roots of the run starts (a component's root, its smallest index, starts a run)
+    for (int e = t; e < nfg; e += CT_THREADS) {
+        const uint32_t en = s_list[e];
+        const int li = int(en & 0xFFFu);
+        if ((li & 63) == int((en >> 14) & 63u))
+            s_par[li] = lds_find(s_par, li);
+    }
+    __syncthreads();
+    // every pixel points at its root; components with a same-class neighbour across
+    // the tile border are open
+    for (int e = t; e < nfg; e += CT_THREADS) {
+        const uint32_t en = s_list[e];
+        const int li = int(en & 0xFFFu), c = li >> 6, rr = li & 63;
+        const uint32_t cl = (en >> 12) & 3u;
+        const int rs = int((en >> 14) & 63u);
+        const uint32_t root = s_par[c * CT_H + rs] & L_KEY;
+        if (rr != rs)
+            s_par[li] = root;
+        if (!(c == 0 || c == nc - 1 || rr == 0 || rr == nr - 1))
             continue;
-        const uint32_t cl = info[j] & 3u;
-        const int c = w + CT_WAVES * j, li = c * CT_H + lane;
-        const uint32_t slot = s_pre[c] + uint32_t(__popcll(s_mask[c] & lt));
-        const uint32_t pv = s_par[li];
+        bool open = false;
+        if (c == 0)
+            open = open || s_fl[rr + 1] == cl || (!conn4 && (s_fl[rr] == cl || s_fl[rr + 2] == cl));
+        if (c == nc - 1)
+            open = open || s_fr[rr + 1] == cl || (!conn4 && (s_fr[rr] == cl || s_fr[rr + 2] == cl));
+        if (rr == 0)
+            open = open || s_ft[c] == cl || (!conn4 && ((c > 0 && s_ft[c - 1] == cl) || (c + 1 < nc && s_ft[c + 1] == cl)));
+        if (rr == nr - 1)
+            open = open || s_fb[c] == cl || (!conn4 && ((c > 0 && s_fb[c - 1] == cl) || (c + 1 < nc && s_fb[c + 1] == cl)));
+        if (open)
+            atomicOr(&s_par[root], L_OPEN);
+    }
+    __syncthreads();
+
+    // 4. roots in key order: their slots, and the closed roots of every segment
+    const int nk = (nfg + CT_THREADS - 1) / CT_THREADS;
+    for (int k = 0; k < nk; k++) {
+        const int e = k * CT_THREADS + t;
+        bool is_root = false;
+        if (e < nfg) {
+            const uint32_t en = s_list[e];
+            const int li = int(en & 0xFFFu);
+            const uint32_t pv = s_par[li];
+            is_root = (pv & L_KEY) == uint32_t(li);
+            if (is_root && !(pv & L_OPEN))
+                atomicOr(((en >> 12) & 3u) == 1u ? &s_rmp[li >> 6] : &s_rmn[li >> 6], 1ull << (li & 63));
+        }
+        const unsigned long long b = __ballot(is_root);
+        if (lane == 0)
+            s_rpre[k * CT_WAVES + w] = uint32_t(__popcll(b));
+    }
+    __syncthreads();
+    if (w == 0) {
+        const int nch = (nfg + 63) / 64;
+        const uint32_t cnt = lane < nch ? s_rpre[lane] : 0u;
+        uint32_t incl = cnt;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t o = __shfl_up(incl, off);
+            if (lane >= off)
+                incl += o;
+        }
+        s_rpre[lane] = incl - cnt;
+        if (lane == 63) {
+            s_rpre[64] = incl;
+            rec.tcount[blockIdx.x] = incl;
+            s_base = incl > rec.rt ? atomicAdd(rec.pool_count, incl - rec.rt) : 0u;
+        }
+    } else if (w == 1 && lane < nc) {
+        const unsigned seg = seg_index(g, cv0 + lane, ty);
+        rmask[seg] = s_rmp[lane];
+        rmask[g.nseg + seg] = s_rmn[lane];
+    }
+    __syncthreads();
+    const uint32_t total = s_rpre[64], pool_base = s_base;
+    const bool ok = total <= rec.rt || size_t(pool_base) + size_t(total - rec.rt) <= size_t(rec.pool_cap);
+    auto ridx = [&](uint32_t slot) -> uint32_t {
+        return slot < rec.rt ? blockIdx.x * rec.rt + slot : rec.ntiles * rec.rt + pool_base + (slot - rec.rt);
+    };
+    for (int k = 0; k < nk; k++) {
+        const int e = k * CT_THREADS + t;
+        bool is_root = false;
+        uint32_t en = 0, pv = 0;
+        if (e < nfg) {
+            en = s_list[e];
+            pv = s_par[en & 0xFFFu];
+            is_root = (pv & L_KEY) == (en & 0xFFFu);
+        }
+        const unsigned long long b = __ballot(is_root);
+        if (!is_root)
+            continue;
+        const int li = int(en & 0xFFFu), c = li >> 6, rr = li & 63;
+        const uint32_t cl = (en >> 12) & 3u;
+        const uint32_t slot = s_rpre[k * CT_WAVES + w] + uint32_t(__popcll(b & lt));
         s_par[li] = pv | (slot << 16);
         const unsigned long long e0 = cl == 1u ? 0ull : ~0ull;
         if (ok) {
-            const uint32_t k = rbase_rec + slot;
-            const uint32_t gkey = uint32_t(cv0 + c) * uint32_t(g.R) + uint32_t(rbase + lane);
-            rec.key[k] = gkey | (cl == 2u ? P_NEG : 0u) | ((pv & L_OPEN) ? P_OPEN : 0u);
-            rec.pos[k] = ((seg_index(g, cv0 + c, ty) + (cl == 2u ? g.nseg : 0u)) << 6) | uint32_t(lane);
+            const uint32_t kk = ridx(slot);
+            const uint32_t gkey = uint32_t(cv0 + c) * uint32_t(g.R) + uint32_t(rbase + rr);
+            rec.key[kk] = gkey | (cl == 2u ? P_NEG : 0u) | ((pv & L_OPEN) ? P_OPEN : 0u);
+            rec.pos[kk] = ((seg_index(g, cv0 + c, ty) + (cl == 2u ? g.nseg : 0u)) << 6) | uint32_t(rr);
         }
         if (slot < uint32_t(CT_RCAP)) {
             r_ext[slot] = e0;
@@ -992,7 +1192,7 @@ __global__ __launch_bounds__(CT_THREADS) void ccl_tile(const T* __restrict__ x, 
             IslandSum z{};
             z.ext = e0;
             z.idx = ~0u;
-            rec.sum[rbase_rec + slot] = z;
+            rec.sum[ridx(slot)] = z;
         }
     }
     // the barrier's workgroup-scope release orders these stores before the atomics
@@ -1000,16 +1200,21 @@ __global__ __launch_bounds__(CT_THREADS) void ccl_tile(const T* __restrict__ x, 
     __syncthreads();
 
     // 5. island sums: count, extremum, moments (sourcefind.cpp:337-628)
+    double ev[CT_PPT];
 #pragma unroll
-    for (int j = 0; j < CT_PPT; j++) {
-        const uint32_t cl = info[j] & 3u;
-        if (!cl)
+    for (int k = 0; k < CT_PPT; k++) {
+        const int e = k * CT_THREADS + t;
+        ev[k] = 0.0;
+        if (e >= nfg)
             continue;
-        const int c = w + CT_WAVES * j, li = c * CT_H + lane;
+        const uint32_t en = s_list[e];
+        const int li = int(en & 0xFFFu), c = li >> 6, rr = li & 63;
+        const uint32_t cl = (en >> 12) & 3u;
         const uint32_t slot = (s_par[s_par[li] & L_KEY] >> 16) & 0xFFFu;
-        const double val = double(vals[j]);
+        const double val = ldv(x, size_t(mem_index(c, rr)));
+        ev[k] = val;
         const unsigned long long kv = okey(val);
-        const double x_idx = double(cv0 + c), y_idx = double(vr0 + lane);
+        const double x_idx = double(cv0 + c), y_idx = double(vr0 + rr);
         const double xb = x_idx * val, yb = y_idx * val;
         const double xx = xb * x_idx, yy = yb * y_idx, xy = xb * y_idx;
         if (slot < uint32_t(CT_RCAP)) {
@@ -1025,7 +1230,7 @@ __global__ __launch_bounds__(CT_THREADS) void ccl_tile(const T* __restrict__ x, 
             unsafeAtomicAdd(&r_mom[4][slot], xy);
             unsafeAtomicAdd(&r_mom[5][slot], val);
         } else if (ok) {
-            IslandSum& s = rec.sum[rbase_rec + slot];
+            IslandSum& s = rec.sum[ridx(slot)];
             atomicAdd(&s.cnt, 1);
             if (cl == 1u)
                 atomicMax(&s.ext, kv);
@@ -1042,21 +1247,21 @@ __global__ __launch_bounds__(CT_THREADS) void ccl_tile(const T* __restrict__ x, 
     __syncthreads();
     // smallest memory index holding the extremum
 #pragma unroll
-    for (int j = 0; j < CT_PPT; j++) {
-        const uint32_t cl = info[j] & 3u;
-        if (!cl)
+    for (int k = 0; k < CT_PPT; k++) {
+        const int e = k * CT_THREADS + t;
+        if (e >= nfg)
             continue;
-        const int c = w + CT_WAVES * j, li = c * CT_H + lane;
+        const uint32_t en = s_list[e];
+        const int li = int(en & 0xFFFu), c = li >> 6, rr = li & 63;
         const uint32_t slot = (s_par[s_par[li] & L_KEY] >> 16) & 0xFFFu;
-        const unsigned long long kv = okey(double(vals[j]));
-        const uint32_t m = uint32_t(mem_col(g, cv0 + c)) * uint32_t(g.R) + uint32_t(rbase + lane);
+        const unsigned long long kv = okey(ev[k]);
         if (slot < uint32_t(CT_RCAP)) {
             if (kv == r_ext[slot])
-                atomicMin(&r_idx[slot], m);
+                atomicMin(&r_idx[slot], mem_index(c, rr));
         } else if (ok) {
-            IslandSum& s = rec.sum[rbase_rec + slot];
+            IslandSum& s = rec.sum[ridx(slot)];
             if (kv == __hip_atomic_load(&s.ext, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                atomicMin(&s.idx, m);
+                atomicMin(&s.idx, mem_index(c, rr));
         }
     }
     __syncthreads();
@@ -1071,7 +1276,7 @@ __global__ __launch_bounds__(CT_THREADS) void ccl_tile(const T* __restrict__ x, 
 #pragma unroll
             for (int q = 0; q < 6; q++)
                 o.mom[q] = r_mom[q][s];
-            rec.sum[rbase_rec + s] = o;
+            rec.sum[ridx(s)] = o;
         }
     }
 #pragma unroll
@@ -1138,11 +1343,10 @@ __global__ __launch_bounds__(128) void ccl_border(uint32_t* P, TileGeom g, int c
 }
 
 // Open components: local root -> global root; global roots join the root masks.
-__global__ __launch_bounds__(256) void ccl_open_roots(uint32_t* P, TileGeom g, CompRecords rec, const uint32_t* rec_count,
-    unsigned long long* rmask)
+__global__ __launch_bounds__(256) void ccl_open_roots(uint32_t* P, TileGeom g, CompRecords rec, unsigned long long* rmask)
 {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x, nrec = *rec_count;
-    if (nrec > rec.cap || i >= nrec)  // overflowed records are incomplete: the host labels again
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (pool_overflow(rec) || !rec_valid(rec, i))
         return;
     const uint32_t k = rec.key[i];
     if (!(k & P_OPEN))
@@ -1157,19 +1361,11 @@ __global__ __launch_bounds__(256) void ccl_open_roots(uint32_t* P, TileGeom g, C
     }
 }
 
-__global__ __launch_bounds__(256) void ccl_seg_counts(const unsigned long long* rmask, size_t m, uint32_t* cnt)
-{
-    const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i < m)
-        cnt[i] = uint32_t(__popcll(rmask[i]));
-}
-
 // Open components whose local root is the global root start their island's sums.
-__global__ __launch_bounds__(256) void ccl_open_init(const uint32_t* P, SegIds s, CompRecords rec, const uint32_t* rec_count,
-    IslandSum* isum)
+__global__ __launch_bounds__(256) void ccl_open_init(const uint32_t* P, SegIds s, CompRecords rec, IslandSum* isum)
 {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x, nrec = *rec_count;
-    if (nrec > rec.cap || i >= nrec)
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (pool_overflow(rec) || !rec_valid(rec, i))
         return;
     const uint32_t k = rec.key[i];
     const uint32_t lr = k & P_KEY;
@@ -1183,11 +1379,10 @@ __global__ __launch_bounds__(256) void ccl_open_init(const uint32_t* P, SegIds s
 }
 
 // Island sums by list position: closed components are stored, open ones combined.
-__global__ __launch_bounds__(256) void ccl_records(const uint32_t* P, SegIds s, CompRecords rec, const uint32_t* rec_count,
-    IslandSum* isum)
+__global__ __launch_bounds__(256) void ccl_records(const uint32_t* P, SegIds s, CompRecords rec, IslandSum* isum)
 {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x, nrec = *rec_count;
-    if (nrec > rec.cap || i >= nrec)
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (pool_overflow(rec) || !rec_valid(rec, i))
         return;
     const uint32_t k = rec.key[i];
     if (!(k & P_OPEN)) {
@@ -1208,11 +1403,10 @@ __global__ __launch_bounds__(256) void ccl_records(const uint32_t* P, SegIds s, 
 }
 
 // Open islands: smallest memory index among the components holding the extremum.
-__global__ __launch_bounds__(256) void ccl_open_idx(const uint32_t* P, SegIds s, CompRecords rec, const uint32_t* rec_count,
-    IslandSum* isum)
+__global__ __launch_bounds__(256) void ccl_open_idx(const uint32_t* P, SegIds s, CompRecords rec, IslandSum* isum)
 {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x, nrec = *rec_count;
-    if (nrec > rec.cap || i >= nrec)
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (pool_overflow(rec) || !rec_valid(rec, i))
         return;
     const uint32_t k = rec.key[i];
     if (!(k & P_OPEN))
@@ -1309,23 +1503,49 @@ __device__ void moments_fit(stp_island& isl, double x_bar, double y_bar, double 
 // valid: extremum beyond the detection threshold (sourcefind.cpp:417-436);
 // keep: valid and at least min_area samples (sourcefind.cpp:240,263).
 __global__ __launch_bounds__(256) void sf_island_flags(const IslandSum* isum, uint32_t kp, uint32_t kn, const SFState* st,
-    int min_area, uint32_t* keep, uint32_t* valid)
+    int min_area, uint32_t* keep, uint32_t* vcnt)
 {
+    __shared__ uint32_t wv[4];
     const uint32_t g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= kp + kn)
-        return;
-    const bool pos = g < kp;
-    const double ext = okey_inv(isum[g].ext);
-    const bool v = pos ? (ext > st->d_pos) : (ext < st->d_neg);
-    valid[g] = v ? 1u : 0u;
-    keep[g] = (v && isum[g].cnt >= min_area) ? 1u : 0u;
+    bool v = false;
+    if (g < kp + kn) {
+        const bool pos = g < kp;
+        const double ext = okey_inv(isum[g].ext);
+        v = pos ? (ext > st->d_pos) : (ext < st->d_neg);
+        keep[g] = (v && isum[g].cnt >= min_area) ? 1u : 0u;
+    }
+    const uint32_t c = uint32_t(__popcll(__ballot(v)));
+    if ((threadIdx.x & 63) == 0)
+        wv[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        vcnt[blockIdx.x] = wv[0] + wv[1] + wv[2] + wv[3];
 }
 
-// One island record per kept label at its rank among the kept labels.
+// One island record per kept label at its rank among the kept labels; block 0
+// also writes the island count and the count of valid labels (vcnt: one per block
+// of sf_island_flags, the same grid).
 __global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, uint32_t kp, uint32_t kn, const uint32_t* keep,
-    const uint32_t* offs, const SFState* st, int rows, int cols, double analysis_n_sigma, stp_island* out)
+    const uint32_t* offs, const uint32_t* vcnt, SFState* st, int rows, int cols, double analysis_n_sigma, stp_island* out)
 {
+    __shared__ uint32_t red[256];
     const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+    if (blockIdx.x == 0) {
+        uint32_t c = 0;
+        for (uint32_t b = threadIdx.x; b < gridDim.x; b += 256)
+            c += vcnt[b];
+        red[threadIdx.x] = c;
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if (threadIdx.x < off)
+                red[threadIdx.x] += red[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            st->num_valid = red[0];
+            st->num_islands = offs[kp + kn];
+        }
+    }
     if (g >= kp + kn || !keep[g])
         return;
     const bool pos = g < kp;
@@ -1357,12 +1577,6 @@ __global__ void sf_label_counts(SFState* st, const uint32_t* segoff, unsigned ns
     st->fallback = ss->fallback;
 }
 
-__global__ void sf_island_counts(SFState* st, const uint32_t* keep_total, const uint32_t* valid_total)
-{
-    st->num_islands = keep_total ? *keep_total : 0u;
-    st->num_valid = valid_total ? *valid_total : 0u;
-}
-
 __global__ void sf_fallback_clear(SelState* ss)
 {
     ss->fallback = 0;
@@ -1381,7 +1595,8 @@ void d2h_sync(stp_ctx* ctx, void* dst, const void* src, size_t bytes)
     HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
-unsigned hist_blocks(size_t n) { return unsigned(std::max<size_t>(1, std::min<size_t>(512, (n + 16383) / 16384))); }
+// streaming passes: HWG threads per block, one block per 64 Ki elements, at most 256 blocks
+unsigned hist_blocks(size_t n) { return unsigned(std::max<size_t>(1, std::min<size_t>(256, (n + 65535) / 65536))); }
 
 
 __global__ void sf_write_value(double* out, unsigned long long key)
@@ -1406,7 +1621,7 @@ void radix_select(stp_ctx* ctx, const T* x, size_t n, size_t k, double* d_out)
     size_t before = 0, cnt = n;
     while (cnt > size_t(SORT_CAP) && shift > 0) {
         shift -= 8;
-        launch(ctx, "sf_radix_hist", sf_radix_hist<T>, dim3(nb), dim3(SWG), 0, x, n, shift, prefix, int(shift < 56), rbh);
+        launch(ctx, "sf_radix_hist", sf_radix_hist<T>, dim3(nb), dim3(HWG), 0, x, n, shift, prefix, int(shift < 56), rbh);
         fill_u32(ctx, (uint32_t*)rhist, 0u, 2 * 256);
         launch(ctx, "sf_hist_reduce", sf_hist_reduce, dim3(1, std::min(64u, nb)), dim3(256), 0, (const uint32_t*)rbh, int(nb),
             256, rhist, (const int*)nullptr);
@@ -1433,7 +1648,7 @@ void radix_select(stp_ctx* ctx, const T* x, size_t n, size_t k, double* d_out)
     gs.shift = shift;
     gs.prefix = prefix;
     fill_u32(ctx, counter, 0u, 1);
-    launch(ctx, "sf_gather", sf_gather<T>, dim3(nb), dim3(SWG), 0, x, n, gs, cand, counter, unsigned(SORT_CAP));
+    launch(ctx, "sf_gather", sf_gather<T>, dim3(nb), dim3(HWG), 0, x, n, gs, cand, counter, unsigned(SORT_CAP));
     launch(ctx, "sf_sort_select", sf_sort_select, dim3(1), dim3(1024), 0, (const unsigned long long*)cand,
         (const unsigned*)counter, int(k - before), int(k - before), tmp);
     HIP_CHECK(hipMemcpyAsync(d_out, tmp, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
@@ -1443,30 +1658,46 @@ void radix_select(stp_ctx* ctx, const T* x, size_t n, size_t k, double* d_out)
 // st->need_exact is set on the device (force sets it), reads st->mean and
 // st->sigma, and sets st->median; ss->fallback reports data the bins cannot
 // resolve.  No host round trip.
+// Selection set-up of the exact median of n values (buffers of the search).
+SelInit sel_setup(stp_ctx* ctx, size_t n, int force)
+{
+    SelInit si{};
+    si.enabled = 1;
+    si.k_hi = n / 2;
+    si.k_lo = (n & 1) ? n / 2 : n / 2 - 1;
+    si.odd = int(n & 1);
+    si.force = force;
+    si.hist1 = wsp<unsigned long long>(ctx, "sf_hist", LIN_BINS + 2);
+    si.hist2 = wsp<unsigned long long>(ctx, "sf_hist2", LIN_BINS);
+    si.counter = wsp<unsigned>(ctx, "sf_counter", 1);
+    return si;
+}
+
+// Enqueues the device-driven exact median search (SelState).  It runs when
+// st->need_exact is set on the device (si.force sets it), reads st->mean and
+// st->sigma, and sets st->median; ss->fallback reports data the bins cannot
+// resolve.  No host round trip.  initialised: the set-up already ran (fused into
+// the statistics kernel).
 template <typename T>
-void enqueue_exact_median(stp_ctx* ctx, const T* x, size_t n, SFState* st, SelState* ss, int force)
+void enqueue_exact_median(stp_ctx* ctx, const T* x, size_t n, SFState* st, SelState* ss, const SelInit& si, bool initialised)
 {
     const unsigned nb = hist_blocks(n);
     uint32_t* bh = wsp<uint32_t>(ctx, "sf_bh", size_t(nb) * (LIN_BINS + 2));
-    unsigned long long* hist = wsp<unsigned long long>(ctx, "sf_hist", LIN_BINS + 2);
     unsigned long long* cand = wsp<unsigned long long>(ctx, "sf_cand", SORT_CAP);
-    unsigned* counter = wsp<unsigned>(ctx, "sf_counter", 1);
-    const unsigned long long k_hi = n / 2, k_lo = (n & 1) ? n / 2 : n / 2 - 1;
     const unsigned slices = std::min(16u, nb);
-    launch(ctx, "sel_init", sel_init, dim3(1), dim3(1), 0, st, ss, k_lo, k_hi, int(n & 1), force);
-    launch(ctx, "sel_hist1", sel_hist1<T>, dim3(nb), dim3(SWG), 0, x, n, (const SelState*)ss, bh);
-    fill_u32(ctx, (uint32_t*)hist, 0u, 2 * size_t(LIN_BINS + 2));
+    if (!initialised)
+        launch(ctx, "sel_init", sel_init, dim3(1), dim3(256), 0, st, ss, si);
+    launch(ctx, "sel_hist1", sel_hist1<T>, dim3(nb), dim3(HWG), 0, x, n, (const SelState*)ss, bh);
     launch(ctx, "sf_hist_reduce", sf_hist_reduce, dim3(div_up(LIN_BINS + 2, 256), slices), dim3(256), 0,
-        (const uint32_t*)bh, int(nb), LIN_BINS + 2, hist, (const int*)&ss->run1);
-    launch(ctx, "sel_pick1", sel_pick1, dim3(1), dim3(1024), 0, ss, (const unsigned long long*)hist);
-    launch(ctx, "sel_hist2", sel_hist2<T>, dim3(nb), dim3(SWG), 0, x, n, (const SelState*)ss, bh);
-    fill_u32(ctx, (uint32_t*)hist, 0u, 2 * size_t(LIN_BINS));
+        (const uint32_t*)bh, int(nb), LIN_BINS + 2, si.hist1, (const int*)&ss->run1);
+    launch(ctx, "sel_pick1", sel_pick1, dim3(1), dim3(1024), 0, ss, (const unsigned long long*)si.hist1);
+    launch(ctx, "sel_hist2", sel_hist2<T>, dim3(nb), dim3(HWG), 0, x, n, (const SelState*)ss, bh);
     launch(ctx, "sf_hist_reduce", sf_hist_reduce, dim3(div_up(LIN_BINS, 256), slices), dim3(256), 0,
-        (const uint32_t*)bh, int(nb), LIN_BINS, hist, (const int*)&ss->run2);
-    launch(ctx, "sel_pick2", sel_pick2, dim3(1), dim3(1024), 0, ss, (const unsigned long long*)hist);
-    fill_u32(ctx, counter, 0u, 1);
-    launch(ctx, "sel_gather", sel_gather<T>, dim3(nb), dim3(SWG), 0, x, n, (const SelState*)ss, cand, counter);
-    launch(ctx, "sel_sort", sel_sort, dim3(1), dim3(1024), 0, ss, (const unsigned long long*)cand, (const unsigned*)counter, st);
+        (const uint32_t*)bh, int(nb), LIN_BINS, si.hist2, (const int*)&ss->run2);
+    launch(ctx, "sel_pick2", sel_pick2, dim3(1), dim3(1024), 0, ss, (const unsigned long long*)si.hist2);
+    launch(ctx, "sel_gather", sel_gather<T>, dim3(nb), dim3(HWG), 0, x, n, (const SelState*)ss, cand, si.counter);
+    launch(ctx, "sel_sort", sel_sort, dim3(1), dim3(1024), 0, ss, (const unsigned long long*)cand, (const unsigned*)si.counter,
+        st);
 }
 
 // Host-driven fallback: radix selection of both ranks, then the median.
@@ -1491,67 +1722,74 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
         throw Error(STP_ERR_UNSUPPORTED, "Gaussian least-squares fitting is not implemented");
     STP_REQUIRE(p.median_method >= STP_MEDIAN_ZEROMEDIAN && p.median_method <= STP_MEDIAN_NTHELEMENT, "unknown median method");
 
-    SFState* st = wsp<SFState>(ctx, "sf_state", 1);
+    // device state of the search: SFState then SelState, cleared by one fill
+    static_assert(sizeof(SFState) % 8 == 0 && sizeof(SelState) % 8 == 0, "state sizes");
+    char* stbuf = wsp<char>(ctx, "sf_state", sizeof(SFState) + sizeof(SelState));
+    SFState* st = (SFState*)stbuf;
+    SelState* ss = (SelState*)(stbuf + sizeof(SFState));
     double* sel = wsp<double>(ctx, "sf_sel", 2);
     const size_t nblk = (n + CANON_BLOCK - 1) / CANON_BLOCK;
     double* part = wsp<double>(ctx, "sf_part", 2 * nblk + 2);
     double* scr = wsp<double>(ctx, "sf_scr", canon_scratch_doubles(nblk));
-    double* sums = wsp<double>(ctx, "sf_sums", 2);
     const bool need_rms = !(std::fabs(p.rms_est) > 0.0);
     const int mm = p.median_method;
     bool stats_valid = false;
-    auto compute_mean_sigma = [&]() {
+    // mean and sigma (canonical sums); si.enabled also sets up the exact-median search
+    auto compute_mean_sigma = [&](const SelInit& si) {
         launch(ctx, "sf_sum", sf_sum_kernel<T>, dim3(unsigned(nblk)), dim3(SWG), 0, x, n, part);
-        canon_reduce_pairs(ctx, part, nblk, scr, sums);
-        launch(ctx, "sf_stats", sf_mean_sigma_kernel, dim3(1), dim3(1), 0, (const double*)sums, (unsigned long long)n, st);
+        size_t mfin = 0;
+        const double* fin = canon_reduce_to_block(ctx, part, nblk, scr, &mfin);
+        launch(ctx, "sf_stats", sf_stats, dim3(1), dim3(CANON_LANES), 0, fin, unsigned(mfin), (unsigned long long)n, st, ss, si);
         stats_valid = true;
     };
-    SelState* ss = wsp<SelState>(ctx, "sf_selstate", 1);
-    fill_f64(ctx, (double*)st, 0.0, sizeof(SFState) / sizeof(double));
-    fill_f64(ctx, (double*)ss, 0.0, sizeof(SelState) / sizeof(double));
+    fill_f64(ctx, (double*)stbuf, 0.0, (sizeof(SFState) + sizeof(SelState)) / sizeof(double));
     // median (sourcefind.cpp:182-197), enqueued without host round trips: the
     // BINAPPROX pick and the exact search are gated on the device state
     if (mm == STP_MEDIAN_BINAPPROX) {
-        compute_mean_sigma();
+        compute_mean_sigma(SelInit{});
         launch(ctx, "sf_ba_gate", sf_ba_gate, dim3(1), dim3(1), 0, st);
         const unsigned nb = hist_blocks(n);
         uint32_t* bh = wsp<uint32_t>(ctx, "sf_babh", size_t(nb) * (BA_BINS + 1));
         unsigned long long* hist = wsp<unsigned long long>(ctx, "sf_bahist", BA_BINS + 1);
-        launch(ctx, "sf_binapprox_hist", sf_binapprox_hist<T>, dim3(nb), dim3(SWG), 0, x, n, (const SFState*)st, bh);
+        launch(ctx, "sf_binapprox_hist", sf_binapprox_hist<T>, dim3(nb), dim3(HWG), 0, x, n, (const SFState*)st, bh);
         fill_u32(ctx, (uint32_t*)hist, 0u, 2 * size_t(BA_BINS + 1));
         launch(ctx, "sf_hist_reduce", sf_hist_reduce, dim3(div_up(BA_BINS + 1, 256), std::min(64u, nb)), dim3(256), 0,
             (const uint32_t*)bh, int(nb), BA_BINS + 1, hist, (const int*)&st->ba_active);
         launch(ctx, "sf_binapprox_pick", sf_binapprox_pick, dim3(1), dim3(1), 0, (const unsigned long long*)hist,
             (unsigned long long)n, st);
-        enqueue_exact_median<T>(ctx, x, n, st, ss, 0);
+        enqueue_exact_median<T>(ctx, x, n, st, ss, sel_setup(ctx, n, 0), false);
     } else if (mm == STP_MEDIAN_BINMEDIAN || mm == STP_MEDIAN_NTHELEMENT) {
         // mean and sigma seed the selection's linear bins (and estimate_rms,
         // sourcefind.cpp:30-36)
-        compute_mean_sigma();
-        enqueue_exact_median<T>(ctx, x, n, st, ss, 1);
+        const SelInit si = sel_setup(ctx, n, 1);
+        compute_mean_sigma(si);
+        enqueue_exact_median<T>(ctx, x, n, st, ss, si, true);
     }
     // ZEROMEDIAN: the state's median stays 0
     if (need_rms && !stats_valid)
-        compute_mean_sigma();
+        compute_mean_sigma(SelInit{});
 
     // RMS (sourcefind.cpp:206) and thresholds (:298-301)
-    auto rms_and_thresholds = [&]() {
-        if (need_rms) {
-            const int iters = std::max(0, p.sigma_clip_iters);
+    // reset: a counter of the labelling cleared with the thresholds
+    auto rms_and_thresholds = [&](uint32_t* reset) {
+        const ThreshArgs ta{ int(need_rms), p.rms_est, p.detection_n_sigma, p.analysis_n_sigma, reset };
+        const int iters = need_rms ? std::max(0, p.sigma_clip_iters) : 0;
+        if (iters > 0) {
             uint32_t* removed = wsp<uint32_t>(ctx, "sf_removed", nblk);
             double* cpart = wsp<double>(ctx, "sf_cpart", 2 * nblk + 2);
-            double* csums = wsp<double>(ctx, "sf_csums", 2);
-            launch(ctx, "sf_clip_init", sf_clip_init, dim3(1), dim3(1), 0, st, (unsigned long long)n, iters, 3.0);
             for (int it = 0; it < iters; it++) {
-                launch(ctx, "sf_clip", sf_clip_kernel<T>, dim3(unsigned(nblk)), dim3(SWG), 0, x, n, (const SFState*)st, cpart,
-                    removed);
-                canon_reduce_pairs(ctx, cpart, nblk, scr, csums);
-                launch(ctx, "sf_clip_update", sf_clip_update, dim3(1), dim3(256), 0, st, (const double*)csums,
-                    (const uint32_t*)removed, int(nblk));
+                launch(ctx, "sf_clip", sf_clip_kernel<T>, dim3(unsigned(nblk)), dim3(SWG), 0, x, n, (const SFState*)st, it, 3.0,
+                    cpart, removed);
+                size_t mfin = 0;
+                const double* fin = canon_reduce_to_block(ctx, cpart, nblk, scr, &mfin);
+                launch(ctx, "sf_clip_update", sf_clip_update, dim3(1), dim3(256), 0, st, fin, unsigned(mfin),
+                    (const uint32_t*)removed, int(nblk), it, (unsigned long long)n, iters, 3.0, int(it == iters - 1), ta);
             }
+        } else {
+            if (need_rms)
+                launch(ctx, "sf_clip_init", sf_clip_init, dim3(1), dim3(1), 0, st, (unsigned long long)n, 0, 3.0);
+            launch(ctx, "sf_thresholds", sf_thresholds, dim3(1), dim3(1), 0, st, ta);
         }
-        launch(ctx, "sf_thresholds", sf_thresholds, dim3(1), dim3(1), 0, st, int(need_rms), p.rms_est,
-            p.detection_n_sigma, p.analysis_n_sigma);
     };
 
     // connected components and island sums by tiles (ccl.h:107-773, sourcefind.cpp:337-628)
@@ -1560,59 +1798,67 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
     const size_t nseg2 = 2 * size_t(tg.nseg);
     uint32_t* P = wsp<uint32_t>(ctx, "ccl_parent", n);
     unsigned long long* rmask = wsp<unsigned long long>(ctx, "ccl_rmask", nseg2);
-    uint32_t* segcnt = wsp<uint32_t>(ctx, "ccl_segcnt", nseg2);
     uint32_t* segoff = wsp<uint32_t>(ctx, "ccl_segoff", nseg2 + 1);
-    uint32_t* nrec = wsp<uint32_t>(ctx, "ccl_nrec", 1);
     const int find_neg = int(p.find_negative_sources != 0), conn4 = int(p.ccl_4connectivity != 0);
-    // component records: a tile component needs one; at most one per foreground pixel
+    // component records: a region of CT_RT per tile, then a pool for tiles with more
+    uint32_t* tcount = wsp<uint32_t>(ctx, "ccl_tcount", ntiles);
+    uint32_t* pool_count = wsp<uint32_t>(ctx, "ccl_npool", 1);
     CompRecords rec{};
-    auto records = [&](size_t want) {
+    auto records = [&](size_t want_pool) {
         constexpr size_t per = sizeof(IslandSum) + 2 * sizeof(uint32_t);
-        const size_t c = std::min(n, std::max(want, ws_bytes(ctx, "ccl_records") / per));
+        const size_t region = size_t(ntiles) * CT_RT;
+        const size_t have = ws_bytes(ctx, "ccl_records") / per;
+        const size_t pool = std::max(want_pool, have > region ? have - region : size_t(0));
+        STP_REQUIRE(region + pool < (size_t(1) << 32), "image too large for 32-bit component records");
+        const size_t c = region + pool;
         char* b = (char*)ws_get(ctx, "ccl_records", c * per);
         rec.sum = (IslandSum*)b;
         rec.key = (uint32_t*)(b + sizeof(IslandSum) * c);
         rec.pos = rec.key + c;
-        rec.cap = uint32_t(c);
+        rec.tcount = tcount;
+        rec.pool_count = pool_count;
+        rec.rt = CT_RT;
+        rec.ntiles = ntiles;
+        rec.pool_cap = uint32_t(pool);
     };
-    records(std::max<size_t>(65536, n / 16));
-    auto label = [&]() {
-        fill_u32(ctx, nrec, 0u, 1);
+    records(std::max<size_t>(4096, n / 256));
+    // clear_pool: the pool counter was not cleared with the thresholds
+    auto label = [&](bool clear_pool) {
+        if (clear_pool)
+            fill_u32(ctx, pool_count, 0u, 1);
         launch(ctx, "ccl_tile", ccl_tile<T>, dim3(ntiles), dim3(CT_THREADS), 0, x, tg, (const SFState*)st, find_neg, conn4, P,
-            rmask, rec, nrec);
+            rmask, rec);
         launch(ctx, "ccl_border", ccl_border, dim3(ntiles), dim3(128), 0, P, tg, conn4);
-        launch(ctx, "ccl_open_roots", ccl_open_roots, dim3(div_up(rec.cap, 256)), dim3(256), 0, P, tg, rec,
-            (const uint32_t*)nrec, rmask);
-        launch(ctx, "ccl_seg_counts", ccl_seg_counts, dim3(div_up(nseg2, 256)), dim3(256), 0, (const unsigned long long*)rmask,
-            nseg2, segcnt);
-        scan_u32(ctx, segcnt, nseg2, segoff);
+        launch(ctx, "ccl_open_roots", ccl_open_roots, dim3(div_up(size_t(ntiles) * CT_RT + rec.pool_cap, 256)), dim3(256), 0, P,
+            tg, rec, rmask);
+        scan_popc64(ctx, rmask, nseg2, segoff);
         launch(ctx, "sf_counts", sf_label_counts, dim3(1), dim3(1), 0, st, (const uint32_t*)segoff, tg.nseg,
-            (const uint32_t*)nrec, (const SelState*)ss);
+            (const uint32_t*)pool_count, (const SelState*)ss);
     };
     SFState* hs = (SFState*)host_stage(ctx, "sf_state", sizeof(SFState));
     auto read_state = [&]() {
         HIP_CHECK(hipMemcpyAsync(hs, st, sizeof(SFState), hipMemcpyDeviceToHost, ctx->stream));
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
     };
-    rms_and_thresholds();
-    label();
+    rms_and_thresholds(pool_count);
+    label(false);
     read_state();
     if (hs->fallback) {
         // data the linear bins cannot resolve (heavy ties): radix selection on
         // the host-driven path, then the stages that depend on the median again
         radix_median<T>(ctx, x, n, st, sel);
         launch(ctx, "sf_fallback_clear", sf_fallback_clear, dim3(1), dim3(1), 0, ss);
-        rms_and_thresholds();
-        label();
+        rms_and_thresholds(pool_count);
+        label(false);
         read_state();
     }
-    if (hs->num_records > rec.cap) {
+    if (hs->num_records > rec.pool_cap) {
         // more tile components than record slots: grow and label again (the
         // capacity is kept for later calls)
         records(size_t(hs->num_records) + size_t(hs->num_records) / 4);
-        label();
+        label(true);
         read_state();
-        if (hs->num_records > rec.cap)
+        if (hs->num_records > rec.pool_cap)
             throw Error(STP_ERR_INTERNAL, "component records overflow");
     }
     const uint32_t kp = hs->num_pos, kn = hs->num_neg;
@@ -1621,34 +1867,25 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
     const size_t nl = size_t(kp) + size_t(kn);
     IslandSum* isum = wsp<IslandSum>(ctx, "sf_isum", nl);
     const SegIds sid{ tg, rmask, segoff };
-    const unsigned rblocks = div_up(rec.cap, 256);
-    launch(ctx, "ccl_open_init", ccl_open_init, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, (const uint32_t*)nrec,
-        isum);
-    launch(ctx, "ccl_records", ccl_records, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, (const uint32_t*)nrec, isum);
-    launch(ctx, "ccl_open_idx", ccl_open_idx, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, (const uint32_t*)nrec,
-        isum);
+    const unsigned rblocks = div_up(size_t(ntiles) * CT_RT + rec.pool_cap, 256);
+    launch(ctx, "ccl_open_init", ccl_open_init, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
+    launch(ctx, "ccl_records", ccl_records, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
+    launch(ctx, "ccl_open_idx", ccl_open_idx, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
     if (d_lab_out)
         launch(ctx, "ccl_label_map", ccl_label_map, dim3(ntiles), dim3(CT_THREADS), 0, (const uint32_t*)P, sid,
             (const SFState*)st, (const IslandSum*)isum, int(p.generate_labelmap != 0), d_lab_out);
 
     // island list: validity, area cut, compaction and moments fit on the device
     uint32_t* keep = wsp<uint32_t>(ctx, "sf_keep", nl);
-    uint32_t* valid = wsp<uint32_t>(ctx, "sf_valid", nl);
     uint32_t* koff = wsp<uint32_t>(ctx, "sf_koff", nl + 1);
-    uint32_t* voff = wsp<uint32_t>(ctx, "sf_voff", nl + 1);
+    uint32_t* vcnt = wsp<uint32_t>(ctx, "sf_vcnt", div_up(nl, 256));
     stp_island* disl = wsp<stp_island>(ctx, "sf_islands", nl);
     if (nl) {
         launch(ctx, "sf_island_flags", sf_island_flags, dim3(unsigned(div_up(nl, 256))), dim3(256), 0, (const IslandSum*)isum,
-            kp, kn, (const SFState*)st, int(p.source_min_area), keep, valid);
+            kp, kn, (const SFState*)st, int(p.source_min_area), keep, vcnt);
         scan_u32(ctx, keep, nl, koff);
-        scan_u32(ctx, valid, nl, voff);
         launch(ctx, "sf_island_emit", sf_island_emit, dim3(unsigned(div_up(nl, 256))), dim3(256), 0, (const IslandSum*)isum, kp,
-            kn, (const uint32_t*)keep, (const uint32_t*)koff, (const SFState*)st, rows, cols, p.analysis_n_sigma, disl);
-        launch(ctx, "sf_island_counts", sf_island_counts, dim3(1), dim3(1), 0, st, (const uint32_t*)(koff + nl),
-            (const uint32_t*)(voff + nl));
-    } else {
-        launch(ctx, "sf_island_counts", sf_island_counts, dim3(1), dim3(1), 0, st, (const uint32_t*)nullptr,
-            (const uint32_t*)nullptr);
+            kn, (const uint32_t*)keep, (const uint32_t*)koff, (const uint32_t*)vcnt, st, rows, cols, p.analysis_n_sigma, disl);
     }
     read_state();
     const size_t found = hs->num_islands;
