@@ -126,25 +126,50 @@ __device__ __forceinline__ void stream_values(const T* x, size_t n, F&& f)
 }
 
 // ------------------------------------------------------------------ sums
+// A workgroup takes CANON_GROUP consecutive canonical blocks (fewer, longer-lived
+// workgroups), loading the next block's terms while it reduces the current one.
+constexpr int CANON_GROUP = 4;
+
 template <typename T>
-__global__ __launch_bounds__(SWG) void sf_sum_kernel(const T* x, size_t n, double* part)
+__device__ __forceinline__ void canon_load(const T* x, size_t n, size_t blk, T (&v)[CANON_PER_LANE])
 {
-    __shared__ double lds[2 * CANON_LANES];
-    const size_t base = size_t(blockIdx.x) * CANON_BLOCK;
-    double a = 0.0, b = 0.0;
+    const size_t base = blk * CANON_BLOCK + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < CANON_PER_LANE; j++) {
-        const size_t i = base + size_t(j) * CANON_LANES + threadIdx.x;
-        if (i < n) {
-            const double v = ldv(x, i);
-            a = a + v;
-            b = b + v * v;
-        }
+        const size_t i = base + size_t(j) * CANON_LANES;
+        v[j] = i < n ? x[i] : T(0);
     }
-    canon_tree(a, b, lds);
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = a;
-        part[2 * blockIdx.x + 1] = b;
+}
+
+template <typename T>
+__global__ __launch_bounds__(SWG) void sf_sum_kernel(const T* x, size_t n, size_t nblk, double* part)
+{
+    __shared__ double lds[2 * CANON_LANES];
+    const size_t b0 = size_t(blockIdx.x) * CANON_GROUP;
+    T cur[CANON_PER_LANE], nxt[CANON_PER_LANE];
+    canon_load(x, n, b0, cur);
+    for (int g = 0; g < CANON_GROUP && b0 + g < nblk; g++) {
+        const size_t blk = b0 + g;
+        if (g + 1 < CANON_GROUP && blk + 1 < nblk)
+            canon_load(x, n, blk + 1, nxt);
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int j = 0; j < CANON_PER_LANE; j++) {
+            if (blk * CANON_BLOCK + size_t(j) * CANON_LANES + threadIdx.x < n) {
+                const double v = double(cur[j]);
+                a = a + v;
+                b = b + v * v;
+            }
+        }
+        canon_tree(a, b, lds);
+        if (threadIdx.x == 0) {
+            part[2 * blk] = a;
+            part[2 * blk + 1] = b;
+        }
+        __syncthreads();  // lds is rewritten by the next block's tree
+#pragma unroll
+        for (int j = 0; j < CANON_PER_LANE; j++)
+            cur[j] = nxt[j];
     }
 }
 
@@ -624,8 +649,8 @@ __global__ __launch_bounds__(1024) void sel_sort(SelState* ss, const unsigned lo
 
 // ------------------------------------------------------------------ sigma clipping
 template <typename T>
-__global__ __launch_bounds__(SWG) void sf_clip_kernel(const T* x, size_t n, const SFState* st, int iter, double num_sigma,
-    double* part, uint32_t* removed)
+__global__ __launch_bounds__(SWG) void sf_clip_kernel(const T* x, size_t n, size_t nblk, const SFState* st, int iter,
+    double num_sigma, double* part, uint32_t* removed)
 {
     __shared__ double lds[2 * CANON_LANES];
     __shared__ uint32_t wc[SWG / 64];
@@ -641,36 +666,44 @@ __global__ __launch_bounds__(SWG) void sf_clip_kernel(const T* x, size_t n, cons
         prev = st->prev_upper;
     }
     const double median = st->median;
-    const size_t base = size_t(blockIdx.x) * CANON_BLOCK;
-    double a = 0.0, b = 0.0;
-    unsigned cnt = 0;
+    const size_t b0 = size_t(blockIdx.x) * CANON_GROUP;
+    T cur[CANON_PER_LANE], nxt[CANON_PER_LANE];
+    canon_load(x, n, b0, cur);
+    for (int g = 0; g < CANON_GROUP && b0 + g < nblk; g++) {
+        const size_t blk = b0 + g;
+        if (g + 1 < CANON_GROUP && blk + 1 < nblk)
+            canon_load(x, n, blk + 1, nxt);
+        double a = 0.0, b = 0.0;
+        unsigned cnt = 0;
 #pragma unroll
-    for (int j = 0; j < CANON_PER_LANE; j++) {
-        const size_t i = base + size_t(j) * CANON_LANES + threadIdx.x;
-        if (i < n) {
-            const double v = ldv(x, i) - median;
-            const double av = fabs(v);
-            const bool rem = (av > upper) && !(av > prev);
-            a = a + (rem ? v : 0.0);
-            b = b + (rem ? v * v : 0.0);
-            cnt += rem ? 1u : 0u;
+        for (int j = 0; j < CANON_PER_LANE; j++) {
+            if (blk * CANON_BLOCK + size_t(j) * CANON_LANES + threadIdx.x < n) {
+                const double v = double(cur[j]) - median;
+                const double av = fabs(v);
+                const bool rem = (av > upper) && !(av > prev);
+                a = a + (rem ? v : 0.0);
+                b = b + (rem ? v * v : 0.0);
+                cnt += rem ? 1u : 0u;
+            }
         }
-    }
-    canon_tree(a, b, lds);
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = a;
-        part[2 * blockIdx.x + 1] = b;
-    }
-    for (int off = 32; off >= 1; off >>= 1)
-        cnt += __shfl_down(cnt, off, 64);
-    if ((threadIdx.x & 63) == 0)
-        wc[threadIdx.x >> 6] = cnt;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t c = 0;
-        for (int w = 0; w < SWG / 64; w++)
-            c += wc[w];
-        removed[blockIdx.x] = c;
+        canon_tree(a, b, lds);
+        for (int off = 32; off >= 1; off >>= 1)
+            cnt += __shfl_down(cnt, off, 64);
+        if ((threadIdx.x & 63) == 0)
+            wc[threadIdx.x >> 6] = cnt;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            part[2 * blk] = a;
+            part[2 * blk + 1] = b;
+            uint32_t c = 0;
+            for (int w = 0; w < SWG / 64; w++)
+                c += wc[w];
+            removed[blk] = c;
+        }
+        __syncthreads();  // lds and wc are rewritten by the next block
+#pragma unroll
+        for (int j = 0; j < CANON_PER_LANE; j++)
+            cur[j] = nxt[j];
     }
 }
 
@@ -1416,9 +1449,11 @@ __global__ __launch_bounds__(256) void ccl_open_idx(const uint32_t* P, SegIds s,
         atomicMin(&d.idx, rec.sum[i].idx);
 }
 
-// Label map in memory order: +-(id + 1), 0 for background and, with gen_map,
+constexpr int LM_THREADS = 256, LM_WAVES = LM_THREADS / 64, LM_PPT = CT_PIX / LM_THREADS;
+
+// Label map in memory order (LM_THREADS threads per tile): +-(id + 1), 0 for background and, with gen_map,
 // for islands whose extremum misses the detection threshold (sourcefind.cpp:462-492).
-__global__ __launch_bounds__(CT_THREADS) void ccl_label_map(const uint32_t* __restrict__ P, SegIds s, const SFState* st,
+__global__ __launch_bounds__(LM_THREADS) void ccl_label_map(const uint32_t* __restrict__ P, SegIds s, const SFState* st,
     const IslandSum* __restrict__ isum, int gen_map, int32_t* __restrict__ lab)
 {
     const TileGeom& g = s.g;
@@ -1431,18 +1466,18 @@ __global__ __launch_bounds__(CT_THREADS) void ccl_label_map(const uint32_t* __re
         return;
     const double d_pos = st->d_pos, d_neg = st->d_neg;
     const uint32_t kp = s.offs[g.nseg];
-    uint32_t pv[CT_PPT], root[CT_PPT];
+    uint32_t pv[LM_PPT], root[LM_PPT];
 #pragma unroll
-    for (int j = 0; j < CT_PPT; j++) {
-        const int c = w + CT_WAVES * j;
+    for (int j = 0; j < LM_PPT; j++) {
+        const int c = w + LM_WAVES * j;
         pv[j] = c < nc ? P[uint32_t(cv0 + c) * uint32_t(g.R) + uint32_t(rbase + lane)] : P_BG;
     }
 #pragma unroll
-    for (int j = 0; j < CT_PPT; j++)
+    for (int j = 0; j < LM_PPT; j++)
         root[j] = (pv[j] != P_BG && (pv[j] & P_OPEN)) ? (P[pv[j] & P_KEY] & P_KEY) : (pv[j] & P_KEY);
 #pragma unroll
-    for (int j = 0; j < CT_PPT; j++) {
-        const int c = w + CT_WAVES * j;
+    for (int j = 0; j < LM_PPT; j++) {
+        const int c = w + LM_WAVES * j;
         if (c >= nc)
             continue;
         int32_t L = 0;
@@ -1736,7 +1771,7 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
     bool stats_valid = false;
     // mean and sigma (canonical sums); si.enabled also sets up the exact-median search
     auto compute_mean_sigma = [&](const SelInit& si) {
-        launch(ctx, "sf_sum", sf_sum_kernel<T>, dim3(unsigned(nblk)), dim3(SWG), 0, x, n, part);
+        launch(ctx, "sf_sum", sf_sum_kernel<T>, dim3(div_up(nblk, CANON_GROUP)), dim3(SWG), 0, x, n, nblk, part);
         size_t mfin = 0;
         const double* fin = canon_reduce_to_block(ctx, part, nblk, scr, &mfin);
         launch(ctx, "sf_stats", sf_stats, dim3(1), dim3(CANON_LANES), 0, fin, unsigned(mfin), (unsigned long long)n, st, ss, si);
@@ -1778,8 +1813,8 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
             uint32_t* removed = wsp<uint32_t>(ctx, "sf_removed", nblk);
             double* cpart = wsp<double>(ctx, "sf_cpart", 2 * nblk + 2);
             for (int it = 0; it < iters; it++) {
-                launch(ctx, "sf_clip", sf_clip_kernel<T>, dim3(unsigned(nblk)), dim3(SWG), 0, x, n, (const SFState*)st, it, 3.0,
-                    cpart, removed);
+                launch(ctx, "sf_clip", sf_clip_kernel<T>, dim3(div_up(nblk, CANON_GROUP)), dim3(SWG), 0, x, n, nblk,
+                    (const SFState*)st, it, 3.0, cpart, removed);
                 size_t mfin = 0;
                 const double* fin = canon_reduce_to_block(ctx, cpart, nblk, scr, &mfin);
                 launch(ctx, "sf_clip_update", sf_clip_update, dim3(1), dim3(256), 0, st, fin, unsigned(mfin),
@@ -1872,7 +1907,7 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
     launch(ctx, "ccl_records", ccl_records, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
     launch(ctx, "ccl_open_idx", ccl_open_idx, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
     if (d_lab_out)
-        launch(ctx, "ccl_label_map", ccl_label_map, dim3(ntiles), dim3(CT_THREADS), 0, (const uint32_t*)P, sid,
+        launch(ctx, "ccl_label_map", ccl_label_map, dim3(ntiles), dim3(LM_THREADS), 0, (const uint32_t*)P, sid,
             (const SFState*)st, (const IslandSum*)isum, int(p.generate_labelmap != 0), d_lab_out);
 
     // island list: validity, area cut, compaction and moments fit on the device
