@@ -185,16 +185,36 @@ __device__ __forceinline__ void add_unique(int* set, int& cnt, int v)
         set[cnt++] = v;
 }
 
+// Tiles lo / w .. hi / w of the index range [lo, hi] (empty when lo > hi).
+__device__ __forceinline__ void add_tile_range(int* set, int& cnt, int lo, int hi, int w)
+{
+    if (lo > hi)
+        return;
+    for (int t = lo / w, te = hi / w; t <= te; t++)
+        add_unique(set, cnt, t);
+}
+
+// Distinct tile rows and tile columns holding the kept taps of a footprint: the
+// same sets as tap_row / tap_col over every tap, from the at most three contiguous
+// pieces of the raw range [centre - s, centre - s + ks - 1] (below 0, inside,
+// at or above N) instead of tap by tap.
 __device__ __forceinline__ void footprint_tiles(const GridGeom& g, int gx, int gy, int* rt, int& nrt, int* ct, int& nct)
 {
     nrt = nct = 0;
-    for (int i = 0; i < g.ks; i++) {
-        const int r = tap_row(g, gy, i);
-        if (r >= 0)
-            add_unique(rt, nrt, r / TILE_R);
-    }
-    for (int j = 0; j < g.ks; j++)
-        add_unique(ct, nct, tap_col(g, gx, j) / g.tc);
+    const int N = g.N, last = g.rows - 1;
+    int a = gy - g.s, b = a + g.ks - 1;
+    if (a < 0 && g.exact)
+        add_tile_range(rt, nrt, a + N, min(min(b, -1) + N, last), TILE_R);
+    add_tile_range(rt, nrt, max(a, 0), min(min(b, N - 1), last), TILE_R);
+    if (b >= N)
+        add_tile_range(rt, nrt, max(a, N) - N, min(b - N, last), TILE_R);
+    a = gx - g.s;
+    b = a + g.ks - 1;
+    if (a < 0)
+        add_tile_range(ct, nct, a + N, min(b, -1) + N, g.tc);
+    add_tile_range(ct, nct, max(a, 0), min(b, N - 1), g.tc);
+    if (b >= N)
+        add_tile_range(ct, nct, max(a, N) - N, b - N, g.tc);
 }
 
 __global__ __launch_bounds__(GWG) void grid_count_kernel(GridGeom g, const double* uv, const double* vis,
@@ -304,9 +324,10 @@ __global__ void zero_u64_kernel(unsigned long long* p)
 
 // ---------------------------------------------------------------------------
 // Binning by per-block LDS counting sort (no global atomics): block b takes
-// visibilities [4096 b, 4096 b + 4096); counts[tile * nblk + b] holds its
-// entries per tile, so one exclusive scan of the flat array yields both the
-// per-tile entry ranges and each block's write cursor inside every tile.
+// visibilities [4096 b, 4096 b + 4096) and writes its entry count of every tile
+// to counts[b * ntiles + tile] (one contiguous row per block).  A column scan over
+// the blocks turns each row into the block's cursor inside every tile and gives
+// the tile totals, whose exclusive scan is the per-tile entry ranges.
 constexpr int BWG = 1024;
 constexpr int VPB = CANON_BLOCK; // visibilities per binning block (also the canonical sum block)
 constexpr int MAX_LDS_TILES = 24576;
@@ -365,7 +386,7 @@ __global__ __launch_bounds__(BWG) void grid_bin_count_kernel(GridGeom g, const d
         wpart[2 * blockIdx.x + 1] = 0.0;
     }
     for (int t = threadIdx.x; t < ntiles; t += BWG)
-        counts[size_t(t) * nblk + blockIdx.x] = hist[t];
+        counts[size_t(blockIdx.x) * ntiles + t] = hist[t];
     if (ngood) {
         for (int off = 32; off >= 1; off >>= 1)
             ng += __shfl_down(ng, off, 64);
@@ -374,13 +395,55 @@ __global__ __launch_bounds__(BWG) void grid_bin_count_kernel(GridGeom g, const d
     }
 }
 
+// Per tile t: counts[b][t] -> entries of tile t in blocks before b (in place); totals[t].
+// A workgroup takes 64 tiles (lanes) x 16 slices of the blocks (waves).
+__global__ __launch_bounds__(1024) void bin_colscan_kernel(uint32_t* counts, int nblk, int ntiles, uint32_t* totals)
+{
+    __shared__ uint32_t part[16][64];
+    const int lane = threadIdx.x & 63, sl = threadIdx.x >> 6;
+    const int t = blockIdx.x * 64 + lane;
+    const int per = (nblk + 15) / 16, b0 = min(nblk, sl * per), b1 = min(nblk, b0 + per);
+    uint32_t s = 0;
+    if (t < ntiles)
+        for (int b = b0; b < b1; b++)
+            s += counts[size_t(b) * ntiles + t];
+    part[sl][lane] = s;
+    __syncthreads();
+    if (sl == 0) {
+        uint32_t run = 0;
+        for (int k = 0; k < 16; k++) {
+            const uint32_t v = part[k][lane];
+            part[k][lane] = run;
+            run += v;
+        }
+        if (t < ntiles)
+            totals[t] = run;
+    }
+    __syncthreads();
+    if (t >= ntiles)
+        return;
+    uint32_t run = part[sl][lane];
+    for (int bb = b0; bb < b1; bb += 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            v[k] = bb + k < b1 ? counts[size_t(bb + k) * ntiles + t] : 0u;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            if (bb + k < b1)
+                counts[size_t(bb + k) * ntiles + t] = run;
+            run += v[k];
+        }
+    }
+}
+
 __global__ __launch_bounds__(BWG) void grid_bin_scatter_kernel(GridGeom g, const double* uv, const double* vis, const double* wts,
-    const uint32_t* flat_off, int nblk, Entry* entries, uint32_t cap)
+    const uint32_t* tile_off, const uint32_t* block_off, Entry* entries, uint32_t cap)
 {
     extern __shared__ uint32_t cur[];
     const int ntiles = g.tiles_r * g.tiles_c;
     for (int t = threadIdx.x; t < ntiles; t += BWG)
-        cur[t] = flat_off[size_t(t) * nblk + blockIdx.x];
+        cur[t] = tile_off[t] + block_off[size_t(blockIdx.x) * ntiles + t];
     __syncthreads();
     const long long base = (long long)blockIdx.x * VPB;
     for (int j = 0; j < VPB / BWG; j++) {
@@ -1107,17 +1170,21 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     if (ntiles <= MAX_LDS_TILES) {
         const size_t m = size_t(ntiles) * nblk;
         uint32_t* counts = (uint32_t*)ws_get(ctx, "grid_bcounts", sizeof(uint32_t) * m);
-        uint32_t* flat = (uint32_t*)ws_get(ctx, "grid_boffsets", sizeof(uint32_t) * (m + 1));
+        uint32_t* totals = (uint32_t*)ws_get(ctx, "grid_btotals", sizeof(uint32_t) * ntiles);
+        uint32_t* toffs = (uint32_t*)ws_get(ctx, "grid_boffsets", sizeof(uint32_t) * (ntiles + 1));
+        uint32_t* tcur = (uint32_t*)ws_get(ctx, "grid_bcursor", sizeof(uint32_t) * ntiles);
         const size_t hist_bytes = sizeof(uint32_t) * size_t(ntiles);
         HIP_CHECK(hipFuncSetAttribute((const void*)grid_bin_count_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(hist_bytes)));
         HIP_CHECK(hipFuncSetAttribute((const void*)grid_bin_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(hist_bytes)));
         launch(ctx, "grid_count", grid_bin_count_kernel, dim3(unsigned(nblk)), dim3(BWG), hist_bytes, g, d_uv, d_vis, d_w, counts,
             int(nblk), wpart, d_ngood);
-        scan_u32(ctx, counts, m, flat);
+        launch(ctx, "grid_colscan", bin_colscan_kernel, dim3(div_up(ntiles, 64)), dim3(1024), 0, counts, int(nblk), int(ntiles),
+            totals);
+        exclusive_scan_u32(ctx, totals, int(ntiles), toffs, tcur);
         launch(ctx, "grid_scatter", grid_bin_scatter_kernel, dim3(unsigned(nblk)), dim3(BWG), hist_bytes, g, d_uv, d_vis, d_w,
-            (const uint32_t*)flat, int(nblk), entries, cap);
-        offsets = flat;
-        ostride = int(nblk);
+            (const uint32_t*)toffs, (const uint32_t*)counts, entries, cap);
+        offsets = toffs;
+        ostride = 1;
     } else {
         // very large grids: global per-tile counters
         uint32_t* counts = (uint32_t*)ws_get(ctx, "grid_counts", sizeof(uint32_t) * ntiles);
