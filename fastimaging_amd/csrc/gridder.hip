@@ -680,6 +680,100 @@ __global__ __launch_bounds__(TWG) void grid_tiles_kernel(GridGeom g, const Entry
     }
 }
 
+// Oversampled-kernel tiles with ks <= KSM taps per side (the common case): the
+// same lane mapping and tap arithmetic as grid_tiles_kernel, specialised so a
+// lane preloads its entry's ks kernel-row values, computes row validity without
+// branches, and sends the taps that miss the tile to a trash cell of its own
+// (one per lane after each LDS plane) instead of masking the atomics.
+template <typename T, typename Acc, int KSM>
+__global__ __launch_bounds__(TWG) void grid_tiles_os_kernel(GridGeom g, const Entry* entries, const uint32_t* offsets,
+    int ostride, uint32_t cap, const double* ktab, cplx<T>* vgrid, cplx<T>* bgrid)
+{
+#pragma clang fp contract(off)
+    using AS = typename Acc::S;
+    extern __shared__ double dyn[];
+    __shared__ Entry ebuf[TEB];
+    const int ks = g.ks, s = g.s;
+    const int S = TILE_R + (ks <= 15 ? ks : 1);     // column stride of the LDS tile
+    const int cells = g.tc * S;
+    const int plane = cells + 64;                    // + one trash cell per lane
+    AS* acc = reinterpret_cast<AS*>(dyn);
+    const int acc_bytes = Acc::arrays * plane * int(sizeof(AS));
+    T* kt = reinterpret_cast<T*>(dyn + (acc_bytes + 7) / 8);
+    const int cs = (g.os / 2) * 2 + 1;
+    for (int k = threadIdx.x; k < cs * ks; k += TWG)
+        kt[k] = T(ktab[k]);
+    for (int k = threadIdx.x; k < Acc::arrays * plane; k += TWG)
+        acc[k] = AS(0);
+    const int t = blockIdx.x;
+    const int rt = t % g.tiles_r, ct = t / g.tiles_r;
+    const int r0 = rt * TILE_R, c0 = ct * g.tc;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int epw = 64 / ks;                         // entries per wave pass
+    const int slot = lane / ks;
+    const int jl = lane - slot * ks;
+    const bool lane_on = slot < epw;
+    const int trash = cells + lane;
+    const uint32_t eb = min(offsets[size_t(t) * ostride], cap), ee = min(offsets[size_t(t + 1) * ostride], cap);
+    for (uint32_t e0 = eb; e0 < ee; e0 += uint32_t(TEB)) {
+        const int cnt = int(min(uint32_t(TEB), ee - e0));
+        __syncthreads();
+        if (int(threadIdx.x) < cnt)
+            ebuf[threadIdx.x] = entries[e0 + threadIdx.x];
+        __syncthreads();
+        for (int q0 = wave * epw; q0 < cnt; q0 += (TWG / 64) * epw) {
+            const int q = q0 + slot;
+            if (!lane_on || q >= cnt)
+                continue;
+            const Entry& E = ebuf[q];
+            const int lc = tap_col(g, E.gx, jl) - c0;
+            if (lc < 0 || lc >= g.tc)
+                continue;
+            const int cpx = E.cp & 0xffff, cpy = E.cp >> 16;
+            const T w = T(E.w), vr = T(E.vr), vim = T(E.vi);
+            const T kx = kt[cpx * ks + jl];
+            T ky[KSM];
+#pragma unroll
+            for (int i = 0; i < KSM; i++)
+                ky[i] = i < ks ? kt[cpy * ks + i] : T(0);
+            const int rb = E.gy - s, cb = lc * S;
+#pragma unroll
+            for (int i0 = 0; i0 < KSM; i0 += RG) {
+                int aa[RG];
+                T xr[RG], xi[RG], xb[RG];
+#pragma unroll
+                for (int u = 0; u < RG; u++) {
+                    const int i = i0 + u;
+                    // tap_row: rows < 0 skipped, rows >= N wrapped, rows >= rows dropped
+                    int r = rb + i;
+                    bool ok = i < ks && r >= 0;
+                    r = r >= g.N ? r - g.N : r;
+                    const int lr = r - r0;
+                    ok = ok && r < g.rows && lr >= 0 && lr < TILE_R;
+                    const T kw = (ky[i] * kx) * w; // gridder.h:854-858
+                    aa[u] = ok ? cb + lr : trash;
+                    xr[u] = vr * kw;
+                    xi[u] = vim * kw;
+                    xb[u] = kw;
+                }
+                Acc::template add_rows<RG, T>(acc, plane, aa, xr, xi, xb, g.beam != 0);
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < g.tc * TILE_R; k += TWG) {
+        const int lr = k % TILE_R, lc = k / TILE_R;
+        const int r = r0 + lr, c = c0 + lc;
+        if (r < g.rows && c < g.N) {
+            const size_t o = size_t(r) + size_t(c) * g.rows;
+            const int a = lc * S + lr;
+            vgrid[o] = { T(Acc::value(acc, plane, 0, a)), T(Acc::value(acc, plane, 1, a)) };
+            if (g.beam)
+                bgrid[o] = { T(Acc::value(acc, plane, 2, a)), T(0) };
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // W-projection (gridder.h:311-367,467-486,767-842; aw_projection.cpp:19-210,366-474)
 //
@@ -1211,11 +1305,22 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         const int S = TILE_R + (g.ks <= 15 ? g.ks : 1);
         const size_t acc_bytes = twofloat ? 6 * size_t(g.tc) * S * sizeof(float) : 3 * size_t(g.tc) * S * sizeof(double);
         const size_t kv_bytes = a.kernel_exact ? sizeof(double) * (KV_DOUBLES + TEB) : sizeof(double) * size_t(cs) * g.ks;
-        const size_t dyn_bytes = ((acc_bytes + 7) / 8) * 8 + kv_bytes;
-        auto kern = twofloat ? grid_tiles_kernel<T, AccTwoFloat> : grid_tiles_kernel<T, AccF64>;
-        HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(dyn_bytes)));
-        launch(ctx, "grid_tiles", kern, dim3(ntiles), dim3(TWG), dyn_bytes, g, (const Entry*)entries, offsets, ostride, cap,
-            d_tab, d_uv, d_vis_grid, d_sampling_grid);
+        if (!a.kernel_exact && g.ks <= 16) {
+            // oversampled kernels up to 16 taps: the specialised tile kernel (trash cells after each plane)
+            const size_t os_acc = (twofloat ? 6 * sizeof(float) : 3 * sizeof(double)) * (size_t(g.tc) * S + 64);
+            const size_t os_bytes = ((os_acc + 7) / 8) * 8 + kv_bytes;
+            auto kern = g.ks <= 8 ? (twofloat ? grid_tiles_os_kernel<T, AccTwoFloat, 8> : grid_tiles_os_kernel<T, AccF64, 8>)
+                                  : (twofloat ? grid_tiles_os_kernel<T, AccTwoFloat, 16> : grid_tiles_os_kernel<T, AccF64, 16>);
+            HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(os_bytes)));
+            launch(ctx, "grid_tiles", kern, dim3(ntiles), dim3(TWG), os_bytes, g, (const Entry*)entries, offsets, ostride, cap,
+                d_tab, d_vis_grid, d_sampling_grid);
+        } else {
+            const size_t dyn_bytes = ((acc_bytes + 7) / 8) * 8 + kv_bytes;
+            auto kern = twofloat ? grid_tiles_kernel<T, AccTwoFloat> : grid_tiles_kernel<T, AccF64>;
+            HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(dyn_bytes)));
+            launch(ctx, "grid_tiles", kern, dim3(ntiles), dim3(TWG), dyn_bytes, g, (const Entry*)entries, offsets, ostride, cap,
+                d_tab, d_uv, d_vis_grid, d_sampling_grid);
+        }
     }
     // sample_grid_total = 2 * sum(good weights) (gridder.h:415-423)
     canon_reduce_pairs(ctx, wpart, nblk, wscr, wres);
