@@ -126,6 +126,14 @@ struct Shape {
     static constexpr int NT = WG / TPT;
     static constexpr int PAD = 1;
 };
+// u passes: two radix-4 butterflies per thread and stage, so a workgroup holds
+// twice the rows (longer contiguous runs of each grid column)
+template <int L>
+struct UShape {
+    static constexpr int TPT = (L / 8 < 1) ? 1 : ((L / 8 > WG) ? WG : L / 8);
+    static constexpr int NT = WG / TPT;
+    static constexpr int PAD = 1;
+};
 
 // U1: for each u2 and block of NT rows, FFT over u1 (stride L2 columns), then
 // multiply by exp(+-2 pi i k1 u2 / N).
@@ -133,7 +141,7 @@ template <typename T, int L1>
 __global__ __launch_bounds__(WG) void fft_u1(cplx<T>* grids, int rows, int N, int L2, size_t grid_elems,
     const T* tw, int inverse)
 {
-    using S = Shape<L1>;
+    using S = UShape<L1>;
     __shared__ cplx<T> buf[S::NT * (L1 + S::PAD)];
     const int u2 = blockIdx.x;
     const int v0 = blockIdx.y * S::NT;
@@ -166,7 +174,7 @@ template <typename T, int L2>
 __global__ __launch_bounds__(WG) void fft_u2(cplx<T>* grids, int rows, int N, int L1, size_t grid_elems,
     const T* tw, int inverse)
 {
-    using S = Shape<L2>;
+    using S = UShape<L2>;
     __shared__ cplx<T> buf[S::NT * (L2 + S::PAD)];
     const int k1 = blockIdx.x;
     const int v0 = blockIdx.y * S::NT;
@@ -330,7 +338,7 @@ void launch_u_passes(stp_ctx* ctx, cplx<T>* grids, int nbatch, int rows, int N, 
     switch (L1) {
 #define U1CASE(LL)                                                                                          \
     case LL:                                                                                                \
-        launch(ctx, "fft_u1", fft_u1<T, LL>, dim3(L2, div_up(rows, Shape<LL>::NT), nbatch), dim3(WG), 0,   \
+        launch(ctx, "fft_u1", fft_u1<T, LL>, dim3(L2, div_up(rows, UShape<LL>::NT), nbatch), dim3(WG), 0,   \
             grids, rows, N, L2, grid_elems, tw, int(inverse));                                              \
         break;
         STP_POW2_CASES(U1CASE)
@@ -341,7 +349,7 @@ void launch_u_passes(stp_ctx* ctx, cplx<T>* grids, int nbatch, int rows, int N, 
     switch (L2) {
 #define U2CASE(LL)                                                                                          \
     case LL:                                                                                                \
-        launch(ctx, "fft_u2", fft_u2<T, LL>, dim3(L1, div_up(rows, Shape<LL>::NT), nbatch), dim3(WG), 0,   \
+        launch(ctx, "fft_u2", fft_u2<T, LL>, dim3(L1, div_up(rows, UShape<LL>::NT), nbatch), dim3(WG), 0,   \
             grids, rows, N, L1, grid_elems, tw, int(inverse));                                              \
         break;
         STP_POW2_CASES(U2CASE)
