@@ -928,13 +928,14 @@ __device__ __forceinline__ bool rec_valid(const CompRecords& rec, uint32_t i)
 // An overflowed pool leaves records unwritten: the host grows it and labels again.
 __device__ __forceinline__ bool pool_overflow(const CompRecords& rec) { return *rec.pool_count > rec.pool_cap; }
 
+// Plain LDS loads: every load has a new address, and the atomics of lds_union
+// order the reloads after a failed link.
 __device__ __forceinline__ uint32_t lds_find(const uint32_t* par, uint32_t a)
 {
-    const volatile uint32_t* vp = par;
-    uint32_t p = vp[a];
+    uint32_t p = par[a];
     while (p != a) {
         a = p;
-        p = vp[a];
+        p = par[a];
     }
     return a;
 }
