@@ -18,6 +18,9 @@ constexpr int CANON_BLOCK = CANON_LANES * CANON_PER_LANE;
 
 // Combines the lane pair (a, b) of all 256 threads of the block in the canonical
 // tree; thread 0 returns the block sums.  lds: 512 doubles.
+// The levels off = 128 and 64 cross waves (LDS); the levels 32 .. 1 stay inside
+// wave 0, where lane t adds lane t + off by a shuffle: the same pairs, so the
+// same sums.
 __device__ __forceinline__ void canon_tree(double& a, double& b, double* lds)
 {
 #pragma clang fp contract(off)
@@ -25,13 +28,24 @@ __device__ __forceinline__ void canon_tree(double& a, double& b, double* lds)
     lds[t] = a;
     lds[CANON_LANES + t] = b;
     __syncthreads();
-    for (int off = CANON_LANES / 2; off >= 1; off >>= 1) {
-        if (t < off) {
-            lds[t] = lds[t] + lds[t + off];
-            lds[CANON_LANES + t] = lds[CANON_LANES + t] + lds[CANON_LANES + t + off];
-        }
-        __syncthreads();
+    if (t < 128) {
+        lds[t] = lds[t] + lds[t + 128];
+        lds[CANON_LANES + t] = lds[CANON_LANES + t] + lds[CANON_LANES + t + 128];
     }
+    __syncthreads();
+    if (t < 64) {
+        double x = lds[t] + lds[t + 64];
+        double y = lds[CANON_LANES + t] + lds[CANON_LANES + t + 64];
+        for (int off = 32; off >= 1; off >>= 1) {
+            x = x + __shfl_down(x, off, 64);
+            y = y + __shfl_down(y, off, 64);
+        }
+        if (t == 0) {
+            lds[0] = x;
+            lds[CANON_LANES] = y;
+        }
+    }
+    __syncthreads();
     a = lds[0];
     b = lds[CANON_LANES];
 }
