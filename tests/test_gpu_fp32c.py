@@ -57,6 +57,18 @@ def test_fp32c_imager_c1(gpu, orc, precision):
     assert e_img > 1e-10
 
 
+@pytest.mark.parametrize("support,precision", [(7, capi.PRECISION_FP32C), (7, capi.PRECISION_FP32_TWOFLOAT),
+                                               (10, capi.PRECISION_FP32C), (10, capi.PRECISION_FP32_TWOFLOAT)])
+def test_fp32_wide_kernels(gpu, orc, support, precision):
+    """Oversampled kernels of 15 taps (the specialised tile kernel's 16-row form) and of
+    21 taps (the general tile kernel) in both FP32 forms, within 1e-4 of the peak."""
+    snap = simdata.snapshot(256, 4.0, 20000, 5, seed=90 + support)
+    pars = capi.imager_pars(256, 4.0, kernel_support=support, oversampling=8)
+    img, beam = gpu.image_visibilities(snap["vis"], snap["weights"], snap["uvw"], pars, precision=precision)
+    oimg, obeam, _, _ = orc.image_visibilities(pars, snap["vis"], snap["weights"], snap["uvw"])
+    assert peak_rel_err(img, oimg) < FP32_TOL and peak_rel_err(beam, obeam) < FP32_TOL
+
+
 @pytest.mark.parametrize("exact,precision", [(False, capi.PRECISION_FP32C), (True, capi.PRECISION_FP32C),
                                              (True, capi.PRECISION_FP32_TWOFLOAT)])
 def test_fp32c_exact_padding_gcf(gpu, orc, exact, precision):

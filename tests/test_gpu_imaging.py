@@ -45,6 +45,7 @@ def test_gpu_gridder_kat(gpu, orc, case):
     (128, 2000, 3, 1, True, True, True),
     (64, 500, 5, 4, False, True, True),
     (128, 1000, 7, 8, False, True, False),
+    (128, 1500, 10, 4, False, True, True),  # 21 taps: the general tile kernel
 ])
 def test_gpu_gridder_random(gpu, orc, N, n, s, os_, exact, halfplane, shift):
     rng = np.random.default_rng(N + n + s)
