@@ -682,9 +682,9 @@ __global__ __launch_bounds__(TWG) void grid_tiles_kernel(GridGeom g, const Entry
 
 // Oversampled-kernel tiles with ks <= KSM taps per side (the common case): the
 // same lane mapping and tap arithmetic as grid_tiles_kernel, specialised so a
-// lane preloads its entry's ks kernel-row values, computes row validity without
-// branches, and sends the taps that miss the tile to a trash cell of its own
-// (one per lane after each LDS plane) instead of masking the atomics.
+// lane preloads its entry's ks kernel-row values and computes row validity
+// without branches; taps that miss the tile issue no LDS atomic (the LDS FP64
+// atomic rate bounds this kernel).
 template <typename T, typename Acc, int KSM>
 __global__ __launch_bounds__(TWG) void grid_tiles_os_kernel(GridGeom g, const Entry* entries, const uint32_t* offsets,
     int ostride, uint32_t cap, const double* ktab, cplx<T>* vgrid, cplx<T>* bgrid)
@@ -695,8 +695,7 @@ __global__ __launch_bounds__(TWG) void grid_tiles_os_kernel(GridGeom g, const En
     __shared__ Entry ebuf[TEB];
     const int ks = g.ks, s = g.s;
     const int S = TILE_R + (ks <= 15 ? ks : 1);     // column stride of the LDS tile
-    const int cells = g.tc * S;
-    const int plane = cells + 64;                    // + one trash cell per lane
+    const int plane = g.tc * S;
     AS* acc = reinterpret_cast<AS*>(dyn);
     const int acc_bytes = Acc::arrays * plane * int(sizeof(AS));
     T* kt = reinterpret_cast<T*>(dyn + (acc_bytes + 7) / 8);
@@ -713,7 +712,6 @@ __global__ __launch_bounds__(TWG) void grid_tiles_os_kernel(GridGeom g, const En
     const int slot = lane / ks;
     const int jl = lane - slot * ks;
     const bool lane_on = slot < epw;
-    const int trash = cells + lane;
     const uint32_t eb = min(offsets[size_t(t) * ostride], cap), ee = min(offsets[size_t(t + 1) * ostride], cap);
     for (uint32_t e0 = eb; e0 < ee; e0 += uint32_t(TEB)) {
         const int cnt = int(min(uint32_t(TEB), ee - e0));
@@ -751,7 +749,7 @@ __global__ __launch_bounds__(TWG) void grid_tiles_os_kernel(GridGeom g, const En
                     const int lr = r - r0;
                     ok = ok && r < g.rows && lr >= 0 && lr < TILE_R;
                     const T kw = (ky[i] * kx) * w; // gridder.h:854-858
-                    aa[u] = ok ? cb + lr : trash;
+                    aa[u] = ok ? cb + lr : -1;
                     xr[u] = vr * kw;
                     xi[u] = vim * kw;
                     xb[u] = kw;
@@ -1323,8 +1321,8 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         const size_t acc_bytes = twofloat ? 6 * size_t(g.tc) * S * sizeof(float) : 3 * size_t(g.tc) * S * sizeof(double);
         const size_t kv_bytes = a.kernel_exact ? sizeof(double) * (KV_DOUBLES + TEB) : sizeof(double) * size_t(cs) * g.ks;
         if (!a.kernel_exact && g.ks <= 16) {
-            // oversampled kernels up to 16 taps: the specialised tile kernel (trash cells after each plane)
-            const size_t os_acc = (twofloat ? 6 * sizeof(float) : 3 * sizeof(double)) * (size_t(g.tc) * S + 64);
+            // oversampled kernels up to 16 taps: the specialised tile kernel
+            const size_t os_acc = (twofloat ? 6 * sizeof(float) : 3 * sizeof(double)) * (size_t(g.tc) * S);
             const size_t os_bytes = ((os_acc + 7) / 8) * 8 + kv_bytes;
             auto kern = g.ks <= 8 ? (twofloat ? grid_tiles_os_kernel<T, AccTwoFloat, 8> : grid_tiles_os_kernel<T, AccF64, 8>)
                                   : (twofloat ? grid_tiles_os_kernel<T, AccTwoFloat, 16> : grid_tiles_os_kernel<T, AccF64, 16>);
