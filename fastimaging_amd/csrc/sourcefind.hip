@@ -1004,8 +1004,7 @@ __global__ __launch_bounds__(CT_THREADS) void ccl_tile(const T* __restrict__ x, 
     __shared__ unsigned long long s_mask[CT_W];        // foreground of each column
     __shared__ unsigned long long s_rmp[CT_W], s_rmn[CT_W];  // closed roots of each column
     __shared__ uint32_t s_pre[CT_W + 1];
-    __shared__ uint32_t s_rpre[CT_PIX / 64 + 1];
-    __shared__ uint32_t s_base;
+    __shared__ uint32_t s_base, s_nroot;
     __shared__ unsigned long long r_ext[CT_RCAP];
     __shared__ uint32_t r_idx[CT_RCAP];
     __shared__ int r_cnt[CT_RCAP];
@@ -1045,6 +1044,8 @@ __global__ __launch_bounds__(CT_THREADS) void ccl_tile(const T* __restrict__ x, 
         s_rmp[t] = 0;
         s_rmn[t] = 0;
     }
+    if (t == 0)
+        s_nroot = 0;
     // classes of the frame around the tile (0 outside the image)
     for (int q = t; q < 2 * (CT_H + 2) + 2 * CT_W; q += CT_THREADS) {
         uint32_t cl = 0;
@@ -1122,8 +1123,8 @@ __global__ __launch_bounds__(CT_THREADS) void ccl_tile(const T* __restrict__ x, 
             s_par[li] = lds_find(s_par, li);
     }
     __syncthreads();
-    // every pixel points at its root; components with a same-class neighbour across
-    // the tile border are open
+    // every pixel points at its root, roots take record slots; components with a
+    // same-class neighbour across the tile border are open
     for (int e = t; e < nfg; e += CT_THREADS) {
         const uint32_t en = s_list[e];
         const int li = int(en & 0xFFFu), c = li >> 6, rr = li & 63;
@@ -1132,6 +1133,8 @@ __global__ __launch_bounds__(CT_THREADS) void ccl_tile(const T* __restrict__ x, 
         const uint32_t root = s_par[c * CT_H + rs] & L_KEY;
         if (rr != rs)
             s_par[li] = root;
+        else if (root == uint32_t(li))  // a root: its slot (any order serves)
+            atomicOr(&s_par[li], atomicAdd(&s_nroot, 1u) << 16);
         if (!(c == 0 || c == nc - 1 || rr == 0 || rr == nr - 1))
             continue;
         bool open = false;
@@ -1148,85 +1151,60 @@ __global__ __launch_bounds__(CT_THREADS) void ccl_tile(const T* __restrict__ x, 
     }
     __syncthreads();
 
-    // 4. roots in key order: their slots, and the closed roots of every segment
-    const int nk = (nfg + CT_THREADS - 1) / CT_THREADS;
-    for (int k = 0; k < nk; k++) {
-        const int e = k * CT_THREADS + t;
-        bool is_root = false;
-        if (e < nfg) {
-            const uint32_t en = s_list[e];
-            const int li = int(en & 0xFFFu);
-            const uint32_t pv = s_par[li];
-            is_root = (pv & L_KEY) == uint32_t(li);
-            if (is_root && !(pv & L_OPEN))
-                atomicOr(((en >> 12) & 3u) == 1u ? &s_rmp[li >> 6] : &s_rmn[li >> 6], 1ull << (li & 63));
-        }
-        const unsigned long long b = __ballot(is_root);
-        if (lane == 0)
-            s_rpre[k * CT_WAVES + w] = uint32_t(__popcll(b));
-    }
-    __syncthreads();
-    if (w == 0) {
-        const int nch = (nfg + 63) / 64;
-        const uint32_t cnt = lane < nch ? s_rpre[lane] : 0u;
-        uint32_t incl = cnt;
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t o = __shfl_up(incl, off);
-            if (lane >= off)
-                incl += o;
-        }
-        s_rpre[lane] = incl - cnt;
-        if (lane == 63) {
-            s_rpre[64] = incl;
-            rec.tcount[blockIdx.x] = incl;
-            s_base = incl > rec.rt ? atomicAdd(rec.pool_count, incl - rec.rt) : 0u;
-        }
-    } else if (w == 1 && lane < nc) {
-        const unsigned seg = seg_index(g, cv0 + lane, ty);
-        rmask[seg] = s_rmp[lane];
-        rmask[g.nseg + seg] = s_rmn[lane];
-    }
-    __syncthreads();
-    const uint32_t total = s_rpre[64], pool_base = s_base;
-    const bool ok = total <= rec.rt || size_t(pool_base) + size_t(total - rec.rt) <= size_t(rec.pool_cap);
-    auto ridx = [&](uint32_t slot) -> uint32_t {
-        return slot < rec.rt ? blockIdx.x * rec.rt + slot : rec.ntiles * rec.rt + pool_base + (slot - rec.rt);
-    };
-    for (int k = 0; k < nk; k++) {
-        const int e = k * CT_THREADS + t;
-        bool is_root = false;
-        uint32_t en = 0, pv = 0;
-        if (e < nfg) {
-            en = s_list[e];
-            pv = s_par[en & 0xFFFu];
-            is_root = (pv & L_KEY) == (en & 0xFFFu);
-        }
-        const unsigned long long b = __ballot(is_root);
-        if (!is_root)
+    // 4. the closed roots of every segment, the LDS sums of the roots' slots, and the
+    // tile's record count (a pool share when it exceeds its region)
+    const uint32_t total = s_nroot;
+    for (int e = t; e < nfg; e += CT_THREADS) {
+        const uint32_t en = s_list[e];
+        const int li = int(en & 0xFFFu);
+        const uint32_t pv = s_par[li];
+        if ((pv & L_KEY) != uint32_t(li))
             continue;
-        const int li = int(en & 0xFFFu), c = li >> 6, rr = li & 63;
-        const uint32_t cl = (en >> 12) & 3u;
-        const uint32_t slot = s_rpre[k * CT_WAVES + w] + uint32_t(__popcll(b & lt));
-        s_par[li] = pv | (slot << 16);
-        const unsigned long long e0 = cl == 1u ? 0ull : ~0ull;
-        if (ok) {
-            const uint32_t kk = ridx(slot);
-            const uint32_t gkey = uint32_t(cv0 + c) * uint32_t(g.R) + uint32_t(rbase + rr);
-            rec.key[kk] = gkey | (cl == 2u ? P_NEG : 0u) | ((pv & L_OPEN) ? P_OPEN : 0u);
-            rec.pos[kk] = ((seg_index(g, cv0 + c, ty) + (cl == 2u ? g.nseg : 0u)) << 6) | uint32_t(rr);
-        }
+        const uint32_t cl = (en >> 12) & 3u, slot = (pv >> 16) & 0xFFFu;
+        if (!(pv & L_OPEN))
+            atomicOr(cl == 1u ? &s_rmp[li >> 6] : &s_rmn[li >> 6], 1ull << (li & 63));
         if (slot < uint32_t(CT_RCAP)) {
-            r_ext[slot] = e0;
+            r_ext[slot] = cl == 1u ? 0ull : ~0ull;
             r_cnt[slot] = 0;
             r_idx[slot] = ~0u;
 #pragma unroll
             for (int q = 0; q < 6; q++)
                 r_mom[q][slot] = 0.0;
-        } else if (ok) {
-            IslandSum z{};
-            z.ext = e0;
-            z.idx = ~0u;
-            rec.sum[ridx(slot)] = z;
+        }
+    }
+    if (t == 0) {
+        rec.tcount[blockIdx.x] = total;
+        s_base = total > rec.rt ? atomicAdd(rec.pool_count, total - rec.rt) : 0u;
+    }
+    __syncthreads();
+    if (w == 1 && lane < nc) {
+        const unsigned seg = seg_index(g, cv0 + lane, ty);
+        rmask[seg] = s_rmp[lane];
+        rmask[g.nseg + seg] = s_rmn[lane];
+    }
+    const uint32_t pool_base = s_base;
+    const bool ok = total <= rec.rt || size_t(pool_base) + size_t(total - rec.rt) <= size_t(rec.pool_cap);
+    auto ridx = [&](uint32_t slot) -> uint32_t {
+        return slot < rec.rt ? blockIdx.x * rec.rt + slot : rec.ntiles * rec.rt + pool_base + (slot - rec.rt);
+    };
+    if (ok) {
+        for (int e = t; e < nfg; e += CT_THREADS) {
+            const uint32_t en = s_list[e];
+            const int li = int(en & 0xFFFu), c = li >> 6, rr = li & 63;
+            const uint32_t pv = s_par[li];
+            if ((pv & L_KEY) != uint32_t(li))
+                continue;
+            const uint32_t cl = (en >> 12) & 3u, slot = (pv >> 16) & 0xFFFu;
+            const uint32_t kk = ridx(slot);
+            const uint32_t gkey = uint32_t(cv0 + c) * uint32_t(g.R) + uint32_t(rbase + rr);
+            rec.key[kk] = gkey | (cl == 2u ? P_NEG : 0u) | ((pv & L_OPEN) ? P_OPEN : 0u);
+            rec.pos[kk] = ((seg_index(g, cv0 + c, ty) + (cl == 2u ? g.nseg : 0u)) << 6) | uint32_t(rr);
+            if (slot >= uint32_t(CT_RCAP)) {
+                IslandSum z{};
+                z.ext = cl == 1u ? 0ull : ~0ull;
+                z.idx = ~0u;
+                rec.sum[kk] = z;
+            }
         }
     }
     // the barrier's workgroup-scope release orders these stores before the atomics
