@@ -13,14 +13,20 @@ __global__ __launch_bounds__(CANON_LANES) void canon_pairs_kernel(const double* 
 {
     __shared__ double lds[2 * CANON_LANES];
     const size_t base = size_t(blockIdx.x) * CANON_BLOCK;
+    // unconditional loads (clamped index, m >= 1) so that all are in flight together
+    double va[CANON_PER_LANE], vb[CANON_PER_LANE];
+#pragma unroll
+    for (int j = 0; j < CANON_PER_LANE; j++) {
+        const size_t i = std::min(base + size_t(j) * CANON_LANES + threadIdx.x, m - 1);
+        va[j] = in[2 * i];
+        vb[j] = in[2 * i + 1];
+    }
     double a = 0.0, b = 0.0;
 #pragma unroll
     for (int j = 0; j < CANON_PER_LANE; j++) {
-        const size_t i = base + size_t(j) * CANON_LANES + threadIdx.x;
-        if (i < m) {
-            a = a + in[2 * i];
-            b = b + in[2 * i + 1];
-        }
+        const bool ok = base + size_t(j) * CANON_LANES + threadIdx.x < m;
+        a = ok ? a + va[j] : a;
+        b = ok ? b + vb[j] : b;
     }
     canon_tree(a, b, lds);
     if (threadIdx.x == 0) {
@@ -149,46 +155,67 @@ struct LoadPopc64 {
     __device__ uint32_t operator()(size_t i) const { return uint32_t(__popcll(p[i])); }
 };
 
+// The 16 values of a thread: straight-line loads when all are in range, so that
+// they are in flight together.
+template <typename L>
+__device__ __forceinline__ void load16(L in, size_t base, size_t m, uint32_t (&v)[16])
+{
+    if (base + 16 <= m) {
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            v[j] = in(base + j);
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+        v[j] = (base + j < m) ? in(base + j) : 0u;
+}
+
+// Inclusive scan over the 256 threads of a block: wave shuffles, then the wave totals.
+__device__ __forceinline__ uint32_t block_scan256(uint32_t a, uint32_t* wsum)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(a, off, 64);
+        if (lane >= off)
+            a += y;
+    }
+    if (lane == 63)
+        wsum[w] = a;
+    __syncthreads();
+    for (int k = 0; k < w; k++)
+        a += wsum[k];
+    return a;
+}
+
 template <typename L>
 __global__ __launch_bounds__(256) void chunk_sum_kernel(L in, size_t m, uint32_t* sums)
 {
-    __shared__ uint32_t s[256];
+    __shared__ uint32_t wsum[4];
     const size_t base = size_t(blockIdx.x) * SCAN_CHUNK + size_t(threadIdx.x) * 16;
+    uint32_t v[16];
+    load16(in, base, m, v);
     uint32_t a = 0;
+#pragma unroll
     for (int j = 0; j < 16; j++)
-        if (base + j < m)
-            a += in(base + j);
-    s[threadIdx.x] = a;
-    __syncthreads();
-    for (int off = 128; off >= 1; off >>= 1) {
-        if (threadIdx.x < off)
-            s[threadIdx.x] += s[threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        sums[blockIdx.x] = s[0];
+        a += v[j];
+    a = block_scan256(a, wsum);
+    if (threadIdx.x == 255)
+        sums[blockIdx.x] = a;
 }
 
 template <typename L>
 __global__ __launch_bounds__(256) void chunk_rescan_kernel(L in, size_t m, const uint32_t* chunk_off, uint32_t* out)
 {
-    __shared__ uint32_t s[256];
+    __shared__ uint32_t wsum[4];
     const size_t base = size_t(blockIdx.x) * SCAN_CHUNK + size_t(threadIdx.x) * 16;
     uint32_t v[16];
+    load16(in, base, m, v);
     uint32_t a = 0;
-    for (int j = 0; j < 16; j++) {
-        v[j] = (base + j < m) ? in(base + j) : 0u;
+#pragma unroll
+    for (int j = 0; j < 16; j++)
         a += v[j];
-    }
-    s[threadIdx.x] = a;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const uint32_t t = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
-        __syncthreads();
-        s[threadIdx.x] += t;
-        __syncthreads();
-    }
-    uint32_t run = chunk_off[blockIdx.x] + s[threadIdx.x] - a;
+    uint32_t run = chunk_off[blockIdx.x] + block_scan256(a, wsum) - a;
     for (int j = 0; j < 16; j++) {
         if (base + j < m)
             out[base + j] = run;

@@ -88,8 +88,9 @@ extern "C" uint64_t stp_job_device_bytes(const stp_job* j)
     b += 4 * 2 * ((rows + 63) / 64) * ((N + 31) / 32) * ((n + 4095) / 4096 + 1); // bin counts + offsets
     b += 2 * I * I * S + 4 * I * I;             // image, beam, label map
     // source-find workspace: parents, tile component records (4.5 B per pixel), segment
-    // masks and offsets, histograms; island sums and list sized for 1 island per 16 pixels
-    b += 10 * I * I + 128 * I * I / 16;
+    // masks and offsets, histograms; island sums and list sized for 1 island per 16 pixels;
+    // sigma-clipping candidates (one value per pixel at most) and their lane table
+    b += 10 * I * I + 128 * I * I / 16 + I * I * S + I * I / 4;
     if (wproj) {
         const uint64_t os = std::max<uint32_t>(1, (p.oversampling >> 1) << 1);
         const uint64_t cs = (os / 2) * 2 + 1;

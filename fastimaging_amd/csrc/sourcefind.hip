@@ -48,8 +48,9 @@ struct SFState {
     // device-side control of the median path
     int need_exact;                // exact order statistics still required
     int ba_active;                 // the BINAPPROX histogram is in use
-    int pad;
+    int compact;                   // the next clipping pass reads the candidates only
     double num_sigma;
+    double cand_bound;             // |v - median| bound of the kept clipping candidates
     // thresholds (sourcefind.cpp:298-301)
     double a_pos, a_neg, d_pos, d_neg;
     // results read back by the host
@@ -134,6 +135,13 @@ template <typename T>
 __device__ __forceinline__ void canon_load(const T* x, size_t n, size_t blk, T (&v)[CANON_PER_LANE])
 {
     const size_t base = blk * CANON_BLOCK + threadIdx.x;
+    if ((blk + 1) * CANON_BLOCK <= n) {
+        // a whole block: straight-line loads, all in flight together
+#pragma unroll
+        for (int j = 0; j < CANON_PER_LANE; j++)
+            v[j] = x[base + size_t(j) * CANON_LANES];
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < CANON_PER_LANE; j++) {
         const size_t i = base + size_t(j) * CANON_LANES;
@@ -155,11 +163,10 @@ __global__ __launch_bounds__(SWG) void sf_sum_kernel(const T* x, size_t n, size_
         double a = 0.0, b = 0.0;
 #pragma unroll
         for (int j = 0; j < CANON_PER_LANE; j++) {
-            if (blk * CANON_BLOCK + size_t(j) * CANON_LANES + threadIdx.x < n) {
-                const double v = double(cur[j]);
-                a = a + v;
-                b = b + v * v;
-            }
+            const bool ok = blk * CANON_BLOCK + size_t(j) * CANON_LANES + threadIdx.x < n;
+            const double v = double(cur[j]);
+            a = ok ? a + v : a;
+            b = ok ? b + v * v : b;
         }
         canon_tree(a, b, lds);
         if (threadIdx.x == 0) {
@@ -186,15 +193,22 @@ __device__ void set_mean_sigma(double s1, double s2, unsigned long long n, SFSta
 // CANON_LANES threads, the order of the canonical pair kernel); thread 0 gets the sums.
 __device__ __forceinline__ void canon_last_level(const double* in, unsigned m, double& a, double& b, double* lds)
 {
+    // every load is issued before the first sum waits on one: unconditional loads
+    // (clamped index; the buffers hold at least one pair)
+    double va[CANON_PER_LANE], vb[CANON_PER_LANE];
+#pragma unroll
+    for (int j = 0; j < CANON_PER_LANE; j++) {
+        const unsigned i = min(unsigned(j) * CANON_LANES + threadIdx.x, m > 0 ? m - 1 : 0u);
+        va[j] = in[2 * i];
+        vb[j] = in[2 * i + 1];
+    }
     a = 0.0;
     b = 0.0;
 #pragma unroll
     for (int j = 0; j < CANON_PER_LANE; j++) {
-        const unsigned i = unsigned(j) * CANON_LANES + threadIdx.x;
-        if (i < m) {
-            a = a + in[2 * i];
-            b = b + in[2 * i + 1];
-        }
+        const bool ok = unsigned(j) * CANON_LANES + threadIdx.x < m;
+        a = ok ? a + va[j] : a;
+        b = ok ? b + vb[j] : b;
     }
     canon_tree(a, b, lds);
 }
@@ -648,13 +662,44 @@ __global__ __launch_bounds__(1024) void sel_sort(SelState* ss, const unsigned lo
 }
 
 // ------------------------------------------------------------------ sigma clipping
+// A pass over the whole image also keeps, per canonical block and lane in the
+// lane's order, the values with |v - median| > CLIP_CAND_FRAC * upper (the
+// candidates).  While the clipping bound stays at or above that value, the next
+// passes read only the candidates: every other term of their sums is an exact
+// zero, so the sums equal those of a full pass bit for bit.  A pass whose bound
+// has dropped below runs over the whole image again and keeps new candidates.
+// 0.5: on dirty images of noise and point sources the bound falls by 10-25 % a pass
+// and the candidates are ~8 % of the pixels.
+constexpr double CLIP_CAND_FRAC = 0.5;
+
+// The terms of one canonical block for this lane: v[j] for j < count.
+template <typename T>
+__device__ __forceinline__ int clip_load(const T* x, size_t n, size_t blk, const T* cand, const uint32_t* ltab,
+    bool compact, T (&v)[CANON_PER_LANE])
+{
+    if (!compact) {
+        canon_load(x, n, blk, v);
+        const size_t base = blk * CANON_BLOCK + threadIdx.x;
+        return base >= n ? 0 : int(std::min<size_t>(CANON_PER_LANE, (n - base + CANON_LANES - 1) / CANON_LANES));
+    }
+    const uint32_t lw = ltab[blk * CANON_LANES + threadIdx.x];
+    const int c = int(lw >> 16);
+    const T* cp = cand + blk * CANON_BLOCK + (lw & 0xFFFFu);
+#pragma unroll
+    for (int j = 0; j < CANON_PER_LANE; j++)
+        v[j] = j < c ? cp[j] : T(0);
+    return c;
+}
+
 template <typename T>
 __global__ __launch_bounds__(SWG) void sf_clip_kernel(const T* x, size_t n, size_t nblk, const SFState* st, int iter,
-    double num_sigma, double* part, uint32_t* removed)
+    double num_sigma, double* part, uint32_t* removed, T* cand, uint32_t* ltab)
 {
     __shared__ double lds[2 * CANON_LANES];
     __shared__ uint32_t wc[SWG / 64];
+    __shared__ uint32_t wk[SWG / 64];
     double upper, prev;
+    bool compact = false;
     if (iter == 0) {
         // the state of sf_clip_init
         upper = num_sigma * st->sigma;
@@ -664,43 +709,71 @@ __global__ __launch_bounds__(SWG) void sf_clip_kernel(const T* x, size_t n, size
             return;
         upper = st->upper;
         prev = st->prev_upper;
+        compact = st->compact != 0;
     }
     const double median = st->median;
+    const double bound = CLIP_CAND_FRAC * upper;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const size_t b0 = size_t(blockIdx.x) * CANON_GROUP;
     T cur[CANON_PER_LANE], nxt[CANON_PER_LANE];
-    canon_load(x, n, b0, cur);
+    int ccur = clip_load(x, n, b0, cand, ltab, compact, cur), cnxt = 0;
     for (int g = 0; g < CANON_GROUP && b0 + g < nblk; g++) {
         const size_t blk = b0 + g;
         if (g + 1 < CANON_GROUP && blk + 1 < nblk)
-            canon_load(x, n, blk + 1, nxt);
+            cnxt = clip_load(x, n, blk + 1, cand, ltab, compact, nxt);
         double a = 0.0, b = 0.0;
-        unsigned cnt = 0;
+        unsigned cnt = 0, keep = 0;
 #pragma unroll
         for (int j = 0; j < CANON_PER_LANE; j++) {
-            if (blk * CANON_BLOCK + size_t(j) * CANON_LANES + threadIdx.x < n) {
+            if (j < ccur) {
                 const double v = double(cur[j]) - median;
                 const double av = fabs(v);
                 const bool rem = (av > upper) && !(av > prev);
                 a = a + (rem ? v : 0.0);
                 b = b + (rem ? v * v : 0.0);
                 cnt += rem ? 1u : 0u;
+                keep |= (av > bound) ? (1u << j) : 0u;
             }
         }
         canon_tree(a, b, lds);
         for (int off = 32; off >= 1; off >>= 1)
             cnt += __shfl_down(cnt, off, 64);
-        if ((threadIdx.x & 63) == 0)
-            wc[threadIdx.x >> 6] = cnt;
+        // candidates of a full pass: exclusive scan of the lanes' counts
+        const uint32_t kc = compact ? 0u : uint32_t(__popc(keep));
+        uint32_t kx = kc;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(kx, off, 64);
+            if (lane >= off)
+                kx += y;
+        }
+        if (lane == 0)
+            wc[w] = cnt;
+        if (lane == 63)
+            wk[w] = kx;
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (t == 0) {
             part[2 * blk] = a;
             part[2 * blk + 1] = b;
             uint32_t c = 0;
-            for (int w = 0; w < SWG / 64; w++)
-                c += wc[w];
+            for (int k = 0; k < SWG / 64; k++)
+                c += wc[k];
             removed[blk] = c;
         }
-        __syncthreads();  // lds and wc are rewritten by the next block
+        if (!compact) {
+            uint32_t off = kx - kc;
+            for (int k = 0; k < w; k++)
+                off += wk[k];
+            ltab[blk * CANON_LANES + t] = off | (kc << 16);
+            T* cp = cand + blk * CANON_BLOCK + off;
+#pragma unroll
+            for (int j = 0; j < CANON_PER_LANE; j++) {
+                if (keep & (1u << j)) {
+                    *cp++ = cur[j];
+                }
+            }
+        }
+        __syncthreads();  // lds, wc and wk are rewritten by the next block
+        ccur = cnxt;
 #pragma unroll
         for (int j = 0; j < CANON_PER_LANE; j++)
             cur[j] = nxt[j];
@@ -722,6 +795,7 @@ __device__ void clip_init_state(SFState* st, unsigned long long n, int iters, do
     st->iter = 0;
     st->iters = iters;
     st->done = iters <= 0;
+    st->compact = 0;
 }
 
 __global__ void sf_clip_init(SFState* st, unsigned long long n, int iters, double num_sigma)
@@ -757,52 +831,58 @@ __global__ void sf_thresholds(SFState* st, ThreshArgs ta)
 // One clipping step (sourcefind.cpp:39-151) from the block partials of sf_clip
 // (m <= CANON_BLOCK pairs after any earlier canonical levels; removed: one count
 // per block).  Iteration 0 first sets up the state; the last iteration also sets
-// the thresholds.
+// the thresholds.  The partials are read before the state is: after the end of
+// the clipping they are stale and go unused.
 __global__ __launch_bounds__(256) void sf_clip_update(SFState* st, const double* cpart, unsigned m, const uint32_t* removed,
     int nblk, int iter, unsigned long long n, int iters, double num_sigma, int last, ThreshArgs ta)
 {
     __shared__ double lds[2 * CANON_LANES];
-    __shared__ unsigned long long red[256];
-    __shared__ int s_done;
+    __shared__ unsigned long long red[256 / 64];
     const int t = threadIdx.x;
-    if (t == 0) {
-        if (iter == 0)
-            clip_init_state(st, n, iters, num_sigma);
-        s_done = st->done;
+    unsigned long long c = 0;
+    for (int k0 = 0; k0 < nblk; k0 += 256 * 16) {
+        // unconditional loads (clamped index) so that none waits on another
+        uint32_t r[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            r[j] = removed[min(k0 + j * 256 + t, nblk - 1)];
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            c += k0 + j * 256 + t < nblk ? r[j] : 0u;
     }
+    double a, b;
+    canon_last_level(cpart, m, a, b, lds);
+    for (int off = 32; off >= 1; off >>= 1)
+        c += __shfl_down(c, off, 64);
+    if ((t & 63) == 0)
+        red[t >> 6] = c;
     __syncthreads();
-    if (!s_done) {
-        double a, b;
-        canon_last_level(cpart, m, a, b, lds);
-        unsigned long long c = 0;
-        for (int k = t; k < nblk; k += 256)
-            c += removed[k];
-        red[t] = c;
-        __syncthreads();
-        for (int off = 128; off > 0; off >>= 1) {
-            if (t < off)
-                red[t] += red[t + off];
-            __syncthreads();
-        }
-        if (t == 0) {
-            const unsigned long long rem = red[0];
-            if (rem == 0) {
+    if (t != 0)
+        return;
+    if (iter == 0)
+        clip_init_state(st, n, iters, num_sigma);
+    if (!st->done) {
+        const unsigned long long rem = red[0] + red[1] + red[2] + red[3];
+        // a full pass kept the candidates of its own bound
+        if (!st->compact)
+            st->cand_bound = CLIP_CAND_FRAC * st->upper;
+        if (rem == 0) {
+            st->done = 1;
+        } else {
+            st->valid_n -= rem;
+            st->d1 -= a;
+            st->d2 -= b;
+            const double vn = double(st->valid_n);
+            st->clip_sigma = sqrt(st->d2 / vn - (st->d1 / vn) * (st->d1 / vn));
+            st->prev_upper = st->upper;
+            st->upper = st->num_sigma * st->clip_sigma;
+            st->compact = st->upper >= st->cand_bound;
+            st->iter++;
+            if (st->iter >= st->iters)
                 st->done = 1;
-            } else {
-                st->valid_n -= rem;
-                st->d1 -= a;
-                st->d2 -= b;
-                const double vn = double(st->valid_n);
-                st->clip_sigma = sqrt(st->d2 / vn - (st->d1 / vn) * (st->d1 / vn));
-                st->prev_upper = st->upper;
-                st->upper = st->num_sigma * st->clip_sigma;
-                st->iter++;
-                if (st->iter >= st->iters)
-                    st->done = 1;
-            }
         }
     }
-    if (last && t == 0)
+    if (last)
         set_thresholds(st, ta);
 }
 
@@ -1445,7 +1525,9 @@ __global__ __launch_bounds__(LM_THREADS) void ccl_label_map(const uint32_t* __re
         return;
     const double d_pos = st->d_pos, d_neg = st->d_neg;
     const uint32_t kp = s.offs[g.nseg];
-    uint32_t pv[LM_PPT], root[LM_PPT];
+    // Each phase only issues loads into registers (none waits on another of its
+    // phase): parents, roots of open components, segment masks and offsets, extrema.
+    uint32_t pv[LM_PPT], rv[LM_PPT];
 #pragma unroll
     for (int j = 0; j < LM_PPT; j++) {
         const int c = w + LM_WAVES * j;
@@ -1453,24 +1535,44 @@ __global__ __launch_bounds__(LM_THREADS) void ccl_label_map(const uint32_t* __re
     }
 #pragma unroll
     for (int j = 0; j < LM_PPT; j++)
-        root[j] = (pv[j] != P_BG && (pv[j] & P_OPEN)) ? (P[pv[j] & P_KEY] & P_KEY) : (pv[j] & P_KEY);
+        rv[j] = (pv[j] != P_BG && (pv[j] & P_OPEN)) ? P[pv[j] & P_KEY] : pv[j];
+    uint32_t so[LM_PPT];
+    unsigned long long sm[LM_PPT];
+    int rr[LM_PPT];
+#pragma unroll
+    for (int j = 0; j < LM_PPT; j++) {
+        const unsigned sseg = pv[j] != P_BG ? seg_of_key(g, rv[j] & P_KEY, pv[j] & P_NEG, rr[j]) : 0u;
+        so[j] = pv[j] != P_BG ? s.offs[sseg] : 0u;
+        sm[j] = pv[j] != P_BG ? s.mask[sseg] : 0ull;
+    }
+    unsigned long long ev[LM_PPT];
+#pragma unroll
+    for (int j = 0; j < LM_PPT; j++) {
+        so[j] += uint32_t(__popcll(sm[j] & ((1ull << rr[j]) - 1ull)));  // the island id
+        ev[j] = (gen_map && pv[j] != P_BG) ? isum[so[j]].ext : 0ull;
+    }
+    // the labels, then the stores (a wait for a load between stores would also
+    // wait for the stores before it)
+    int32_t L[LM_PPT];
+#pragma unroll
+    for (int j = 0; j < LM_PPT; j++) {
+        L[j] = 0;
+        if (pv[j] != P_BG) {
+            const bool neg = pv[j] & P_NEG;
+            const uint32_t gid = so[j];
+            bool keep = true;
+            if (gen_map) {
+                const double e = okey_inv(ev[j]);
+                keep = neg ? (e < d_neg) : (e > d_pos);
+            }
+            L[j] = keep ? (neg ? -int32_t(gid - kp + 1) : int32_t(gid + 1)) : 0;
+        }
+    }
 #pragma unroll
     for (int j = 0; j < LM_PPT; j++) {
         const int c = w + LM_WAVES * j;
-        if (c >= nc)
-            continue;
-        int32_t L = 0;
-        if (pv[j] != P_BG) {
-            const bool neg = pv[j] & P_NEG;
-            const uint32_t gid = root_gid(s, root[j], neg);
-            bool keep = true;
-            if (gen_map) {
-                const double e = okey_inv(isum[gid].ext);
-                keep = neg ? (e < d_neg) : (e > d_pos);
-            }
-            L = keep ? (neg ? -int32_t(gid - kp + 1) : int32_t(gid + 1)) : 0;
-        }
-        lab[size_t(mem_col(g, cv0 + c)) * size_t(g.R) + size_t(rbase + lane)] = L;
+        if (c < nc)
+            lab[size_t(mem_col(g, cv0 + c)) * size_t(g.R) + size_t(rbase + lane)] = L[j];
     }
 }
 
@@ -1791,9 +1893,11 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
         if (iters > 0) {
             uint32_t* removed = wsp<uint32_t>(ctx, "sf_removed", nblk);
             double* cpart = wsp<double>(ctx, "sf_cpart", 2 * nblk + 2);
+            T* cand = wsp<T>(ctx, "sf_clip_cand", nblk * CANON_BLOCK);
+            uint32_t* ltab = wsp<uint32_t>(ctx, "sf_clip_lanes", nblk * CANON_LANES);
             for (int it = 0; it < iters; it++) {
                 launch(ctx, "sf_clip", sf_clip_kernel<T>, dim3(div_up(nblk, CANON_GROUP)), dim3(SWG), 0, x, n, nblk,
-                    (const SFState*)st, it, 3.0, cpart, removed);
+                    (const SFState*)st, it, 3.0, cpart, removed, cand, ltab);
                 size_t mfin = 0;
                 const double* fin = canon_reduce_to_block(ctx, cpart, nblk, scr, &mfin);
                 launch(ctx, "sf_clip_update", sf_clip_update, dim3(1), dim3(256), 0, st, fin, unsigned(mfin),

@@ -66,6 +66,48 @@ def test_gpu_sourcefind_given_rms_and_iters(gpu, orc):
         compare(gpu.source_find(img, pars), orc.source_find(img, pars))
 
 
+def _clip_bounds(img, iters, num_sigma=3.0):
+    """Bounds of the clipping passes (sourcefind.cpp:39-151) with numpy sums: only
+    used to show which passes of a case read the candidates, with a wide margin."""
+    v = img.ravel(order="F") - np.median(img)
+    keep = np.ones(v.size, bool)
+    bounds = [num_sigma * img.std()]
+    for _ in range(iters):
+        rem = keep & (np.abs(v) > bounds[-1])
+        if not rem.any():
+            break
+        keep &= ~rem
+        bounds.append(num_sigma * v[keep].std())
+    return bounds
+
+
+@pytest.mark.parametrize("case", ["noise", "bright", "ragged"])
+def test_gpu_sigma_clip_candidates(gpu, orc, case):
+    """Clipping passes after a full one read only the pixels beyond 0.5 x the full
+    pass's bound; once the bound falls below that, a pass runs over the whole image
+    again.  noise: every later pass reads candidates; bright: a few very bright
+    pixels make the first bound ~10x the next (a second full pass, then candidates);
+    ragged: a partial last canonical block (and sources that move the bound by 15 %).  Bar: statistics and catalogue bit-exact."""
+    rng = np.random.default_rng(21)
+    if case == "ragged":
+        img = _field(300, 302, seed=21)
+    else:
+        img = np.asfortranarray(noise(512, 512, sigma=1.0, seed=21))
+        if case == "bright":
+            img.ravel(order="F")[rng.choice(img.size, 60, replace=False)] = 500.0
+    iters = 12
+    b = _clip_bounds(img, iters)
+    assert len(b) >= 3
+    if case == "bright":
+        assert b[1] < 0.5 * 0.5 * b[0] and all(b[k + 1] > 0.97 * b[k] for k in range(1, len(b) - 1))
+    elif case == "noise":
+        assert all(b[k] > 0.95 * b[0] for k in range(len(b)))
+    else:
+        assert all(b[k] > 0.6 * b[0] for k in range(len(b)))
+    pars = capi.sourcefind_pars(5.0, 3.0, sigma_clip_iters=iters, median_method=capi.MEDIAN_BINMEDIAN)
+    compare(gpu.source_find(img, pars), orc.source_find(img, pars))
+
+
 def test_gpu_sourcefind_degenerate(gpu, orc):
     """Constant image (sigma 0: exact-median fallback), sparse image (most pixels exactly 0:
     radix selection), all-source image (every pixel above threshold)."""
