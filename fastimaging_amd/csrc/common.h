@@ -95,6 +95,9 @@ struct stp_ctx {
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, stp_hip::TimingAcc> timings;
     int num_cus = 256;
+    // islands found by the last source find: the size of the island copy enqueued
+    // with the final state copy (one host round trip when it suffices)
+    size_t sf_islands_hint = 64;
 };
 
 namespace stp_hip {

@@ -57,7 +57,8 @@ struct SFState {
     unsigned int num_pos, num_neg;           // labels of each sign
     unsigned int num_valid, num_islands;     // detections; islands after the area cut
     int fallback;                            // exact median needs the radix selection
-    int pad2;
+    int skip;                                // labelling incomplete (records overflow or median
+                                             // fallback): the island stages do nothing
     unsigned int num_records;                // pool component records requested (capacity check)
     unsigned int pad3;
 };
@@ -672,74 +673,195 @@ __global__ __launch_bounds__(1024) void sel_sort(SelState* ss, const unsigned lo
 // and the candidates are ~8 % of the pixels.
 constexpr double CLIP_CAND_FRAC = 0.5;
 
-// The terms of one canonical block for this lane: v[j] for j < count.
-template <typename T>
-__device__ __forceinline__ int clip_load(const T* x, size_t n, size_t blk, const T* cand, const uint32_t* ltab,
-    bool compact, T (&v)[CANON_PER_LANE])
+// Removal of one term (sourcefind.cpp:39-151): its share of the sums of the removed.
+__device__ __forceinline__ void clip_term(double x, double median, double upper, double prev, double& a, double& b,
+    unsigned& cnt)
 {
-    if (!compact) {
-        canon_load(x, n, blk, v);
-        const size_t base = blk * CANON_BLOCK + threadIdx.x;
-        return base >= n ? 0 : int(std::min<size_t>(CANON_PER_LANE, (n - base + CANON_LANES - 1) / CANON_LANES));
-    }
-    const uint32_t lw = ltab[blk * CANON_LANES + threadIdx.x];
-    const int c = int(lw >> 16);
-    const T* cp = cand + blk * CANON_BLOCK + (lw & 0xFFFFu);
-#pragma unroll
-    for (int j = 0; j < CANON_PER_LANE; j++)
-        v[j] = j < c ? cp[j] : T(0);
-    return c;
+    const double v = x - median;
+    const double av = fabs(v);
+    const bool rem = (av > upper) && !(av > prev);
+    a = a + (rem ? v : 0.0);
+    b = b + (rem ? v * v : 0.0);
+    cnt += rem ? 1u : 0u;
 }
 
+// A candidate pass over one canonical block by one wave: lane l stands for the
+// canonical lanes l + 64 q (q < 4).  The levels of canon_tree follow: lanes t and
+// t + 128, then t and t + 64, then the shuffles of the wave.  The first CLIP_BATCH
+// candidates of every lane are loaded together; the rest (rare) in a second batch.
+constexpr int CLIP_BATCH = 8;
 template <typename T>
-__global__ __launch_bounds__(SWG) void sf_clip_kernel(const T* x, size_t n, size_t nblk, const SFState* st, int iter,
+__device__ __forceinline__ void clip_candidates(const T* cand, const uint32_t* ltab, size_t blk, double median,
+    double upper, double prev, double* part, uint32_t* removed)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t lw[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        lw[q] = ltab[blk * CANON_LANES + 64 * q + lane];
+    T v[4][CLIP_BATCH];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const T* cp = cand + blk * CANON_BLOCK + (lw[q] & 0xFFFFu);
+        const int c = int(lw[q] >> 16);
+#pragma unroll
+        for (int j = 0; j < CLIP_BATCH; j++)
+            v[q][j] = j < c ? cp[j] : T(0);
+    }
+    double a[4], b[4];
+    unsigned cnt = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        a[q] = 0.0;
+        b[q] = 0.0;
+        const int c = int(lw[q] >> 16);
+#pragma unroll
+        for (int j = 0; j < CLIP_BATCH; j++)
+            if (j < c)
+                clip_term(double(v[q][j]), median, upper, prev, a[q], b[q], cnt);
+    }
+    const uint32_t cmax = max(max(lw[0], lw[1]), max(lw[2], lw[3])) >> 16;
+    if (__any(cmax > CLIP_BATCH)) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const T* cp = cand + blk * CANON_BLOCK + (lw[q] & 0xFFFFu);
+            const int c = int(lw[q] >> 16);
+#pragma unroll
+            for (int j = CLIP_BATCH; j < CANON_PER_LANE; j++)
+                v[q][j - CLIP_BATCH] = j < c ? cp[j] : T(0);
+#pragma unroll
+            for (int j = CLIP_BATCH; j < CANON_PER_LANE; j++)
+                if (j < c)
+                    clip_term(double(v[q][j - CLIP_BATCH]), median, upper, prev, a[q], b[q], cnt);
+        }
+    }
+    double x = (a[0] + a[2]) + (a[1] + a[3]);
+    double y = (b[0] + b[2]) + (b[1] + b[3]);
+    for (int off = 32; off >= 1; off >>= 1) {
+        x = x + __shfl_down(x, off, 64);
+        y = y + __shfl_down(y, off, 64);
+        cnt += __shfl_down(cnt, off, 64);
+    }
+    if (lane == 0) {
+        part[2 * blk] = x;
+        part[2 * blk + 1] = y;
+        removed[blk] = cnt;
+    }
+}
+
+// A pass over one canonical block by one wave (lanes as in clip_candidates) that
+// reads the whole block and keeps its candidates: the pass after a drop of the
+// bound below that of the kept candidates.
+template <typename T>
+__device__ __forceinline__ void clip_block_full(const T* x, size_t n, size_t blk, double median, double upper,
+    double prev, double bound, double* part, uint32_t* removed, T* cand, uint32_t* ltab)
+{
+    const int lane = threadIdx.x & 63;
+    double a[4], b[4];
+    unsigned cnt = 0;
+    uint32_t base = 0;  // candidates of the canonical lanes before 64 q
+    for (int q = 0; q < 4; q++) {
+        const size_t i0 = blk * CANON_BLOCK + 64 * q + lane;
+        T v[CANON_PER_LANE];
+#pragma unroll
+        for (int j = 0; j < CANON_PER_LANE; j++)
+            v[j] = i0 + size_t(j) * CANON_LANES < n ? x[i0 + size_t(j) * CANON_LANES] : T(0);
+        double aq = 0.0, bq = 0.0;
+        uint32_t keep = 0;
+#pragma unroll
+        for (int j = 0; j < CANON_PER_LANE; j++) {
+            if (i0 + size_t(j) * CANON_LANES < n) {
+                clip_term(double(v[j]), median, upper, prev, aq, bq, cnt);
+                keep |= (fabs(double(v[j]) - median) > bound) ? (1u << j) : 0u;
+            }
+        }
+        a[q] = aq;
+        b[q] = bq;
+        const uint32_t kc = uint32_t(__popc(keep));
+        uint32_t kx = kc;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(kx, off, 64);
+            if (lane >= off)
+                kx += y;
+        }
+        const uint32_t off = base + kx - kc;
+        ltab[blk * CANON_LANES + 64 * q + lane] = off | (kc << 16);
+        T* cp = cand + blk * CANON_BLOCK + off;
+#pragma unroll
+        for (int j = 0; j < CANON_PER_LANE; j++) {
+            if (keep & (1u << j))
+                *cp++ = v[j];
+        }
+        base += __shfl(kx, 63, 64);
+    }
+    double xs = (a[0] + a[2]) + (a[1] + a[3]);
+    double ys = (b[0] + b[2]) + (b[1] + b[3]);
+    for (int off = 32; off >= 1; off >>= 1) {
+        xs = xs + __shfl_down(xs, off, 64);
+        ys = ys + __shfl_down(ys, off, 64);
+        cnt += __shfl_down(cnt, off, 64);
+    }
+    if (lane == 0) {
+        part[2 * blk] = xs;
+        part[2 * blk + 1] = ys;
+        removed[blk] = cnt;
+    }
+}
+
+// Clipping passes after the first: one wave per canonical block, reading the kept
+// candidates while the bound allows it (every lane of the grid takes one branch).
+template <typename T>
+__global__ __launch_bounds__(SWG) void sf_clip_next_kernel(const T* x, size_t n, size_t nblk, const SFState* st,
+    double* part, uint32_t* removed, T* cand, uint32_t* ltab)
+{
+    if (st->done)
+        return;
+    const double upper = st->upper, prev = st->prev_upper, median = st->median;
+    const size_t blk = size_t(blockIdx.x) * (SWG / 64) + (threadIdx.x >> 6);
+    if (blk >= nblk)
+        return;
+    if (st->compact)
+        clip_candidates(cand, ltab, blk, median, upper, prev, part, removed);
+    else
+        clip_block_full(x, n, blk, median, upper, prev, CLIP_CAND_FRAC * upper, part, removed, cand, ltab);
+}
+
+// The first clipping pass (the state of sf_clip_init) over the whole image; keeps
+// the candidates of its bound.
+template <typename T>
+__global__ __launch_bounds__(SWG) void sf_clip_kernel(const T* x, size_t n, size_t nblk, const SFState* st,
     double num_sigma, double* part, uint32_t* removed, T* cand, uint32_t* ltab)
 {
     __shared__ double lds[2 * CANON_LANES];
     __shared__ uint32_t wc[SWG / 64];
     __shared__ uint32_t wk[SWG / 64];
-    double upper, prev;
-    bool compact = false;
-    if (iter == 0) {
-        // the state of sf_clip_init
-        upper = num_sigma * st->sigma;
-        prev = DBL_MAX;
-    } else {
-        if (st->done)
-            return;
-        upper = st->upper;
-        prev = st->prev_upper;
-        compact = st->compact != 0;
-    }
+    const double upper = num_sigma * st->sigma, prev = DBL_MAX;
     const double median = st->median;
     const double bound = CLIP_CAND_FRAC * upper;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const size_t b0 = size_t(blockIdx.x) * CANON_GROUP;
     T cur[CANON_PER_LANE], nxt[CANON_PER_LANE];
-    int ccur = clip_load(x, n, b0, cand, ltab, compact, cur), cnxt = 0;
+    canon_load(x, n, b0, cur);
     for (int g = 0; g < CANON_GROUP && b0 + g < nblk; g++) {
         const size_t blk = b0 + g;
         if (g + 1 < CANON_GROUP && blk + 1 < nblk)
-            cnxt = clip_load(x, n, blk + 1, cand, ltab, compact, nxt);
+            canon_load(x, n, blk + 1, nxt);
+        const size_t base = blk * CANON_BLOCK + t;
+        const int nv = base >= n ? 0 : int(std::min<size_t>(CANON_PER_LANE, (n - base + CANON_LANES - 1) / CANON_LANES));
         double a = 0.0, b = 0.0;
         unsigned cnt = 0, keep = 0;
 #pragma unroll
         for (int j = 0; j < CANON_PER_LANE; j++) {
-            if (j < ccur) {
-                const double v = double(cur[j]) - median;
-                const double av = fabs(v);
-                const bool rem = (av > upper) && !(av > prev);
-                a = a + (rem ? v : 0.0);
-                b = b + (rem ? v * v : 0.0);
-                cnt += rem ? 1u : 0u;
-                keep |= (av > bound) ? (1u << j) : 0u;
+            if (j < nv) {
+                clip_term(double(cur[j]), median, upper, prev, a, b, cnt);
+                keep |= (fabs(double(cur[j]) - median) > bound) ? (1u << j) : 0u;
             }
         }
         canon_tree(a, b, lds);
         for (int off = 32; off >= 1; off >>= 1)
             cnt += __shfl_down(cnt, off, 64);
-        // candidates of a full pass: exclusive scan of the lanes' counts
-        const uint32_t kc = compact ? 0u : uint32_t(__popc(keep));
+        // exclusive scan of the lanes' candidate counts
+        const uint32_t kc = uint32_t(__popc(keep));
         uint32_t kx = kc;
         for (int off = 1; off < 64; off <<= 1) {
             const uint32_t y = __shfl_up(kx, off, 64);
@@ -759,21 +881,17 @@ __global__ __launch_bounds__(SWG) void sf_clip_kernel(const T* x, size_t n, size
                 c += wc[k];
             removed[blk] = c;
         }
-        if (!compact) {
-            uint32_t off = kx - kc;
-            for (int k = 0; k < w; k++)
-                off += wk[k];
-            ltab[blk * CANON_LANES + t] = off | (kc << 16);
-            T* cp = cand + blk * CANON_BLOCK + off;
+        uint32_t off = kx - kc;
+        for (int k = 0; k < w; k++)
+            off += wk[k];
+        ltab[blk * CANON_LANES + t] = off | (kc << 16);
+        T* cp = cand + blk * CANON_BLOCK + off;
 #pragma unroll
-            for (int j = 0; j < CANON_PER_LANE; j++) {
-                if (keep & (1u << j)) {
-                    *cp++ = cur[j];
-                }
-            }
+        for (int j = 0; j < CANON_PER_LANE; j++) {
+            if (keep & (1u << j))
+                *cp++ = cur[j];
         }
         __syncthreads();  // lds, wc and wk are rewritten by the next block
-        ccur = cnxt;
 #pragma unroll
         for (int j = 0; j < CANON_PER_LANE; j++)
             cur[j] = nxt[j];
@@ -1521,7 +1639,7 @@ __global__ __launch_bounds__(LM_THREADS) void ccl_label_map(const uint32_t* __re
     const int cv0 = tx * CT_W, nc = min(CT_W, g.C - cv0);
     const int vr0 = tile_vr0(g, ty), nr = tile_vr1(g, ty) - vr0;
     const int rbase = flip_row(g, vr0);
-    if (lane >= nr)
+    if (lane >= nr || st->skip)
         return;
     const double d_pos = st->d_pos, d_neg = st->d_neg;
     const uint32_t kp = s.offs[g.nseg];
@@ -1618,10 +1736,14 @@ __device__ void moments_fit(stp_island& isl, double x_bar, double y_bar, double 
 // labels -1..-kn (the island order of sourcefind.cpp:235-278).
 // valid: extremum beyond the detection threshold (sourcefind.cpp:417-436);
 // keep: valid and at least min_area samples (sourcefind.cpp:240,263).
-__global__ __launch_bounds__(256) void sf_island_flags(const IslandSum* isum, uint32_t kp, uint32_t kn, const SFState* st,
+// The grid covers cap >= kp + kn labels (kp, kn from the state); keep is 0 beyond.
+__global__ __launch_bounds__(256) void sf_island_flags(const IslandSum* isum, uint32_t cap, const SFState* st,
     int min_area, uint32_t* keep, uint32_t* vcnt)
 {
     __shared__ uint32_t wv[4];
+    if (st->skip)
+        return;
+    const uint32_t kp = st->num_pos, kn = st->num_neg;
     const uint32_t g = blockIdx.x * 256 + threadIdx.x;
     bool v = false;
     if (g < kp + kn) {
@@ -1629,6 +1751,8 @@ __global__ __launch_bounds__(256) void sf_island_flags(const IslandSum* isum, ui
         const double ext = okey_inv(isum[g].ext);
         v = pos ? (ext > st->d_pos) : (ext < st->d_neg);
         keep[g] = (v && isum[g].cnt >= min_area) ? 1u : 0u;
+    } else if (g < cap) {
+        keep[g] = 0u;
     }
     const uint32_t c = uint32_t(__popcll(__ballot(v)));
     if ((threadIdx.x & 63) == 0)
@@ -1641,15 +1765,28 @@ __global__ __launch_bounds__(256) void sf_island_flags(const IslandSum* isum, ui
 // One island record per kept label at its rank among the kept labels; block 0
 // also writes the island count and the count of valid labels (vcnt: one per block
 // of sf_island_flags, the same grid).
-__global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, uint32_t kp, uint32_t kn, const uint32_t* keep,
+__global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, const uint32_t* keep,
     const uint32_t* offs, const uint32_t* vcnt, SFState* st, int rows, int cols, double analysis_n_sigma, stp_island* out)
 {
     __shared__ uint32_t red[256];
+    if (st->skip)
+        return;
+    const uint32_t kp = st->num_pos, kn = st->num_neg;
     const uint32_t g = blockIdx.x * 256 + threadIdx.x;
     if (blockIdx.x == 0) {
+        // the blocks holding labels (unconditional loads with a clamped index, so
+        // that a batch is in flight together)
+        const uint32_t nb = (kp + kn + 255) / 256;
         uint32_t c = 0;
-        for (uint32_t b = threadIdx.x; b < gridDim.x; b += 256)
-            c += vcnt[b];
+        for (uint32_t b0 = 0; b0 < nb; b0 += 256 * 8) {
+            uint32_t v[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                v[j] = vcnt[min(b0 + 256 * j + threadIdx.x, nb > 0 ? nb - 1 : 0u)];
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                c += b0 + 256 * j + threadIdx.x < nb ? v[j] : 0u;
+        }
         red[threadIdx.x] = c;
         __syncthreads();
         for (int off = 128; off > 0; off >>= 1) {
@@ -1685,12 +1822,16 @@ __global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, uin
     out[offs[g]] = isl;
 }
 
-__global__ void sf_label_counts(SFState* st, const uint32_t* segoff, unsigned nseg, const uint32_t* nrec, const SelState* ss)
+// Label counts, and whether the island stages can run: every tile component has
+// a record (so the labels fit the cap of the island arrays) and the median held.
+__global__ void sf_label_counts(SFState* st, const uint32_t* segoff, unsigned nseg, const uint32_t* nrec, uint32_t pool_cap,
+    unsigned long long cap, const SelState* ss)
 {
     st->num_pos = segoff[nseg];
     st->num_neg = segoff[2 * nseg] - segoff[nseg];
     st->num_records = *nrec;
     st->fallback = ss->fallback;
+    st->skip = *nrec > pool_cap || ss->fallback || (unsigned long long)segoff[2 * nseg] > cap;
 }
 
 __global__ void sf_fallback_clear(SelState* ss)
@@ -1896,8 +2037,12 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
             T* cand = wsp<T>(ctx, "sf_clip_cand", nblk * CANON_BLOCK);
             uint32_t* ltab = wsp<uint32_t>(ctx, "sf_clip_lanes", nblk * CANON_LANES);
             for (int it = 0; it < iters; it++) {
-                launch(ctx, "sf_clip", sf_clip_kernel<T>, dim3(div_up(nblk, CANON_GROUP)), dim3(SWG), 0, x, n, nblk,
-                    (const SFState*)st, it, 3.0, cpart, removed, cand, ltab);
+                if (it == 0)
+                    launch(ctx, "sf_clip", sf_clip_kernel<T>, dim3(div_up(nblk, CANON_GROUP)), dim3(SWG), 0, x, n, nblk,
+                        (const SFState*)st, 3.0, cpart, removed, cand, ltab);
+                else
+                    launch(ctx, "sf_clip_next", sf_clip_next_kernel<T>, dim3(div_up(nblk, SWG / 64)), dim3(SWG), 0, x, n,
+                        nblk, (const SFState*)st, cpart, removed, cand, ltab);
                 size_t mfin = 0;
                 const double* fin = canon_reduce_to_block(ctx, cpart, nblk, scr, &mfin);
                 launch(ctx, "sf_clip_update", sf_clip_update, dim3(1), dim3(256), 0, st, fin, unsigned(mfin),
@@ -1951,15 +2096,50 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
             tg, rec, rmask);
         scan_popc64(ctx, rmask, nseg2, segoff);
         launch(ctx, "sf_counts", sf_label_counts, dim3(1), dim3(1), 0, st, (const uint32_t*)segoff, tg.nseg,
-            (const uint32_t*)pool_count, (const SelState*)ss);
+            (const uint32_t*)pool_count, rec.pool_cap, (unsigned long long)(size_t(ntiles) * CT_RT + rec.pool_cap),
+            (const SelState*)ss);
+    };
+    // island sums in list order (positive labels, then negative); label map
+    // (sourcefind.cpp:337-628); the island list: validity, area cut, compaction and
+    // moments fit.  Sized for one label per record (the label counts stay on the
+    // device: no host round trip), skipped by the kernels when the labelling is
+    // incomplete.
+    stp_island* disl = nullptr;
+    auto islands_of_labels = [&]() {
+        const size_t nl = size_t(ntiles) * CT_RT + rec.pool_cap;
+        IslandSum* isum = wsp<IslandSum>(ctx, "sf_isum", nl);
+        const SegIds sid{ tg, rmask, segoff };
+        const unsigned rblocks = div_up(nl, 256);
+        launch(ctx, "ccl_open_init", ccl_open_init, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
+        launch(ctx, "ccl_records", ccl_records, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
+        launch(ctx, "ccl_open_idx", ccl_open_idx, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
+        if (d_lab_out)
+            launch(ctx, "ccl_label_map", ccl_label_map, dim3(ntiles), dim3(LM_THREADS), 0, (const uint32_t*)P, sid,
+                (const SFState*)st, (const IslandSum*)isum, int(p.generate_labelmap != 0), d_lab_out);
+        uint32_t* keep = wsp<uint32_t>(ctx, "sf_keep", nl);
+        uint32_t* koff = wsp<uint32_t>(ctx, "sf_koff", nl + 1);
+        uint32_t* vcnt = wsp<uint32_t>(ctx, "sf_vcnt", div_up(nl, 256));
+        disl = wsp<stp_island>(ctx, "sf_islands", nl);
+        launch(ctx, "sf_island_flags", sf_island_flags, dim3(rblocks), dim3(256), 0, (const IslandSum*)isum, uint32_t(nl),
+            (const SFState*)st, int(p.source_min_area), keep, vcnt);
+        scan_u32(ctx, keep, nl, koff);
+        launch(ctx, "sf_island_emit", sf_island_emit, dim3(rblocks), dim3(256), 0, (const IslandSum*)isum,
+            (const uint32_t*)keep, (const uint32_t*)koff, (const uint32_t*)vcnt, st, rows, cols, p.analysis_n_sigma, disl);
     };
     SFState* hs = (SFState*)host_stage(ctx, "sf_state", sizeof(SFState));
+    // the state, and as many islands as the last call found (a guess: any missing
+    // ones are copied afterwards), in one host round trip
+    const size_t spec = std::min({ cap, ctx->sf_islands_hint, size_t(ntiles) * CT_RT + rec.pool_cap });
+    stp_island* stage = (stp_island*)host_stage(ctx, "sf_islands", std::max<size_t>(1, spec) * sizeof(stp_island));
     auto read_state = [&]() {
         HIP_CHECK(hipMemcpyAsync(hs, st, sizeof(SFState), hipMemcpyDeviceToHost, ctx->stream));
+        if (spec)
+            HIP_CHECK(hipMemcpyAsync(stage, disl, spec * sizeof(stp_island), hipMemcpyDeviceToHost, ctx->stream));
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
     };
     rms_and_thresholds(pool_count);
     label(false);
+    islands_of_labels();
     read_state();
     if (hs->fallback) {
         // data the linear bins cannot resolve (heavy ties): radix selection on
@@ -1968,6 +2148,7 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
         launch(ctx, "sf_fallback_clear", sf_fallback_clear, dim3(1), dim3(1), 0, ss);
         rms_and_thresholds(pool_count);
         label(false);
+        islands_of_labels();
         read_state();
     }
     if (hs->num_records > rec.pool_cap) {
@@ -1975,45 +2156,24 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
         // capacity is kept for later calls)
         records(size_t(hs->num_records) + size_t(hs->num_records) / 4);
         label(true);
+        islands_of_labels();
         read_state();
         if (hs->num_records > rec.pool_cap)
             throw Error(STP_ERR_INTERNAL, "component records overflow");
     }
+    if (hs->skip)
+        throw Error(STP_ERR_INTERNAL, "labelling incomplete");
     const uint32_t kp = hs->num_pos, kn = hs->num_neg;
-
-    // island sums in list order (positive labels, then negative); label map (sourcefind.cpp:337-628)
-    const size_t nl = size_t(kp) + size_t(kn);
-    IslandSum* isum = wsp<IslandSum>(ctx, "sf_isum", nl);
-    const SegIds sid{ tg, rmask, segoff };
-    const unsigned rblocks = div_up(size_t(ntiles) * CT_RT + rec.pool_cap, 256);
-    launch(ctx, "ccl_open_init", ccl_open_init, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
-    launch(ctx, "ccl_records", ccl_records, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
-    launch(ctx, "ccl_open_idx", ccl_open_idx, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
-    if (d_lab_out)
-        launch(ctx, "ccl_label_map", ccl_label_map, dim3(ntiles), dim3(LM_THREADS), 0, (const uint32_t*)P, sid,
-            (const SFState*)st, (const IslandSum*)isum, int(p.generate_labelmap != 0), d_lab_out);
-
-    // island list: validity, area cut, compaction and moments fit on the device
-    uint32_t* keep = wsp<uint32_t>(ctx, "sf_keep", nl);
-    uint32_t* koff = wsp<uint32_t>(ctx, "sf_koff", nl + 1);
-    uint32_t* vcnt = wsp<uint32_t>(ctx, "sf_vcnt", div_up(nl, 256));
-    stp_island* disl = wsp<stp_island>(ctx, "sf_islands", nl);
-    if (nl) {
-        launch(ctx, "sf_island_flags", sf_island_flags, dim3(unsigned(div_up(nl, 256))), dim3(256), 0, (const IslandSum*)isum,
-            kp, kn, (const SFState*)st, int(p.source_min_area), keep, vcnt);
-        scan_u32(ctx, keep, nl, koff);
-        launch(ctx, "sf_island_emit", sf_island_emit, dim3(unsigned(div_up(nl, 256))), dim3(256), 0, (const IslandSum*)isum, kp,
-            kn, (const uint32_t*)keep, (const uint32_t*)koff, (const uint32_t*)vcnt, st, rows, cols, p.analysis_n_sigma, disl);
-    }
-    read_state();
     const size_t found = hs->num_islands;
     const size_t ncopy = std::min(found, cap);
-    if (ncopy) {
-        stp_island* stage = (stp_island*)host_stage(ctx, "sf_islands", ncopy * sizeof(stp_island));
+    if (ncopy > spec) {
+        stage = (stp_island*)host_stage(ctx, "sf_islands", ncopy * sizeof(stp_island));
         HIP_CHECK(hipMemcpyAsync(stage, disl, ncopy * sizeof(stp_island), hipMemcpyDeviceToHost, ctx->stream));
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
-        std::memcpy(islands, stage, ncopy * sizeof(stp_island));
     }
+    if (ncopy)
+        std::memcpy(islands, stage, ncopy * sizeof(stp_island));
+    ctx->sf_islands_hint = std::max<size_t>(64, found + found / 4);
     *n_islands = found;
     if (stats) {
         stats->bg_level = hs->median;
