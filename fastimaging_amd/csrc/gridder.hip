@@ -62,17 +62,34 @@ struct Prep {
     double wl;   // w (wavelengths) after the half-plane conversion (W-projection)
 };
 
-__device__ __forceinline__ Prep prep_vis(const GridGeom& g, const double* uv, const double* vis, const double* wts, long long k)
+// The inputs of one visibility (loads only, so that several can be in flight).
+struct VisIn {
+    double u, v, vr, vi, w, wl;
+};
+
+__device__ __forceinline__ VisIn load_vis(const GridGeom& g, const double* uv, const double* vis, const double* wts, long long k)
+{
+    VisIn r;
+    const long long i = g.perm ? (long long)g.perm[k] : k;
+    r.u = uv[i];
+    r.v = uv[g.n + i];
+    r.vr = vis[2 * i];
+    r.vi = vis[2 * i + 1];
+    r.w = wts[i];
+    r.wl = g.wproj ? uv[2 * g.n + i] : 0.0;
+    return r;
+}
+
+__device__ __forceinline__ Prep prep_in(const GridGeom& g, const VisIn& r)
 {
     Prep p;
-    const long long i = g.perm ? (long long)g.perm[k] : k;
-    double u = uv[i] * g.scale;
-    double v = uv[g.n + i] * g.scale;
-    p.vr = vis[2 * i];
-    p.vi = vis[2 * i + 1];
-    p.w = wts[i];
+    double u = r.u * g.scale;
+    double v = r.v * g.scale;
+    p.vr = r.vr;
+    p.vi = r.vi;
+    p.w = r.w;
     p.dup = 0;
-    p.wl = g.wproj ? uv[2 * g.n + i] : 0.0;
+    p.wl = r.wl;
     p.good = (fabs(u) < 1.0e9 && fabs(v) < 1.0e9) ? 1 : 0; // also rejects NaN
     if (!p.good)
         u = v = 0.0;
@@ -106,6 +123,11 @@ __device__ __forceinline__ Prep prep_vis(const GridGeom& g, const double* uv, co
     p.cpx = cx + rmax;
     p.cpy = cy + rmax;
     return p;
+}
+
+__device__ __forceinline__ Prep prep_vis(const GridGeom& g, const double* uv, const double* vis, const double* wts, long long k)
+{
+    return prep_in(g, load_vis(g, uv, vis, wts, k));
 }
 
 // Centre / kernel indices of pass `pass` (1 = conjugate copy, gridder.h:794-805).
@@ -217,14 +239,14 @@ __device__ __forceinline__ void footprint_tiles(const GridGeom& g, int gx, int g
         add_tile_range(ct, nct, max(a, N) - N, b - N, g.tc);
 }
 
+// wpart: per block the canonical sum of the good weights and the good count.
 __global__ __launch_bounds__(GWG) void grid_count_kernel(GridGeom g, const double* uv, const double* vis,
-    const double* wts, uint32_t* counts, double* wpart, unsigned long long* ngood)
+    const double* wts, uint32_t* counts, double* wpart)
 {
 #pragma clang fp contract(off)
     __shared__ double lds[2 * CANON_LANES];
     const long long base = (long long)blockIdx.x * CANON_BLOCK;
-    double wsum = 0.0, nsum = 0.0;
-    unsigned long long ng = 0;
+    double wsum = 0.0, nsum = 0.0;  // good weights, good count (exact in double)
     for (int j = 0; j < CANON_PER_LANE; j++) {
         const long long i = base + (long long)j * CANON_LANES + threadIdx.x;
         if (i >= g.n)
@@ -233,7 +255,7 @@ __global__ __launch_bounds__(GWG) void grid_count_kernel(GridGeom g, const doubl
         if (!p.good)
             continue;
         wsum = wsum + p.w;
-        ng++;
+        nsum = nsum + 1.0;
         for (int pass = 0; pass <= p.dup; pass++) {
             int gx, gy, cpx, cpy;
             pass_geom(g, p, pass, gx, gy, cpx, cpy);
@@ -247,14 +269,7 @@ __global__ __launch_bounds__(GWG) void grid_count_kernel(GridGeom g, const doubl
     canon_tree(wsum, nsum, lds);
     if (threadIdx.x == 0) {
         wpart[2 * blockIdx.x] = wsum;
-        wpart[2 * blockIdx.x + 1] = 0.0;
-    }
-    if (ngood) {
-        // wave-level sum of the good count, one atomic per wave
-        for (int off = 32; off >= 1; off >>= 1)
-            ng += __shfl_down(ng, off, 64);
-        if ((threadIdx.x & 63) == 0 && ng)
-            atomicAdd(ngood, ng);
+        wpart[2 * blockIdx.x + 1] = nsum;
     }
 }
 
@@ -312,14 +327,12 @@ __device__ double dev_kernel(const GridGeom& g, double r)
     return v;
 }
 
-__global__ void scale_total_kernel(const double* s, double* out)
+// s: the reduced (good weights, good count) pair.
+__global__ void scale_total_kernel(const double* s, double* out, unsigned long long* ngood)
 {
     out[0] = s[0] * 2.0;
-}
-
-__global__ void zero_u64_kernel(unsigned long long* p)
-{
-    p[0] = 0ull;
+    if (ngood)
+        *ngood = (unsigned long long)s[1];
 }
 
 // ---------------------------------------------------------------------------
@@ -333,23 +346,32 @@ constexpr int VPB = CANON_BLOCK; // visibilities per binning block (also the can
 constexpr int MAX_LDS_TILES = 24576;
 
 __global__ __launch_bounds__(BWG) void grid_bin_count_kernel(GridGeom g, const double* uv, const double* vis, const double* wts,
-    uint32_t* counts, int nblk, double* wpart, unsigned long long* ngood)
+    uint32_t* counts, int nblk, double* wpart)
 {
 #pragma clang fp contract(off)
     extern __shared__ uint32_t hist[];
     __shared__ double wl[VPB];
+    __shared__ unsigned long long wng[BWG / 64];
     const int ntiles = g.tiles_r * g.tiles_c;
     for (int t = threadIdx.x; t < ntiles; t += BWG)
         hist[t] = 0;
     __syncthreads();
     const long long base = (long long)blockIdx.x * VPB;
     unsigned long long ng = 0;
+    VisIn in[VPB / BWG];
+#pragma unroll
+    for (int j = 0; j < VPB / BWG; j++) {
+        const long long i = base + threadIdx.x + j * BWG;
+        if (i < g.n)
+            in[j] = load_vis(g, uv, vis, wts, i);
+    }
+#pragma unroll
     for (int j = 0; j < VPB / BWG; j++) {
         const int local = threadIdx.x + j * BWG;
         const long long i = base + local;
         double wgood = 0.0;
         if (i < g.n) {
-            const Prep p = prep_vis(g, uv, vis, wts, i);
+            const Prep p = prep_in(g, in[j]);
             if (p.good) {
                 wgood = p.w;
                 ng++;
@@ -380,19 +402,21 @@ __global__ __launch_bounds__(BWG) void grid_bin_count_kernel(GridGeom g, const d
         if (threadIdx.x < off)
             wl[threadIdx.x] = wl[threadIdx.x] + wl[threadIdx.x + off];
     }
+    // the block's good count (exact in double) beside its weight sum
+    for (int off = 32; off >= 1; off >>= 1)
+        ng += __shfl_down(ng, off, 64);
+    if ((threadIdx.x & 63) == 0)
+        wng[threadIdx.x >> 6] = ng;
     __syncthreads();
     if (threadIdx.x == 0) {
+        unsigned long long c = 0;
+        for (int k = 0; k < BWG / 64; k++)
+            c += wng[k];
         wpart[2 * blockIdx.x] = wl[0];
-        wpart[2 * blockIdx.x + 1] = 0.0;
+        wpart[2 * blockIdx.x + 1] = double(c);
     }
     for (int t = threadIdx.x; t < ntiles; t += BWG)
         counts[size_t(blockIdx.x) * ntiles + t] = hist[t];
-    if (ngood) {
-        for (int off = 32; off >= 1; off >>= 1)
-            ng += __shfl_down(ng, off, 64);
-        if ((threadIdx.x & 63) == 0 && ng)
-            atomicAdd(ngood, ng);
-    }
 }
 
 // Per tile t: counts[b][t] -> entries of tile t in blocks before b (in place); totals[t].
@@ -446,11 +470,19 @@ __global__ __launch_bounds__(BWG) void grid_bin_scatter_kernel(GridGeom g, const
         cur[t] = tile_off[t] + block_off[size_t(blockIdx.x) * ntiles + t];
     __syncthreads();
     const long long base = (long long)blockIdx.x * VPB;
+    VisIn in[VPB / BWG];
+#pragma unroll
+    for (int j = 0; j < VPB / BWG; j++) {
+        const long long i = base + threadIdx.x + j * BWG;
+        if (i < g.n)
+            in[j] = load_vis(g, uv, vis, wts, i);
+    }
+#pragma unroll
     for (int j = 0; j < VPB / BWG; j++) {
         const long long i = base + threadIdx.x + j * BWG;
         if (i >= g.n)
             break;
-        const Prep p = prep_vis(g, uv, vis, wts, i);
+        const Prep p = prep_in(g, in[j]);
         if (!p.good)
             continue;
         for (int pass = 0; pass <= p.dup; pass++) {
@@ -1198,8 +1230,6 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     STP_REQUIRE(cap64 < (size_t(1) << 32), "too many gridding entries for one call");
     const uint32_t cap = uint32_t(cap64);
     Entry* entries = (Entry*)ws_get(ctx, "grid_entries", sizeof(Entry) * cap64);
-    if (d_ngood)
-        launch(ctx, "fill", zero_u64_kernel, dim3(1), dim3(1), 0, d_ngood);
 
     // W-kernel cache in FP64 for every grid precision: with an FP32 cache the W tile
     // kernel measured 1.8x slower (44 vs 24 ms at N = 4096, 2M visibilities, 32 planes)
@@ -1286,7 +1316,7 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         HIP_CHECK(hipFuncSetAttribute((const void*)grid_bin_count_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(hist_bytes)));
         HIP_CHECK(hipFuncSetAttribute((const void*)grid_bin_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(hist_bytes)));
         launch(ctx, "grid_count", grid_bin_count_kernel, dim3(unsigned(nblk)), dim3(BWG), hist_bytes, g, d_uv, d_vis, d_w, counts,
-            int(nblk), wpart, d_ngood);
+            int(nblk), wpart);
         launch(ctx, "grid_colscan", bin_colscan_kernel, dim3(div_up(ntiles, 64)), dim3(1024), 0, counts, int(nblk), int(ntiles),
             totals);
         exclusive_scan_u32(ctx, totals, int(ntiles), toffs, tcur);
@@ -1300,8 +1330,7 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         uint32_t* offs = (uint32_t*)ws_get(ctx, "grid_offsets", sizeof(uint32_t) * (ntiles + 1));
         uint32_t* cursor = (uint32_t*)ws_get(ctx, "grid_cursor", sizeof(uint32_t) * ntiles);
         fill_u32(ctx, counts, 0u, ntiles);
-        launch(ctx, "grid_count", grid_count_kernel, dim3(unsigned(nblk)), dim3(GWG), 0, g, d_uv, d_vis, d_w, counts, wpart,
-            d_ngood);
+        launch(ctx, "grid_count", grid_count_kernel, dim3(unsigned(nblk)), dim3(GWG), 0, g, d_uv, d_vis, d_w, counts, wpart);
         exclusive_scan_u32(ctx, counts, ntiles, offs, cursor);
         launch(ctx, "grid_scatter", grid_scatter_kernel, dim3(div_up(n, GWG)), dim3(GWG), 0, g, d_uv, d_vis, d_w, cursor,
             entries, cap);
@@ -1337,9 +1366,9 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
                 d_tab, d_uv, d_vis_grid, d_sampling_grid);
         }
     }
-    // sample_grid_total = 2 * sum(good weights) (gridder.h:415-423)
+    // sample_grid_total = 2 * sum(good weights) (gridder.h:415-423); the good count
     canon_reduce_pairs(ctx, wpart, nblk, wscr, wres);
-    launch(ctx, "grid_total", scale_total_kernel, dim3(1), dim3(1), 0, (const double*)wres, d_total);
+    launch(ctx, "grid_total", scale_total_kernel, dim3(1), dim3(1), 0, (const double*)wres, d_total, d_ngood);
 }
 
 template void grid_visibilities<double>(stp_ctx*, const GridderArgs&, size_t, const double*, const double*,
