@@ -39,6 +39,56 @@ __device__ __forceinline__ cplx<T> load_tw(const T* tw, int idx, bool inverse)
     return w;
 }
 
+// Twiddle sources of lds_fft: tw(e) = exp(+-2 pi i e / L) for the length-L
+// transform (+ for the inverse).
+// From a global table of exp(+2 pi i m / (L * tws)).
+template <typename T>
+struct GlobalTw {
+    const T* tw;
+    int tws;
+    bool inverse;
+    __device__ cplx<T> operator()(int e) const { return load_tw(tw, e * tws, inverse); }
+};
+
+// exp(+-2 pi i m / n) from an LDS quarter table q[r] = exp(+2 pi i r / n), r < n/4
+// (m = k n/4 + r: the entry times i^k, an exact rotation).
+template <typename T>
+__device__ __forceinline__ cplx<T> quarter_tw(const cplx<T>* q, int qlog, int m, bool inverse)
+{
+    const int k = m >> qlog;
+    cplx<T> w = q[m & ((1 << qlog) - 1)];
+    if (k & 1)
+        w = { -w.y, w.x };
+    if (k & 2)
+        w = { -w.x, -w.y };
+    if (!inverse)
+        w.y = -w.y;
+    return w;
+}
+
+// From an LDS quarter table of length n = L * tws.
+template <typename T>
+struct QuarterTw {
+    const cplx<T>* q;
+    int qlog, tws;
+    bool inverse;
+    __device__ cplx<T> operator()(int e) const { return quarter_tw(q, qlog, e * tws, inverse); }
+};
+
+// From an LDS table t[e] = exp(+2 pi i e / L), e < L.
+template <typename T>
+struct LdsTw {
+    const cplx<T>* t;
+    bool inverse;
+    __device__ cplx<T> operator()(int e) const
+    {
+        cplx<T> w = t[e];
+        if (!inverse)
+            w.y = -w.y;
+        return w;
+    }
+};
+
 template <typename T>
 __device__ __forceinline__ void bfly2(cplx<T>* v)
 {
@@ -63,8 +113,8 @@ __device__ __forceinline__ void bfly4(cplx<T>* v, bool inverse)
 }
 
 // One Stockham stage of radix R on a length-L transform held at buf.
-template <typename T, int L, int TPT, int R>
-__device__ __forceinline__ void stockham_stage(cplx<T>* buf, int lt, int Ns, const T* tw, int tws, bool inverse)
+template <typename T, int L, int TPT, int R, typename TW>
+__device__ __forceinline__ void stockham_stage(cplx<T>* buf, int lt, int Ns, const TW& tw, bool inverse)
 {
     constexpr int NB = L / R;
     constexpr int BPT = (NB + TPT - 1) / TPT;
@@ -88,7 +138,7 @@ __device__ __forceinline__ void stockham_stage(cplx<T>* buf, int lt, int Ns, con
                 const int step = L / (Ns * R);
 #pragma unroll
                 for (int r = 1; r < R; r++)
-                    v[b][r] = cmul(v[b][r], load_tw(tw, r * k * step * tws, inverse));
+                    v[b][r] = cmul(v[b][r], tw(r * k * step));
             }
             if (R == 4)
                 bfly4(v[b], inverse);
@@ -104,19 +154,19 @@ __device__ __forceinline__ void stockham_stage(cplx<T>* buf, int lt, int Ns, con
 }
 
 // In-place unnormalised FFT of length L at buf, TPT cooperating threads (lt in [0, TPT)).
-// tw: table exp(+2 pi i m / (L*tws)); inverse selects the + sign.
-template <typename T, int L, int TPT>
-__device__ __forceinline__ void lds_fft(cplx<T>* buf, int lt, const T* tw, int tws, bool inverse)
+// tw: a twiddle source (exponents below 3L/4); inverse selects the + sign.
+template <typename T, int L, int TPT, typename TW>
+__device__ __forceinline__ void lds_fft(cplx<T>* buf, int lt, const TW& tw, bool inverse)
 {
     constexpr int LOG = Log2<L>::value;
     int Ns = 1;
     if (LOG & 1) {
-        stockham_stage<T, L, TPT, 2>(buf, lt, Ns, tw, tws, inverse);
+        stockham_stage<T, L, TPT, 2>(buf, lt, Ns, tw, inverse);
         Ns = 2;
     }
     if constexpr (L >= 4) {
         for (; Ns < L; Ns *= 4)
-            stockham_stage<T, L, TPT, 4>(buf, lt, Ns, tw, tws, inverse);
+            stockham_stage<T, L, TPT, 4>(buf, lt, Ns, tw, inverse);
     }
 }
 
@@ -143,10 +193,15 @@ __global__ __launch_bounds__(WG) void fft_u1(cplx<T>* grids, int rows, int N, in
 {
     using S = UShape<L1>;
     __shared__ cplx<T> buf[S::NT * (L1 + S::PAD)];
+    __shared__ cplx<T> s_tw[L1], s_post[L1];  // exp(+2 pi i e / L1); exp(+2 pi i k1 u2 / N)
     const int u2 = blockIdx.x;
     const int v0 = blockIdx.y * S::NT;
     cplx<T>* g = grids + size_t(blockIdx.z) * grid_elems;
     const int tid = threadIdx.x;
+    for (int e = tid; e < L1; e += WG) {
+        s_tw[e] = load_tw(tw, e * (N / L1), true);
+        s_post[e] = load_tw(tw, (e * u2) & (N - 1), true);
+    }
     for (int idx = tid; idx < S::NT * L1; idx += WG) {
         const int vv = idx % S::NT, u1 = idx / S::NT;
         const int v = v0 + vv;
@@ -157,13 +212,13 @@ __global__ __launch_bounds__(WG) void fft_u1(cplx<T>* grids, int rows, int N, in
     }
     __syncthreads();
     const int t = tid / S::TPT, lt = tid % S::TPT;
-    lds_fft<T, L1, S::TPT>(buf + t * (L1 + S::PAD), lt, tw, N / L1, inverse != 0);
+    lds_fft<T, L1, S::TPT>(buf + t * (L1 + S::PAD), lt, LdsTw<T>{ s_tw, inverse != 0 }, inverse != 0);
     for (int idx = tid; idx < S::NT * L1; idx += WG) {
         const int vv = idx % S::NT, k1 = idx / S::NT;
         const int v = v0 + vv;
         if (v < rows) {
             cplx<T> x = buf[vv * (L1 + S::PAD) + k1];
-            x = cmul(x, load_tw(tw, (k1 * u2) & (N - 1), inverse != 0));
+            x = cmul(x, LdsTw<T>{ s_post, inverse != 0 }(k1));
             g[size_t(v) + size_t(k1 * L2 + u2) * rows] = x;
         }
     }
@@ -176,10 +231,13 @@ __global__ __launch_bounds__(WG) void fft_u2(cplx<T>* grids, int rows, int N, in
 {
     using S = UShape<L2>;
     __shared__ cplx<T> buf[S::NT * (L2 + S::PAD)];
+    __shared__ cplx<T> s_tw[L2];  // exp(+2 pi i e / L2)
     const int k1 = blockIdx.x;
     const int v0 = blockIdx.y * S::NT;
     cplx<T>* g = grids + size_t(blockIdx.z) * grid_elems;
     const int tid = threadIdx.x;
+    for (int e = tid; e < L2; e += WG)
+        s_tw[e] = load_tw(tw, e * (N / L2), true);
     for (int idx = tid; idx < S::NT * L2; idx += WG) {
         const int vv = idx % S::NT, u2 = idx / S::NT;
         const int v = v0 + vv;
@@ -190,7 +248,7 @@ __global__ __launch_bounds__(WG) void fft_u2(cplx<T>* grids, int rows, int N, in
     }
     __syncthreads();
     const int t = tid / S::TPT, lt = tid % S::TPT;
-    lds_fft<T, L2, S::TPT>(buf + t * (L2 + S::PAD), lt, tw, N / L2, inverse != 0);
+    lds_fft<T, L2, S::TPT>(buf + t * (L2 + S::PAD), lt, LdsTw<T>{ s_tw, inverse != 0 }, inverse != 0);
     for (int idx = tid; idx < S::NT * L2; idx += WG) {
         const int vv = idx % S::NT, k2 = idx / S::NT;
         const int v = v0 + vv;
@@ -211,8 +269,12 @@ __global__ __launch_bounds__(WG) void fft_v_c2r(const cplx<T>* grids, size_t gri
     using S = Shape<M>;
     constexpr int n = 2 * M;
     constexpr int rows = M + 1;
+    constexpr int QN = (n / 4 > 0) ? n / 4 : 1;  // quarter table: exp(+2 pi i r / n), r < n/4
     __shared__ cplx<T> buf[S::NT * (M + S::PAD)];
+    __shared__ cplx<T> s_q[QN];
     const int tid = threadIdx.x;
+    for (int r = tid; r < QN; r += WG)
+        s_q[r] = load_tw(tw_n, r, true);
     const int t = tid / S::TPT, lt = tid % S::TPT;
     const int x = blockIdx.x * S::NT + t;
     const int gi = blockIdx.y;
@@ -221,6 +283,8 @@ __global__ __launch_bounds__(WG) void fft_v_c2r(const cplx<T>* grids, size_t gri
     const int slot = active ? ((x % L1) * L2 + x / L1) : 0;
     cplx<T>* b = buf + t * (M + S::PAD);
     const cplx<T>* col = g + size_t(slot) * rows;
+    __syncthreads();
+    constexpr int QLOG = Log2<QN>::value;
     // pack Z[k] = E[k] + i O[k]; thread handles the pair (k, M - k)
     for (int k = lt; k <= M / 2; k += S::TPT) {
         if (!active)
@@ -235,17 +299,17 @@ __global__ __launch_bounds__(WG) void fft_v_c2r(const cplx<T>* grids, size_t gri
         // E[k] = X[k] + conj(X[M-k]);  O[k] = (X[k] - conj(X[M-k])) W^k, W = exp(+2 pi i / n)
         const cplx<T> e1 = { a.x + c.x, a.y - c.y };
         const cplx<T> d1 = { a.x - c.x, a.y + c.y };
-        const cplx<T> o1 = cmul(d1, load_tw(tw_n, k, true));
+        const cplx<T> o1 = cmul(d1, quarter_tw(s_q, QLOG, k, true));
         b[k] = { e1.x - o1.y, e1.y + o1.x };
         if (km != k) {
             const cplx<T> e2 = { c.x + a.x, c.y - a.y };
             const cplx<T> d2 = { c.x - a.x, c.y + a.y };
-            const cplx<T> o2 = cmul(d2, load_tw(tw_n, km, true));
+            const cplx<T> o2 = cmul(d2, quarter_tw(s_q, QLOG, km, true));
             b[km] = { e2.x - o2.y, e2.y + o2.x };
         }
     }
     __syncthreads();
-    lds_fft<T, M, S::TPT>(b, lt, tw_n, 2, true);
+    lds_fft<T, M, S::TPT>(b, lt, QuarterTw<T>{ s_q, QLOG, 2, true }, true);
     if (!active)
         return;
     // crop mapping of the padded column x (imager.h:99-150)
@@ -303,7 +367,7 @@ __global__ __launch_bounds__(WG) void fft_cols_c2c(const cplx<T>* in, int rows, 
         for (int r = lt; r < L; r += S::TPT)
             b[r] = in[size_t(slot) * rows + r];
     __syncthreads();
-    lds_fft<T, L, S::TPT>(b, lt, tw_rows, 1, inverse != 0);
+    lds_fft<T, L, S::TPT>(b, lt, GlobalTw<T>{ tw_rows, 1, inverse != 0 }, inverse != 0);
     if (active)
         for (int r = lt; r < L; r += S::TPT)
             out[size_t(x) * rows + r] = b[r];
