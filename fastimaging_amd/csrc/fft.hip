@@ -270,32 +270,47 @@ __global__ __launch_bounds__(WG) void fft_v_c2r(const cplx<T>* grids, size_t gri
     constexpr int n = 2 * M;
     constexpr int rows = M + 1;
     constexpr int QN = (n / 4 > 0) ? n / 4 : 1;  // quarter table: exp(+2 pi i r / n), r < n/4
+    constexpr int QLOG = Log2<QN>::value;
+    constexpr int KI = (M / 2) / S::TPT + 1;      // pack pairs (k, M - k) per thread
     __shared__ cplx<T> buf[S::NT * (M + S::PAD)];
     __shared__ cplx<T> s_q[QN];
     const int tid = threadIdx.x;
     for (int r = tid; r < QN; r += WG)
         s_q[r] = load_tw(tw_n, r, true);
     const int t = tid / S::TPT, lt = tid % S::TPT;
-    const int x = blockIdx.x * S::NT + t;
     const int gi = blockIdx.y;
     const cplx<T>* g = grids + size_t(gi) * grid_elems;
+    cplx<T>* b = buf + t * (M + S::PAD);
+    const int hi = ns.image_size / 2, hp = n / 2;
+    const bool scale = ns.total_ptr != nullptr;
+    const double factor = scale ? (1.0 / ns.total_ptr[0]) : 1.0;
+    const int x = blockIdx.x * S::NT + t;
     const bool active = x < N;
     const int slot = active ? ((x % L1) * L2 + x / L1) : 0;
-    cplx<T>* b = buf + t * (M + S::PAD);
     const cplx<T>* col = g + size_t(slot) * rows;
-    __syncthreads();
-    constexpr int QLOG = Log2<QN>::value;
+    // the column's terms first (all loads in flight together)
+    cplx<T> av[KI], cv[KI];
+#pragma unroll
+    for (int i = 0; i < KI; i++) {
+        const int k = lt + i * S::TPT;
+        if (active && k <= M / 2) {
+            av[i] = col[k];
+            cv[i] = col[M - k];
+        }
+    }
+    __syncthreads();  // the table is ready
     // pack Z[k] = E[k] + i O[k]; thread handles the pair (k, M - k)
-    for (int k = lt; k <= M / 2; k += S::TPT) {
-        if (!active)
-            break;
+#pragma unroll
+    for (int i = 0; i < KI; i++) {
+        const int k = lt + i * S::TPT;
+        if (!active || k > M / 2)
+            continue;
         const int km = M - k;
+        const cplx<T> a = av[i], c = cv[i];
         if (k == 0) {
-            const T x0 = col[0].x, xm = col[M].x; // imaginary parts of v=0 and v=n/2 ignored
-            b[0] = { x0 + xm, x0 - xm };
+            b[0] = { a.x + c.x, a.x - c.x }; // imaginary parts of v=0 and v=n/2 ignored
             continue;
         }
-        const cplx<T> a = col[k], c = col[km];
         // E[k] = X[k] + conj(X[M-k]);  O[k] = (X[k] - conj(X[M-k])) W^k, W = exp(+2 pi i / n)
         const cplx<T> e1 = { a.x + c.x, a.y - c.y };
         const cplx<T> d1 = { a.x - c.x, a.y + c.y };
@@ -310,21 +325,18 @@ __global__ __launch_bounds__(WG) void fft_v_c2r(const cplx<T>* grids, size_t gri
     }
     __syncthreads();
     lds_fft<T, M, S::TPT>(b, lt, QuarterTw<T>{ s_q, QLOG, 2, true }, true);
-    if (!active)
-        return;
     // crop mapping of the padded column x (imager.h:99-150)
-    const int hi = ns.image_size / 2, hp = n / 2;
-    int j;
-    if (x < hi)
-        j = x;
-    else if (x >= N - (ns.image_size - hi))
-        j = x - N + ns.image_size;
-    else
+    int j = -1;
+    if (active) {
+        if (x < hi)
+            j = x;
+        else if (x >= N - (ns.image_size - hi))
+            j = x - N + ns.image_size;
+    }
+    if (j < 0)
         return;
     const int joff = (j < hi) ? j : j - ns.image_size;
     OT* out = outs.p[gi] + size_t(j) * ns.image_size;
-    const bool scale = ns.total_ptr != nullptr;
-    const double factor = scale ? (1.0 / ns.total_ptr[0]) : 1.0;
     for (int m = lt; m < M; m += S::TPT) {
         const cplx<T> z = b[m];
         const int ys[2] = { 2 * m, 2 * m + 1 };
