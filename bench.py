@@ -89,7 +89,7 @@ def pmc_traffic(workload, path, kernel):
 
 def measure(ctx, capi, args, pars, sfp, prec, n, N, bufs, do_sf):
     d_vis, d_w, d_uvw, d_img, d_beam, d_lab = bufs
-    island_buf = (capi.Island * (1 << 16))()
+    island_buf = ctx.island_buffer(1 << 16)  # page-locked: the catalogue arrives by DMA
     S = 8 if prec == capi.PRECISION_FP64 else 4
     rows = N // 2 + 1
 
@@ -152,6 +152,8 @@ def measure(ctx, capi, args, pars, sfp, prec, n, N, bufs, do_sf):
     B = 48 * n + 2 * (2 * rows * N * 2 * S) + 2 * N * N * S + (6 * N * N * S + 4 * N * N if do_sf else 0)
     image = np.empty((N, N), order="F", dtype=np.float64 if S == 8 else np.float32)
     ctx.dtoh(image, d_img)
+    island_list = [dict(i) for i in islands] if islands is not None else None
+    ctx.free_host(island_buf)
     return {
         "value": args.steps / t_total,
         "ms_per_step": ms_per_step,
@@ -163,7 +165,7 @@ def measure(ctx, capi, args, pars, sfp, prec, n, N, bufs, do_sf):
         "path_roofline": {"compulsory_bytes": B, "achieved_GBs": round(B / (ms_per_step * 1e-3) / 1e9, 1),
                           "frac": round(B / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
-        "islands": [dict(i) for i in islands] if islands is not None else None,
+        "islands": island_list,
         "image": image,
     }
 

@@ -309,6 +309,8 @@ _SIG = {
                                               ctypes.POINTER(SourceFindStats)]),
     "stp_device_malloc": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_size_t]),
     "stp_device_free": (ctypes.c_int, [_P, _P]),
+    "stp_host_malloc": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_size_t]),
+    "stp_host_free": (ctypes.c_int, [_P, _P]),
     "stp_memcpy_htod": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "stp_memcpy_dtoh": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "stp_ctx_set_autotune": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p]),

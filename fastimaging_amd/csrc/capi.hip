@@ -309,6 +309,12 @@ stp_status stp_ctx_destroy(stp_ctx* ctx)
     }
     for (auto e : ctx->branch_events)
         (void)hipEventDestroy(e);
+    if (ctx->copy_stream) {
+        (void)hipStreamSynchronize(ctx->copy_stream);
+        (void)hipStreamDestroy(ctx->copy_stream);
+    }
+    if (ctx->copy_event)
+        (void)hipEventDestroy(ctx->copy_event);
     for (auto& kv : ctx->ws)
         if (kv.second.ptr)
             (void)hipFree(kv.second.ptr);
@@ -619,6 +625,24 @@ stp_status stp_device_malloc(stp_ctx* ctx, void** ptr, size_t bytes)
     return guarded(ctx, [&] {
         STP_REQUIRE(ptr != nullptr, "NULL pointer");
         HIP_CHECK(hipMalloc(ptr, bytes ? bytes : 1));
+    });
+}
+
+stp_status stp_host_malloc(stp_ctx* ctx, void** ptr, size_t bytes)
+{
+    return guarded(ctx, [&] {
+        STP_REQUIRE(ptr != nullptr, "NULL pointer");
+        HIP_CHECK(hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault));
+    });
+}
+
+stp_status stp_host_free(stp_ctx* ctx, void* ptr)
+{
+    return guarded(ctx, [&] {
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (ctx->copy_stream)
+            HIP_CHECK(hipStreamSynchronize(ctx->copy_stream));
+        HIP_CHECK(hipHostFree(ptr));
     });
 }
 

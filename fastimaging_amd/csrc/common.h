@@ -98,12 +98,19 @@ struct stp_ctx {
     // islands found by the last source find: the size of the island copy enqueued
     // with the final state copy (one host round trip when it suffices)
     size_t sf_islands_hint = 64;
+    // second stream for result copies that overlap the last kernels of a call
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t copy_event = nullptr;
 };
 
 namespace stp_hip {
 
 // Returns a device buffer of at least `bytes` for slot `name` (contents undefined).
 void* ws_get(stp_ctx* ctx, const std::string& name, size_t bytes);
+// The context's copy stream (created on first use, with ctx->copy_event).
+hipStream_t copy_stream(stp_ctx* ctx);
+// Whether host pointer p lies in page-locked memory of the HIP runtime.
+bool pinned_host(const void* p);
 // Bytes held by workspace slot `name` (0 when it has none).
 size_t ws_bytes(stp_ctx* ctx, const std::string& name);
 // Destroys the context's captured batch graphs (their buffers are about to change).

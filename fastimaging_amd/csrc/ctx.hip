@@ -28,6 +28,25 @@ void* ws_get(stp_ctx* ctx, const std::string& name, size_t bytes)
     return b.ptr;
 }
 
+hipStream_t copy_stream(stp_ctx* ctx)
+{
+    if (!ctx->copy_stream) {
+        HIP_CHECK(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&ctx->copy_event, hipEventDisableTiming));
+    }
+    return ctx->copy_stream;
+}
+
+bool pinned_host(const void* p)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // ordinary (pageable) memory
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
 size_t ws_bytes(stp_ctx* ctx, const std::string& name)
 {
     auto it = ctx->ws.find(ctx->ws_slot ? name + "#" + std::to_string(ctx->ws_slot) : name);

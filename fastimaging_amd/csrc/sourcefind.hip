@@ -2105,7 +2105,7 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
     // device: no host round trip), skipped by the kernels when the labelling is
     // incomplete.
     stp_island* disl = nullptr;
-    auto islands_of_labels = [&]() {
+    auto islands_of_labels = [&]() -> const IslandSum* {
         const size_t nl = size_t(ntiles) * CT_RT + rec.pool_cap;
         IslandSum* isum = wsp<IslandSum>(ctx, "sf_isum", nl);
         const SegIds sid{ tg, rmask, segoff };
@@ -2113,9 +2113,6 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
         launch(ctx, "ccl_open_init", ccl_open_init, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
         launch(ctx, "ccl_records", ccl_records, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
         launch(ctx, "ccl_open_idx", ccl_open_idx, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
-        if (d_lab_out)
-            launch(ctx, "ccl_label_map", ccl_label_map, dim3(ntiles), dim3(LM_THREADS), 0, (const uint32_t*)P, sid,
-                (const SFState*)st, (const IslandSum*)isum, int(p.generate_labelmap != 0), d_lab_out);
         uint32_t* keep = wsp<uint32_t>(ctx, "sf_keep", nl);
         uint32_t* koff = wsp<uint32_t>(ctx, "sf_koff", nl + 1);
         uint32_t* vcnt = wsp<uint32_t>(ctx, "sf_vcnt", div_up(nl, 256));
@@ -2125,22 +2122,35 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
         scan_u32(ctx, keep, nl, koff);
         launch(ctx, "sf_island_emit", sf_island_emit, dim3(rblocks), dim3(256), 0, (const IslandSum*)isum,
             (const uint32_t*)keep, (const uint32_t*)koff, (const uint32_t*)vcnt, st, rows, cols, p.analysis_n_sigma, disl);
+        return isum;
     };
     SFState* hs = (SFState*)host_stage(ctx, "sf_state", sizeof(SFState));
-    // the state, and as many islands as the last call found (a guess: any missing
-    // ones are copied afterwards), in one host round trip
+    // the island list goes straight to a page-locked caller buffer, else through a stage;
+    // as many islands as the last call found are copied with the state (a guess: any
+    // missing ones are copied afterwards)
+    const bool pinned = pinned_host(islands);
     const size_t spec = std::min({ cap, ctx->sf_islands_hint, size_t(ntiles) * CT_RT + rec.pool_cap });
-    stp_island* stage = (stp_island*)host_stage(ctx, "sf_islands", std::max<size_t>(1, spec) * sizeof(stp_island));
-    auto read_state = [&]() {
-        HIP_CHECK(hipMemcpyAsync(hs, st, sizeof(SFState), hipMemcpyDeviceToHost, ctx->stream));
+    stp_island* stage =
+        pinned ? islands : (stp_island*)host_stage(ctx, "sf_islands", std::max<size_t>(1, spec) * sizeof(stp_island));
+    // the island stages; then the state and the island list are copied on the copy
+    // stream while the label map runs, and one host wait covers both
+    auto finish = [&]() {
+        const IslandSum* isum = islands_of_labels();
+        hipStream_t cs = copy_stream(ctx);
+        HIP_CHECK(hipEventRecord(ctx->copy_event, ctx->stream));
+        HIP_CHECK(hipStreamWaitEvent(cs, ctx->copy_event, 0));
+        HIP_CHECK(hipMemcpyAsync(hs, st, sizeof(SFState), hipMemcpyDeviceToHost, cs));
         if (spec)
-            HIP_CHECK(hipMemcpyAsync(stage, disl, spec * sizeof(stp_island), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_CHECK(hipMemcpyAsync(stage, disl, spec * sizeof(stp_island), hipMemcpyDeviceToHost, cs));
+        if (d_lab_out)
+            launch(ctx, "ccl_label_map", ccl_label_map, dim3(ntiles), dim3(LM_THREADS), 0, (const uint32_t*)P,
+                SegIds{ tg, rmask, segoff }, (const SFState*)st, isum, int(p.generate_labelmap != 0), d_lab_out);
+        HIP_CHECK(hipStreamSynchronize(cs));
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
     };
     rms_and_thresholds(pool_count);
     label(false);
-    islands_of_labels();
-    read_state();
+    finish();
     if (hs->fallback) {
         // data the linear bins cannot resolve (heavy ties): radix selection on
         // the host-driven path, then the stages that depend on the median again
@@ -2148,16 +2158,14 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
         launch(ctx, "sf_fallback_clear", sf_fallback_clear, dim3(1), dim3(1), 0, ss);
         rms_and_thresholds(pool_count);
         label(false);
-        islands_of_labels();
-        read_state();
+        finish();
     }
     if (hs->num_records > rec.pool_cap) {
         // more tile components than record slots: grow and label again (the
         // capacity is kept for later calls)
         records(size_t(hs->num_records) + size_t(hs->num_records) / 4);
         label(true);
-        islands_of_labels();
-        read_state();
+        finish();
         if (hs->num_records > rec.pool_cap)
             throw Error(STP_ERR_INTERNAL, "component records overflow");
     }
@@ -2167,13 +2175,18 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
     const size_t found = hs->num_islands;
     const size_t ncopy = std::min(found, cap);
     if (ncopy > spec) {
-        stage = (stp_island*)host_stage(ctx, "sf_islands", ncopy * sizeof(stp_island));
-        HIP_CHECK(hipMemcpyAsync(stage, disl, ncopy * sizeof(stp_island), hipMemcpyDeviceToHost, ctx->stream));
+        if (pinned) {
+            HIP_CHECK(hipMemcpyAsync(islands + spec, disl + spec, (ncopy - spec) * sizeof(stp_island),
+                hipMemcpyDeviceToHost, ctx->stream));
+        } else {
+            stage = (stp_island*)host_stage(ctx, "sf_islands", ncopy * sizeof(stp_island));
+            HIP_CHECK(hipMemcpyAsync(stage, disl, ncopy * sizeof(stp_island), hipMemcpyDeviceToHost, ctx->stream));
+        }
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
     }
-    if (ncopy)
+    if (ncopy && !pinned)
         std::memcpy(islands, stage, ncopy * sizeof(stp_island));
-    ctx->sf_islands_hint = std::max<size_t>(64, found + found / 4);
+    ctx->sf_islands_hint = std::max<size_t>(64, found + found / 8);
     *n_islands = found;
     if (stats) {
         stats->bg_level = hs->median;

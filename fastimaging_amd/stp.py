@@ -176,6 +176,18 @@ class Context:
     def free(self, p):
         self._check(self.lib.stp_device_free(self.h, p))
 
+    def island_buffer(self, capacity):
+        """A (capi.Island * capacity) array in page-locked host memory: island
+        lists written to it arrive by DMA (free with free_host)."""
+        p = _P()
+        self._check(self.lib.stp_host_malloc(self.h, ctypes.byref(p), ctypes.sizeof(capi.Island) * capacity))
+        arr = (capi.Island * capacity).from_address(p.value)
+        arr._stp_host_ptr = p
+        return arr
+
+    def free_host(self, arr):
+        self._check(self.lib.stp_host_free(self.h, arr._stp_host_ptr))
+
     def htod(self, dptr, arr):
         arr = np.ascontiguousarray(arr) if not arr.flags.f_contiguous else arr
         self._check(self.lib.stp_memcpy_htod(self.h, dptr, _ptr(arr), arr.nbytes))

@@ -327,7 +327,9 @@ stp_status stp_source_find(stp_ctx* ctx,
 
 /* Same, with the image (and label map, if not NULL) in device memory.
  * image_precision selects the element type of d_image (double for FP64,
- * float for FP32C).  Islands and stats are returned in host memory. */
+ * float for FP32C).  Islands and stats are returned in host memory; an island
+ * buffer from stp_host_malloc is written by DMA (its entries past *n_islands,
+ * within the capacity, may then be overwritten). */
 stp_status stp_source_find_device(stp_ctx* ctx,
     const void* d_image,
     int32_t image_precision,
@@ -347,6 +349,11 @@ stp_status stp_device_malloc(stp_ctx* ctx, void** ptr, size_t bytes);
 stp_status stp_device_free(stp_ctx* ctx, void* ptr);
 stp_status stp_memcpy_htod(stp_ctx* ctx, void* dst, const void* src, size_t bytes);
 stp_status stp_memcpy_dtoh(stp_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* No reference counterpart.  Page-locked host memory: an island list written by
+ * stp_source_find / stp_source_find_device into such a buffer arrives by DMA,
+ * with no staging copy on the host (ordinary host memory works too). */
+stp_status stp_host_malloc(stp_ctx* ctx, void** ptr, size_t bytes);
+stp_status stp_host_free(stp_ctx* ctx, void* ptr);
 
 /* ------------------------------------------------------------------------- */
 /* Autotuner and batches                                                      */

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from fastimaging_amd import capi, simdata
-from tests.helpers import compare, fftshift, gaussian2d, noise
+from tests.helpers import ISLAND_EXACT, compare, fftshift, fits_agree, gaussian2d, noise
 
 pytestmark = pytest.mark.gpu
 
@@ -106,6 +106,40 @@ def test_gpu_sigma_clip_candidates(gpu, orc, case):
         assert all(b[k] > 0.6 * b[0] for k in range(len(b)))
     pars = capi.sourcefind_pars(5.0, 3.0, sigma_clip_iters=iters, median_method=capi.MEDIAN_BINMEDIAN)
     compare(gpu.source_find(img, pars), orc.source_find(img, pars))
+
+
+def test_gpu_island_list_pinned_buffer(gpu, orc):
+    """An island list written by DMA into a page-locked buffer (copied with the state
+    while the label map runs) equals the one through ordinary memory and the
+    oracle's: small, large, small catalogues on one context, so that the copy sized
+    by the previous call's count falls short, then overshoots."""
+    fields = [_field(256, 256, seed=41, n_src=2), _field(512, 512, seed=42, n_src=60), _field(256, 256, seed=43, n_src=3)]
+    pars = capi.sourcefind_pars(4.0, 2.5, median_method=capi.MEDIAN_BINMEDIAN, source_min_area=1)
+    pinned = gpu.island_buffer(1 << 14)
+    try:
+        counts = []
+        for img in fields:
+            rows, cols = img.shape
+            d_img = gpu.malloc(8 * rows * cols)
+            try:
+                gpu.htod(d_img, img)
+                a, sa = gpu.source_find_device(d_img, rows, cols, pars, precision=capi.PRECISION_FP64, out=pinned)
+                a = list(a)
+                b, sb = gpu.source_find_device(d_img, rows, cols, pars, precision=capi.PRECISION_FP64, capacity=1 << 14)
+                b = list(b)
+            finally:
+                gpu.free(d_img)
+            assert sa.num_islands == sb.num_islands == len(a) == len(b)
+            _, oisl, _ = orc.source_find(img, pars)
+            assert len(a) == len(oisl)
+            # moments: sums in atomic order, so within fits_agree() from call to call
+            for x, y, z in zip(a, b, oisl):
+                assert all(x[f] == y[f] == z[f] for f in ISLAND_EXACT)
+                assert fits_agree(x["moments_fit"], y["moments_fit"]) and fits_agree(x["moments_fit"], z["moments_fit"])
+            counts.append(len(a))
+        assert counts[1] > 2 * counts[0] + 64 and counts[2] < counts[1]
+    finally:
+        gpu.free_host(pinned)
 
 
 def test_gpu_sourcefind_degenerate(gpu, orc):
