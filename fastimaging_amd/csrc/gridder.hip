@@ -327,12 +327,19 @@ __device__ double dev_kernel(const GridGeom& g, double r)
     return v;
 }
 
-// s: the reduced (good weights, good count) pair.
-__global__ void scale_total_kernel(const double* s, double* out, unsigned long long* ngood)
+// Last canonical level of the (good weights, good count) block pairs: the sampling
+// grid total 2 * sum(weights) and the good count.
+__global__ __launch_bounds__(CANON_LANES) void grid_total_kernel(const double* part, unsigned m, double* total,
+    unsigned long long* ngood)
 {
-    out[0] = s[0] * 2.0;
-    if (ngood)
-        *ngood = (unsigned long long)s[1];
+    __shared__ double lds[2 * CANON_LANES];
+    double a, b;
+    canon_last_level(part, m, a, b, lds);
+    if (threadIdx.x == 0) {
+        total[0] = a * 2.0;
+        if (ngood)
+            *ngood = (unsigned long long)b;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1221,7 +1228,6 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     const size_t nblk = std::max<size_t>(1, (n + VPB - 1) / VPB);
     double* wpart = (double*)ws_get(ctx, "grid_wpart", sizeof(double) * (2 * nblk + 2));
     double* wscr = (double*)ws_get(ctx, "grid_wscr", sizeof(double) * canon_scratch_doubles(nblk));
-    double* wres = (double*)ws_get(ctx, "grid_wres", sizeof(double) * 2);
     // worst-case entries: a footprint of ks cells spans at most ceil((ks-1)/T)+1 tiles per
     // axis, one more where it wraps across the grid edge; two passes
     const size_t per_pass = size_t(std::min(MAXT, int(div_up(g.ks - 1, TILE_R)) + 2)) *
@@ -1367,8 +1373,9 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         }
     }
     // sample_grid_total = 2 * sum(good weights) (gridder.h:415-423); the good count
-    canon_reduce_pairs(ctx, wpart, nblk, wscr, wres);
-    launch(ctx, "grid_total", scale_total_kernel, dim3(1), dim3(1), 0, (const double*)wres, d_total, d_ngood);
+    size_t mfin = 0;
+    const double* fin = canon_reduce_to_block(ctx, wpart, nblk, wscr, &mfin);
+    launch(ctx, "grid_total", grid_total_kernel, dim3(1), dim3(CANON_LANES), 0, fin, unsigned(mfin), d_total, d_ngood);
 }
 
 template void grid_visibilities<double>(stp_ctx*, const GridderArgs&, size_t, const double*, const double*,

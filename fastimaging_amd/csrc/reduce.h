@@ -50,6 +50,30 @@ __device__ __forceinline__ void canon_tree(double& a, double& b, double* lds)
     b = lds[CANON_LANES];
 }
 
+// Last level of the canonical reduction (m <= CANON_BLOCK pairs, one block of
+// CANON_LANES threads, the order of the canonical pair kernel); thread 0 gets the sums.
+__device__ __forceinline__ void canon_last_level(const double* in, unsigned m, double& a, double& b, double* lds)
+{
+    // every load is issued before the first sum waits on one: unconditional loads
+    // (clamped index; the buffers hold at least one pair)
+    double va[CANON_PER_LANE], vb[CANON_PER_LANE];
+#pragma unroll
+    for (int j = 0; j < CANON_PER_LANE; j++) {
+        const unsigned i = min(unsigned(j) * CANON_LANES + threadIdx.x, m > 0 ? m - 1 : 0u);
+        va[j] = in[2 * i];
+        vb[j] = in[2 * i + 1];
+    }
+    a = 0.0;
+    b = 0.0;
+#pragma unroll
+    for (int j = 0; j < CANON_PER_LANE; j++) {
+        const bool ok = unsigned(j) * CANON_LANES + threadIdx.x < m;
+        a = ok ? a + va[j] : a;
+        b = ok ? b + vb[j] : b;
+    }
+    canon_tree(a, b, lds);
+}
+
 // Reduces m pairs (2m doubles at `partials`) to one pair written to `result`
 // (device, 2 doubles), with the canonical order.  `scratch` holds
 // canon_scratch_doubles(m) doubles.

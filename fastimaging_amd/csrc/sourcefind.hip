@@ -190,30 +190,6 @@ __device__ void set_mean_sigma(double s1, double s2, unsigned long long n, SFSta
     st->sigma = sqrt(s2 / double(n) - (mean * mean));
 }
 
-// Last level of the canonical reduction (m <= CANON_BLOCK pairs, one block of
-// CANON_LANES threads, the order of the canonical pair kernel); thread 0 gets the sums.
-__device__ __forceinline__ void canon_last_level(const double* in, unsigned m, double& a, double& b, double* lds)
-{
-    // every load is issued before the first sum waits on one: unconditional loads
-    // (clamped index; the buffers hold at least one pair)
-    double va[CANON_PER_LANE], vb[CANON_PER_LANE];
-#pragma unroll
-    for (int j = 0; j < CANON_PER_LANE; j++) {
-        const unsigned i = min(unsigned(j) * CANON_LANES + threadIdx.x, m > 0 ? m - 1 : 0u);
-        va[j] = in[2 * i];
-        vb[j] = in[2 * i + 1];
-    }
-    a = 0.0;
-    b = 0.0;
-#pragma unroll
-    for (int j = 0; j < CANON_PER_LANE; j++) {
-        const bool ok = unsigned(j) * CANON_LANES + threadIdx.x < m;
-        a = ok ? a + va[j] : a;
-        b = ok ? b + vb[j] : b;
-    }
-    canon_tree(a, b, lds);
-}
-
 // ------------------------------------------------------------------ histograms
 // bin index of matrix_math.cpp:370-392 with the x86 out-of-range conversion made explicit
 __device__ __forceinline__ int ba_bin(double x, double leftend, double scale)
@@ -515,13 +491,17 @@ __global__ __launch_bounds__(256) void sel_init(SFState* st, SelState* ss, SelIn
 }
 
 // Mean and sigma from the block partials of sf_sum (last canonical level), then
-// optionally the selection set-up.
+// optionally the selection set-up.  clear (nclear doubles, may be null): the state
+// buffer, zeroed first.
 __global__ __launch_bounds__(CANON_LANES) void sf_stats(const double* part, unsigned m, unsigned long long n, SFState* st,
-    SelState* ss, SelInit si)
+    SelState* ss, SelInit si, double* clear, unsigned nclear)
 {
     __shared__ double lds[2 * CANON_LANES];
+    if (clear)
+        for (unsigned k = threadIdx.x; k < nclear; k += CANON_LANES)
+            clear[k] = 0.0;
     double a, b;
-    canon_last_level(part, m, a, b, lds);
+    canon_last_level(part, m, a, b, lds);  // its barriers also order the clearing before the writes
     if (threadIdx.x == 0)
         set_mean_sigma(a, b, n, st);
     if (si.enabled)
@@ -1991,15 +1971,24 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
     const bool need_rms = !(std::fabs(p.rms_est) > 0.0);
     const int mm = p.median_method;
     bool stats_valid = false;
+    // the state starts zeroed: by the first statistics kernel, else by a fill
+    const unsigned nstate = unsigned((sizeof(SFState) + sizeof(SelState)) / sizeof(double));
+    bool cleared = false;
+    auto clear_state = [&]() {
+        if (!cleared)
+            fill_f64(ctx, (double*)stbuf, 0.0, nstate);
+        cleared = true;
+    };
     // mean and sigma (canonical sums); si.enabled also sets up the exact-median search
     auto compute_mean_sigma = [&](const SelInit& si) {
         launch(ctx, "sf_sum", sf_sum_kernel<T>, dim3(div_up(nblk, CANON_GROUP)), dim3(SWG), 0, x, n, nblk, part);
         size_t mfin = 0;
         const double* fin = canon_reduce_to_block(ctx, part, nblk, scr, &mfin);
-        launch(ctx, "sf_stats", sf_stats, dim3(1), dim3(CANON_LANES), 0, fin, unsigned(mfin), (unsigned long long)n, st, ss, si);
+        launch(ctx, "sf_stats", sf_stats, dim3(1), dim3(CANON_LANES), 0, fin, unsigned(mfin), (unsigned long long)n, st, ss, si,
+            cleared ? (double*)nullptr : (double*)stbuf, nstate);
+        cleared = true;
         stats_valid = true;
     };
-    fill_f64(ctx, (double*)stbuf, 0.0, (sizeof(SFState) + sizeof(SelState)) / sizeof(double));
     // median (sourcefind.cpp:182-197), enqueued without host round trips: the
     // BINAPPROX pick and the exact search are gated on the device state
     if (mm == STP_MEDIAN_BINAPPROX) {
@@ -2025,6 +2014,7 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
     // ZEROMEDIAN: the state's median stays 0
     if (need_rms && !stats_valid)
         compute_mean_sigma(SelInit{});
+    clear_state();  // no statistics kernel ran (ZEROMEDIAN with a given RMS)
 
     // RMS (sourcefind.cpp:206) and thresholds (:298-301)
     // reset: a counter of the labelling cleared with the thresholds
