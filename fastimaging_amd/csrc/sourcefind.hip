@@ -255,8 +255,15 @@ __global__ __launch_bounds__(256) void sf_hist_reduce(const uint32_t* bh, int nb
     const int per = (nblk + int(gridDim.y) - 1) / int(gridDim.y);
     const int b0 = int(blockIdx.y) * per, b1 = min(nblk, b0 + per);
     unsigned long long s = 0;
-    for (int b = b0; b < b1; b++)
-        s += bh[size_t(b) * nb + k];
+    for (int c = b0; c < b1; c += 8) {
+        uint32_t v[8];  // a batch of loads in flight together (clamped index)
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            v[j] = bh[size_t(min(c + j, b1 - 1)) * nb + k];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            s += c + j < b1 ? v[j] : 0u;
+    }
     if (s)
         atomicAdd(&out[k], s);
 }
@@ -404,39 +411,55 @@ __device__ __forceinline__ int sub_bin(double t, double base2, double scale2)
     return b < 0 ? 0 : (b >= LIN_BINS ? LIN_BINS - 1 : b);
 }
 
-// Bins holding ranks ka <= kb of a histogram (1024 threads).  Writes the bin
-// indices, the count before bin ia and the count through bin ib; -1 if absent.
+// Bins holding ranks ka <= kb of a histogram of nb <= 1024 * RB_PER bins (1024
+// threads).  Writes the bin indices, the count before bin ia and the count
+// through bin ib; -1 if absent.
+constexpr int RB_PER = 9;
+static_assert(LIN_BINS + 2 <= 1024 * RB_PER, "rank_bins capacity");
 __device__ void rank_bins(const unsigned long long* h, int nb, unsigned long long ka, unsigned long long kb, int* ia,
     int* ib, unsigned long long* before_a, unsigned long long* through_b)
 {
-    __shared__ unsigned long long part[1024];
-    const int t = threadIdx.x;
+    __shared__ unsigned long long wsum[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t == 0) {
         *ia = -1;
         *ib = -1;
     }
     const int per = (nb + 1023) / 1024;
     const int i0 = min(nb, t * per), i1 = min(nb, i0 + per);
+    // the thread's bins, all loads in flight together (clamped index)
+    unsigned long long v[RB_PER];
+#pragma unroll
+    for (int j = 0; j < RB_PER; j++)
+        v[j] = h[min(i0 + j, nb - 1)];
     unsigned long long s = 0;
-    for (int i = i0; i < i1; i++)
-        s += h[i];
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const unsigned long long v = (t >= off) ? part[t - off] : 0ull;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RB_PER; j++)
+        s += i0 + j < i1 ? v[j] : 0ull;
+    // exclusive scan over the threads: wave shuffles, then the wave totals
+    unsigned long long x = s;
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned long long y = __shfl_up(x, off, 64);
+        if (lane >= off)
+            x += y;
     }
-    unsigned long long cum = part[t] - s;
-    for (int i = i0; i < i1; i++) {
-        const unsigned long long c = cum + h[i];
+    if (lane == 63)
+        wsum[w] = x;
+    __syncthreads();
+    unsigned long long cum = x - s;
+    for (int k = 0; k < w; k++)
+        cum += wsum[k];
+#pragma unroll
+    for (int j = 0; j < RB_PER; j++) {
+        if (i0 + j >= i1)
+            break;
+        const unsigned long long c = cum + v[j];
         if (cum <= ka && c > ka) {
-            *ia = i;
+            *ia = i0 + j;
             *before_a = cum;
         }
         if (cum <= kb && c > kb) {
-            *ib = i;
+            *ib = i0 + j;
             *through_b = c;
         }
         cum = c;
