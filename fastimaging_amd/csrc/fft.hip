@@ -198,17 +198,27 @@ __global__ __launch_bounds__(WG) void fft_u1(cplx<T>* grids, int rows, int N, in
     const int v0 = blockIdx.y * S::NT;
     cplx<T>* g = grids + size_t(blockIdx.z) * grid_elems;
     const int tid = threadIdx.x;
+    // NT * L1 / WG elements per thread, all loads in flight before the LDS stores
+    constexpr int EPT = S::NT * L1 / WG;
+    static_assert(EPT * WG == S::NT * L1, "whole elements per thread");
+    cplx<T> xv[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const int idx = tid + e * WG;
+        const int vv = idx % S::NT, u1 = idx / S::NT;
+        const int v = v0 + vv;
+        xv[e] = { T(0), T(0) };
+        if (v < rows)
+            xv[e] = g[size_t(v) + size_t(u1 * L2 + u2) * rows];
+    }
     for (int e = tid; e < L1; e += WG) {
         s_tw[e] = load_tw(tw, e * (N / L1), true);
         s_post[e] = load_tw(tw, (e * u2) & (N - 1), true);
     }
-    for (int idx = tid; idx < S::NT * L1; idx += WG) {
-        const int vv = idx % S::NT, u1 = idx / S::NT;
-        const int v = v0 + vv;
-        cplx<T> x = { T(0), T(0) };
-        if (v < rows)
-            x = g[size_t(v) + size_t(u1 * L2 + u2) * rows];
-        buf[vv * (L1 + S::PAD) + u1] = x;
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const int idx = tid + e * WG;
+        buf[(idx % S::NT) * (L1 + S::PAD) + idx / S::NT] = xv[e];
     }
     __syncthreads();
     const int t = tid / S::TPT, lt = tid % S::TPT;
@@ -236,15 +246,24 @@ __global__ __launch_bounds__(WG) void fft_u2(cplx<T>* grids, int rows, int N, in
     const int v0 = blockIdx.y * S::NT;
     cplx<T>* g = grids + size_t(blockIdx.z) * grid_elems;
     const int tid = threadIdx.x;
-    for (int e = tid; e < L2; e += WG)
-        s_tw[e] = load_tw(tw, e * (N / L2), true);
-    for (int idx = tid; idx < S::NT * L2; idx += WG) {
+    constexpr int EPT = S::NT * L2 / WG;
+    static_assert(EPT * WG == S::NT * L2, "whole elements per thread");
+    cplx<T> xv[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const int idx = tid + e * WG;
         const int vv = idx % S::NT, u2 = idx / S::NT;
         const int v = v0 + vv;
-        cplx<T> x = { T(0), T(0) };
+        xv[e] = { T(0), T(0) };
         if (v < rows)
-            x = g[size_t(v) + size_t(k1 * L2 + u2) * rows];
-        buf[vv * (L2 + S::PAD) + u2] = x;
+            xv[e] = g[size_t(v) + size_t(k1 * L2 + u2) * rows];
+    }
+    for (int e = tid; e < L2; e += WG)
+        s_tw[e] = load_tw(tw, e * (N / L2), true);
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const int idx = tid + e * WG;
+        buf[(idx % S::NT) * (L2 + S::PAD) + idx / S::NT] = xv[e];
     }
     __syncthreads();
     const int t = tid / S::TPT, lt = tid % S::TPT;
