@@ -394,9 +394,20 @@ __global__ __launch_bounds__(WG) void fft_cols_c2c(const cplx<T>* in, int rows, 
     const bool active = x < N;
     const int slot = active ? ((x % L1) * L2 + x / L1) : 0;
     cplx<T>* b = buf + t * (L + S::PAD);
-    if (active)
-        for (int r = lt; r < L; r += S::TPT)
-            b[r] = in[size_t(slot) * rows + r];
+    constexpr int EPT = (L + S::TPT - 1) / S::TPT;  // loads in flight together
+    cplx<T> xv[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const int r = lt + e * S::TPT;
+        if (active && r < L)
+            xv[e] = in[size_t(slot) * rows + r];
+    }
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const int r = lt + e * S::TPT;
+        if (active && r < L)
+            b[r] = xv[e];
+    }
     __syncthreads();
     lds_fft<T, L, S::TPT>(b, lt, GlobalTw<T>{ tw_rows, 1, inverse != 0 }, inverse != 0);
     if (active)
