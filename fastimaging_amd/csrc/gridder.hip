@@ -468,13 +468,67 @@ __global__ __launch_bounds__(1024) void bin_colscan_kernel(uint32_t* counts, int
     }
 }
 
+// Exclusive scan of the tile totals into out[0..ntiles) (LDS) by the BWG threads of a
+// block, each taking a run of at most TSCAN_PER tiles; returns the total.
+constexpr int TSCAN_PER = (MAX_LDS_TILES + BWG - 1) / BWG;
+__device__ uint32_t scan_tile_totals(const uint32_t* totals, int ntiles, uint32_t* out, uint32_t* wsum)
+{
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int per = (ntiles + BWG - 1) / BWG;
+    const int i0 = min(ntiles, t * per), i1 = min(ntiles, i0 + per);
+    uint32_t v[TSCAN_PER];
+#pragma unroll
+    for (int j = 0; j < TSCAN_PER; j++)
+        v[j] = i0 + j < i1 ? totals[i0 + j] : 0u;
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < TSCAN_PER; j++)
+        s += v[j];
+    uint32_t x = s;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off)
+            x += y;
+    }
+    if (lane == 63)
+        wsum[w] = x;
+    __syncthreads();
+    uint32_t total = 0;
+    for (int k = 0; k < BWG / 64; k++) {
+        total += wsum[k];
+        if (k < w)
+            x += wsum[k];
+    }
+    uint32_t run = x - s;
+#pragma unroll
+    for (int j = 0; j < TSCAN_PER; j++) {
+        if (i0 + j < i1) {
+            out[i0 + j] = run;
+            run += v[j];
+        }
+    }
+    return total;
+}
+
+// Block b's entries go to cursors tile offset + block offset; the tile offsets (the
+// scan of the tile totals) are computed by every block in LDS, and block 0 also
+// writes them to tile_off[0..ntiles] for the tile kernel.
 __global__ __launch_bounds__(BWG) void grid_bin_scatter_kernel(GridGeom g, const double* uv, const double* vis, const double* wts,
-    const uint32_t* tile_off, const uint32_t* block_off, Entry* entries, uint32_t cap)
+    const uint32_t* totals, const uint32_t* block_off, Entry* entries, uint32_t cap, uint32_t* tile_off)
 {
     extern __shared__ uint32_t cur[];
+    __shared__ uint32_t wsum[BWG / 64];
     const int ntiles = g.tiles_r * g.tiles_c;
+    const uint32_t total = scan_tile_totals(totals, ntiles, cur, wsum);
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        for (int t = threadIdx.x; t < ntiles; t += BWG)
+            tile_off[t] = cur[t];
+        if (threadIdx.x == 0)
+            tile_off[ntiles] = total;
+    }
     for (int t = threadIdx.x; t < ntiles; t += BWG)
-        cur[t] = tile_off[t] + block_off[size_t(blockIdx.x) * ntiles + t];
+        cur[t] += block_off[size_t(blockIdx.x) * ntiles + t];
     __syncthreads();
     const long long base = (long long)blockIdx.x * VPB;
     VisIn in[VPB / BWG];
@@ -1317,7 +1371,6 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         uint32_t* counts = (uint32_t*)ws_get(ctx, "grid_bcounts", sizeof(uint32_t) * m);
         uint32_t* totals = (uint32_t*)ws_get(ctx, "grid_btotals", sizeof(uint32_t) * ntiles);
         uint32_t* toffs = (uint32_t*)ws_get(ctx, "grid_boffsets", sizeof(uint32_t) * (ntiles + 1));
-        uint32_t* tcur = (uint32_t*)ws_get(ctx, "grid_bcursor", sizeof(uint32_t) * ntiles);
         const size_t hist_bytes = sizeof(uint32_t) * size_t(ntiles);
         HIP_CHECK(hipFuncSetAttribute((const void*)grid_bin_count_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(hist_bytes)));
         HIP_CHECK(hipFuncSetAttribute((const void*)grid_bin_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(hist_bytes)));
@@ -1325,9 +1378,8 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
             int(nblk), wpart);
         launch(ctx, "grid_colscan", bin_colscan_kernel, dim3(div_up(ntiles, 64)), dim3(1024), 0, counts, int(nblk), int(ntiles),
             totals);
-        exclusive_scan_u32(ctx, totals, int(ntiles), toffs, tcur);
         launch(ctx, "grid_scatter", grid_bin_scatter_kernel, dim3(unsigned(nblk)), dim3(BWG), hist_bytes, g, d_uv, d_vis, d_w,
-            (const uint32_t*)toffs, (const uint32_t*)counts, entries, cap);
+            (const uint32_t*)totals, (const uint32_t*)counts, entries, cap, toffs);
         offsets = toffs;
         ostride = 1;
     } else {

@@ -204,18 +204,43 @@ __global__ __launch_bounds__(256) void chunk_sum_kernel(L in, size_t m, uint32_t
         sums[blockIdx.x] = a;
 }
 
+// chunk_off: the chunks' offsets; when null, a block adds up the chunk sums before
+// it itself (at most RESCAN_SELF chunks: 16 loads per thread, in flight together).
+constexpr int RESCAN_SELF = 4096;
 template <typename L>
-__global__ __launch_bounds__(256) void chunk_rescan_kernel(L in, size_t m, const uint32_t* chunk_off, uint32_t* out)
+__global__ __launch_bounds__(256) void chunk_rescan_kernel(L in, size_t m, const uint32_t* chunk_off, const uint32_t* sums,
+    uint32_t* out)
 {
-    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t wsum[4], wsum2[4];
+    __shared__ uint32_t s_off;
     const size_t base = size_t(blockIdx.x) * SCAN_CHUNK + size_t(threadIdx.x) * 16;
     uint32_t v[16];
     load16(in, base, m, v);
+    uint32_t off;
+    if (chunk_off) {
+        off = chunk_off[blockIdx.x];
+    } else {
+        uint32_t w[RESCAN_SELF / 256];
+#pragma unroll
+        for (int j = 0; j < RESCAN_SELF / 256; j++) {
+            const unsigned c = threadIdx.x + 256u * j;
+            w[j] = c < blockIdx.x ? sums[c] : 0u;
+        }
+        uint32_t p = 0;
+#pragma unroll
+        for (int j = 0; j < RESCAN_SELF / 256; j++)
+            p += w[j];
+        p = block_scan256(p, wsum2);
+        if (threadIdx.x == 255)
+            s_off = p;
+        __syncthreads();
+        off = s_off;
+    }
     uint32_t a = 0;
 #pragma unroll
     for (int j = 0; j < 16; j++)
         a += v[j];
-    uint32_t run = chunk_off[blockIdx.x] + block_scan256(a, wsum) - a;
+    uint32_t run = off + block_scan256(a, wsum) - a;
     for (int j = 0; j < 16; j++) {
         if (base + j < m)
             out[base + j] = run;
@@ -238,8 +263,15 @@ void scan_impl(stp_ctx* ctx, L in, size_t m, uint32_t* out)
     uint32_t* offs = (uint32_t*)ws_get(ctx, "scan_offs", sizeof(uint32_t) * (nb + 1));
     uint32_t* cur = (uint32_t*)ws_get(ctx, "scan_cur", sizeof(uint32_t) * nb);
     launch(ctx, "scan_sums", chunk_sum_kernel<L>, dim3(unsigned(nb)), dim3(256), 0, in, m, sums);
+    if (nb <= size_t(RESCAN_SELF)) {
+        // few chunks: every rescan block adds up the sums before it (no scan launch)
+        launch(ctx, "scan_rescan", chunk_rescan_kernel<L>, dim3(unsigned(nb)), dim3(256), 0, in, m, (const uint32_t*)nullptr,
+            (const uint32_t*)sums, out);
+        return;
+    }
     exclusive_scan_u32(ctx, sums, int(nb), offs, cur);
-    launch(ctx, "scan_rescan", chunk_rescan_kernel<L>, dim3(unsigned(nb)), dim3(256), 0, in, m, (const uint32_t*)offs, out);
+    launch(ctx, "scan_rescan", chunk_rescan_kernel<L>, dim3(unsigned(nb)), dim3(256), 0, in, m, (const uint32_t*)offs,
+        (const uint32_t*)sums, out);
 }
 
 void scan_u32(stp_ctx* ctx, const uint32_t* in, size_t m, uint32_t* out)

@@ -1574,10 +1574,29 @@ __global__ __launch_bounds__(256) void ccl_open_roots(uint32_t* P, TileGeom g, C
     }
 }
 
-// Open components whose local root is the global root start their island's sums.
-__global__ __launch_bounds__(256) void ccl_open_init(const uint32_t* P, SegIds s, CompRecords rec, IslandSum* isum)
+// Label counts, and whether the island stages can run: every tile component has
+// a record (so the labels fit the cap of the island arrays) and the median held.
+struct LabelCounts {
+    SFState* st;
+    const SelState* ss;
+    unsigned long long cap;  // labels the island arrays hold
+};
+
+// Open components whose local root is the global root start their island's sums;
+// thread 0 also sets the label counts (read by the later island kernels).
+__global__ __launch_bounds__(256) void ccl_open_init(const uint32_t* P, SegIds s, CompRecords rec, IslandSum* isum,
+    LabelCounts lc)
 {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i == 0) {
+        SFState* st = lc.st;
+        const uint32_t nrec = *rec.pool_count, kp = s.offs[s.g.nseg], kall = s.offs[2 * s.g.nseg];
+        st->num_pos = kp;
+        st->num_neg = kall - kp;
+        st->num_records = nrec;
+        st->fallback = lc.ss->fallback;
+        st->skip = nrec > rec.pool_cap || lc.ss->fallback || (unsigned long long)kall > lc.cap;
+    }
     if (pool_overflow(rec) || !rec_valid(rec, i))
         return;
     const uint32_t k = rec.key[i];
@@ -1823,18 +1842,6 @@ __global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, con
     moments_fit(isl, xb, yb, m[2] / m[5] - xb * xb, m[3] / m[5] - yb * yb, m[4] / m[5] - xb * yb, st->rms,
         analysis_n_sigma);
     out[offs[g]] = isl;
-}
-
-// Label counts, and whether the island stages can run: every tile component has
-// a record (so the labels fit the cap of the island arrays) and the median held.
-__global__ void sf_label_counts(SFState* st, const uint32_t* segoff, unsigned nseg, const uint32_t* nrec, uint32_t pool_cap,
-    unsigned long long cap, const SelState* ss)
-{
-    st->num_pos = segoff[nseg];
-    st->num_neg = segoff[2 * nseg] - segoff[nseg];
-    st->num_records = *nrec;
-    st->fallback = ss->fallback;
-    st->skip = *nrec > pool_cap || ss->fallback || (unsigned long long)segoff[2 * nseg] > cap;
 }
 
 __global__ void sf_fallback_clear(SelState* ss)
@@ -2108,9 +2115,6 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
         launch(ctx, "ccl_open_roots", ccl_open_roots, dim3(div_up(size_t(ntiles) * CT_RT + rec.pool_cap, 256)), dim3(256), 0, P,
             tg, rec, rmask);
         scan_popc64(ctx, rmask, nseg2, segoff);
-        launch(ctx, "sf_counts", sf_label_counts, dim3(1), dim3(1), 0, st, (const uint32_t*)segoff, tg.nseg,
-            (const uint32_t*)pool_count, rec.pool_cap, (unsigned long long)(size_t(ntiles) * CT_RT + rec.pool_cap),
-            (const SelState*)ss);
     };
     // island sums in list order (positive labels, then negative); label map
     // (sourcefind.cpp:337-628); the island list: validity, area cut, compaction and
@@ -2123,7 +2127,8 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
         IslandSum* isum = wsp<IslandSum>(ctx, "sf_isum", nl);
         const SegIds sid{ tg, rmask, segoff };
         const unsigned rblocks = div_up(nl, 256);
-        launch(ctx, "ccl_open_init", ccl_open_init, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
+        launch(ctx, "ccl_open_init", ccl_open_init, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum,
+            LabelCounts{ st, (const SelState*)ss, (unsigned long long)nl });
         launch(ctx, "ccl_records", ccl_records, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
         launch(ctx, "ccl_open_idx", ccl_open_idx, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
         uint32_t* keep = wsp<uint32_t>(ctx, "sf_keep", nl);
