@@ -437,7 +437,7 @@ stp_status stp_image_visibilities_device(stp_ctx* ctx, const stp_imager_pars* p,
         if (ctx->autotune)
             autotune_shape(ctx, p, wp && wp->num_wplanes > 0 ? wp : nullptr, kernel, n, d_vis, d_w, d_uvw, precision, d_image,
                 d_beam, d_total, d_ngood);
-        LaunchTimer whole(ctx, "image_visibilities");
+        LaunchTimer whole(ctx, "image_visibilities", ctx->stream);
         if (precision == STP_PRECISION_FP64)
             run_imager<double, double>(ctx, p, wp, kernel, n, d_vis, d_w, d_uvw, (double*)d_image, (double*)d_beam, d_total,
                 d_ngood, false);

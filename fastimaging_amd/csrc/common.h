@@ -109,8 +109,9 @@ namespace stp_hip {
 void* ws_get(stp_ctx* ctx, const std::string& name, size_t bytes);
 // The context's copy stream (created on first use, with ctx->copy_event).
 hipStream_t copy_stream(stp_ctx* ctx);
-// Whether host pointer p lies in page-locked memory of the HIP runtime.
-bool pinned_host(const void* p);
+// The device address of host pointer p when it lies in page-locked memory of the
+// HIP runtime (kernels may write it directly), else null.
+void* pinned_device_ptr(void* p);
 // Bytes held by workspace slot `name` (0 when it has none).
 size_t ws_bytes(stp_ctx* ctx, const std::string& name);
 // Destroys the context's captured batch graphs (their buffers are about to change).
@@ -126,8 +127,9 @@ const Twiddles& twiddles_for(stp_ctx* ctx, int n);
 struct LaunchTimer {
     stp_ctx* ctx;
     const char* name;
+    hipStream_t stream;
     hipEvent_t a = nullptr, b = nullptr;
-    LaunchTimer(stp_ctx* c, const char* n);
+    LaunchTimer(stp_ctx* c, const char* n, hipStream_t s);
     ~LaunchTimer();
 };
 
@@ -138,8 +140,20 @@ inline void launch(stp_ctx* ctx, const char* name, K kernel, dim3 grid, dim3 blo
 {
     if (grid.x == 0 || grid.y == 0 || grid.z == 0)
         return;
-    LaunchTimer t(ctx, name);
+    LaunchTimer t(ctx, name, ctx->stream);
     hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream, args...);
+    HIP_CHECK(hipGetLastError());
+}
+
+// The same on another stream of the context (its copy stream).
+template <typename K, typename... Args>
+inline void launch_on(stp_ctx* ctx, hipStream_t stream, const char* name, K kernel, dim3 grid, dim3 block, size_t shmem,
+    Args... args)
+{
+    if (grid.x == 0 || grid.y == 0 || grid.z == 0)
+        return;
+    LaunchTimer t(ctx, name, stream);
+    hipLaunchKernelGGL(kernel, grid, block, shmem, stream, args...);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -37,14 +37,18 @@ hipStream_t copy_stream(stp_ctx* ctx)
     return ctx->copy_stream;
 }
 
-bool pinned_host(const void* p)
+void* pinned_device_ptr(void* p)
 {
     hipPointerAttribute_t a;
     if (hipPointerGetAttributes(&a, p) != hipSuccess) {
         (void)hipGetLastError();  // ordinary (pageable) memory
-        return false;
+        return nullptr;
     }
-    return a.type == hipMemoryTypeHost;
+    if (a.type != hipMemoryTypeHost)
+        return nullptr;
+    void* d = nullptr;
+    HIP_CHECK(hipHostGetDevicePointer(&d, p, 0));
+    return d;
 }
 
 size_t ws_bytes(stp_ctx* ctx, const std::string& name)
@@ -135,9 +139,10 @@ static hipEvent_t take_event(stp_ctx* ctx)
     return e;
 }
 
-LaunchTimer::LaunchTimer(stp_ctx* c, const char* n)
+LaunchTimer::LaunchTimer(stp_ctx* c, const char* n, hipStream_t s)
     : ctx(c)
     , name(n)
+    , stream(s)
 {
     bool on = ctx->timing;
     if (on && !ctx->timing_filter.empty()) {
@@ -151,14 +156,14 @@ LaunchTimer::LaunchTimer(stp_ctx* c, const char* n)
     if (on) {
         a = take_event(ctx);
         b = take_event(ctx);
-        HIP_CHECK(hipEventRecord(a, ctx->stream));
+        HIP_CHECK(hipEventRecord(a, stream));
     }
 }
 
 LaunchTimer::~LaunchTimer()
 {
     if (a) {
-        if (hipEventRecord(b, ctx->stream) == hipSuccess)
+        if (hipEventRecord(b, stream) == hipSuccess)
             ctx->pending.push_back({ name, a, b });
     }
 }

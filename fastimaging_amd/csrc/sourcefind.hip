@@ -1787,8 +1787,11 @@ __global__ __launch_bounds__(256) void sf_island_flags(const IslandSum* isum, ui
 // One island record per kept label at its rank among the kept labels; block 0
 // also writes the island count and the count of valid labels (vcnt: one per block
 // of sf_island_flags, the same grid).
+// out holds out_cap records (device memory, or page-locked host memory written over
+// the bus); islands beyond it are counted, not written.
 __global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, const uint32_t* keep,
-    const uint32_t* offs, const uint32_t* vcnt, SFState* st, int rows, int cols, double analysis_n_sigma, stp_island* out)
+    const uint32_t* offs, const uint32_t* vcnt, SFState* st, int rows, int cols, double analysis_n_sigma, stp_island* out,
+    uint32_t out_cap)
 {
     __shared__ uint32_t red[256];
     if (st->skip)
@@ -1841,7 +1844,8 @@ __global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, con
     const double xb = m[0] / m[5], yb = m[1] / m[5];
     moments_fit(isl, xb, yb, m[2] / m[5] - xb * xb, m[3] / m[5] - yb * yb, m[4] / m[5] - xb * yb, st->rms,
         analysis_n_sigma);
-    out[offs[g]] = isl;
+    if (offs[g] < out_cap)
+        out[offs[g]] = isl;
 }
 
 __global__ void sf_fallback_clear(SelState* ss)
@@ -2122,7 +2126,10 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
     // device: no host round trip), skipped by the kernels when the labelling is
     // incomplete.
     stp_island* disl = nullptr;
-    auto islands_of_labels = [&]() -> const IslandSum* {
+    // a page-locked caller buffer is written by the emitting kernel itself
+    stp_island* islands_dev = (stp_island*)pinned_device_ptr(islands);
+    const bool pinned = islands_dev != nullptr;
+    auto island_sums = [&]() -> const IslandSum* {
         const size_t nl = size_t(ntiles) * CT_RT + rec.pool_cap;
         IslandSum* isum = wsp<IslandSum>(ctx, "sf_isum", nl);
         const SegIds sid{ tg, rmask, segoff };
@@ -2131,6 +2138,11 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
             LabelCounts{ st, (const SelState*)ss, (unsigned long long)nl });
         launch(ctx, "ccl_records", ccl_records, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
         launch(ctx, "ccl_open_idx", ccl_open_idx, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
+        return isum;
+    };
+    auto island_list = [&](const IslandSum* isum) {
+        const size_t nl = size_t(ntiles) * CT_RT + rec.pool_cap;
+        const unsigned rblocks = div_up(nl, 256);
         uint32_t* keep = wsp<uint32_t>(ctx, "sf_keep", nl);
         uint32_t* koff = wsp<uint32_t>(ctx, "sf_koff", nl + 1);
         uint32_t* vcnt = wsp<uint32_t>(ctx, "sf_vcnt", div_up(nl, 256));
@@ -2139,32 +2151,34 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
             (const SFState*)st, int(p.source_min_area), keep, vcnt);
         scan_u32(ctx, keep, nl, koff);
         launch(ctx, "sf_island_emit", sf_island_emit, dim3(rblocks), dim3(256), 0, (const IslandSum*)isum,
-            (const uint32_t*)keep, (const uint32_t*)koff, (const uint32_t*)vcnt, st, rows, cols, p.analysis_n_sigma, disl);
-        return isum;
+            (const uint32_t*)keep, (const uint32_t*)koff, (const uint32_t*)vcnt, st, rows, cols, p.analysis_n_sigma,
+            pinned ? islands_dev : disl, pinned ? uint32_t(std::min<size_t>(cap, 0xFFFFFFFFu)) : uint32_t(nl));
     };
     SFState* hs = (SFState*)host_stage(ctx, "sf_state", sizeof(SFState));
-    // the island list goes straight to a page-locked caller buffer, else through a stage;
-    // as many islands as the last call found are copied with the state (a guess: any
-    // missing ones are copied afterwards)
-    const bool pinned = pinned_host(islands);
-    const size_t spec = std::min({ cap, ctx->sf_islands_hint, size_t(ntiles) * CT_RT + rec.pool_cap });
+    // an ordinary caller buffer gets the island list through a stage: as many islands as
+    // the last call found are copied with the state (a guess: any missing ones are
+    // copied afterwards)
+    const size_t spec = pinned ? 0 : std::min({ cap, ctx->sf_islands_hint, size_t(ntiles) * CT_RT + rec.pool_cap });
     stp_island* stage =
         pinned ? islands : (stp_island*)host_stage(ctx, "sf_islands", std::max<size_t>(1, spec) * sizeof(stp_island));
-    // the island stages; then the state and the island list are copied on the copy
-    // stream while the label map runs, and one host wait covers both
+    // the island sums; then the label map on the copy stream while the island list is
+    // built (and written or copied) and the state copied on the main stream; one host
+    // wait covers both
     auto finish = [&]() {
-        const IslandSum* isum = islands_of_labels();
+        const IslandSum* isum = island_sums();
         hipStream_t cs = copy_stream(ctx);
-        HIP_CHECK(hipEventRecord(ctx->copy_event, ctx->stream));
-        HIP_CHECK(hipStreamWaitEvent(cs, ctx->copy_event, 0));
-        HIP_CHECK(hipMemcpyAsync(hs, st, sizeof(SFState), hipMemcpyDeviceToHost, cs));
-        if (spec)
-            HIP_CHECK(hipMemcpyAsync(stage, disl, spec * sizeof(stp_island), hipMemcpyDeviceToHost, cs));
-        if (d_lab_out)
-            launch(ctx, "ccl_label_map", ccl_label_map, dim3(ntiles), dim3(LM_THREADS), 0, (const uint32_t*)P,
+        if (d_lab_out) {
+            HIP_CHECK(hipEventRecord(ctx->copy_event, ctx->stream));
+            HIP_CHECK(hipStreamWaitEvent(cs, ctx->copy_event, 0));
+            launch_on(ctx, cs, "ccl_label_map", ccl_label_map, dim3(ntiles), dim3(LM_THREADS), 0, (const uint32_t*)P,
                 SegIds{ tg, rmask, segoff }, (const SFState*)st, isum, int(p.generate_labelmap != 0), d_lab_out);
-        HIP_CHECK(hipStreamSynchronize(cs));
+        }
+        island_list(isum);
+        HIP_CHECK(hipMemcpyAsync(hs, st, sizeof(SFState), hipMemcpyDeviceToHost, ctx->stream));
+        if (spec)
+            HIP_CHECK(hipMemcpyAsync(stage, disl, spec * sizeof(stp_island), hipMemcpyDeviceToHost, ctx->stream));
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(cs));
     };
     rms_and_thresholds(pool_count);
     label(false);
@@ -2192,14 +2206,9 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
     const uint32_t kp = hs->num_pos, kn = hs->num_neg;
     const size_t found = hs->num_islands;
     const size_t ncopy = std::min(found, cap);
-    if (ncopy > spec) {
-        if (pinned) {
-            HIP_CHECK(hipMemcpyAsync(islands + spec, disl + spec, (ncopy - spec) * sizeof(stp_island),
-                hipMemcpyDeviceToHost, ctx->stream));
-        } else {
-            stage = (stp_island*)host_stage(ctx, "sf_islands", ncopy * sizeof(stp_island));
-            HIP_CHECK(hipMemcpyAsync(stage, disl, ncopy * sizeof(stp_island), hipMemcpyDeviceToHost, ctx->stream));
-        }
+    if (!pinned && ncopy > spec) {
+        stage = (stp_island*)host_stage(ctx, "sf_islands", ncopy * sizeof(stp_island));
+        HIP_CHECK(hipMemcpyAsync(stage, disl, ncopy * sizeof(stp_island), hipMemcpyDeviceToHost, ctx->stream));
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
     }
     if (ncopy && !pinned)

@@ -328,8 +328,7 @@ stp_status stp_source_find(stp_ctx* ctx,
 /* Same, with the image (and label map, if not NULL) in device memory.
  * image_precision selects the element type of d_image (double for FP64,
  * float for FP32C).  Islands and stats are returned in host memory; an island
- * buffer from stp_host_malloc is written by DMA (its entries past *n_islands,
- * within the capacity, may then be overwritten). */
+ * buffer from stp_host_malloc is written by the device directly (no copy). */
 stp_status stp_source_find_device(stp_ctx* ctx,
     const void* d_image,
     int32_t image_precision,
