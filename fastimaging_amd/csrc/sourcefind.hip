@@ -1787,13 +1787,42 @@ __global__ __launch_bounds__(256) void sf_island_flags(const IslandSum* isum, ui
 // One island record per kept label at its rank among the kept labels; block 0
 // also writes the island count and the count of valid labels (vcnt: one per block
 // of sf_island_flags, the same grid).
+// One island record (sourcefind.cpp:235-278, 592-613) from the sums of label g.
+__device__ void emit_island(const IslandSum& s, uint32_t g, uint32_t kp, int rows, int cols, double rms,
+    double analysis_n_sigma, stp_island* dst)
+{
+    const bool pos = g < kp;
+    const uint32_t i = pos ? g : g - kp;
+    const size_t li = size_t(s.idx);
+    const int r = int(li % size_t(rows)), c = int(li / size_t(rows));
+    const int h_shift = cols / 2, v_shift = rows / 2;
+    stp_island isl{};
+    isl.label_idx = pos ? int32_t(i + 1) : -int32_t(i + 1);
+    isl.sign = pos ? 1 : -1;
+    isl.extremum_y_idx = r < v_shift ? r + v_shift : r - v_shift;
+    isl.extremum_x_idx = c < h_shift ? c + h_shift : c - h_shift;
+    isl.num_samples = s.cnt;
+    isl.extremum_val = okey_inv(s.ext);
+    // moments normalised by the flux sum (sourcefind.cpp:594-613)
+    const double* m = s.mom;
+    const double xb = m[0] / m[5], yb = m[1] / m[5];
+    moments_fit(isl, xb, yb, m[2] / m[5] - xb * xb, m[3] / m[5] - yb * yb, m[4] / m[5] - xb * yb, rms,
+        analysis_n_sigma);
+    *dst = isl;
+}
+
 // out holds out_cap records (device memory, or page-locked host memory written over
-// the bus); islands beyond it are counted, not written.
+// the bus); islands beyond it are counted, not written.  A block's islands form one
+// run of the list: they are assembled in LDS and written as consecutive 16-byte
+// chunks by consecutive lanes (whole lines on the bus).
+static_assert(sizeof(stp_island) % 16 == 0, "island records of whole 16-byte chunks");
+constexpr int ISL_CHUNKS = int(sizeof(stp_island) / 16);
 __global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, const uint32_t* keep,
     const uint32_t* offs, const uint32_t* vcnt, SFState* st, int rows, int cols, double analysis_n_sigma, stp_island* out,
     uint32_t out_cap)
 {
     __shared__ uint32_t red[256];
+    __shared__ uint4 s_isl[256 * ISL_CHUNKS];
     if (st->skip)
         return;
     const uint32_t kp = st->num_pos, kn = st->num_neg;
@@ -1824,29 +1853,19 @@ __global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, con
             st->num_islands = offs[kp + kn];
         }
     }
-    if (g >= kp + kn || !keep[g])
+    const uint32_t nlab = kp + kn, g0 = blockIdx.x * 256;
+    if (g0 >= nlab)
         return;
-    const bool pos = g < kp;
-    const uint32_t i = pos ? g : g - kp;
-    const IslandSum& s = isum[g];
-    const size_t li = size_t(s.idx);
-    const int r = int(li % size_t(rows)), c = int(li / size_t(rows));
-    const int h_shift = cols / 2, v_shift = rows / 2;
-    stp_island isl{};
-    isl.label_idx = pos ? int32_t(i + 1) : -int32_t(i + 1);
-    isl.sign = pos ? 1 : -1;
-    isl.extremum_y_idx = r < v_shift ? r + v_shift : r - v_shift;
-    isl.extremum_x_idx = c < h_shift ? c + h_shift : c - h_shift;
-    isl.num_samples = s.cnt;
-    isl.extremum_val = okey_inv(s.ext);
-    // moments normalised by the flux sum (sourcefind.cpp:594-613)
-    const double* m = s.mom;
-    const double xb = m[0] / m[5], yb = m[1] / m[5];
-    moments_fit(isl, xb, yb, m[2] / m[5] - xb * xb, m[3] / m[5] - yb * yb, m[4] / m[5] - xb * yb, st->rms,
-        analysis_n_sigma);
-    if (offs[g] < out_cap)
-        out[offs[g]] = isl;
+    const uint32_t base = offs[g0], cnt = offs[min(g0 + 256, nlab)] - base;
+    if (g < nlab && keep[g])
+        emit_island(isum[g], g, kp, rows, cols, st->rms, analysis_n_sigma, (stp_island*)s_isl + (offs[g] - base));
+    __syncthreads();
+    uint4* o = (uint4*)out + size_t(base) * ISL_CHUNKS;
+    for (uint32_t q = threadIdx.x; q < cnt * ISL_CHUNKS; q += 256)
+        if (base + q / ISL_CHUNKS < out_cap)
+            o[q] = s_isl[q];
 }
+
 
 __global__ void sf_fallback_clear(SelState* ss)
 {
