@@ -1104,6 +1104,16 @@ __device__ __forceinline__ uint32_t root_gid(const SegIds& s, uint32_t key, bool
     return gid_at(s, sseg, rr);
 }
 
+// Label g's sums: lrec[g] is the record of a closed component, or LREC_OPEN for an
+// open island, whose combined sums are isum[g].
+constexpr uint32_t LREC_OPEN = 0xFFFFFFFFu;
+__device__ __forceinline__ const IslandSum& label_sum(const IslandSum* isum, const uint32_t* lrec, const IslandSum* rsum,
+    uint32_t g)
+{
+    const uint32_t r = lrec[g];
+    return r == LREC_OPEN ? isum[g] : rsum[r];
+}
+
 // Per-component records written by ccl_tile: tile b owns records [b rt, b rt + rt);
 // components beyond rt take records of a shared pool after the tile regions (one
 // atomic per overflowing tile: a device-wide counter per tile would serialise).
@@ -1585,7 +1595,7 @@ struct LabelCounts {
 // Open components whose local root is the global root start their island's sums;
 // thread 0 also sets the label counts (read by the later island kernels).
 __global__ __launch_bounds__(256) void ccl_open_init(const uint32_t* P, SegIds s, CompRecords rec, IslandSum* isum,
-    LabelCounts lc)
+    uint32_t* lrec, LabelCounts lc)
 {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i == 0) {
@@ -1607,11 +1617,15 @@ __global__ __launch_bounds__(256) void ccl_open_init(const uint32_t* P, SegIds s
     IslandSum z{};
     z.ext = neg ? ~0ull : 0ull;
     z.idx = ~0u;
-    isum[root_gid(s, lr, neg)] = z;
+    const uint32_t gid = root_gid(s, lr, neg);
+    isum[gid] = z;
+    lrec[gid] = LREC_OPEN;
 }
 
-// Island sums by list position: closed components are stored, open ones combined.
-__global__ __launch_bounds__(256) void ccl_records(const uint32_t* P, SegIds s, CompRecords rec, IslandSum* isum)
+// Island sums by list position: a closed component's label points at its record,
+// open ones are combined into the island's sums.
+__global__ __launch_bounds__(256) void ccl_records(const uint32_t* P, SegIds s, CompRecords rec, IslandSum* isum,
+    uint32_t* lrec)
 {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (pool_overflow(rec) || !rec_valid(rec, i))
@@ -1619,7 +1633,7 @@ __global__ __launch_bounds__(256) void ccl_records(const uint32_t* P, SegIds s, 
     const uint32_t k = rec.key[i];
     if (!(k & P_OPEN)) {
         const uint32_t pos = rec.pos[i];
-        isum[gid_at(s, pos >> 6, int(pos & 63u))] = rec.sum[i];
+        lrec[gid_at(s, pos >> 6, int(pos & 63u))] = i;
         return;
     }
     const bool neg = k & P_NEG;
@@ -1653,7 +1667,8 @@ constexpr int LM_THREADS = 256, LM_WAVES = LM_THREADS / 64, LM_PPT = CT_PIX / LM
 // Label map in memory order (LM_THREADS threads per tile): +-(id + 1), 0 for background and, with gen_map,
 // for islands whose extremum misses the detection threshold (sourcefind.cpp:462-492).
 __global__ __launch_bounds__(LM_THREADS) void ccl_label_map(const uint32_t* __restrict__ P, SegIds s, const SFState* st,
-    const IslandSum* __restrict__ isum, int gen_map, int32_t* __restrict__ lab)
+    const IslandSum* __restrict__ isum, const uint32_t* __restrict__ lrec, const IslandSum* __restrict__ rsum, int gen_map,
+    int32_t* __restrict__ lab)
 {
     const TileGeom& g = s.g;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -1685,12 +1700,16 @@ __global__ __launch_bounds__(LM_THREADS) void ccl_label_map(const uint32_t* __re
         so[j] = pv[j] != P_BG ? s.offs[sseg] : 0u;
         sm[j] = pv[j] != P_BG ? s.mask[sseg] : 0ull;
     }
-    unsigned long long ev[LM_PPT];
+    uint32_t lr[LM_PPT];
 #pragma unroll
     for (int j = 0; j < LM_PPT; j++) {
         so[j] += uint32_t(__popcll(sm[j] & ((1ull << rr[j]) - 1ull)));  // the island id
-        ev[j] = (gen_map && pv[j] != P_BG) ? isum[so[j]].ext : 0ull;
+        lr[j] = (gen_map && pv[j] != P_BG) ? lrec[so[j]] : 0u;
     }
+    unsigned long long ev[LM_PPT];
+#pragma unroll
+    for (int j = 0; j < LM_PPT; j++)
+        ev[j] = (gen_map && pv[j] != P_BG) ? (lr[j] == LREC_OPEN ? isum[so[j]].ext : rsum[lr[j]].ext) : 0ull;
     // the labels, then the stores (a wait for a load between stores would also
     // wait for the stores before it)
     int32_t L[LM_PPT];
@@ -1759,8 +1778,8 @@ __device__ void moments_fit(stp_island& isl, double x_bar, double y_bar, double 
 // valid: extremum beyond the detection threshold (sourcefind.cpp:417-436);
 // keep: valid and at least min_area samples (sourcefind.cpp:240,263).
 // The grid covers cap >= kp + kn labels (kp, kn from the state); keep is 0 beyond.
-__global__ __launch_bounds__(256) void sf_island_flags(const IslandSum* isum, uint32_t cap, const SFState* st,
-    int min_area, uint32_t* keep, uint32_t* vcnt)
+__global__ __launch_bounds__(256) void sf_island_flags(const IslandSum* isum, const uint32_t* lrec, const IslandSum* rsum,
+    uint32_t cap, const SFState* st, int min_area, uint32_t* keep, uint32_t* vcnt)
 {
     __shared__ uint32_t wv[4];
     if (st->skip)
@@ -1770,9 +1789,10 @@ __global__ __launch_bounds__(256) void sf_island_flags(const IslandSum* isum, ui
     bool v = false;
     if (g < kp + kn) {
         const bool pos = g < kp;
-        const double ext = okey_inv(isum[g].ext);
+        const IslandSum& sm = label_sum(isum, lrec, rsum, g);
+        const double ext = okey_inv(sm.ext);
         v = pos ? (ext > st->d_pos) : (ext < st->d_neg);
-        keep[g] = (v && isum[g].cnt >= min_area) ? 1u : 0u;
+        keep[g] = (v && sm.cnt >= min_area) ? 1u : 0u;
     } else if (g < cap) {
         keep[g] = 0u;
     }
@@ -1817,7 +1837,8 @@ __device__ void emit_island(const IslandSum& s, uint32_t g, uint32_t kp, int row
 // chunks by consecutive lanes (whole lines on the bus).
 static_assert(sizeof(stp_island) % 16 == 0, "island records of whole 16-byte chunks");
 constexpr int ISL_CHUNKS = int(sizeof(stp_island) / 16);
-__global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, const uint32_t* keep,
+__global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, const uint32_t* lrec, const IslandSum* rsum,
+    const uint32_t* keep,
     const uint32_t* offs, const uint32_t* vcnt, SFState* st, int rows, int cols, double analysis_n_sigma, stp_island* out,
     uint32_t out_cap)
 {
@@ -1858,7 +1879,8 @@ __global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, con
         return;
     const uint32_t base = offs[g0], cnt = offs[min(g0 + 256, nlab)] - base;
     if (g < nlab && keep[g])
-        emit_island(isum[g], g, kp, rows, cols, st->rms, analysis_n_sigma, (stp_island*)s_isl + (offs[g] - base));
+        emit_island(label_sum(isum, lrec, rsum, g), g, kp, rows, cols, st->rms, analysis_n_sigma,
+            (stp_island*)s_isl + (offs[g] - base));
     __syncthreads();
     uint4* o = (uint4*)out + size_t(base) * ISL_CHUNKS;
     for (uint32_t q = threadIdx.x; q < cnt * ISL_CHUNKS; q += 256)
@@ -2148,14 +2170,16 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
     // a page-locked caller buffer is written by the emitting kernel itself
     stp_island* islands_dev = (stp_island*)pinned_device_ptr(islands);
     const bool pinned = islands_dev != nullptr;
+    uint32_t* lrec = nullptr;
     auto island_sums = [&]() -> const IslandSum* {
         const size_t nl = size_t(ntiles) * CT_RT + rec.pool_cap;
         IslandSum* isum = wsp<IslandSum>(ctx, "sf_isum", nl);
+        lrec = wsp<uint32_t>(ctx, "sf_lrec", nl);
         const SegIds sid{ tg, rmask, segoff };
         const unsigned rblocks = div_up(nl, 256);
-        launch(ctx, "ccl_open_init", ccl_open_init, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum,
+        launch(ctx, "ccl_open_init", ccl_open_init, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum, lrec,
             LabelCounts{ st, (const SelState*)ss, (unsigned long long)nl });
-        launch(ctx, "ccl_records", ccl_records, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
+        launch(ctx, "ccl_records", ccl_records, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum, lrec);
         launch(ctx, "ccl_open_idx", ccl_open_idx, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
         return isum;
     };
@@ -2166,11 +2190,12 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
         uint32_t* koff = wsp<uint32_t>(ctx, "sf_koff", nl + 1);
         uint32_t* vcnt = wsp<uint32_t>(ctx, "sf_vcnt", div_up(nl, 256));
         disl = wsp<stp_island>(ctx, "sf_islands", nl);
-        launch(ctx, "sf_island_flags", sf_island_flags, dim3(rblocks), dim3(256), 0, (const IslandSum*)isum, uint32_t(nl),
-            (const SFState*)st, int(p.source_min_area), keep, vcnt);
+        launch(ctx, "sf_island_flags", sf_island_flags, dim3(rblocks), dim3(256), 0, (const IslandSum*)isum,
+            (const uint32_t*)lrec, (const IslandSum*)rec.sum, uint32_t(nl), (const SFState*)st, int(p.source_min_area), keep,
+            vcnt);
         scan_u32(ctx, keep, nl, koff);
         launch(ctx, "sf_island_emit", sf_island_emit, dim3(rblocks), dim3(256), 0, (const IslandSum*)isum,
-            (const uint32_t*)keep, (const uint32_t*)koff, (const uint32_t*)vcnt, st, rows, cols, p.analysis_n_sigma,
+            (const uint32_t*)lrec, (const IslandSum*)rec.sum, (const uint32_t*)keep, (const uint32_t*)koff, (const uint32_t*)vcnt, st, rows, cols, p.analysis_n_sigma,
             pinned ? islands_dev : disl, pinned ? uint32_t(std::min<size_t>(cap, 0xFFFFFFFFu)) : uint32_t(nl));
     };
     SFState* hs = (SFState*)host_stage(ctx, "sf_state", sizeof(SFState));
@@ -2190,7 +2215,8 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
             HIP_CHECK(hipEventRecord(ctx->copy_event, ctx->stream));
             HIP_CHECK(hipStreamWaitEvent(cs, ctx->copy_event, 0));
             launch_on(ctx, cs, "ccl_label_map", ccl_label_map, dim3(ntiles), dim3(LM_THREADS), 0, (const uint32_t*)P,
-                SegIds{ tg, rmask, segoff }, (const SFState*)st, isum, int(p.generate_labelmap != 0), d_lab_out);
+                SegIds{ tg, rmask, segoff }, (const SFState*)st, isum, (const uint32_t*)lrec, (const IslandSum*)rec.sum,
+                int(p.generate_labelmap != 0), d_lab_out);
         }
         island_list(isum);
         HIP_CHECK(hipMemcpyAsync(hs, st, sizeof(SFState), hipMemcpyDeviceToHost, ctx->stream));
