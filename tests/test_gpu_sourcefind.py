@@ -142,6 +142,34 @@ def test_gpu_island_list_pinned_buffer(gpu, orc):
         gpu.free_host(pinned)
 
 
+def test_gpu_island_capacity_too_small(gpu):
+    """A catalogue larger than the island buffer: STP_ERR_CAPACITY naming the count, and the
+    first `capacity` islands delivered, into a page-locked buffer (written by the emitting
+    kernel, which stops at the capacity) as into an ordinary one."""
+    from fastimaging_amd.stp import STPError
+    img = _field(512, 512, seed=42, n_src=60)
+    rows, cols = img.shape
+    pars = capi.sourcefind_pars(4.0, 2.5, median_method=capi.MEDIAN_BINMEDIAN, source_min_area=1)
+    d_img = gpu.malloc(8 * rows * cols)
+    pinned = gpu.island_buffer(32)
+    try:
+        gpu.htod(d_img, img)
+        full, st = gpu.source_find_device(d_img, rows, cols, pars, precision=capi.PRECISION_FP64, capacity=1 << 12)
+        full = list(full)
+        assert len(full) > 64
+        plain = (capi.Island * 32)()
+        for buf in (pinned, plain):
+            with pytest.raises(STPError) as e:
+                gpu.source_find_device(d_img, rows, cols, pars, precision=capi.PRECISION_FP64, out=buf)
+            assert e.value.status == 6 and f"{len(full)} islands found" in str(e.value)
+            got = [capi.island_tuple(buf[i]) for i in range(32)]
+            for x, y in zip(got, full[:32]):
+                assert all(x[f] == y[f] for f in ISLAND_EXACT) and fits_agree(x["moments_fit"], y["moments_fit"])
+    finally:
+        gpu.free(d_img)
+        gpu.free_host(pinned)
+
+
 def test_gpu_sourcefind_degenerate(gpu, orc):
     """Constant image (sigma 0: exact-median fallback), sparse image (most pixels exactly 0:
     radix selection), all-source image (every pixel above threshold)."""
