@@ -244,6 +244,22 @@ def test_gpu_source_detection_kats(gpu):
         assert abs(r0.islands[0]["extremum_y_idx"] - 52.66) < 0.5
 
 
+def test_gpu_sourcefind_many_canonical_blocks(gpu, orc):
+    """6144 x 6144 pixels, more than 4096 canonical blocks: the statistics and the clipping
+    passes reduce their block partials through an extra canonical level.  Noise with 40 very
+    bright 3 x 3 blocks, so the first clipping bound is far above the next (a second full
+    pass, then candidate passes).  Bar: compare() (statistics bit-exact)."""
+    img = np.asfortranarray(noise(6144, 6144, sigma=1.0, seed=5))
+    rng = np.random.default_rng(5)
+    for _ in range(40):
+        r, c = rng.integers(8, 6136, 2)
+        img[r - 1:r + 2, c - 1:c + 2] += rng.uniform(2000.0, 8000.0)
+    b = _clip_bounds(img, 5)
+    assert b[1] < 0.25 * b[0]
+    pars = capi.sourcefind_pars(5.0, 3.0, median_method=capi.MEDIAN_BINMEDIAN)
+    compare(gpu.source_find(img, pars), orc.source_find(img, pars))
+
+
 def test_gpu_pipeline_c2_full_size(gpu, orc):
     """Workload c2 at full size: the oracle image through both source finders is bit-exact,
     and the GPU image gives the same islands as the oracle image."""
