@@ -74,14 +74,12 @@ __device__ __forceinline__ void canon_last_level(const double* in, unsigned m, d
     canon_tree(a, b, lds);
 }
 
-// Reduces m pairs (2m doubles at `partials`) to one pair written to `result`
-// (device, 2 doubles), with the canonical order.  `scratch` holds
+// Canonical reduction of m pairs (2m doubles at `partials`): runs the levels of
+// CANON_BLOCK pairs per block while more than CANON_BLOCK pairs remain and returns
+// the remaining pairs (count in *m_out) for a last level done by the caller with one
+// block in the same order (canon_last_level).  `scratch` holds
 // canon_scratch_doubles(m) doubles.
 inline size_t canon_scratch_doubles(size_t m) { return 4 * ((m + CANON_BLOCK - 1) / CANON_BLOCK) + 4; }
-void canon_reduce_pairs(stp_ctx* ctx, const double* partials, size_t m, double* scratch, double* result);
-// Runs the canonical levels of canon_reduce_pairs while more than CANON_BLOCK pairs
-// remain; returns the remaining pairs (count in *m_out) for a last level done by
-// the caller with one block in the same order.
 const double* canon_reduce_to_block(stp_ctx* ctx, const double* partials, size_t m, double* scratch, size_t* m_out);
 
 // Fill kernels (used instead of hipMemsetAsync so the work stays capturable).

@@ -83,30 +83,6 @@ __global__ __launch_bounds__(SCAN_WG) void exclusive_scan_kernel(const uint32_t*
 
 namespace stp_hip {
 
-void canon_reduce_pairs(stp_ctx* ctx, const double* partials, size_t m, double* scratch, double* result)
-{
-    if (m <= 1) {
-        if (m == 1)
-            HIP_CHECK(hipMemcpyAsync(result, partials, 2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-        else
-            fill_f64(ctx, result, 0.0, 2);
-        return;
-    }
-    const double* src = partials;
-    size_t cur = m;
-    // ping-pong between the two halves of scratch; the last level writes result
-    double* bufs[2] = { scratch, scratch + 2 * ((m + CANON_BLOCK - 1) / CANON_BLOCK) };
-    int which = 0;
-    while (cur > 1) {
-        const size_t nb = (cur + CANON_BLOCK - 1) / CANON_BLOCK;
-        double* dst = (nb == 1) ? result : bufs[which];
-        launch(ctx, "canon_reduce", canon_pairs_kernel, dim3(unsigned(nb)), dim3(CANON_LANES), 0, src, cur, dst);
-        src = dst;
-        cur = nb;
-        which ^= 1;
-    }
-}
-
 const double* canon_reduce_to_block(stp_ctx* ctx, const double* partials, size_t m, double* scratch, size_t* m_out)
 {
     const double* src = partials;
