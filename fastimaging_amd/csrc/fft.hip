@@ -11,6 +11,8 @@
 //            padding crop and grid correction of the imager are fused here.
 // Each transform runs in LDS with 64-wide waves; a workgroup holds NT
 // transforms of length L, TPT = 256/NT threads per transform.
+#include <type_traits>
+
 #include "common.h"
 #include "fft.h"
 
@@ -66,13 +68,36 @@ __device__ __forceinline__ cplx<T> quarter_tw(const cplx<T>* q, int qlog, int m,
     return w;
 }
 
-// From an LDS quarter table of length n = L * tws.
+// exp(+2 pi i m / n) from two small LDS tables: r = m mod n/4 = a 2^blog + b,
+// W^r = hi[a] * lo[b] (hi[a] = exp(2 pi i a 2^blog / n), lo[b] = exp(2 pi i b / n)),
+// then the exact rotation by i^(m div n/4).
 template <typename T>
-struct QuarterTw {
-    const cplx<T>* q;
-    int qlog, tws;
-    bool inverse;
-    __device__ cplx<T> operator()(int e) const { return quarter_tw(q, qlog, e * tws, inverse); }
+__device__ __forceinline__ cplx<T> split_tw(const cplx<T>* hi, const cplx<T>* lo, int qlog, int blog, int m)
+{
+    const int k = m >> qlog, r = m & ((1 << qlog) - 1);
+    cplx<T> w = cmul(hi[r >> blog], lo[r & ((1 << blog) - 1)]);
+    if (k & 1)
+        w = { -w.y, w.x };
+    if (k & 2)
+        w = { -w.x, -w.y };
+    return w;
+}
+
+// Twiddles of the c2r pass (inverse sign, table of n = 2M): tw(e) = exp(2 pi i 2e / n)
+// for the length-M transform, at(m) = exp(2 pi i m / n) for the packing.  SPLIT: the
+// two small tables of split_tw at t (lo, then hi); else a quarter table at t.
+template <typename T, bool SPLIT>
+struct C2rTw {
+    const cplx<T>* t;
+    int qlog, blog;
+    __device__ cplx<T> at(int m) const
+    {
+        if constexpr (SPLIT)
+            return split_tw(t + (1 << blog), t, qlog, blog, m);
+        else
+            return quarter_tw(t, qlog, m, true);
+    }
+    __device__ cplx<T> operator()(int e) const { return at(2 * e); }
 };
 
 // From an LDS table t[e] = exp(+2 pi i e / L), e < L.
@@ -288,14 +313,20 @@ __global__ __launch_bounds__(WG) void fft_v_c2r(const cplx<T>* grids, size_t gri
     using S = Shape<M>;
     constexpr int n = 2 * M;
     constexpr int rows = M + 1;
-    constexpr int QN = (n / 4 > 0) ? n / 4 : 1;  // quarter table: exp(+2 pi i r / n), r < n/4
+    constexpr int QN = (n / 4 > 0) ? n / 4 : 1;  // quarter wave: r < n/4
     constexpr int QLOG = Log2<QN>::value;
+    // FP32: two small tables (a complex multiply per twiddle; the smaller LDS footprint
+    // lets more workgroups share a CU: 80 -> 74 us at c2); FP64: the quarter table
+    constexpr bool SPLIT = std::is_same<T, float>::value;
+    constexpr int BLOG = QLOG < 5 ? QLOG : 5;
+    constexpr int NTW = SPLIT ? (1 << BLOG) + (QN >> BLOG) : QN;
     constexpr int KI = (M / 2) / S::TPT + 1;      // pack pairs (k, M - k) per thread
     __shared__ cplx<T> buf[S::NT * (M + S::PAD)];
-    __shared__ cplx<T> s_q[QN];
+    __shared__ cplx<T> s_tw[NTW];
     const int tid = threadIdx.x;
-    for (int r = tid; r < QN; r += WG)
-        s_q[r] = load_tw(tw_n, r, true);
+    for (int r = tid; r < NTW; r += WG)
+        s_tw[r] = load_tw(tw_n, (SPLIT && r >= (1 << BLOG)) ? (r - (1 << BLOG)) << BLOG : r, true);
+    const C2rTw<T, SPLIT> tw{ s_tw, QLOG, BLOG };
     const int t = tid / S::TPT, lt = tid % S::TPT;
     const int gi = blockIdx.y;
     const cplx<T>* g = grids + size_t(gi) * grid_elems;
@@ -333,17 +364,17 @@ __global__ __launch_bounds__(WG) void fft_v_c2r(const cplx<T>* grids, size_t gri
         // E[k] = X[k] + conj(X[M-k]);  O[k] = (X[k] - conj(X[M-k])) W^k, W = exp(+2 pi i / n)
         const cplx<T> e1 = { a.x + c.x, a.y - c.y };
         const cplx<T> d1 = { a.x - c.x, a.y + c.y };
-        const cplx<T> o1 = cmul(d1, quarter_tw(s_q, QLOG, k, true));
+        const cplx<T> o1 = cmul(d1, tw.at(k));
         b[k] = { e1.x - o1.y, e1.y + o1.x };
         if (km != k) {
             const cplx<T> e2 = { c.x + a.x, c.y - a.y };
             const cplx<T> d2 = { c.x - a.x, c.y + a.y };
-            const cplx<T> o2 = cmul(d2, quarter_tw(s_q, QLOG, km, true));
+            const cplx<T> o2 = cmul(d2, tw.at(km));
             b[km] = { e2.x - o2.y, e2.y + o2.x };
         }
     }
     __syncthreads();
-    lds_fft<T, M, S::TPT>(b, lt, QuarterTw<T>{ s_q, QLOG, 2, true }, true);
+    lds_fft<T, M, S::TPT>(b, lt, tw, true);
     // crop mapping of the padded column x (imager.h:99-150)
     int j = -1;
     if (active) {
