@@ -1779,30 +1779,36 @@ __device__ void moments_fit(stp_island& isl, double x_bar, double y_bar, double 
 // valid: extremum beyond the detection threshold (sourcefind.cpp:417-436);
 // keep: valid and at least min_area samples (sourcefind.cpp:240,263).
 // The grid covers cap >= kp + kn labels (kp, kn from the state); keep is 0 beyond.
+// vcnt / kcnt: per block of 256 labels, the valid and the kept labels.
 __global__ __launch_bounds__(256) void sf_island_flags(const IslandSum* isum, const uint32_t* lrec, const IslandSum* rsum,
-    uint32_t cap, const SFState* st, int min_area, uint32_t* keep, uint32_t* vcnt)
+    uint32_t cap, const SFState* st, int min_area, uint32_t* keep, uint32_t* vcnt, uint32_t* kcnt)
 {
-    __shared__ uint32_t wv[4];
+    __shared__ uint32_t wv[4], wk[4];
     if (st->skip)
         return;
     const uint32_t kp = st->num_pos, kn = st->num_neg;
     const uint32_t g = blockIdx.x * 256 + threadIdx.x;
-    bool v = false;
+    bool v = false, k = false;
     if (g < kp + kn) {
         const bool pos = g < kp;
         const IslandSum& sm = label_sum(isum, lrec, rsum, g);
         const double ext = okey_inv(sm.ext);
         v = pos ? (ext > st->d_pos) : (ext < st->d_neg);
-        keep[g] = (v && sm.cnt >= min_area) ? 1u : 0u;
+        k = v && sm.cnt >= min_area;
+        keep[g] = k ? 1u : 0u;
     } else if (g < cap) {
         keep[g] = 0u;
     }
-    const uint32_t c = uint32_t(__popcll(__ballot(v)));
-    if ((threadIdx.x & 63) == 0)
+    const uint32_t c = uint32_t(__popcll(__ballot(v))), ck = uint32_t(__popcll(__ballot(k)));
+    if ((threadIdx.x & 63) == 0) {
         wv[threadIdx.x >> 6] = c;
+        wk[threadIdx.x >> 6] = ck;
+    }
     __syncthreads();
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         vcnt[blockIdx.x] = wv[0] + wv[1] + wv[2] + wv[3];
+        kcnt[blockIdx.x] = wk[0] + wk[1] + wk[2] + wk[3];
+    }
 }
 
 // One island record per kept label at its rank among the kept labels; block 0
@@ -1838,12 +1844,14 @@ __device__ void emit_island(const IslandSum& s, uint32_t g, uint32_t kp, int row
 // chunks by consecutive lanes (whole lines on the bus).
 static_assert(sizeof(stp_island) % 16 == 0, "island records of whole 16-byte chunks");
 constexpr int ISL_CHUNKS = int(sizeof(stp_island) / 16);
+// A block's islands start after the kept labels of the blocks before it (kcnt, added
+// up by the block itself) and follow in label order (wave ballots of keep).
 __global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, const uint32_t* lrec, const IslandSum* rsum,
-    const uint32_t* keep,
-    const uint32_t* offs, const uint32_t* vcnt, SFState* st, int rows, int cols, double analysis_n_sigma, stp_island* out,
-    uint32_t out_cap)
+    const uint32_t* keep, const uint32_t* kcnt, const uint32_t* vcnt, SFState* st, int rows, int cols,
+    double analysis_n_sigma, stp_island* out, uint32_t out_cap)
 {
-    __shared__ uint32_t red[256];
+    __shared__ uint32_t red[256], red2[256];
+    __shared__ uint32_t wk[4];
     __shared__ uint4 s_isl[256 * ISL_CHUNKS];
     if (st->skip)
         return;
@@ -1853,35 +1861,70 @@ __global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, con
         // the blocks holding labels (unconditional loads with a clamped index, so
         // that a batch is in flight together)
         const uint32_t nb = (kp + kn + 255) / 256;
-        uint32_t c = 0;
+        uint32_t c = 0, ck = 0;
         for (uint32_t b0 = 0; b0 < nb; b0 += 256 * 8) {
-            uint32_t v[8];
+            uint32_t v[8], u[8];
 #pragma unroll
-            for (int j = 0; j < 8; j++)
+            for (int j = 0; j < 8; j++) {
                 v[j] = vcnt[min(b0 + 256 * j + threadIdx.x, nb > 0 ? nb - 1 : 0u)];
+                u[j] = kcnt[min(b0 + 256 * j + threadIdx.x, nb > 0 ? nb - 1 : 0u)];
+            }
 #pragma unroll
-            for (int j = 0; j < 8; j++)
+            for (int j = 0; j < 8; j++) {
                 c += b0 + 256 * j + threadIdx.x < nb ? v[j] : 0u;
+                ck += b0 + 256 * j + threadIdx.x < nb ? u[j] : 0u;
+            }
         }
         red[threadIdx.x] = c;
+        red2[threadIdx.x] = ck;
         __syncthreads();
         for (int off = 128; off > 0; off >>= 1) {
-            if (threadIdx.x < off)
+            if (threadIdx.x < off) {
                 red[threadIdx.x] += red[threadIdx.x + off];
+                red2[threadIdx.x] += red2[threadIdx.x + off];
+            }
             __syncthreads();
         }
         if (threadIdx.x == 0) {
             st->num_valid = red[0];
-            st->num_islands = offs[kp + kn];
+            st->num_islands = red2[0];
         }
+        __syncthreads();  // red is reused below
     }
     const uint32_t nlab = kp + kn, g0 = blockIdx.x * 256;
     if (g0 >= nlab)
         return;
-    const uint32_t base = offs[g0], cnt = offs[min(g0 + 256, nlab)] - base;
-    if (g < nlab && keep[g])
+    // kept islands of the blocks before this one (loads in flight together)
+    uint32_t pb = 0;
+    for (uint32_t b0 = 0; b0 < blockIdx.x; b0 += 256 * 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            v[j] = kcnt[min(b0 + 256 * j + threadIdx.x, blockIdx.x - 1)];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            pb += b0 + 256 * j + threadIdx.x < blockIdx.x ? v[j] : 0u;
+    }
+    red[threadIdx.x] = pb;
+    // the position of this label's island within the block
+    const bool kept = g < nlab && keep[g];
+    const unsigned long long bal = __ballot(kept);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+        wk[w] = uint32_t(__popcll(bal));
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    const uint32_t base = red[0], cnt = wk[0] + wk[1] + wk[2] + wk[3];
+    uint32_t pos = uint32_t(__popcll(bal & ((1ull << lane) - 1ull)));
+    for (int k = 0; k < w; k++)
+        pos += wk[k];
+    if (kept)
         emit_island(label_sum(isum, lrec, rsum, g), g, kp, rows, cols, st->rms, analysis_n_sigma,
-            (stp_island*)s_isl + (offs[g] - base));
+            (stp_island*)s_isl + pos);
     __syncthreads();
     uint4* o = (uint4*)out + size_t(base) * ISL_CHUNKS;
     for (uint32_t q = threadIdx.x; q < cnt * ISL_CHUNKS; q += 256)
@@ -2188,16 +2231,16 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
         const size_t nl = size_t(ntiles) * CT_RT + rec.pool_cap;
         const unsigned rblocks = div_up(nl, 256);
         uint32_t* keep = wsp<uint32_t>(ctx, "sf_keep", nl);
-        uint32_t* koff = wsp<uint32_t>(ctx, "sf_koff", nl + 1);
-        uint32_t* vcnt = wsp<uint32_t>(ctx, "sf_vcnt", div_up(nl, 256));
+        uint32_t* vcnt = wsp<uint32_t>(ctx, "sf_vcnt", 2 * size_t(rblocks));
+        uint32_t* kcnt = vcnt + rblocks;
         disl = wsp<stp_island>(ctx, "sf_islands", nl);
         launch(ctx, "sf_island_flags", sf_island_flags, dim3(rblocks), dim3(256), 0, (const IslandSum*)isum,
             (const uint32_t*)lrec, (const IslandSum*)rec.sum, uint32_t(nl), (const SFState*)st, int(p.source_min_area), keep,
-            vcnt);
-        scan_u32(ctx, keep, nl, koff);
+            vcnt, kcnt);
         launch(ctx, "sf_island_emit", sf_island_emit, dim3(rblocks), dim3(256), 0, (const IslandSum*)isum,
-            (const uint32_t*)lrec, (const IslandSum*)rec.sum, (const uint32_t*)keep, (const uint32_t*)koff, (const uint32_t*)vcnt, st, rows, cols, p.analysis_n_sigma,
-            pinned ? islands_dev : disl, pinned ? uint32_t(std::min<size_t>(cap, 0xFFFFFFFFu)) : uint32_t(nl));
+            (const uint32_t*)lrec, (const IslandSum*)rec.sum, (const uint32_t*)keep, (const uint32_t*)kcnt,
+            (const uint32_t*)vcnt, st, rows, cols, p.analysis_n_sigma, pinned ? islands_dev : disl,
+            pinned ? uint32_t(std::min<size_t>(cap, 0xFFFFFFFFu)) : uint32_t(nl));
     };
     SFState* hs = (SFState*)host_stage(ctx, "sf_state", sizeof(SFState));
     // an ordinary caller buffer gets the island list through a stage: as many islands as
