@@ -2224,12 +2224,14 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
         launch(ctx, "ccl_open_init", ccl_open_init, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum, lrec,
             LabelCounts{ st, (const SelState*)ss, (unsigned long long)nl });
         launch(ctx, "ccl_records", ccl_records, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum, lrec);
-        launch(ctx, "ccl_open_idx", ccl_open_idx, dim3(rblocks), dim3(256), 0, (const uint32_t*)P, sid, rec, isum);
         return isum;
     };
     auto island_list = [&](const IslandSum* isum) {
         const size_t nl = size_t(ntiles) * CT_RT + rec.pool_cap;
         const unsigned rblocks = div_up(nl, 256);
+        // the open islands' memory index (read by the emit only)
+        launch(ctx, "ccl_open_idx", ccl_open_idx, dim3(rblocks), dim3(256), 0, (const uint32_t*)P,
+            SegIds{ tg, rmask, segoff }, rec, (IslandSum*)isum);
         uint32_t* keep = wsp<uint32_t>(ctx, "sf_keep", nl);
         uint32_t* vcnt = wsp<uint32_t>(ctx, "sf_vcnt", 2 * size_t(rblocks));
         uint32_t* kcnt = vcnt + rblocks;
