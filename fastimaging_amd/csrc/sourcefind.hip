@@ -22,6 +22,7 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 
 #include "reduce.h"
@@ -1846,15 +1847,25 @@ static_assert(sizeof(stp_island) % 16 == 0, "island records of whole 16-byte chu
 constexpr int ISL_CHUNKS = int(sizeof(stp_island) / 16);
 // A block's islands start after the kept labels of the blocks before it (kcnt, added
 // up by the block itself) and follow in label order (wave ballots of keep).
+// Block 0 also writes the final state to host_st (page-locked host memory, over the
+// bus) when it is given, also when the labelling is incomplete: the state the host
+// reads after the call, with no copy launch.
+constexpr unsigned SF_STATE_WORDS = unsigned(sizeof(SFState) / 4);
+static_assert(sizeof(SFState) % 4 == 0, "state of whole 32-bit words");
 __global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, const uint32_t* lrec, const IslandSum* rsum,
     const uint32_t* keep, const uint32_t* kcnt, const uint32_t* vcnt, SFState* st, int rows, int cols,
-    double analysis_n_sigma, stp_island* out, uint32_t out_cap)
+    double analysis_n_sigma, stp_island* out, uint32_t out_cap, SFState* host_st)
 {
     __shared__ uint32_t red[256], red2[256];
     __shared__ uint32_t wk[4];
     __shared__ uint4 s_isl[256 * ISL_CHUNKS];
-    if (st->skip)
+    const int skip = st->skip;
+    if (skip) {
+        if (blockIdx.x == 0 && host_st)
+            for (unsigned t = threadIdx.x; t < SF_STATE_WORDS; t += 256)
+                ((uint32_t*)host_st)[t] = ((const uint32_t*)st)[t];
         return;
+    }
     const uint32_t kp = st->num_pos, kn = st->num_neg;
     const uint32_t g = blockIdx.x * 256 + threadIdx.x;
     if (blockIdx.x == 0) {
@@ -1885,9 +1896,19 @@ __global__ __launch_bounds__(256) void sf_island_emit(const IslandSum* isum, con
             }
             __syncthreads();
         }
+        const uint32_t nv = red[0], ni = red2[0];
         if (threadIdx.x == 0) {
-            st->num_valid = red[0];
-            st->num_islands = red2[0];
+            st->num_valid = nv;
+            st->num_islands = ni;
+        }
+        if (host_st) {
+            // the two counts come from this block's sums, every other word is final
+            constexpr unsigned WV = unsigned(offsetof(SFState, num_valid) / 4);
+            constexpr unsigned WI = unsigned(offsetof(SFState, num_islands) / 4);
+            for (unsigned t = threadIdx.x; t < SF_STATE_WORDS; t += 256) {
+                const uint32_t v = ((const uint32_t*)st)[t];
+                ((uint32_t*)host_st)[t] = t == WV ? nv : t == WI ? ni : v;
+            }
         }
         __syncthreads();  // red is reused below
     }
@@ -2215,6 +2236,9 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
     stp_island* islands_dev = (stp_island*)pinned_device_ptr(islands);
     const bool pinned = islands_dev != nullptr;
     uint32_t* lrec = nullptr;
+    // the state for the host: written by the emitting kernel into the page-locked stage
+    SFState* hs = (SFState*)host_stage(ctx, "sf_state", sizeof(SFState));
+    SFState* hs_dev = (SFState*)pinned_device_ptr(hs);
     auto island_sums = [&]() -> const IslandSum* {
         const size_t nl = size_t(ntiles) * CT_RT + rec.pool_cap;
         IslandSum* isum = wsp<IslandSum>(ctx, "sf_isum", nl);
@@ -2242,9 +2266,8 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
         launch(ctx, "sf_island_emit", sf_island_emit, dim3(rblocks), dim3(256), 0, (const IslandSum*)isum,
             (const uint32_t*)lrec, (const IslandSum*)rec.sum, (const uint32_t*)keep, (const uint32_t*)kcnt,
             (const uint32_t*)vcnt, st, rows, cols, p.analysis_n_sigma, pinned ? islands_dev : disl,
-            pinned ? uint32_t(std::min<size_t>(cap, 0xFFFFFFFFu)) : uint32_t(nl));
+            pinned ? uint32_t(std::min<size_t>(cap, 0xFFFFFFFFu)) : uint32_t(nl), hs_dev);
     };
-    SFState* hs = (SFState*)host_stage(ctx, "sf_state", sizeof(SFState));
     // an ordinary caller buffer gets the island list through a stage: as many islands as
     // the last call found are copied with the state (a guess: any missing ones are
     // copied afterwards)
@@ -2265,7 +2288,8 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
                 int(p.generate_labelmap != 0), d_lab_out);
         }
         island_list(isum);
-        HIP_CHECK(hipMemcpyAsync(hs, st, sizeof(SFState), hipMemcpyDeviceToHost, ctx->stream));
+        if (!hs_dev)
+            HIP_CHECK(hipMemcpyAsync(hs, st, sizeof(SFState), hipMemcpyDeviceToHost, ctx->stream));
         if (spec)
             HIP_CHECK(hipMemcpyAsync(stage, disl, spec * sizeof(stp_island), hipMemcpyDeviceToHost, ctx->stream));
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
