@@ -32,4 +32,9 @@ template <typename T>
 void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const double* d_uv, const double* d_vis,
     const double* d_w, cplx<T>* d_vis_grid, cplx<T>* d_sampling_grid, double* d_total, unsigned long long* d_ngood);
 
+// Entries (40 bytes each) the gridder's binning reserves for n visibilities: the
+// most tiles one footprint can touch on this grid, times two passes, times n.
+size_t gridder_entry_capacity(int N, bool halfplane, int support, bool exact, int tile_cols, size_t n);
+constexpr size_t GRID_ENTRY_BYTES = 40;
+
 } // namespace stp_hip

@@ -29,7 +29,6 @@ namespace {
 constexpr int TILE_R = 64;  // grid rows (v) per tile: 64 contiguous complex per column
 constexpr int TILE_C = 32;  // default grid columns (u) per tile; GridderArgs::tile_cols overrides
 constexpr int GWG = 256;
-constexpr int MAXT = 8;     // max distinct tiles per dimension of one footprint
 
 struct GridGeom {
     int N, rows, half, s, ks, os;
@@ -198,45 +197,92 @@ __device__ __forceinline__ int tap_col(const GridGeom& g, int gx, int j)
     return c;
 }
 
-__device__ __forceinline__ void add_unique(int* set, int& cnt, int v)
+// The tiles of one axis that hold kept taps of a footprint: three intervals of
+// tile indices [lo[k], hi[k]] (empty when lo > hi), one per contiguous piece of
+// the raw range [centre - s, centre - s + ks - 1]: below 0, inside, at or above N.
+// Pieces of a small grid can share tiles; visit_tiles skips the repeats.
+struct TileSpan {
+    int lo[3], hi[3];
+};
+
+__host__ __device__ __forceinline__ void span_set(TileSpan& sp, int k, int lo, int hi, int w)
 {
-    for (int k = 0; k < cnt; k++)
-        if (set[k] == v)
-            return;
-    if (cnt < MAXT)
-        set[cnt++] = v;
+    const bool on = lo <= hi;
+    sp.lo[k] = on ? lo / w : 1;
+    sp.hi[k] = on ? hi / w : 0;
 }
 
-// Tiles lo / w .. hi / w of the index range [lo, hi] (empty when lo > hi).
-__device__ __forceinline__ void add_tile_range(int* set, int& cnt, int lo, int hi, int w)
+// Rows: the oversampled path skips rows < 0 and wraps rows >= N, the exact path
+// also wraps rows < 0; rows beyond `last` (the half plane) are dropped.
+// Columns wrap both ways (the same sets as tap_row / tap_col over every tap).
+__host__ __device__ __forceinline__ void footprint_spans(int N, int last, int s, int ks, int exact, int tile_r, int tile_c,
+    int gx, int gy, TileSpan& rs, TileSpan& cs)
 {
-    if (lo > hi)
-        return;
-    for (int t = lo / w, te = hi / w; t <= te; t++)
-        add_unique(set, cnt, t);
+    int a = gy - s, b = a + ks - 1;
+    span_set(rs, 0, a + N, (a < 0 && exact) ? min(min(b, -1) + N, last) : -1, tile_r);
+    span_set(rs, 1, max(a, 0), min(min(b, N - 1), last), tile_r);
+    span_set(rs, 2, max(a, N) - N, (b >= N) ? min(b - N, last) : -1, tile_r);
+    a = gx - s;
+    b = a + ks - 1;
+    span_set(cs, 0, a + N, (a < 0) ? min(b, -1) + N : -1, tile_c);
+    span_set(cs, 1, max(a, 0), min(b, N - 1), tile_c);
+    span_set(cs, 2, max(a, N) - N, (b >= N) ? b - N : -1, tile_c);
 }
 
-// Distinct tile rows and tile columns holding the kept taps of a footprint: the
-// same sets as tap_row / tap_col over every tap, from the at most three contiguous
-// pieces of the raw range [centre - s, centre - s + ks - 1] (below 0, inside,
-// at or above N) instead of tap by tap.
-__device__ __forceinline__ void footprint_tiles(const GridGeom& g, int gx, int gy, int* rt, int& nrt, int* ct, int& nct)
+// Whether tile t lies in one of the intervals before k.
+__host__ __device__ __forceinline__ bool span_seen(const TileSpan& sp, int k, int t)
 {
-    nrt = nct = 0;
-    const int N = g.N, last = g.rows - 1;
-    int a = gy - g.s, b = a + g.ks - 1;
-    if (a < 0 && g.exact)
-        add_tile_range(rt, nrt, a + N, min(min(b, -1) + N, last), TILE_R);
-    add_tile_range(rt, nrt, max(a, 0), min(min(b, N - 1), last), TILE_R);
-    if (b >= N)
-        add_tile_range(rt, nrt, max(a, N) - N, min(b - N, last), TILE_R);
-    a = gx - g.s;
-    b = a + g.ks - 1;
-    if (a < 0)
-        add_tile_range(ct, nct, a + N, min(b, -1) + N, g.tc);
-    add_tile_range(ct, nct, max(a, 0), min(b, N - 1), g.tc);
-    if (b >= N)
-        add_tile_range(ct, nct, max(a, N) - N, b - N, g.tc);
+    bool seen = false;
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+        seen = seen || (q < k && t >= sp.lo[q] && t <= sp.hi[q]);
+    return seen;
+}
+
+// Calls f(tile column, tile row) once for every distinct tile of the footprint.
+template <typename F>
+__device__ __forceinline__ void visit_tiles(const GridGeom& g, int gx, int gy, F f)
+{
+    TileSpan rs, cs;
+    footprint_spans(g.N, g.rows - 1, g.s, g.ks, g.exact, TILE_R, g.tc, gx, gy, rs, cs);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        for (int ct = cs.lo[a]; ct <= cs.hi[a]; ct++) {
+            if (span_seen(cs, a, ct))
+                continue;
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                for (int rt = rs.lo[b]; rt <= rs.hi[b]; rt++)
+                    if (!span_seen(rs, b, rt))
+                        f(ct, rt);
+            }
+        }
+    }
+}
+
+// Largest number of distinct tiles one footprint can touch along an axis of
+// length N (kept indices 0..last), over every centre the gridder can produce.
+int max_span_tiles(int N, int last, int s, int ks, int exact, int w, bool rows)
+{
+    int best = 1;
+    for (int c = -1; c <= N + 1; c++) {
+        TileSpan rs, cs;
+        footprint_spans(N, last, s, ks, exact, w, w, rows ? 0 : c, rows ? c : 0, rs, cs);
+        const TileSpan& sp = rows ? rs : cs;
+        int cnt = 0;
+        for (int k = 0; k < 3; k++)
+            for (int t = sp.lo[k]; t <= sp.hi[k]; t++)
+                if (!span_seen(sp, k, t))
+                    cnt++;
+        best = std::max(best, cnt);
+    }
+    return best;
+}
+
+size_t footprint_tiles_max(const GridGeom& g)
+{
+    return size_t(max_span_tiles(g.N, g.rows - 1, g.s, g.ks, g.exact, TILE_R, true)) *
+        size_t(max_span_tiles(g.N, g.rows - 1, g.s, g.ks, g.exact, g.tc, false));
 }
 
 // wpart: per block the canonical sum of the good weights and the good count.
@@ -259,11 +305,7 @@ __global__ __launch_bounds__(GWG) void grid_count_kernel(GridGeom g, const doubl
         for (int pass = 0; pass <= p.dup; pass++) {
             int gx, gy, cpx, cpy;
             pass_geom(g, p, pass, gx, gy, cpx, cpy);
-            int rt[MAXT], ct[MAXT], nrt, nct;
-            footprint_tiles(g, gx, gy, rt, nrt, ct, nct);
-            for (int a = 0; a < nct; a++)
-                for (int b = 0; b < nrt; b++)
-                    atomicAdd(&counts[ct[a] * g.tiles_r + rt[b]], 1u);
+            visit_tiles(g, gx, gy, [&](int ct, int rt) { atomicAdd(&counts[ct * g.tiles_r + rt], 1u); });
         }
     }
     canon_tree(wsum, nsum, lds);
@@ -284,14 +326,11 @@ __global__ __launch_bounds__(GWG) void grid_scatter_kernel(GridGeom g, const dou
         return;
     for (int pass = 0; pass <= p.dup; pass++) {
         const Entry e = make_entry(g, p, pass, i);
-        int rt[MAXT], ct[MAXT], nrt, nct;
-        footprint_tiles(g, e.gx, e.gy, rt, nrt, ct, nct);
-        for (int a = 0; a < nct; a++)
-            for (int b = 0; b < nrt; b++) {
-                const uint32_t slot = atomicAdd(&cursor[ct[a] * g.tiles_r + rt[b]], 1u);
-                if (slot < cap)
-                    entries[slot] = e;
-            }
+        visit_tiles(g, e.gx, e.gy, [&](int ct, int rt) {
+            const uint32_t slot = atomicAdd(&cursor[ct * g.tiles_r + rt], 1u);
+            if (slot < cap)
+                entries[slot] = e;
+        });
     }
 }
 
@@ -385,11 +424,7 @@ __global__ __launch_bounds__(BWG) void grid_bin_count_kernel(GridGeom g, const d
                 for (int pass = 0; pass <= p.dup; pass++) {
                     int gx, gy, cpx, cpy;
                     pass_geom(g, p, pass, gx, gy, cpx, cpy);
-                    int rt[MAXT], ct[MAXT], nrt, nct;
-                    footprint_tiles(g, gx, gy, rt, nrt, ct, nct);
-                    for (int a = 0; a < nct; a++)
-                        for (int b = 0; b < nrt; b++)
-                            atomicAdd(&hist[ct[a] * g.tiles_r + rt[b]], 1u);
+                    visit_tiles(g, gx, gy, [&](int ct, int rt) { atomicAdd(&hist[ct * g.tiles_r + rt], 1u); });
                 }
             }
         }
@@ -548,14 +583,11 @@ __global__ __launch_bounds__(BWG) void grid_bin_scatter_kernel(GridGeom g, const
             continue;
         for (int pass = 0; pass <= p.dup; pass++) {
             const Entry e = make_entry(g, p, pass, i);
-            int rt[MAXT], ct[MAXT], nrt, nct;
-            footprint_tiles(g, e.gx, e.gy, rt, nrt, ct, nct);
-            for (int a = 0; a < nct; a++)
-                for (int b = 0; b < nrt; b++) {
-                    const uint32_t slot = atomicAdd(&cur[ct[a] * g.tiles_r + rt[b]], 1u);
-                    if (slot < cap)
-                        entries[slot] = e;
-                }
+            visit_tiles(g, e.gx, e.gy, [&](int ct, int rt) {
+                const uint32_t slot = atomicAdd(&cur[ct * g.tiles_r + rt], 1u);
+                if (slot < cap)
+                    entries[slot] = e;
+            });
         }
     }
 }
@@ -1222,7 +1254,7 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     // W-projection grids with max_wpconv_support (gridder.h:260-268)
     const int support = wproj ? int(a.wproj.max_wpconv_support) : a.support;
     STP_REQUIRE(N >= 4 && (N % 2) == 0, "image_size must be even and >= 4");
-    STP_REQUIRE(support >= 1 && support <= 64, "kernel support must be in [1, 64]");
+    STP_REQUIRE(support >= 1 && support <= 127, "kernel support must be in [1, 127]");
     STP_REQUIRE(2 * support + 1 <= N, "kernel wider than the grid");
     STP_REQUIRE(n < (size_t(1) << 30), "at most 2^30 visibilities per call");
     if (wproj) {
@@ -1282,11 +1314,9 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     const size_t nblk = std::max<size_t>(1, (n + VPB - 1) / VPB);
     double* wpart = (double*)ws_get(ctx, "grid_wpart", sizeof(double) * (2 * nblk + 2));
     double* wscr = (double*)ws_get(ctx, "grid_wscr", sizeof(double) * canon_scratch_doubles(nblk));
-    // worst-case entries: a footprint of ks cells spans at most ceil((ks-1)/T)+1 tiles per
-    // axis, one more where it wraps across the grid edge; two passes
-    const size_t per_pass = size_t(std::min(MAXT, int(div_up(g.ks - 1, TILE_R)) + 2)) *
-        size_t(std::min(MAXT, int(div_up(g.ks - 1, g.tc)) + 2));
-    const size_t cap64 = std::max<size_t>(1, 2 * per_pass * n);
+    // worst-case entries: the most tiles one footprint can touch (every centre
+    // checked on the host), times two passes
+    const size_t cap64 = std::max<size_t>(1, 2 * footprint_tiles_max(g) * n);
     STP_REQUIRE(cap64 < (size_t(1) << 32), "too many gridding entries for one call");
     const uint32_t cap = uint32_t(cap64);
     Entry* entries = (Entry*)ws_get(ctx, "grid_entries", sizeof(Entry) * cap64);
@@ -1428,6 +1458,18 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     size_t mfin = 0;
     const double* fin = canon_reduce_to_block(ctx, wpart, nblk, wscr, &mfin);
     launch(ctx, "grid_total", grid_total_kernel, dim3(1), dim3(CANON_LANES), 0, fin, unsigned(mfin), d_total, d_ngood);
+}
+
+size_t gridder_entry_capacity(int N, bool halfplane, int support, bool exact, int tile_cols, size_t n)
+{
+    GridGeom g;
+    g.N = N;
+    g.rows = halfplane ? N / 2 + 1 : N;
+    g.s = support;
+    g.ks = 2 * support + 1;
+    g.exact = exact ? 1 : 0;
+    g.tc = tile_cols > 0 ? tile_cols : TILE_C;
+    return std::max<size_t>(1, 2 * footprint_tiles_max(g) * n);
 }
 
 template void grid_visibilities<double>(stp_ctx*, const GridderArgs&, size_t, const double*, const double*,

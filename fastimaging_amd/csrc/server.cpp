@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/stp_capi.h"
+#include "gridder.h"
 
 namespace {
 
@@ -79,12 +80,13 @@ extern "C" uint64_t stp_job_device_bytes(const stp_job* j)
     const bool wproj = wplanes_of(*j) > 0;
     const uint64_t s = wproj ? j->w_proj.max_wpconv_support : p.kernel_support;
     const uint64_t ks = 2 * s + 1;
-    // gridder entry capacity: (ceil((ks-1)/64)+2) x (ceil((ks-1)/32)+2) tiles per pass, two passes
-    const uint64_t per_pass = std::min<uint64_t>(8, (ks + 62) / 64 + 2) * std::min<uint64_t>(8, (ks + 30) / 32 + 2);
+    // gridder entry capacity at the narrowest tile width the autotuner may pick (the most
+    // tiles per footprint)
+    const uint64_t entries = stp_hip::gridder_entry_capacity(int(N), true, int(s), p.kernel_exact != 0, 16, size_t(n));
     uint64_t b = 0;
     b += 48 * n;                                // inputs
     b += 2 * rows * N * 2 * S;                  // vis + sampling grids
-    b += 40 * 2 * per_pass * n;                 // binned entries
+    b += stp_hip::GRID_ENTRY_BYTES * entries;   // binned entries
     b += 4 * 2 * ((rows + 63) / 64) * ((N + 31) / 32) * ((n + 4095) / 4096 + 1); // bin counts + offsets
     b += 2 * I * I * S + 4 * I * I;             // image, beam, label map
     // source-find workspace: parents, tile component records (4.5 B per pixel), segment
