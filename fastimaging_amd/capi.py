@@ -134,6 +134,7 @@ class ImagerDiag(ctypes.Structure):
         ("image_rows", ctypes.c_uint32),
         ("image_cols", ctypes.c_uint32),
         ("ms_total", ctypes.c_double),
+        ("num_grid_entries", ctypes.c_uint64),
     ]
 
 
@@ -190,6 +191,9 @@ class JobResult(ctypes.Structure):
         ("t_submit", ctypes.c_double),
         ("t_start", ctypes.c_double),
         ("t_end", ctypes.c_double),
+        ("job_bytes", ctypes.c_uint64),
+        ("slot_bytes_before", ctypes.c_uint64),
+        ("slot_bytes_after", ctypes.c_uint64),
         ("error", ctypes.c_char * 256),
     ]
 
@@ -300,6 +304,9 @@ _SIG = {
                                             ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, _P, _D]),
     "stp_fft_c2r": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, _P]),
     "stp_fft_c2c": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, _P]),
+    "stp_generate_visibilities_from_local_skymodel": (ctypes.c_int, [_P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, _P]),
+    "stp_model_visibilities_device": (ctypes.c_int, [_P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, _P, ctypes.c_int32,
+                                                     ctypes.c_int32]),
     "stp_source_find": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(SourceFindPars), _P,
                                        ctypes.POINTER(Island), ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
                                        ctypes.POINTER(SourceFindStats)]),

@@ -14,6 +14,7 @@
 #include "kernel_tables.h"
 #include "reduce.h"
 #include "sourcefind.h"
+#include "visibility.h"
 
 using namespace stp_hip;
 
@@ -111,7 +112,7 @@ std::string tune_key(const stp_imager_pars* p, const stp_wproj_pars* wp, size_t 
 template <typename T, typename OT>
 void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp, const stp_kernel_spec* kspec, size_t n,
     const double* d_vis, const double* d_w, const double* d_uvw, OT* d_image, OT* d_beam, double* d_total,
-    unsigned long long* d_ngood, bool twofloat)
+    unsigned long long* d_ngood, bool twofloat, unsigned long long* d_nentries = nullptr)
 {
     const stp_kernel_spec kern = kspec ? *kspec : dispatch_kernel(p);
     check_kernel_spec(&kern);
@@ -133,6 +134,7 @@ void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp
         a.wproj = *wp;
     a.cell_rad = (p->cell_size / 3600.0) * (M_PI / 180.0);
     a.analytic_gcf = p->analytic_gcf != 0;
+    a.d_nentries = d_nentries;
     int fft_l1 = 0;
     auto tc = ctx->tune.find(tune_key(p, wp, n, std::is_same<T, double>::value ? 0 : (twofloat ? 2 : 1)));
     if (tc != ctx->tune.end()) {
@@ -164,15 +166,15 @@ void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp
 // Imaging on device buffers in the requested precision.
 void run_imaging(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp, const stp_kernel_spec* kernel, size_t n,
     const double* d_vis, const double* d_w, const double* d_uvw, int32_t precision, void* d_image, void* d_beam,
-    double* d_total, unsigned long long* d_ngood)
+    double* d_total, unsigned long long* d_ngood, unsigned long long* d_nentries = nullptr)
 {
     STP_REQUIRE(precision >= STP_PRECISION_FP64 && precision <= STP_PRECISION_FP32_TWOFLOAT, "unknown precision");
     if (precision == STP_PRECISION_FP64)
         run_imager<double, double>(ctx, p, wp, kernel, n, d_vis, d_w, d_uvw, (double*)d_image, (double*)d_beam, d_total, d_ngood,
-            false);
+            false, d_nentries);
     else
         run_imager<float, float>(ctx, p, wp, kernel, n, d_vis, d_w, d_uvw, (float*)d_image, (float*)d_beam, d_total, d_ngood,
-            precision == STP_PRECISION_FP32_TWOFLOAT);
+            precision == STP_PRECISION_FP32_TWOFLOAT, d_nentries);
 }
 
 // Autotuner: times the imaging of this very input with each candidate launch
@@ -263,6 +265,25 @@ void load_tune_file(stp_ctx* ctx, const std::string& fn)
     std::fclose(f);
 }
 
+// Waits for the call and fills diag from the device triple (sample_grid_total,
+// good count, gridding entries) at d_total.
+void read_diag(stp_ctx* ctx, const stp_imager_pars* p, const double* d_total, stp_imager_diag* diag)
+{
+    unsigned long long h[3] = { 0, 0, 0 };
+    HIP_CHECK(hipMemcpyAsync(h, d_total, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (!diag)
+        return;
+    double tot;
+    std::memcpy(&tot, &h[0], sizeof(tot));
+    diag->sample_grid_total = tot;
+    diag->num_good_vis = h[1];
+    diag->num_grid_entries = h[2];
+    diag->image_rows = tot > 0.0 ? p->image_size : 0;
+    diag->image_cols = tot > 0.0 ? p->image_size : 0;
+    diag->ms_total = 0.0;
+}
+
 } // namespace
 
 extern "C" {
@@ -315,6 +336,8 @@ stp_status stp_ctx_destroy(stp_ctx* ctx)
     }
     if (ctx->copy_event)
         (void)hipEventDestroy(ctx->copy_event);
+    if (ctx->skymodel_event)
+        (void)hipEventDestroy(ctx->skymodel_event);
     for (auto& kv : ctx->ws)
         if (kv.second.ptr)
             (void)hipFree(kv.second.ptr);
@@ -349,12 +372,22 @@ stp_status stp_ctx_synchronize(stp_ctx* ctx)
 stp_status stp_ctx_release_workspace(stp_ctx* ctx)
 {
     return guarded(ctx, [&] {
-        HIP_CHECK(hipDeviceSynchronize());
+        // only this context's queued work can still use its buffers
+        sync_streams(ctx);
         invalidate_graphs(ctx);
         for (auto& kv : ctx->ws)
             if (kv.second.ptr)
                 HIP_CHECK(hipFree(kv.second.ptr));
         ctx->ws.clear();
+        for (auto& kv : ctx->tables)
+            if (kv.second.ptr)
+                HIP_CHECK(hipFree(kv.second.ptr));
+        ctx->tables.clear();
+        for (auto& kv : ctx->twiddles) {
+            HIP_CHECK(hipFree(kv.second.d));
+            HIP_CHECK(hipFree(kv.second.f));
+        }
+        ctx->twiddles.clear();
     });
 }
 
@@ -365,6 +398,10 @@ size_t stp_ctx_workspace_bytes(const stp_ctx* ctx)
     size_t s = 0;
     for (auto& kv : ctx->ws)
         s += kv.second.bytes;
+    for (auto& kv : ctx->tables)
+        s += kv.second.bytes;
+    for (auto& kv : ctx->twiddles)
+        s += size_t(kv.second.n) * (2 * sizeof(double) + 2 * sizeof(float));
     return s;
 }
 
@@ -431,33 +468,17 @@ stp_status stp_image_visibilities_device(stp_ctx* ctx, const stp_imager_pars* p,
         check_imager_pars(p, wp, ap);
         STP_REQUIRE(d_image != nullptr, "image buffer is NULL");
         STP_REQUIRE(n == 0 || (d_vis && d_w && d_uvw), "visibility buffers are NULL");
-        double* d_total = (double*)ws_get(ctx, "imager_total", sizeof(double) * 2);
+        double* d_total = (double*)ws_get(ctx, "imager_total", sizeof(double) * 3);
         unsigned long long* d_ngood = (unsigned long long*)(d_total + 1);
         STP_REQUIRE(precision >= STP_PRECISION_FP64 && precision <= STP_PRECISION_FP32_TWOFLOAT, "unknown precision");
         if (ctx->autotune)
             autotune_shape(ctx, p, wp && wp->num_wplanes > 0 ? wp : nullptr, kernel, n, d_vis, d_w, d_uvw, precision, d_image,
                 d_beam, d_total, d_ngood);
         LaunchTimer whole(ctx, "image_visibilities", ctx->stream);
-        if (precision == STP_PRECISION_FP64)
-            run_imager<double, double>(ctx, p, wp, kernel, n, d_vis, d_w, d_uvw, (double*)d_image, (double*)d_beam, d_total,
-                d_ngood, false);
-        else
-            run_imager<float, float>(ctx, p, wp, kernel, n, d_vis, d_w, d_uvw, (float*)d_image, (float*)d_beam, d_total,
-                d_ngood, precision == STP_PRECISION_FP32_TWOFLOAT);
-        if (synchronize || diag) {
-            double tot = 0.0;
-            unsigned long long ng = 0;
-            HIP_CHECK(hipMemcpyAsync(&tot, d_total, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-            HIP_CHECK(hipMemcpyAsync(&ng, d_ngood, sizeof(ng), hipMemcpyDeviceToHost, ctx->stream));
-            HIP_CHECK(hipStreamSynchronize(ctx->stream));
-            if (diag) {
-                diag->sample_grid_total = tot;
-                diag->num_good_vis = ng;
-                diag->image_rows = tot > 0.0 ? p->image_size : 0;
-                diag->image_cols = tot > 0.0 ? p->image_size : 0;
-                diag->ms_total = 0.0;
-            }
-        }
+        run_imaging(ctx, p, wp, kernel, n, d_vis, d_w, d_uvw, precision, d_image, d_beam, d_total, d_ngood,
+            (unsigned long long*)(d_total + 2));
+        if (synchronize || diag)
+            read_diag(ctx, p, d_total, diag);
     });
 }
 
@@ -587,6 +608,31 @@ stp_status stp_fft_c2c(stp_ctx* ctx, const double* in, int32_t n, int32_t forwar
         fft_c2c_2d<double>(ctx, a, b, n, forward != 0);
         HIP_CHECK(hipMemcpyAsync(out, b, sizeof(cplx<double>) * ne, hipMemcpyDeviceToHost, ctx->stream));
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+stp_status stp_generate_visibilities_from_local_skymodel(stp_ctx* ctx, size_t n_src, const double* skymodel, size_t n_vis,
+    const double* uvw, double* model_vis)
+{
+    return guarded(ctx, [&] {
+        STP_REQUIRE(n_vis == 0 || (uvw && model_vis), "visibility buffers are NULL");
+        if (n_vis == 0)
+            return;
+        double* d = (double*)ws_get(ctx, "host_model", sizeof(double) * 5 * n_vis);
+        HIP_CHECK(hipMemcpyAsync(d + 2 * n_vis, uvw, sizeof(double) * 3 * n_vis, hipMemcpyHostToDevice, ctx->stream));
+        model_visibilities(ctx, n_src, skymodel, n_vis, d + 2 * n_vis, d, false);
+        HIP_CHECK(hipMemcpyAsync(model_vis, d, sizeof(double) * 2 * n_vis, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+stp_status stp_model_visibilities_device(stp_ctx* ctx, size_t n_src, const double* skymodel, size_t n_vis,
+    const double* d_uvw, double* d_vis, int32_t subtract, int32_t synchronize)
+{
+    return guarded(ctx, [&] {
+        model_visibilities(ctx, n_src, skymodel, n_vis, d_uvw, d_vis, subtract != 0);
+        if (synchronize)
+            HIP_CHECK(hipStreamSynchronize(ctx->stream));
     });
 }
 

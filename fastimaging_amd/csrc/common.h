@@ -101,12 +101,16 @@ struct stp_ctx {
     // second stream for result copies that overlap the last kernels of a call
     hipStream_t copy_stream = nullptr;
     hipEvent_t copy_event = nullptr;
+    // the last upload of sky-model terms from the page-locked stage
+    hipEvent_t skymodel_event = nullptr;
 };
 
 namespace stp_hip {
 
 // Returns a device buffer of at least `bytes` for slot `name` (contents undefined).
 void* ws_get(stp_ctx* ctx, const std::string& name, size_t bytes);
+// Waits for every stream of the context (main, batch branches, copy stream).
+void sync_streams(stp_ctx* ctx);
 // The context's copy stream (created on first use, with ctx->copy_event).
 hipStream_t copy_stream(stp_ctx* ctx);
 // The device address of host pointer p when it lies in page-locked memory of the

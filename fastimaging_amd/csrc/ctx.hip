@@ -14,9 +14,9 @@ void* ws_get(stp_ctx* ctx, const std::string& name, size_t bytes)
     if (b.bytes >= bytes)
         return b.ptr;
     if (b.ptr) {
-        // the old buffer may still be read by queued work, and captured batch
-        // graphs refer to it
-        HIP_CHECK(hipDeviceSynchronize());
+        // the old buffer may still be read by this context's queued work, and
+        // captured batch graphs refer to it
+        sync_streams(ctx);
         HIP_CHECK(hipFree(b.ptr));
         b.ptr = nullptr;
         b.bytes = 0;
@@ -26,6 +26,15 @@ void* ws_get(stp_ctx* ctx, const std::string& name, size_t bytes)
     HIP_CHECK(hipMalloc(&b.ptr, alloc));
     b.bytes = alloc;
     return b.ptr;
+}
+
+void sync_streams(stp_ctx* ctx)
+{
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (auto st : ctx->branch_streams)
+        HIP_CHECK(hipStreamSynchronize(st));
+    if (ctx->copy_stream)
+        HIP_CHECK(hipStreamSynchronize(ctx->copy_stream));
 }
 
 hipStream_t copy_stream(stp_ctx* ctx)
@@ -101,13 +110,10 @@ const double* dev_table(stp_ctx* ctx, const std::string& key, const std::vector<
 }
 
 // exp(+2 pi i k / n) for k < n, evaluated in long double and rounded once.
-const Twiddles& twiddles_for(stp_ctx* ctx, int n)
+static void twiddle_tables(int n, std::vector<double>& hd, std::vector<float>& hf)
 {
-    auto it = ctx->twiddles.find(n);
-    if (it != ctx->twiddles.end())
-        return it->second;
-    std::vector<double> hd(2 * size_t(n));
-    std::vector<float> hf(2 * size_t(n));
+    hd.resize(2 * size_t(n));
+    hf.resize(2 * size_t(n));
     const long double two_pi = 6.283185307179586476925286766559005768L;
     for (int k = 0; k < n; k++) {
         const long double a = two_pi * (long double)k / (long double)n;
@@ -117,6 +123,16 @@ const Twiddles& twiddles_for(stp_ctx* ctx, int n)
         hf[2 * k] = float(c);
         hf[2 * k + 1] = float(s);
     }
+}
+
+const Twiddles& twiddles_for(stp_ctx* ctx, int n)
+{
+    auto it = ctx->twiddles.find(n);
+    if (it != ctx->twiddles.end())
+        return it->second;
+    std::vector<double> hd;
+    std::vector<float> hf;
+    twiddle_tables(n, hd, hf);
     Twiddles t;
     t.n = n;
     HIP_CHECK(hipMalloc(&t.d, hd.size() * sizeof(double)));

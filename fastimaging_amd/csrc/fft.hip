@@ -316,8 +316,9 @@ __global__ __launch_bounds__(WG) void fft_v_c2r(const cplx<T>* grids, size_t gri
     constexpr int QN = (n / 4 > 0) ? n / 4 : 1;  // quarter wave: r < n/4
     constexpr int QLOG = Log2<QN>::value;
     // FP32: two small tables (a complex multiply per twiddle; the smaller LDS footprint
-    // lets more workgroups share a CU: 80 -> 74 us at c2); FP64: the quarter table
-    constexpr bool SPLIT = std::is_same<T, float>::value;
+    // lets more workgroups share a CU: 80 -> 74 us at c2); FP64: the quarter table,
+    // except at n = 16384, whose 128 KB column leaves no room for it
+    constexpr bool SPLIT = std::is_same<T, float>::value || M >= 8192;
     constexpr int BLOG = QLOG < 5 ? QLOG : 5;
     constexpr int NTW = SPLIT ? (1 << BLOG) + (QN >> BLOG) : QN;
     constexpr int KI = (M / 2) / S::TPT + 1;      // pack pairs (k, M - k) per thread
@@ -523,7 +524,7 @@ void fft_c2r_2d(stp_ctx* ctx, cplx<T>* grids, int nbatch, int n_rows, int N, con
             (const cplx<T>*)grids, grid_elems, N, L1, L2, tvn, outs, ns);                                      \
         break;
         VCASE(2) VCASE(4) VCASE(8) VCASE(16) VCASE(32) VCASE(64) VCASE(128) VCASE(256) VCASE(512) VCASE(1024)
-        VCASE(2048) VCASE(4096)
+        VCASE(2048) VCASE(4096) VCASE(8192)
 #undef VCASE
     default:
         throw Error(STP_ERR_UNSUPPORTED, "c2r length not supported: " + std::to_string(n_rows));

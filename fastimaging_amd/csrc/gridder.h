@@ -21,6 +21,7 @@ struct GridderArgs {
     double cell_rad = 0.0;  // cell size in radians (W-kernel image-domain sampling)
     bool analytic_gcf = false; // image-domain AA kernel of the W-kernels from gcf() (conv_func.h:351-412)
     int tile_cols = 0;      // tile width (16, 32 or 64 grid columns); 0 = default
+    unsigned long long* d_nentries = nullptr; // device u64: receives the gridding entries written (may be NULL)
 };
 
 // Grids n visibilities into vis_grid (and sampling_grid), each rows x N complex

@@ -335,6 +335,27 @@ __global__ __launch_bounds__(GWG) void grid_scatter_kernel(GridGeom g, const dou
 }
 
 // Kernel functors on the device (exact gridding only; conv_func.cpp:10-202).
+// Spheroidal function of the PSWF functor (conv_func.cpp:112-182): F. Schwab's
+// rational approximation (Indirect Imaging, 1984), M = 6, alpha = 1; the same
+// coefficients and evaluation order as the host table (kernel_tables.cpp).
+__device__ double dev_spheroidal(double nu)
+{
+    constexpr double p0[5] = { 8.203343e-2, -3.644705e-1, 6.278660e-1, -5.335581e-1, 2.312756e-1 };
+    constexpr double p1[5] = { 4.028559e-3, -3.697768e-2, 1.021332e-1, -1.201436e-1, 6.412774e-2 };
+    constexpr double q0[3] = { 1.0, 8.212018e-1, 2.078043e-1 };
+    constexpr double q1[3] = { 1.0, 9.599102e-1, 2.918724e-1 };
+    nu = fabs(nu);
+    const bool part = nu >= 0.75 && nu <= 1.0;
+    const double nuend = (nu >= 0.0 && nu < 0.75) ? 0.75 : ((nu <= 1.0) ? 1.0 : 0.0);
+    const double d = pow(nu, 2.0) - pow(nuend, 2.0);
+    double top = part ? p1[0] : p0[0], bot = part ? q1[0] : q0[0];
+    for (int k = 1; k < 5; k++)
+        top += (part ? p1[k] : p0[k]) * pow(d, double(k));
+    for (int k = 1; k < 3; k++)
+        bot += (part ? q1[k] : q0[k]) * pow(d, double(k));
+    return bot > 0.0 ? top / bot : 0.0;
+}
+
 __device__ double dev_kernel(const GridGeom& g, double r)
 {
     double v;
@@ -353,6 +374,11 @@ __device__ double dev_kernel(const GridGeom& g, double r)
     case STP_KERNEL_GAUSSIAN:
         v = exp((-1.0 / (g.kwg * g.kwg)) * (r * r));
         break;
+    case STP_KERNEL_PSWF: { // conv_func.cpp:185-196
+        const double nu = r / g.kparam;
+        v = dev_spheroidal(nu) * (1.0 - pow(nu, 2.0));
+        break;
+    }
     default: { // GaussianSinc
         double x = r * (M_PI / g.kws);
         if (x == 0.0)
@@ -368,8 +394,9 @@ __device__ double dev_kernel(const GridGeom& g, double r)
 
 // Last canonical level of the (good weights, good count) block pairs: the sampling
 // grid total 2 * sum(weights) and the good count.
+// Also copies the number of gridding entries (the end of the last tile's range).
 __global__ __launch_bounds__(CANON_LANES) void grid_total_kernel(const double* part, unsigned m, double* total,
-    unsigned long long* ngood)
+    unsigned long long* ngood, const uint32_t* entries_end, unsigned long long* nentries)
 {
     __shared__ double lds[2 * CANON_LANES];
     double a, b;
@@ -378,6 +405,8 @@ __global__ __launch_bounds__(CANON_LANES) void grid_total_kernel(const double* p
         total[0] = a * 2.0;
         if (ngood)
             *ngood = (unsigned long long)b;
+        if (nentries)
+            *nentries = *entries_end;
     }
 }
 
@@ -1150,34 +1179,43 @@ __global__ __launch_bounds__(256) void wk_extract_kernel(const cplx<double>* con
     }
 }
 
-// W-projection tile accumulation: one wave per entry, lane = tile row, the lane
-// walks the tile's columns.  Per tap (gridder.h:826-861): kval = K * w with K the
-// plane's complex kernel (conjugated for w < 0), grid += vis * kval, beam += kval.
-// LDS: FP64 sums of grid (re, im) and beam (re, im).
-constexpr int WTWG = 512;
-constexpr int WEB = 256;
+// W-projection tile accumulation without atomics.  The tile's 64 rows are the
+// lanes and its columns are split among the workgroup's eight waves (CPW columns
+// each), so every grid cell belongs to exactly one lane, which keeps the cell's
+// sums in registers.  Every wave walks all of the tile's entries (staged in LDS)
+// and adds the taps of its own columns.  Per tap (gridder.h:826-861): kval = K * w
+// with K the plane's complex kernel (conjugated for w < 0), grid += vis * kval,
+// beam += kval, summed in FP64 on every grid precision.  The tile is written once.
+constexpr int WTW = 8;   // waves per W tile workgroup
+constexpr int WEB = 256; // entries staged per batch
 
-constexpr int WCB = 8;  // tile columns whose W-kernel values a lane loads at once
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ double uni(double x)
+{
+    const unsigned long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readfirstlane(int(b & 0xffffffffu)), hi = __builtin_amdgcn_readfirstlane(int(b >> 32));
+    return __longlong_as_double((long long)((unsigned long long)(unsigned)hi << 32 | (unsigned)lo));
+}
 
-template <typename T, typename KT>
-__global__ __launch_bounds__(WTWG) void grid_wtiles_kernel(GridGeom g, const Entry* entries, const uint32_t* offsets,
+template <typename T, typename KT, int CPW>
+__global__ __launch_bounds__(WTW * 64) void grid_wtiles_kernel(GridGeom g, const Entry* entries, const uint32_t* offsets,
     int ostride, uint32_t cap, const cplx<KT>* cache, cplx<T>* vgrid, cplx<T>* bgrid)
 {
 #pragma clang fp contract(off)
-    extern __shared__ double acc[];
     __shared__ Entry ebuf[WEB];
     const int ks = g.ks, s = g.s, N = g.N;
-    const int S = TILE_R + 1;
-    const int plane = g.tc * S;
     const int cs = (g.os / 2) * 2 + 1;
     const size_t kk = size_t(ks) * ks;
-    for (int k = threadIdx.x; k < 4 * plane; k += WTWG)
-        acc[k] = 0.0;
     const int t = blockIdx.x;
     const int rt = t % g.tiles_r, ct = t / g.tiles_r;
     const int r0 = rt * TILE_R, c0 = ct * g.tc;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = r0 + lane;
+    const int wc = c0 + wave * CPW; // first grid column of this wave
+    double ar[CPW], ai[CPW], br[CPW], bi[CPW];
+#pragma unroll
+    for (int u = 0; u < CPW; u++)
+        ar[u] = ai[u] = br[u] = bi[u] = 0.0;
     const uint32_t eb = min(offsets[size_t(t) * ostride], cap), ee = min(offsets[size_t(t + 1) * ostride], cap);
     for (uint32_t e0 = eb; e0 < ee; e0 += WEB) {
         const int cnt = int(min(uint32_t(WEB), ee - e0));
@@ -1185,60 +1223,61 @@ __global__ __launch_bounds__(WTWG) void grid_wtiles_kernel(GridGeom g, const Ent
         if (int(threadIdx.x) < cnt)
             ebuf[threadIdx.x] = entries[e0 + threadIdx.x];
         __syncthreads();
-        for (int q = wave; q < cnt; q += WTWG / 64) {
-            const Entry& E = ebuf[q];
+        for (int q = 0; q < cnt; q++) {
+            const int gx = uni(ebuf[q].gx), gy = uni(ebuf[q].gy), cp = uni(ebuf[q].cp), pl = uni(ebuf[q].idx);
+            // kernel column of the wave's first column, mod N; the wave's columns run on from it
+            int j0 = wc - (gx - s);
+            j0 = j0 < 0 ? j0 + N : (j0 >= N ? j0 - N : j0);
+            bool any = false;
+#pragma unroll
+            for (int u = 0; u < CPW; u++) {
+                const int j = j0 + u >= N ? j0 + u - N : j0 + u;
+                any = any || (j < ks && wc + u < N);
+            }
+            if (!any)
+                continue;
             // kernel row of my grid row: direct, or wrapped from >= N (rows < 0 are skipped)
-            int i = r - (E.gy - s);
+            int i = r - (gy - s);
             if (i < 0 || i >= ks)
                 i += N;
-            if (i < 0 || i >= ks || r >= g.rows)
-                continue;
-            const bool conj = E.cp < 0;
-            const int cpx = E.cp & 0x7ff, cpy = (E.cp >> 16) & 0x7ff;
-            const cplx<KT>* K = cache + (size_t(E.idx) * cs * cs + size_t(cpy) * cs + cpx) * kk;
-            const double w = E.w, vr = E.vr, vi = E.vi;
-            int j0 = c0 - (E.gx - s); // kernel column of the tile's first column, mod N
-            j0 = j0 < 0 ? j0 + N : (j0 >= N ? j0 - N : j0);
-            const int a0 = lane;
-            // WCB columns at a time: their kernel values are loaded together, then summed
-            for (int lc0 = 0; lc0 < g.tc; lc0 += WCB) {
-                cplx<KT> kv[WCB];
-                bool ok[WCB];
+            const bool rowok = i >= 0 && i < ks && r < g.rows;
+            const bool conj = cp < 0;
+            const int cpx = cp & 0x7ff, cpy = (cp >> 16) & 0x7ff;
+            const cplx<KT>* K = cache + (size_t(pl) * cs * cs + size_t(cpy) * cs + cpx) * kk;
+            const double w = uni(ebuf[q].w), vr = uni(ebuf[q].vr), vi = uni(ebuf[q].vi);
+            cplx<KT> kv[CPW];
 #pragma unroll
-                for (int u = 0; u < WCB; u++) {
-                    const int lc = lc0 + u;
-                    int j = j0 + lc;
-                    j = j >= N ? j - N : j;
-                    ok[u] = lc < g.tc && j < ks && c0 + lc < N;
-                    kv[u] = ok[u] ? K[size_t(j) * ks + i] : cplx<KT>{ KT(0), KT(0) };
-                }
+            for (int u = 0; u < CPW; u++) {
+                const int j = j0 + u >= N ? j0 + u - N : j0 + u;
+                const bool ok = rowok && j < ks && wc + u < N;
+                kv[u] = ok ? K[size_t(j) * ks + i] : cplx<KT>{ KT(0), KT(0) };
+            }
+            if (rowok) {
 #pragma unroll
-                for (int u = 0; u < WCB; u++) {
-                    if (!ok[u])
+                for (int u = 0; u < CPW; u++) {
+                    const int j = j0 + u >= N ? j0 + u - N : j0 + u;
+                    if (j >= ks || wc + u >= N)
                         continue;
                     const double kr = double(kv[u].x) * w;
                     const double ki = (conj ? -double(kv[u].y) : double(kv[u].y)) * w;
-                    const int a = (lc0 + u) * S + a0;
-                    unsafeAtomicAdd(&acc[a], vr * kr - vi * ki);
-                    unsafeAtomicAdd(&acc[plane + a], vr * ki + vi * kr);
-                    if (g.beam) {
-                        unsafeAtomicAdd(&acc[2 * plane + a], kr);
-                        unsafeAtomicAdd(&acc[3 * plane + a], ki);
-                    }
+                    ar[u] += vr * kr - vi * ki;
+                    ai[u] += vr * ki + vi * kr;
+                    br[u] += kr;
+                    bi[u] += ki;
                 }
             }
         }
     }
-    __syncthreads();
-    for (int k = threadIdx.x; k < g.tc * TILE_R; k += WTWG) {
-        const int lr = k % TILE_R, lc = k / TILE_R;
-        const int rr = r0 + lr, c = c0 + lc;
-        if (rr < g.rows && c < N) {
-            const size_t o = size_t(rr) + size_t(c) * g.rows;
-            const int a = lc * S + lr;
-            vgrid[o] = { T(acc[a]), T(acc[plane + a]) };
-            if (g.beam)
-                bgrid[o] = { T(acc[2 * plane + a]), T(acc[3 * plane + a]) };
+    if (r < g.rows) {
+#pragma unroll
+        for (int u = 0; u < CPW; u++) {
+            const int c = wc + u;
+            if (c < N && c < c0 + g.tc) {
+                const size_t o = size_t(r) + size_t(c) * g.rows;
+                vgrid[o] = { T(ar[u]), T(ai[u]) };
+                if (g.beam)
+                    bgrid[o] = { T(br[u]), T(bi[u]) };
+            }
         }
     }
 }
@@ -1294,8 +1333,6 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     g.kparam = kf.param;
     g.kwg = kf.width_g;
     g.kws = kf.width_s;
-    if (a.kernel_exact)
-        STP_REQUIRE(kf.kind != STP_KERNEL_PSWF, "exact gridding with PSWF is not implemented on the GPU");
 
     const int ntiles = g.tiles_r * g.tiles_c;
     // kernel table (oversampled 1-D kernels), computed on the host once per configuration
@@ -1321,9 +1358,9 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     const uint32_t cap = uint32_t(cap64);
     Entry* entries = (Entry*)ws_get(ctx, "grid_entries", sizeof(Entry) * cap64);
 
-    // W-kernel cache in FP64 for every grid precision: with an FP32 cache the W tile
-    // kernel measured 1.8x slower (44 vs 24 ms at N = 4096, 2M visibilities, 32 planes)
-    using KT = double;
+    // W-kernel cache in the grid precision: FP64 kernels on the FP64 path, FP32
+    // kernels (half the bytes the W tile kernel reads per tap) on the FP32 paths
+    using KT = T;
     cplx<KT>* wcache = nullptr;
     if (wproj) {
         const int P = int(a.wproj.num_wplanes);
@@ -1426,11 +1463,10 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         ostride = 1;
     }
     if (wproj) {
-        const size_t dyn_bytes = sizeof(double) * 4 * size_t(g.tc) * (TILE_R + 1);
-        HIP_CHECK(hipFuncSetAttribute((const void*)grid_wtiles_kernel<T, KT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-            int(dyn_bytes)));
-        launch(ctx, "grid_wtiles", grid_wtiles_kernel<T, KT>, dim3(ntiles), dim3(WTWG), dyn_bytes, g, (const Entry*)entries,
-            offsets, ostride, cap, (const cplx<KT>*)wcache, d_vis_grid, d_sampling_grid);
+        // eight waves split the tile's columns: 2, 4 or 8 columns per wave
+        auto kern = g.tc == 16 ? grid_wtiles_kernel<T, KT, 2> : (g.tc == 32 ? grid_wtiles_kernel<T, KT, 4> : grid_wtiles_kernel<T, KT, 8>);
+        launch(ctx, "grid_wtiles", kern, dim3(ntiles), dim3(WTW * 64), 0, g, (const Entry*)entries, offsets, ostride, cap,
+            (const cplx<KT>*)wcache, d_vis_grid, d_sampling_grid);
     } else {
         STP_REQUIRE(a.kernel_exact || cs * g.ks <= KV_DOUBLES, "oversampled kernel table too large");
         const bool twofloat = std::is_same<T, float>::value && a.twofloat;
@@ -1457,7 +1493,8 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     // sample_grid_total = 2 * sum(good weights) (gridder.h:415-423); the good count
     size_t mfin = 0;
     const double* fin = canon_reduce_to_block(ctx, wpart, nblk, wscr, &mfin);
-    launch(ctx, "grid_total", grid_total_kernel, dim3(1), dim3(CANON_LANES), 0, fin, unsigned(mfin), d_total, d_ngood);
+    launch(ctx, "grid_total", grid_total_kernel, dim3(1), dim3(CANON_LANES), 0, fin, unsigned(mfin), d_total, d_ngood,
+        offsets + size_t(ntiles) * ostride, a.d_nentries);
 }
 
 size_t gridder_entry_capacity(int N, bool halfplane, int support, bool exact, int tile_cols, size_t n)

@@ -35,6 +35,7 @@ struct JobRec {
     uint64_t bytes = 0;
     stp_job_result res{};
     bool done = false;
+    bool claimed = false; // a stp_server_wait has taken the result
 };
 
 // Per-slot device buffers, grown on demand and kept between jobs.
@@ -59,7 +60,11 @@ struct stp_server {
     std::deque<JobRec*> queue;
     std::map<uint64_t, std::unique_ptr<JobRec>> jobs;
     uint64_t next_id = 1;
+    // admission ledger: the estimate of every running job plus the device memory each
+    // slot keeps between jobs (its workspace, tables, twiddles and staging buffers)
     uint64_t reserved = 0;
+    std::vector<uint64_t> resident;  // per slot
+    bool release_idle = false;       // the queue head waits for memory idle slots hold
     uint32_t running = 0;
     bool stop = false;
     std::vector<std::thread> workers;
@@ -204,17 +209,49 @@ void run_job(stp_server* s, stp_ctx* ctx, SlotBufs& b, JobRec* r)
     r->res.status = STP_OK;
 }
 
+// Device bytes a slot keeps between jobs.
+uint64_t slot_bytes(stp_ctx* ctx, const SlotBufs& b)
+{
+    return (ctx ? stp_ctx_workspace_bytes(ctx) : 0) + b.in_bytes + b.out_bytes;
+}
+
+// Frees everything a slot keeps between jobs.
+void slot_release(stp_ctx* ctx, SlotBufs& b)
+{
+    if (!ctx)
+        return;
+    stp_ctx_release_workspace(ctx);
+    if (b.in)
+        stp_device_free(ctx, b.in);
+    if (b.out)
+        stp_device_free(ctx, b.out);
+    b = SlotBufs();
+}
+
 void worker(stp_server* s, int slot)
 {
     stp_ctx* ctx = nullptr;
     const stp_status cst = stp_ctx_create(&ctx, s->cfg.device);
     SlotBufs bufs;
-    const uint64_t share = s->budget / std::max<uint32_t>(1, s->cfg.num_slots);
     for (;;) {
         JobRec* r = nullptr;
+        bool admitted = false;
+        bool released = false; // at most one release per idle period
         {
             std::unique_lock<std::mutex> lk(s->mu);
             for (;;) {
+                if (s->release_idle && s->resident[slot] > 0 && !released) {
+                    released = true;
+                    // the queue head does not fit: give back what this idle slot holds
+                    lk.unlock();
+                    slot_release(ctx, bufs);
+                    lk.lock();
+                    s->reserved -= s->resident[slot];
+                    s->resident[slot] = slot_bytes(ctx, bufs);
+                    s->reserved += s->resident[slot];
+                    s->cv_work.notify_all();
+                    continue;
+                }
                 if (!s->queue.empty()) {
                     JobRec* head = s->queue.front();
                     if (head->bytes > s->budget || cst != STP_OK) {
@@ -227,11 +264,26 @@ void worker(stp_server* s, int slot)
                         s->queue.pop_front();
                         s->reserved += head->bytes;
                         s->running++;
+                        s->release_idle = false;
                         r = head;
+                        admitted = true;
                         break;
+                    }
+                    // it does not fit beside what is running and resident: this slot's
+                    // own memory first, then every idle slot's (one notification when
+                    // the request is raised; each release notifies again)
+                    if (s->resident[slot] > 0 && !released) {
+                        s->release_idle = true;
+                        continue;
+                    }
+                    if (!s->release_idle) {
+                        s->release_idle = true;
+                        s->cv_work.notify_all();
                     }
                 } else if (s->stop) {
                     break;
+                } else {
+                    s->release_idle = false;
                 }
                 s->cv_work.wait(lk);
             }
@@ -240,21 +292,25 @@ void worker(stp_server* s, int slot)
             break;
         r->res.slot = slot;
         r->res.t_start = s->now();
-        const bool admitted = !(r->bytes > s->budget || cst != STP_OK);
+        r->res.job_bytes = r->bytes;
         if (cst != STP_OK)
             fail(r, cst, "stp_ctx_create failed for the server's device");
         else if (!admitted)
             fail(r, STP_ERR_OUT_OF_MEMORY, "job needs more device memory than the server's HBM budget");
-        else
+        else {
+            r->res.slot_bytes_before = slot_bytes(ctx, bufs);
             run_job(s, ctx, bufs, r);
+            r->res.slot_bytes_after = slot_bytes(ctx, bufs);
+        }
         r->res.t_end = s->now();
-        // keep the slot's resident workspace within its share of the budget
-        if (ctx && admitted && stp_ctx_workspace_bytes(ctx) + bufs.in_bytes + bufs.out_bytes > share)
-            stp_ctx_release_workspace(ctx);
         {
             std::lock_guard<std::mutex> lk(s->mu);
             if (admitted) {
+                // the job's estimate is replaced by what the slot now holds
                 s->reserved -= r->bytes;
+                s->reserved -= s->resident[slot];
+                s->resident[slot] = r->res.slot_bytes_after;
+                s->reserved += s->resident[slot];
                 s->running--;
             }
             r->done = true;
@@ -263,12 +319,12 @@ void worker(stp_server* s, int slot)
         s->cv_work.notify_all();
     }
     if (ctx) {
-        if (bufs.in)
-            stp_device_free(ctx, bufs.in);
-        if (bufs.out)
-            stp_device_free(ctx, bufs.out);
+        slot_release(ctx, bufs);
         stp_ctx_destroy(ctx);
     }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->reserved -= s->resident[slot];
+    s->resident[slot] = 0;
 }
 
 } // namespace
@@ -289,6 +345,7 @@ stp_status stp_server_create(stp_server** out, const stp_server_cfg* cfg)
     if (!s)
         return STP_ERR_OUT_OF_MEMORY;
     s->cfg = *cfg;
+    s->resident.assign(cfg->num_slots, 0);
     s->budget = cfg->hbm_budget_bytes;
     if (s->budget == 0) {
         size_t fr = 0, tot = 0;
@@ -387,10 +444,15 @@ stp_status stp_server_wait(stp_server* s, uint64_t job_id, stp_job_result* resul
         return STP_ERR_INVALID_ARGUMENT;
     }
     JobRec* r = it->second.get();
+    if (r->claimed) {
+        s->last_error = "job is already being waited for";
+        return STP_ERR_INVALID_ARGUMENT;
+    }
+    r->claimed = true;
     s->cv_done.wait(lk, [&] { return r->done; });
     if (result)
         *result = r->res;
-    s->jobs.erase(it);
+    s->jobs.erase(job_id);
     return STP_OK;
 }
 

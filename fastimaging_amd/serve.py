@@ -107,6 +107,8 @@ class Server:
             "diag": r.diag,
             "slot": r.slot,
             "t_submit": r.t_submit, "t_start": r.t_start, "t_end": r.t_end,
+            "job_bytes": r.job_bytes, "slot_bytes_before": r.slot_bytes_before,
+            "slot_bytes_after": r.slot_bytes_after,
         }
 
     def job_bytes(self, img_pars, n_vis, precision=capi.PRECISION_FP64, w_proj=None):

@@ -155,6 +155,22 @@ class Context:
         return out
 
     # ------------------------------------------------------------ source find
+    def generate_visibilities_from_local_skymodel(self, skymodel, uvw_lambda):
+        """stp::generate_visibilities_from_local_skymodel (visibility.cpp:31-41): skymodel
+        rows (l, m, flux), uvw (n, 3) in wavelengths; returns complex128 (n,)."""
+        sm = np.asfortranarray(skymodel, dtype=np.float64).reshape(-1, 3)
+        uvw = np.asfortranarray(uvw_lambda, dtype=np.float64)
+        out = np.empty(uvw.shape[0], dtype=np.complex128)
+        self._check(self.lib.stp_generate_visibilities_from_local_skymodel(self.h, sm.shape[0], _ptr(sm), uvw.shape[0],
+                                                                           _ptr(uvw), _ptr(out)))
+        return out
+
+    def model_visibilities_device(self, skymodel, n_vis, d_uvw, d_vis, subtract=True, synchronize=False):
+        """Model (subtract=False) or residual (d_vis -= model) visibilities on device buffers."""
+        sm = np.asfortranarray(skymodel, dtype=np.float64).reshape(-1, 3)
+        self._check(self.lib.stp_model_visibilities_device(self.h, sm.shape[0], _ptr(sm), n_vis, d_uvw, d_vis,
+                                                           int(subtract), int(synchronize)))
+
     def source_find(self, image, pars, capacity=1 << 16, label_map=True):
         """stp::SourceFindImage (sourcefind.cpp:156-292)."""
         img = np.asfortranarray(image, dtype=np.float64)

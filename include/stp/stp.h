@@ -329,6 +329,26 @@ public:
     mat sampling_grid;
 };
 
+/* ------------------------------ model visibilities (visibility.h:34-59) */
+/* skymodel rows (l, m, flux_jy); uvw_baselines n x 3 (wavelengths). */
+inline cx_mat generate_visibilities_from_local_skymodel(const mat& skymodel, const mat& uvw_baselines)
+{
+    if (skymodel.n_cols != 3 || uvw_baselines.n_cols != 3)
+        throw std::runtime_error("stp: skymodel and uvw_baselines must have 3 columns");
+    cx_mat out(uvw_baselines.n_rows, 1);
+    detail::check(stp_generate_visibilities_from_local_skymodel(detail::ctx(), skymodel.n_rows, skymodel.memptr(),
+        uvw_baselines.n_rows, uvw_baselines.memptr(), reinterpret_cast<double*>(out.memptr())));
+    return out;
+}
+inline cx_mat visibilities_for_point_source(const mat& dist_uvw, double l, double m, double flux)
+{
+    mat sky(1, 3);
+    sky.at(0, 0) = l;
+    sky.at(0, 1) = m;
+    sky.at(0, 2) = flux;
+    return generate_visibilities_from_local_skymodel(sky, dist_uvw);
+}
+
 /* ---------------------------------------- source finder (sourcefind.h) */
 struct BoundingBox {
     int top = 0, bottom = 0, left = 0, right = 0;

@@ -192,6 +192,7 @@ typedef struct stp_imager_diag {
     uint32_t image_rows;        /* 0 when sample_grid_total <= 0 (empty result, imager.h:355) */
     uint32_t image_cols;
     double ms_total;            /* device time of the whole call (ms) */
+    uint64_t num_grid_entries;  /* (visibility pass, grid tile) entries the gridder binned */
 } stp_imager_diag;
 
 /* ------------------------------------------------------------------------- */
@@ -206,9 +207,11 @@ stp_status stp_ctx_create(stp_ctx** ctx, int device);
 stp_status stp_ctx_destroy(stp_ctx* ctx);
 const char* stp_ctx_last_error(const stp_ctx* ctx);
 stp_status stp_ctx_synchronize(stp_ctx* ctx);
-/* Releases every cached device buffer of the context. */
+/* Releases every cached device buffer of the context (workspace, kernel tables,
+ * FFT twiddles) after waiting for the context's own streams. */
 stp_status stp_ctx_release_workspace(stp_ctx* ctx);
-/* Bytes of device workspace the context currently holds. */
+/* Bytes of device memory the context currently holds (workspace, kernel tables,
+ * FFT twiddles). */
 size_t stp_ctx_workspace_bytes(const stp_ctx* ctx);
 
 /* Fills the defaults of the reference's ImagerPars full constructor
@@ -300,6 +303,37 @@ stp_status stp_fft_c2r(stp_ctx* ctx, const double* in, int32_t in_rows, int32_t 
  * Unnormalised 2-D complex FFT (sign -1 forward, +1 inverse) of a square
  * n x n column-major complex matrix. */
 stp_status stp_fft_c2c(stp_ctx* ctx, const double* in, int32_t n, int32_t forward, double* out);
+
+/* ------------------------------------------------------------------------- */
+/* Model and residual visibilities                                            */
+/* ------------------------------------------------------------------------- */
+
+/* stp::generate_visibilities_from_local_skymodel(skymodel, uvw_baselines)
+ *   src/stp/visibility/visibility.cpp:31-41 (per source: visibilities_for_point_source,
+ *   visibility.cpp:15-29).
+ * skymodel: n_src x 3 column-major, rows (l, m, flux_jy) with l^2 + m^2 <= 1;
+ * uvw_lambda: n_vis x 3 column-major (wavelengths); model_vis: n_vis complex,
+ * caller-allocated.  Synchronous. */
+stp_status stp_generate_visibilities_from_local_skymodel(stp_ctx* ctx,
+    size_t n_src,
+    const double* skymodel,
+    size_t n_vis,
+    const double* uvw_lambda,
+    double* model_vis);
+
+/* The same with the baselines and visibilities in device memory (the sky model
+ * stays in host memory).  subtract == 0: d_vis = model; subtract != 0:
+ * d_vis -= model in place, the residual visibilities the reduce driver images
+ * with its -d option (src/reduce/reduce.cpp:113-122).  Enqueued on the context
+ * stream; returns before the work finishes unless `synchronize` is non-zero. */
+stp_status stp_model_visibilities_device(stp_ctx* ctx,
+    size_t n_src,
+    const double* skymodel,
+    size_t n_vis,
+    const double* d_uvw_lambda,
+    double* d_vis,
+    int32_t subtract,
+    int32_t synchronize);
 
 /* ------------------------------------------------------------------------- */
 /* Source finding                                                             */
@@ -438,6 +472,9 @@ typedef struct stp_job_result {
     stp_sourcefind_stats stats;
     stp_imager_diag diag;
     double t_submit, t_start, t_end;   /* seconds on the server's steady clock */
+    uint64_t job_bytes;          /* what admission reserved for the job (stp_job_device_bytes) */
+    uint64_t slot_bytes_before;  /* device bytes the slot held when it took the job */
+    uint64_t slot_bytes_after;   /* ... and after it: at most slot_bytes_before + job_bytes */
     char error[256];
 } stp_job_result;
 
@@ -452,7 +489,9 @@ stp_status stp_server_poll(stp_server* server, uint64_t job_id, int32_t* done);
 stp_status stp_server_wait(stp_server* server, uint64_t job_id, stp_job_result* result);
 /* Estimated device bytes of a job (what admission reserves). */
 uint64_t stp_job_device_bytes(const stp_job* job);
-/* Jobs queued, running, and bytes reserved right now. */
+/* Jobs queued, running, and bytes reserved right now: the estimates of the running
+ * jobs plus the device memory every slot keeps between jobs.  A job that does not
+ * fit makes the idle slots release what they keep. */
 stp_status stp_server_load(stp_server* server, uint32_t* queued, uint32_t* running, uint64_t* reserved_bytes);
 const char* stp_server_last_error(const stp_server* server);
 

@@ -139,6 +139,9 @@ SourceFindResult source_find(const double* img, int rows, int cols, const stp_so
 void estimate_moments_fit(stp_island& isl, double x_bar, double y_bar, double xx_bar, double yy_bar, double xy_bar,
     double rms_est, double analysis_n_sigma);
 
+/* ---------------- model visibilities: src/stp/visibility/visibility.cpp ---------------- */
+void model_visibilities(const double* skymodel, size_t n_src, const double* uvw, size_t n_vis, cx* out, int nthreads);
+
 int default_threads();
 
 } // namespace oracle

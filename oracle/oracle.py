@@ -269,3 +269,15 @@ def populate_kernel_cache(kind, param, support, oversampling, widths=None):
     _check(lib().oracle_populate_kernel_cache(_spec(kind, param, widths), ctypes.c_uint(support),
                                               ctypes.c_uint(oversampling), _ptr(out)))
     return np.ascontiguousarray(np.swapaxes(out, 2, 3))
+
+
+def model_visibilities(skymodel, uvw, nthreads=None):
+    """generate_visibilities_from_local_skymodel (visibility.cpp:31-41): skymodel (n_src, 3)
+    rows (l, m, flux), uvw (n_vis, 3) in wavelengths; returns complex128 (n_vis,)."""
+    sm = np.asfortranarray(skymodel, dtype=np.float64).reshape(-1, 3)
+    uvw = np.asfortranarray(uvw, dtype=np.float64)
+    out = np.zeros(uvw.shape[0], dtype=np.complex128)
+    _check(lib().oracle_model_visibilities(_ptr(sm), ctypes.c_size_t(sm.shape[0]), _ptr(uvw),
+                                           ctypes.c_size_t(uvw.shape[0]), _ptr(out),
+                                           nthreads if nthreads is not None else threads()))
+    return out

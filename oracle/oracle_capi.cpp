@@ -27,6 +27,11 @@ static int guard(F f)
 extern "C" {
 
 const char* oracle_last_error() { return g_err.c_str(); }
+
+int oracle_model_visibilities(const double* skymodel, size_t n_src, const double* uvw, size_t n_vis, double* out, int nthreads)
+{
+    return guard([&] { model_visibilities(skymodel, n_src, uvw, n_vis, reinterpret_cast<cx*>(out), nthreads); });
+}
 int oracle_default_threads() { return default_threads(); }
 
 int oracle_kernel_eval(const stp_kernel_spec* k, const double* x, size_t n, double* out)

@@ -1,0 +1,177 @@
+"""Workloads c4 (48 snapshots in one batch) and c5 (a served mix) at their full
+sizes, checked snapshot by snapshot against the FP64 oracle.
+
+c4: the 48 snapshots of simdata.batch_specs() (N in {2048, 4096}, 2e4-2e6
+visibilities log-uniform, 1-40 sources) through stp_image_batch: FP64 eager within
+1e-10 of each snapshot's peak, FP32 (FP64 tile sums) captured as a hipGraph with
+autotuning and replayed, within 1e-4.
+c5: a served mix with an 8192^2 image, a 4M-visibility job and a 32-plane
+support-32 W-projection job, each the same alone and in the mix and against the
+oracle (images; island lists and label maps bit-exact against the oracle's source
+finder run on the served image).
+"""
+import threading
+
+import numpy as np
+import pytest
+
+from fastimaging_amd import capi, simdata
+from fastimaging_amd.serve import Server
+from tests.helpers import peak_rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c4(gpu, orc):
+    specs = simdata.batch_specs()
+    assert len(specs) == 48
+    snaps, oracle, bufs = [], [], []
+    for k, sp in enumerate(specs):
+        snap = simdata.snapshot(**sp)
+        N, nv = sp["image_size"], sp["n_vis"]
+        pars = capi.imager_pars(N, sp["cell_arcsec"])
+        oimg, obeam, _, _ = orc.image_visibilities(pars, snap["vis"], snap["weights"], snap["uvw"])
+        b = [gpu.malloc(16 * nv), gpu.malloc(8 * nv), gpu.malloc(24 * nv)]
+        gpu.htod(b[0], np.ascontiguousarray(snap["vis"]))
+        gpu.htod(b[1], np.ascontiguousarray(snap["weights"]))
+        gpu.htod(b[2], np.asfortranarray(snap["uvw"]))
+        snaps.append((pars, nv))
+        oracle.append((oimg, obeam))
+        bufs.append(b)
+    print(f"c4: {sum(nv for _, nv in snaps)} visibilities in 48 snapshots", flush=True)
+    yield snaps, oracle, bufs
+    for b in bufs:
+        for x in b:
+            gpu.free(x)
+
+
+def _items(gpu, snaps, bufs, osz):
+    items, outs = [], []
+    for (pars, nv), b in zip(snaps, bufs):
+        N = pars.image_size
+        o = [gpu.malloc(osz * N * N), gpu.malloc(osz * N * N)]
+        it = capi.BatchItem()
+        it.img_pars = pars
+        it.n_vis = nv
+        it.d_vis, it.d_vis_weights, it.d_uvw_lambda = [x.value for x in b]
+        it.d_image, it.d_beam = o[0].value, o[1].value
+        items.append(it)
+        outs.append(o)
+    return items, outs
+
+
+def _check(gpu, items, oracle, dt, tol, what):
+    worst = 0.0
+    for k, (it, (oimg, obeam)) in enumerate(zip(items, oracle)):
+        N = it.img_pars.image_size
+        img = np.empty((N, N), order="F", dtype=dt)
+        beam = np.empty((N, N), order="F", dtype=dt)
+        gpu.dtoh(img, it.d_image)
+        gpu.dtoh(beam, it.d_beam)
+        e = max(peak_rel_err(img.astype(np.float64), oimg), peak_rel_err(beam.astype(np.float64), obeam))
+        assert e < tol, (what, k, e)
+        worst = max(worst, e)
+    print(f"c4 {what}: worst peak-relative error over 48 snapshots {worst:.2e}", flush=True)
+
+
+def test_c4_fp64_batch(gpu, c4):
+    snaps, oracle, bufs = c4
+    items, outs = _items(gpu, snaps, bufs, 8)
+    try:
+        totals = gpu.image_batch(items, precision=capi.PRECISION_FP64, max_concurrent=4)
+        assert np.all(totals > 0)
+        _check(gpu, items, oracle, np.float64, 1e-10, "FP64 eager")
+    finally:
+        for o in outs:
+            for x in o:
+                gpu.free(x)
+
+
+def test_c4_fp32_graph_replay(gpu, c4):
+    snaps, oracle, bufs = c4
+    items, outs = _items(gpu, snaps, bufs, 4)
+    flags = capi.BATCH_GRAPH | capi.BATCH_AUTOTUNE
+    try:
+        # first call: autotune every new shape, run eagerly and capture; then replays
+        for rep in range(3):
+            for it in items:
+                N = it.img_pars.image_size
+                gpu.htod(it.d_image, np.zeros(N * N, dtype=np.float32))
+            gpu.image_batch(items, precision=capi.PRECISION_FP32C, max_concurrent=4, flags=flags)
+            _check(gpu, items, oracle, np.float32, 1e-4, f"FP32 graph call {rep}")
+    finally:
+        for o in outs:
+            for x in o:
+                gpu.free(x)
+
+
+# ---------------------------------------------------------------------------- c5
+def _c5_jobs():
+    sf = capi.sourcefind_pars(5.0, 3.0, median_method=capi.MEDIAN_BINMEDIAN)
+    specs = [
+        # (N, cell, n_vis, sources, seed, precision, W-projection planes/support, w_max)
+        (8192, 1.0, 1_000_000, 30, 951, capi.PRECISION_FP32C, None, 0.0),
+        (2048, 1.0, 4_000_000, 40, 952, capi.PRECISION_FP64, None, 0.0),
+        (2048, 2.0, 1_000_000, 20, 953, capi.PRECISION_FP32C, (32, 32), 1500.0),
+        (1024, 1.0, 50_000, 5, 954, capi.PRECISION_FP64, None, 0.0),
+        (4096, 1.0, 500_000, 25, 955, capi.PRECISION_FP32_TWOFLOAT, None, 0.0),
+        (2048, 2.0, 300_000, 12, 956, capi.PRECISION_FP64, (32, 32), 1500.0),
+    ]
+    jobs = []
+    for N, cell, nv, ns, seed, prec, wpp, wmax in specs:
+        snap = simdata.snapshot(N, cell, nv, ns, seed=seed, field_frac=0.5 if wpp else 0.25, w_max=wmax)
+        wp = capi.wproj_pars(*wpp, undersampling_opt=1) if wpp else None
+        pars = capi.imager_pars(N, cell, oversampling=8 if wpp else 9)
+        jobs.append(dict(snap=snap, pars=pars, wp=wp, prec=prec, sf=sf))
+    return jobs
+
+
+def _submit(srv, j):
+    s = j["snap"]
+    return srv.submit(s["vis"], s["weights"], s["uvw"], j["pars"], w_proj=j["wp"], precision=j["prec"],
+                      sourcefind=j["sf"], label_map=True)
+
+
+def test_c5_served_mix(gpu, orc):
+    jobs = _c5_jobs()
+    with Server(num_slots=1, hbm_budget_bytes=128 << 30) as srv:
+        alone = [srv.wait(_submit(srv, j)) for j in jobs]
+    with Server(num_slots=4, hbm_budget_bytes=128 << 30) as srv:
+        ids = [None] * len(jobs)
+
+        def sub(idx):
+            for k in idx:
+                ids[k] = _submit(srv, jobs[k])
+
+        th = [threading.Thread(target=sub, args=(range(t, len(jobs), 2),)) for t in range(2)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        mixed = [srv.wait(i) for i in ids]
+    assert len({m["slot"] for m in mixed}) > 1
+    assert any(a["t_start"] < b["t_end"] and b["t_start"] < a["t_end"]
+               for x, a in enumerate(mixed) for b in mixed[x + 1:])
+    key = lambda i: (i["label_idx"], i["sign"], i["extremum_x_idx"], i["extremum_y_idx"], i["num_samples"],
+                     i["extremum_val"])
+    for k, (j, a, b) in enumerate(zip(jobs, alone, mixed)):
+        fp64 = j["prec"] == capi.PRECISION_FP64
+        tol_mix = 1e-12 if fp64 else 1e-6
+        # alone vs in the mix
+        assert peak_rel_err(b["image"], a["image"]) <= tol_mix, k
+        assert peak_rel_err(b["beam"], a["beam"]) <= tol_mix, k
+        # against the oracle
+        s = j["snap"]
+        oimg, obeam, _, _ = orc.image_visibilities(j["pars"], s["vis"], s["weights"], s["uvw"], wproj=j["wp"])
+        e = peak_rel_err(b["image"], oimg)
+        print(f"c5 job {k}: N={j['pars'].image_size} n_vis={s['weights'].size} precision={j['prec']} "
+              f"wproj={j['wp'] is not None}: image err {e:.2e}, {len(b['islands'])} islands", flush=True)
+        assert e < (1e-10 if fp64 else 1e-4), k
+        assert peak_rel_err(b["beam"], obeam) < (1e-10 if fp64 else 1e-4), k
+        # the source finder on the served image is bit-exact against the oracle's
+        img_in = b["image"] if fp64 else b["image"].astype(np.float32).astype(np.float64)
+        lm, isl, _ = orc.source_find(img_in, j["sf"])
+        assert np.array_equal(b["label_map"], lm), k
+        assert [key(i) for i in b["islands"]] == [key(i) for i in isl], k
+        assert [key(i) for i in a["islands"]] == [key(i) for i in b["islands"]], k

@@ -71,6 +71,35 @@ def test_gpu_fft_c2r(gpu, orc, n):
     assert peak_rel_err(out, ref) < 1e-12
 
 
+@pytest.mark.parametrize("n,cols", [(16384, 64), (16384, 512)])
+def test_gpu_fft_c2r_16384(gpu, n, cols):
+    """The 2^14-point c2r along v (the reference's production image size,
+    configs/reduce/fastimg_simple_config.json), on narrow grids."""
+    rng = np.random.default_rng(n + cols)
+    g = rng.normal(size=(n // 2 + 1, cols)) + 1j * rng.normal(size=(n // 2 + 1, cols))
+    out = gpu.fft_c2r(g)
+    ref = (np.fft.irfftn(g.T, s=(cols, n)) * n * cols).T
+    assert peak_rel_err(out, ref) < 1e-12
+
+
+def test_gpu_imager_16384(gpu, orc):
+    """A 16384^2 image with the reference's production imager settings (PSWF support 3,
+    oversampling 8, analytic grid correction, no beam) against the oracle."""
+    snap = simdata.snapshot(16384, 0.5, 300_000, 20, seed=81)
+    img, _, oimg, _ = _imaging_case(orc, gpu, snap, kernel_function=capi.KERNEL_PSWF, kernel_support=3,
+                                    oversampling=8, generate_beam=False, gridding_correction=True, analytic_gcf=True)
+    assert img.shape == (16384, 16384)
+    e = peak_rel_err(img, oimg)
+    print(f"N=16384: image err {e:.2e}", flush=True)
+    assert e < TOL
+    # and the source finder on the 2^28-pixel image, bit-exact against the oracle's
+    sfp = capi.sourcefind_pars(5.0, 3.0, median_method=capi.MEDIAN_BINMEDIAN)
+    sf = gpu.source_find(img, sfp)
+    lm, isl, _ = orc.source_find(img, sfp)
+    assert np.array_equal(sf.label_map, lm)
+    assert len(sf.islands) == len(isl) > 0
+
+
 @pytest.mark.parametrize("n", [4, 16, 64, 256, 1024])
 def test_gpu_fft_c2c(gpu, n):
     rng = np.random.default_rng(n + 1)
@@ -109,6 +138,43 @@ def test_gpu_imager_exact_and_gcf(gpu, orc):
                                            analytic_gcf=False)
     assert img.shape == (128, 128)
     assert peak_rel_err(img, oimg) < TOL and peak_rel_err(beam, obeam) < TOL
+
+
+def test_gpu_imager_pswf_defaults(gpu, orc):
+    """The ImagerPars defaults (types.h:132-158): PSWF support 3, exact kernels,
+    analytic grid correction, and the stp_python wrapper's defaults
+    (stp_python.cpp:244-246) through the Python mirror."""
+    from fastimaging_amd import stp_python
+    snap = simdata.snapshot(256, 4.0, 20_000, 4, seed=79)
+    pars = capi.ImagerPars()
+    capi.load().stp_imager_pars_init(pars, 256, 4.0)
+    assert pars.kernel_function == capi.KERNEL_PSWF and pars.kernel_exact == 1 and pars.gridding_correction == 1
+    pars.generate_beam = 1
+    img, beam = gpu.image_visibilities(snap["vis"], snap["weights"], snap["uvw"], pars)
+    oimg, obeam, _, _ = orc.image_visibilities(pars, snap["vis"], snap["weights"], snap["uvw"])
+    e_i, e_b = peak_rel_err(img, oimg), peak_rel_err(beam, obeam)
+    print(f"PSWF exact + analytic gcf: image err {e_i:.2e}, beam err {e_b:.2e}", flush=True)
+    assert e_i < TOL and e_b < TOL
+    # the Python wrapper's defaults: PSWF exact, gridding correction from the
+    # DFT of the kernel (analytic_gcf false), no beam
+    img2, _ = stp_python.image_visibilities_wrapper(snap["vis"], snap["weights"], snap["uvw"], 256, 4.0)
+    wpars = capi.imager_pars(256, 4.0, kernel_function=capi.KERNEL_PSWF, kernel_support=3, kernel_exact=True,
+                             oversampling=8, generate_beam=False, gridding_correction=True, analytic_gcf=False)
+    oimg2, _, _, _ = orc.image_visibilities(wpars, snap["vis"], snap["weights"], snap["uvw"])
+    assert peak_rel_err(img2, oimg2) < TOL
+
+
+@pytest.mark.parametrize("N,padding", [(512, 1.0), (256, 2.0)])
+def test_gpu_imager_pswf_oversampled_analytic_gcf(gpu, orc, N, padding):
+    """The reference's production imager settings (configs/reduce/fastimg_simple_config.json):
+    PSWF support 3, oversampling 8, gridding correction with the analytic gcf, no beam;
+    with padding 2 the image is cropped from the padded grid."""
+    snap = simdata.snapshot(N, 2.0, 60_000, 8, seed=80)
+    img, _, oimg, _ = _imaging_case(orc, gpu, snap, kernel_function=capi.KERNEL_PSWF, kernel_support=3,
+                                    oversampling=8, generate_beam=False, gridding_correction=True,
+                                    analytic_gcf=True, padding_factor=padding)
+    assert img.shape == (N, N)
+    assert peak_rel_err(img, oimg) < TOL
 
 
 def test_gpu_imager_empty_and_errors(gpu, orc):

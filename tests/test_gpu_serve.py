@@ -54,6 +54,11 @@ def test_serve_alone_vs_mix(gpu, orc):
     jobs = _jobs()
     with Server(num_slots=1, hbm_budget_bytes=64 << 30) as srv:
         alone = [srv.wait(_submit(srv, j)) for j in jobs]
+    # the admission estimate bounds what a job makes the slot hold
+    assert alone[0]["slot_bytes_before"] == 0
+    for a in alone:
+        assert 0 < a["slot_bytes_after"] <= a["slot_bytes_before"] + a["job_bytes"]
+        assert a["slot_bytes_after"] >= a["slot_bytes_before"]
     # one of them against the oracle: the served result is the imager's result
     j0 = jobs[0]
     oimg, obeam, _, _ = orc.image_visibilities(j0["pars"], j0["snap"]["vis"], j0["snap"]["weights"], j0["snap"]["uvw"])
@@ -99,6 +104,12 @@ def test_serve_admission_budget(gpu):
             time.sleep(0.002)
         res = [srv.wait(i) for i in ids]
         assert all(b <= budget for _, b in seen)
+        # resident slot memory is part of the ledger: what the slots keep after the
+        # jobs is reserved, and never more than the budget
+        q, r, b = srv.load()
+        assert r == 0 and 0 < b <= budget
+        for x in res:
+            assert x["slot_bytes_after"] <= x["slot_bytes_before"] + x["job_bytes"]
         assert all(r["image"] is not None for r in res)
         # bigger than the budget: rejected when it reaches a slot
         big = all_jobs[3]

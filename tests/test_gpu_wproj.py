@@ -79,12 +79,31 @@ def test_gpu_wproj_errors(gpu):
     assert e.value.status == capi.STATUS_CODES["STP_ERR_UNSUPPORTED"]
 
 
-def test_gpu_wproj_c3_full_size(gpu, orc):
-    """Workload c3 at its full size: N=4096, cell 2", 2M visibilities with |w| <= 1500,
-    40 sources over the full field, 32 planes, W-kernel support 32, oversampling 8."""
+C3_PARS = dict(kernel_support=3, oversampling=8)
+
+
+@pytest.fixture(scope="module")
+def c3_case(orc):
+    """Workload c3 at its full size and its FP64 oracle image, computed once:
+    N=4096, cell 2", 2M visibilities with |w| <= 1500, 40 sources over the full
+    field, 32 planes, W-kernel support 32, oversampling 8."""
     snap = simdata.workload_snapshot("c3")
     wp = capi.wproj_pars(32, 32, undersampling_opt=1)
-    img, beam, oimg, obeam = _wcase(gpu, orc, snap, 4096, 2.0, wp, kernel_support=3, oversampling=8)
+    pars = capi.imager_pars(4096, 2.0, **C3_PARS)
+    oimg, obeam, ototal, ongood = orc.image_visibilities(pars, snap["vis"], snap["weights"], snap["uvw"], wproj=wp)
+    return snap, wp, pars, oimg, obeam, ototal, ongood
+
+
+@pytest.mark.parametrize("precision,tol", [(capi.PRECISION_FP64, TOL), (capi.PRECISION_FP32C, 1e-4),
+                                           (capi.PRECISION_FP32_TWOFLOAT, 1e-4)])
+def test_gpu_wproj_c3_full_size(gpu, c3_case, precision, tol):
+    """c3 at full size on every precision path against the FP64 oracle: FP64 within
+    1e-10 of the peak, the FP32 forms within 1e-4."""
+    snap, wp, pars, oimg, obeam, ototal, ongood = c3_case
+    img, beam, diag = gpu.image_visibilities(snap["vis"], snap["weights"], snap["uvw"], pars, w_proj=wp,
+                                             precision=precision, return_diag=True)
+    assert diag.num_good_vis == ongood
+    assert abs(diag.sample_grid_total - ototal) <= 1e-13 * ototal
     e_i, e_b = peak_rel_err(img, oimg), peak_rel_err(beam, obeam)
-    print(f"c3: image err {e_i:.3e}, beam err {e_b:.3e}")
-    assert e_i < TOL and e_b < TOL
+    print(f"c3 precision {precision}: image err {e_i:.3e}, beam err {e_b:.3e}", flush=True)
+    assert e_i < tol and e_b < tol
