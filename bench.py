@@ -174,7 +174,8 @@ def bench_batch(args):
     """Workload c4: 48 snapshots (N in {2048, 4096}, 2e4-2e6 visibilities) imaged per
     call of stp_image_batch, FP32 path, autotuned shapes, captured once as a hipGraph
     and replayed; compared with the same batch on the FP64 path run eagerly."""
-    from fastimaging_amd import capi, simdata
+    from fastimaging_amd import capi
+    from tests import simdata
     from fastimaging_amd.stp import Context
     specs = simdata.batch_specs(args.snapshots)
     t0 = time.time()
@@ -249,7 +250,8 @@ def bench_wproj(args):
     paths.  The W gridding is VALU/LDS work: its roofline is FLOP-based (12 flop
     per tap with the beam, ks^2 taps per good visibility pass) against the vector
     peak of the path's arithmetic."""
-    from fastimaging_amd import capi, simdata
+    from fastimaging_amd import capi
+    from tests import simdata
     from fastimaging_amd.stp import Context
     wl = simdata.WORKLOADS["c3"]
     t0 = time.time()
@@ -345,7 +347,8 @@ def bench_serve(args):
     process at ~75 % of the server's measured capacity; reports latency
     percentiles (submit to result, host buffers in and out) and throughput."""
     import threading
-    from fastimaging_amd import capi, simdata
+    from fastimaging_amd import capi
+    from tests import simdata
     from fastimaging_amd.serve import Server
     specs = simdata.serve_specs()
     t0 = time.time()
@@ -422,7 +425,8 @@ def main():
     if args.workload == "c5":
         return bench_serve(args)
 
-    from fastimaging_amd import capi, simdata
+    from fastimaging_amd import capi
+    from tests import simdata
     from fastimaging_amd.stp import Context
 
     wl = simdata.WORKLOADS[args.workload]

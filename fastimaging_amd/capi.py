@@ -265,6 +265,36 @@ def sourcefind_pars(detection_n_sigma, analysis_n_sigma, rms_est=0.0, find_negat
     return p
 
 
+class FitResult(ctypes.Structure):
+    """Least-squares fit of one island (IslandParams::leastsq_fit, bounding_box)."""
+    _fields_ = [("leastsq_fit", Gaussian2d), ("bounding_box", BoundingBox), ("iterations", ctypes.c_int32),
+                ("status", ctypes.c_int32)]
+
+
+def fit_tuple(f):
+    """Plain-Python view of a FitResult record."""
+    g, b = f.leastsq_fit, f.bounding_box
+    return dict(leastsq_fit=(g.amplitude, g.x_centre, g.y_centre, g.semimajor, g.semiminor, g.theta),
+                bounding_box=(b.top, b.bottom, b.left, b.right), iterations=f.iterations, status=f.status)
+
+
+def islands_to_records(islands):
+    """A ctypes Island array from island dicts (island_tuple) or Island records."""
+    islands = list(islands)
+    arr = (Island * max(1, len(islands)))()
+    for k, isl in enumerate(islands):
+        if isinstance(isl, Island):
+            arr[k] = isl
+            continue
+        r = arr[k]
+        r.label_idx, r.sign = isl["label_idx"], isl["sign"]
+        r.extremum_y_idx, r.extremum_x_idx = isl["extremum_y_idx"], isl["extremum_x_idx"]
+        r.num_samples, r.extremum_val = isl["num_samples"], isl["extremum_val"]
+        (r.moments_fit.amplitude, r.moments_fit.x_centre, r.moments_fit.y_centre, r.moments_fit.semimajor,
+         r.moments_fit.semiminor, r.moments_fit.theta) = isl["moments_fit"]
+    return arr
+
+
 def island_tuple(isl):
     """Plain-Python view of an Island record."""
     g = isl.moments_fit
@@ -299,6 +329,9 @@ _SIG = {
                                                      ctypes.c_size_t, _P, _P, _P,
                                                      ctypes.c_int32, _P, _P, ctypes.c_int32,
                                                      ctypes.POINTER(ImagerDiag)]),
+    "stp_fit_islands": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, ctypes.c_size_t, _P]),
+    "stp_fit_islands_device": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, _P,
+                                              ctypes.c_size_t, _P]),
     "stp_convolve_to_grid": (ctypes.c_int, [_P, ctypes.POINTER(KernelSpec), ctypes.c_uint32, ctypes.c_int32,
                                             ctypes.c_size_t, _P, _P, _P, ctypes.c_int32, ctypes.c_uint32,
                                             ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, _P, _D]),

@@ -666,6 +666,29 @@ stp_status stp_source_find_device(stp_ctx* ctx, const void* d_image, int32_t pre
     });
 }
 
+stp_status stp_fit_islands_device(stp_ctx* ctx, const void* d_image, int32_t prec, int32_t rows, int32_t cols,
+    const int32_t* d_label_map, const stp_island* islands, size_t n, stp_fit_result* results)
+{
+    return guarded(ctx, [&] { fit_islands_device(ctx, d_image, prec, rows, cols, d_label_map, islands, n, results); });
+}
+
+stp_status stp_fit_islands(stp_ctx* ctx, const double* image, int32_t rows, int32_t cols, const int32_t* label_map,
+    const stp_island* islands, size_t n, stp_fit_result* results)
+{
+    return guarded(ctx, [&] {
+        STP_REQUIRE(rows >= 2 && cols >= 2, "image must be at least 2 x 2");
+        STP_REQUIRE(image && label_map, "image or label map is NULL");
+        if (n == 0)
+            return;
+        const size_t ne = size_t(rows) * cols;
+        double* d_img = (double*)ws_get(ctx, "host_fit_img", sizeof(double) * ne);
+        int32_t* d_lab = (int32_t*)ws_get(ctx, "host_fit_lab", sizeof(int32_t) * ne);
+        HIP_CHECK(hipMemcpyAsync(d_img, image, sizeof(double) * ne, hipMemcpyHostToDevice, ctx->stream));
+        HIP_CHECK(hipMemcpyAsync(d_lab, label_map, sizeof(int32_t) * ne, hipMemcpyHostToDevice, ctx->stream));
+        fit_islands_device(ctx, d_img, STP_PRECISION_FP64, rows, cols, d_lab, islands, n, results);
+    });
+}
+
 stp_status stp_device_malloc(stp_ctx* ctx, void** ptr, size_t bytes)
 {
     return guarded(ctx, [&] {

@@ -25,8 +25,9 @@ struct OutPtrs {
 template <typename T, typename OT>
 void fft_c2r_2d(stp_ctx* ctx, cplx<T>* grids, int nbatch, int n_rows, int N, const OutPtrs<OT>& outs, const NormSpec& ns);
 
-// 2-D c2c of an N x N column-major matrix; `data` is overwritten, result in `out`.
+// 2-D c2c of nbatch N x N column-major matrices (contiguous); `data` is
+// overwritten, results in `out`.
 template <typename T>
-void fft_c2c_2d(stp_ctx* ctx, cplx<T>* data, cplx<T>* out, int N, bool forward);
+void fft_c2c_2d(stp_ctx* ctx, cplx<T>* data, cplx<T>* out, int N, bool forward, int nbatch = 1);
 
 } // namespace stp_hip

@@ -420,6 +420,9 @@ __global__ __launch_bounds__(WG) void fft_cols_c2c(const cplx<T>* in, int rows, 
 {
     using S = Shape<L>;
     __shared__ cplx<T> buf[S::NT * (L + S::PAD)];
+    // blockIdx.y: array of the batch (rows x N elements each)
+    in += size_t(blockIdx.y) * rows * N;
+    out += size_t(blockIdx.y) * rows * N;
     const int tid = threadIdx.x;
     const int t = tid / S::TPT, lt = tid % S::TPT;
     const int x = blockIdx.x * S::NT + t;
@@ -532,19 +535,20 @@ void fft_c2r_2d(stp_ctx* ctx, cplx<T>* grids, int nbatch, int n_rows, int N, con
 }
 
 template <typename T>
-void fft_c2c_2d(stp_ctx* ctx, cplx<T>* data, cplx<T>* out, int N, bool forward)
+void fft_c2c_2d(stp_ctx* ctx, cplx<T>* data, cplx<T>* out, int N, bool forward, int nbatch)
 {
     STP_REQUIRE(N >= 4 && (N & (N - 1)) == 0, "c2c size must be a power of two >= 4");
+    STP_REQUIRE(nbatch >= 1 && nbatch <= 65535, "1 to 65535 arrays per c2c launch");
     const Twiddles& twn = twiddles_for(ctx, N);
     const T* tw = std::is_same<T, double>::value ? (const T*)twn.d : (const T*)twn.f;
     // rows: transform along columns index (u) for every row, in place (four-step)
-    launch_u_passes<T>(ctx, data, 1, N, N, size_t(N) * N, tw, !forward);
+    launch_u_passes<T>(ctx, data, nbatch, N, N, size_t(N) * N, tw, !forward);
     int L1, L2;
     split_len<T>(N, L1, L2);
     switch (N) {
 #define CCASE(LL)                                                                                           \
     case LL:                                                                                                \
-        launch(ctx, "fft_cols_c2c", fft_cols_c2c<T, LL>, dim3(div_up(N, Shape<LL>::NT)), dim3(WG), 0,      \
+        launch(ctx, "fft_cols_c2c", fft_cols_c2c<T, LL>, dim3(div_up(N, Shape<LL>::NT), nbatch), dim3(WG), 0, \
             (const cplx<T>*)data, N, N, L1, L2, out, tw, int(!forward));                                    \
         break;
         CCASE(4) CCASE(8) CCASE(16) CCASE(32) CCASE(64) CCASE(128) CCASE(256) CCASE(512) CCASE(1024)
@@ -558,6 +562,6 @@ void fft_c2c_2d(stp_ctx* ctx, cplx<T>* data, cplx<T>* out, int N, bool forward)
 template void fft_c2r_2d<double, double>(stp_ctx*, cplx<double>*, int, int, int, const OutPtrs<double>&, const NormSpec&);
 template void fft_c2r_2d<float, float>(stp_ctx*, cplx<float>*, int, int, int, const OutPtrs<float>&, const NormSpec&);
 template void fft_c2r_2d<float, double>(stp_ctx*, cplx<float>*, int, int, int, const OutPtrs<double>&, const NormSpec&);
-template void fft_c2c_2d<double>(stp_ctx*, cplx<double>*, cplx<double>*, int, bool);
+template void fft_c2c_2d<double>(stp_ctx*, cplx<double>*, cplx<double>*, int, bool, int);
 
 } // namespace stp_hip

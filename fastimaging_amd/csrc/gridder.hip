@@ -1107,12 +1107,15 @@ __global__ void wplane_ids_kernel(const uint32_t* first, int P, long long n, uin
 
 // generate_image_domain_convolution_kernel (aw_projection.cpp:50-111): the
 // array x array image-domain kernel, zero outside the +-workarea/2 corners.
-__global__ void wk_fill_kernel(cplx<double>* comb, int arr, int half, const double* aa, const double* w_plane, int p,
+// blockIdx.y: plane p0 + blockIdx.y, written to array blockIdx.y of comb.
+__global__ void wk_fill_kernel(cplx<double>* comb, int arr, int half, const double* aa, const double* w_plane, int p0,
     double scaled_cell)
 {
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     if (e >= (long long)arr * arr)
         return;
+    const int p = p0 + int(blockIdx.y);
+    comb += size_t(blockIdx.y) * arr * arr;
     const int i = int(e % arr), j = int(e / arr);
     // distance of the sample from the origin, folded: i and arr - i map to px = i
     const int pi = (i < half) ? i : ((i > arr - half) ? arr - i : -1);
@@ -1141,22 +1144,52 @@ __global__ void wk_fill_kernel(cplx<double>* comb, int arr, int half, const doub
     comb[e] = v;
 }
 
+// Truncated support of each plane of a batch (aw_projection.cpp:176-179,197-209):
+// the largest s <= support whose sample conv(s * os, 0) exceeds trunc_frac times
+// |conv(0, 0)|.  A plane truncated to 0 (the reference asserts there, gridder.h:584)
+// keeps support 1.
+__global__ void wk_trunc_kernel(const cplx<double>* conv, int arr, int os, int support, double trunc_frac, int nb,
+    int* plane_support)
+{
+    const int b = blockIdx.x * 64 + threadIdx.x;
+    if (b >= nb)
+        return;
+    const cplx<double>* c = conv + size_t(b) * arr * arr;
+    const double min_value = hypot(c[0].x, c[0].y) * trunc_frac;
+    long long i = min((long long)support * os, (long long)(arr >> 1));
+    while (i > 0) {
+        if (hypot(c[i].x, c[i].y) > min_value)
+            break;
+        i -= os;
+    }
+    plane_support[b] = max(1, min(int(i / os), support));
+}
+
 // generate_kernel_cache, non-Hankel branch (aw_projection.cpp:415-443): kernel
 // (cp_x = x, cp_y = y), element (row i, column j) = conv[cxi(x, j) + cyi(y, i) * arr],
-// normalised by the sum of its real parts.  One block per (x, y).
+// normalised by the sum of its real parts.  One block per (x, y); blockIdx.y:
+// array blockIdx.y of conv, written to plane blockIdx.y of cache.  With a truncated
+// plane (plane_support non-NULL) the kernel of support s_p sits centred in the
+// (2 support + 1)^2 slot, normalised over its own taps, zero outside: gridding
+// those zeros adds nothing, as the reference's smaller loop adds nothing.
 template <typename KT>
 __global__ __launch_bounds__(256) void wk_extract_kernel(const cplx<double>* conv, int arr, int os, int support,
-    cplx<KT>* cache)
+    const int* plane_support, cplx<KT>* cache)
 {
     __shared__ double red[256];
+    conv += size_t(blockIdx.y) * arr * arr;
     const int cs = (os / 2) * 2 + 1;
     const int x = blockIdx.x % cs, y = blockIdx.x / cs;
     const int ks = 2 * support + 1;
+    const int sp = plane_support ? plane_support[blockIdx.y] : support;
+    const int lo = support - sp, hi = support + sp;
     const long long ctr = arr / 2, op = os / 2;
     const long long oc = ctr - (long long)support * os + op;
     double sum = 0.0;
     for (int e = threadIdx.x; e < ks * ks; e += 256) {
         const int i = e % ks, j = e / ks;
+        if (i < lo || i > hi || j < lo || j > hi)
+            continue;
         const long long cxi = (oc - x + (long long)j * os + ctr) % arr;
         const long long cyi = (oc - y + (long long)i * os + ctr) % arr;
         sum += conv[cxi + cyi * arr].x;
@@ -1169,9 +1202,13 @@ __global__ __launch_bounds__(256) void wk_extract_kernel(const cplx<double>* con
         __syncthreads();
     }
     const double inv = 1.0 / red[0];
-    cplx<KT>* out = cache + size_t(blockIdx.x) * ks * ks;
+    cplx<KT>* out = cache + (size_t(blockIdx.y) * cs * cs + blockIdx.x) * ks * ks;
     for (int e = threadIdx.x; e < ks * ks; e += 256) {
         const int i = e % ks, j = e / ks;
+        if (i < lo || i > hi || j < lo || j > hi) {
+            out[e] = { KT(0), KT(0) };
+            continue;
+        }
         const long long cxi = (oc - x + (long long)j * os + ctr) % arr;
         const long long cyi = (oc - y + (long long)i * os + ctr) % arr;
         const cplx<double> v = conv[cxi + cyi * arr];
@@ -1418,16 +1455,28 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
             a.analytic_gcf ? 1 : 0);
         const double* d_aa = has_table(ctx, key) ? dev_table(ctx, key, {})
                                                  : dev_table(ctx, key, img_dom_kernel(kf, size_t(workarea), false, a.analytic_gcf));
-        cplx<double>* comb = (cplx<double>*)ws_get(ctx, "w_comb", sizeof(cplx<double>) * size_t(arr) * arr);
-        cplx<double>* conv = (cplx<double>*)ws_get(ctx, "w_conv", sizeof(cplx<double>) * size_t(arr) * arr);
+        // planes in groups of PB: one fill, one batched 2-D c2c and one extraction per
+        // group, the image-domain arrays of a group at most 256 MB each
+        const size_t arr2 = size_t(arr) * arr;
+        const int PB = int(std::max<size_t>(1, std::min<size_t>(size_t(P), (size_t(1) << 28) / (arr2 * sizeof(cplx<double>)))));
+        cplx<double>* comb = (cplx<double>*)ws_get(ctx, "w_comb", sizeof(cplx<double>) * arr2 * PB);
+        cplx<double>* conv = (cplx<double>*)ws_get(ctx, "w_conv", sizeof(cplx<double>) * arr2 * PB);
         const size_t per_plane = size_t(cs) * cs * size_t(g.ks) * g.ks;
         wcache = (cplx<KT>*)ws_get(ctx, "w_cache", sizeof(cplx<KT>) * per_plane * P);
-        for (int p = 0; p < P; p++) {
-            launch(ctx, "w_kernel_fill", wk_fill_kernel, dim3(unsigned(div_up(size_t(arr) * arr, 256))), dim3(256), 0, comb, arr,
-                workarea / 2, d_aa, (const double*)w_plane, p, a.cell_rad * scaling);
-            fft_c2c_2d<double>(ctx, comb, conv, arr, true);
-            launch(ctx, "w_kernel_extract", wk_extract_kernel<KT>, dim3(unsigned(cs * cs)), dim3(256), 0,
-                (const cplx<double>*)conv, arr, os, g.s, wcache + size_t(p) * per_plane);
+        // kernel_trunc_perc > 0: every plane grids with its own truncated support
+        const bool trunc = a.wproj.kernel_trunc_perc > 0.0;
+        int* psup = trunc ? (int*)ws_get(ctx, "w_psupport", sizeof(int) * P) : nullptr;
+        for (int p0 = 0; p0 < P; p0 += PB) {
+            const int nb = std::min(PB, P - p0);
+            launch(ctx, "w_kernel_fill", wk_fill_kernel, dim3(unsigned(div_up(arr2, 256)), unsigned(nb)), dim3(256), 0, comb,
+                arr, workarea / 2, d_aa, (const double*)w_plane, p0, a.cell_rad * scaling);
+            fft_c2c_2d<double>(ctx, comb, conv, arr, true, nb);
+            if (trunc)
+                launch(ctx, "w_kernel_trunc", wk_trunc_kernel, dim3(div_up(nb, 64)), dim3(64), 0, (const cplx<double>*)conv, arr,
+                    os, g.s, a.wproj.kernel_trunc_perc / 100.0, nb, psup + p0);
+            launch(ctx, "w_kernel_extract", wk_extract_kernel<KT>, dim3(unsigned(cs * cs), unsigned(nb)), dim3(256), 0,
+                (const cplx<double>*)conv, arr, os, g.s, (const int*)(trunc ? psup + p0 : nullptr),
+                wcache + size_t(p0) * per_plane);
         }
     }
 

@@ -2096,7 +2096,8 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
     const size_t n = size_t(rows) * size_t(cols);
     STP_REQUIRE(n < (size_t(1) << 30), "image too large for 30-bit label keys");
     if (p.gaussian_fitting)
-        throw Error(STP_ERR_UNSUPPORTED, "Gaussian least-squares fitting is not implemented");
+        throw Error(STP_ERR_UNSUPPORTED, "gaussian_fitting: fit the islands with stp_fit_islands[_device] on this "
+                                         "call's image and label map (the island record has no fit member)");
     STP_REQUIRE(p.median_method >= STP_MEDIAN_ZEROMEDIAN && p.median_method <= STP_MEDIAN_NTHELEMENT, "unknown median method");
 
     // device state of the search: SFState then SelState, cleared by one fill

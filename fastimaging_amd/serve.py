@@ -45,13 +45,17 @@ class Server:
         self.close()
 
     def submit(self, vis, vis_weights, uvw_lambda, img_pars, w_proj=None, kernel=None,
-               precision=capi.PRECISION_FP64, sourcefind=None, label_map=False, island_cap=1 << 16,
+               precision=capi.PRECISION_FP64, sourcefind=None, label_map=False, island_cap=None,
                want_image=True, want_beam=True):
         uvw = np.asfortranarray(uvw_lambda, dtype=np.float64)
         n = uvw.shape[0]
         vis = np.ascontiguousarray(vis, dtype=np.complex128).reshape(n)
         w = np.ascontiguousarray(vis_weights, dtype=np.float64).reshape(n)
         N = img_pars.image_size
+        if island_cap is None:
+            # one island per 256 pixels, at least 65536 (an 8192^2 image of the served
+            # mix has held 92 000 islands)
+            island_cap = max(1 << 16, N * N // 256)
         image = np.zeros((N, N), order="F") if want_image else None
         beam = np.zeros((N, N), order="F") if (want_beam and img_pars.generate_beam) else None
         lab = np.zeros((N, N), dtype=np.int32, order="F") if (label_map and sourcefind is not None) else None

@@ -184,6 +184,31 @@ class Context:
         return SourceFindResult(lm, [capi.island_tuple(isl[i]) for i in range(n.value)], st)
 
     # ------------------------------------------------------- device buffers
+    def fit_islands(self, image, label_map, islands):
+        """IslandParams::leastsq_fit_gaussian_2d (sourcefind.cpp:719-866) for each island
+        of a source find (dicts or capi.Island records) on its image and label map;
+        returns a list of dicts (leastsq_fit, bounding_box, iterations, status)."""
+        img = np.asfortranarray(image, dtype=np.float64)
+        lm = np.asfortranarray(label_map, dtype=np.int32)
+        rows, cols = img.shape
+        islands = list(islands.raw()[:len(islands)] if isinstance(islands, IslandList) else islands)
+        n = len(islands)
+        recs = capi.islands_to_records(islands)
+        out = (capi.FitResult * max(1, n))()
+        self._check(self.lib.stp_fit_islands(self.h, _ptr(img), rows, cols, _ptr(lm), ctypes.cast(recs, _P), n,
+                                             ctypes.cast(out, _P)))
+        return [capi.fit_tuple(out[i]) for i in range(n)]
+
+    def fit_islands_device(self, d_image, rows, cols, d_label_map, islands, precision=0):
+        """The same on a device image (element type by precision) and device label map."""
+        islands = list(islands.raw()[:len(islands)] if isinstance(islands, IslandList) else islands)
+        n = len(islands)
+        recs = capi.islands_to_records(islands)
+        out = (capi.FitResult * max(1, n))()
+        self._check(self.lib.stp_fit_islands_device(self.h, d_image, precision, rows, cols, d_label_map,
+                                                    ctypes.cast(recs, _P), n, ctypes.cast(out, _P)))
+        return [capi.fit_tuple(out[i]) for i in range(n)]
+
     def malloc(self, nbytes):
         p = _P()
         self._check(self.lib.stp_device_malloc(self.h, ctypes.byref(p), nbytes))

@@ -157,13 +157,23 @@ def source_find_wrapper(image_data, detection_n_sigma, analysis_n_sigma, rms_est
         raise ValueError("image_data must be 2-D (stp_python.cpp:158)")
     pars = capi.sourcefind_pars(detection_n_sigma, analysis_n_sigma, rms_est=rms_est,
                                 find_negative_sources=find_negative_sources, sigma_clip_iters=sigma_clip_iters,
-                                median_method=int(median_method), gaussian_fitting=gaussian_fitting,
+                                median_method=int(median_method), gaussian_fitting=False,
                                 ccl_4connectivity=ccl_4connectivity, generate_labelmap=generate_labelmap,
                                 source_min_area=source_min_area)
-    res = _ctx().source_find(np.asfortranarray(img, dtype=np.float64), pars, label_map=True)
+    img64 = np.asfortranarray(img, dtype=np.float64)
+    ctx = _ctx()
+    res = ctx.source_find(img64, pars, label_map=True)
+    # gaussian_fitting: the least-squares fit of every island (sourcefind.cpp:281-289)
+    fits = ctx.fit_islands(img64, res.label_map, res.islands) if gaussian_fitting else None
     out = []
-    for i in res.islands:
+    for k, i in enumerate(res.islands):
         mf = Gaussian2dParams(*i["moments_fit"])
+        lf, report = Gaussian2dParams(), ""
+        if fits is not None:
+            lf = Gaussian2dParams(*fits[k]["leastsq_fit"])
+            report = (f"Levenberg-Marquardt: {fits[k]['iterations']} iterations, "
+                      + ("not solved (fewer than 3 samples)" if fits[k]["status"] == 2 else
+                         "converged" if fits[k]["status"] == 0 else "iteration limit"))
         out.append((i["sign"], i["extremum_val"], i["extremum_x_idx"], i["extremum_y_idx"], i["num_samples"], mf,
-                    Gaussian2dParams(), ""))
+                    lf, report))
     return out

@@ -389,7 +389,7 @@ public:
         p.find_negative_sources = find_negative_sources;
         p.sigma_clip_iters = sigma_clip_iters;
         p.median_method = int32_t(median_method);
-        p.gaussian_fitting = gaussian_fitting;
+        p.gaussian_fitting = 0; // the fit runs below, on this call's image and label map
         p.ccl_4connectivity = ccl_4connectivity;
         p.generate_labelmap = generate_labelmap;
         p.source_min_area = source_min_area;
@@ -421,6 +421,22 @@ public:
             const stp_gaussian2d& g = isl[i].moments_fit;
             ip.moments_fit = { g.amplitude, g.x_centre, g.y_centre, g.semimajor, g.semiminor, g.theta };
             islands.push_back(std::move(ip));
+        }
+        if (gaussian_fitting && n > 0) {
+            // IslandParams::leastsq_fit_gaussian_2d for every island (sourcefind.cpp:281-289)
+            std::vector<stp_fit_result> fits(n);
+            detail::check(stp_fit_islands(detail::ctx(), input_data.memptr(), int32_t(input_data.n_rows),
+                int32_t(input_data.n_cols), label_map.memptr(), isl.data(), n, fits.data()));
+            for (size_t i = 0; i < n; i++) {
+                const stp_gaussian2d& f = fits[i].leastsq_fit;
+                islands[i].leastsq_fit = { f.amplitude, f.x_centre, f.y_centre, f.semimajor, f.semiminor, f.theta };
+                const stp_bounding_box& b = fits[i].bounding_box;
+                islands[i].bounding_box = { b.top, b.bottom, b.left, b.right };
+                islands[i].ceres_report = fits[i].status == 2
+                    ? "not solved (fewer than 3 samples)"
+                    : "Levenberg-Marquardt, " + std::to_string(fits[i].iterations) + " iterations, " +
+                        (fits[i].status == 0 ? "converged" : "iteration limit");
+            }
         }
     }
     Mat<int> label_map;

@@ -30,7 +30,7 @@ extern "C" {
 typedef enum stp_status {
     STP_OK = 0,
     STP_ERR_INVALID_ARGUMENT = 1, /* reference: assert(...) in imager.h:257-287, gridder.h:270-295 */
-    STP_ERR_UNSUPPORTED = 2,      /* feature outside this implementation (A-projection, Hankel, Ceres fit) */
+    STP_ERR_UNSUPPORTED = 2,      /* feature outside this implementation (A-projection, Hankel) */
     STP_ERR_DEVICE = 3,           /* HIP runtime error */
     STP_ERR_OUT_OF_MEMORY = 4,    /* device allocation failed or HBM budget exceeded */
     STP_ERR_INTERNAL = 5,         /* internal consistency check failed */
@@ -146,7 +146,7 @@ typedef struct stp_sourcefind_pars {
     int32_t find_negative_sources;
     int32_t sigma_clip_iters;
     int32_t median_method;       /* stp_median_method */
-    int32_t gaussian_fitting;    /* must be 0 (Ceres fit not implemented) */
+    int32_t gaussian_fitting;    /* must be 0 here; the fit is stp_fit_islands */
     int32_t ccl_4connectivity;
     int32_t generate_labelmap;
     int32_t source_min_area;
@@ -172,6 +172,25 @@ typedef struct stp_island {
     stp_bounding_box bounding_box;
     stp_gaussian2d moments_fit;
 } stp_island;
+
+/* Result of the least-squares Gaussian fit of one island
+ * (IslandParams::leastsq_fit and bounding_box, sourcefind.h:95-133). */
+typedef struct stp_fit_result {
+    stp_gaussian2d leastsq_fit;     /* constrained as convert_to_constrained_parameters */
+    stp_bounding_box bounding_box;  /* visual coordinates (sourcefind.cpp:449-666) */
+    int32_t iterations;             /* linear solves of the minimiser (0: not solved) */
+    int32_t status;                 /* 0 converged, 1 iteration limit, 2 fewer than 3 samples */
+} stp_fit_result;
+
+/* Convergence parameters of the island fit (Levenberg-Marquardt with Marquardt's
+ * diagonal scaling and Nielsen's damping update; see stp_fit_islands). */
+#define STP_FIT_MU0 1.0e-4        /* initial damping */
+#define STP_FIT_MU_MAX 1.0e32     /* damping at which the search stops */
+#define STP_FIT_DIAG_MIN 1.0e-6   /* clamp of the J^T J diagonal used for scaling */
+#define STP_FIT_DIAG_MAX 1.0e32
+#define STP_FIT_FTOL 1.0e-14      /* accepted step: relative decrease of the cost */
+#define STP_FIT_XTOL 1.0e-12      /* accepted step: |step| <= XTOL (|x| + XTOL) */
+#define STP_FIT_MAX_ITERS 200     /* linear solves per island */
 
 /* Scalars the reference exposes as SourceFindImage members (sourcefind.h:146-164). */
 typedef struct stp_sourcefind_stats {
@@ -374,6 +393,41 @@ stp_status stp_source_find_device(stp_ctx* ctx,
     size_t island_capacity,
     size_t* n_islands,
     stp_sourcefind_stats* stats);
+
+/* IslandParams::leastsq_fit_gaussian_2d (src/stp/sourcefind/sourcefind.cpp:719-866),
+ * which SourceFindImage runs for every island when gaussian_fitting is set
+ * (sourcefind.cpp:281-289), with the bounding boxes it then computes
+ * (sourcefind.cpp:449-666).  Call it after stp_source_find[_device] with the same
+ * image, that call's label map and its islands.  Each island's pixels are the
+ * label map entries equal to its label_idx; the residuals are the reference's
+ * model minus data (fitting.cpp:59-120), started from the moments fit
+ * (non-positive axes start at 1, sourcefind.cpp:736-743); islands of fewer than
+ * 3 samples get (extremum_val, 0, 0, 0, 0, 0) and status 2 (sourcefind.cpp:724-728).
+ * The reference minimises with Ceres (LBFGS by default); this library runs
+ * Levenberg-Marquardt on the GPU (one wavefront per island) to the convergence
+ * parameters STP_FIT_*, i.e. to the least-squares minimum itself.  A label map
+ * that disagrees with an island's num_samples fails with STP_ERR_INVALID_ARGUMENT.
+ * results: n_islands records (host).  Synchronous. */
+stp_status stp_fit_islands(stp_ctx* ctx,
+    const double* image,            /* rows x cols, host */
+    int32_t rows,
+    int32_t cols,
+    const int32_t* label_map,       /* rows x cols, host */
+    const stp_island* islands,
+    size_t n_islands,
+    stp_fit_result* results);
+
+/* The same with the image (element type by image_precision) and the label map
+ * in device memory. */
+stp_status stp_fit_islands_device(stp_ctx* ctx,
+    const void* d_image,
+    int32_t image_precision,
+    int32_t rows,
+    int32_t cols,
+    const int32_t* d_label_map,
+    const stp_island* islands,
+    size_t n_islands,
+    stp_fit_result* results);
 
 /* ------------------------------------------------------------------------- */
 /* Device memory helpers (so callers need no HIP headers)                     */

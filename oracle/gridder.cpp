@@ -144,7 +144,8 @@ static cx combine_2d_kernel_value(const std::vector<double>& aa, size_t px, size
 }
 
 std::vector<std::vector<cx>> wproj_kernel_cache(double w_value, const std::vector<double>& aa, unsigned workarea_size,
-    double cell_rad, unsigned oversampling, double scaling_factor, unsigned support, int nthreads)
+    double cell_rad, unsigned oversampling, double scaling_factor, unsigned support, int nthreads, double trunc_perc,
+    unsigned* trunc_support)
 {
     // generate_image_domain_convolution_kernel (aw_projection.cpp:50-111)
     const size_t arr = size_t(workarea_size) * oversampling;
@@ -171,6 +172,22 @@ std::vector<std::vector<cx>> wproj_kernel_cache(double w_value, const std::vecto
     // forward 2-D c2c (aw_projection.cpp:194)
     std::vector<cx> conv(arr * arr);
     fft_c2c_2d(comb.data(), int(arr), int(arr), true, conv.data(), nthreads);
+    // truncation at kernel_trunc_perc % of the peak (aw_projection.cpp:176-179,197-209):
+    // the largest support whose sample conv(s * os, 0) still exceeds that fraction
+    if (trunc_perc > 0.0) {
+        const double min_value = std::abs(conv[0]) * (trunc_perc / 100.0);
+        size_t i = std::min(size_t(support) * oversampling, arr >> 1);
+        while (i > 0) {
+            if (std::abs(conv[i]) > min_value)
+                break;
+            i -= oversampling;
+        }
+        support = std::min(unsigned(i / oversampling), support);
+        if (support == 0)
+            throw std::invalid_argument("W-kernel truncated to support 0 (gridder.h:584 asserts support > 0)");
+    }
+    if (trunc_support)
+        *trunc_support = support;
     // generate_kernel_cache, non-Hankel branch (aw_projection.cpp:415-443)
     const size_t ctr_idx = arr / 2, op = oversampling / 2, cache_size = 2 * op + 1;
     const size_t tcs = 2 * size_t(support) + 1;
@@ -334,7 +351,11 @@ GridderOutput convolve_to_grid(const KernelFunc& kf, unsigned support, int image
             const size_t ve = (pi == num_planes - 1) ? n : first_idx[pi + 1];
             int cs_plane = conv_support;
             if (use_wproj) {
-                cache = wproj_kernel_cache(w_avg[pi], aa, workarea, cell_rad, oversampling, scaling, unsigned(conv_support), nthreads);
+                unsigned ts = 0;
+                cache = wproj_kernel_cache(w_avg[pi], aa, workarea, cell_rad, oversampling, scaling, unsigned(conv_support), nthreads,
+                    wp.kernel_trunc_perc, &ts);
+                // the plane grids with its truncated support (gridder.h:756-758)
+                cs_plane = int(ts);
             }
             const int ks = 2 * cs_plane + 1;
             // column-owner parallelism: per pixel, contributions add in visibility order,

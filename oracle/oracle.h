@@ -95,8 +95,11 @@ GridderOutput convolve_to_grid(const KernelFunc& kf, unsigned support, int image
 
 // W-projection kernel cache for one plane (aw_projection.cpp:50-111,173-210,366-452).
 // Returns cache[(y*cache_size + x)] of (2s+1)^2 complex each, column-major (row = y tap).
+// With trunc_perc > 0 the support is first truncated (aw_projection.cpp:197-209); the
+// support used is returned in *trunc_support (may be NULL).
 std::vector<std::vector<cx>> wproj_kernel_cache(double w_value, const std::vector<double>& aa_kernel_img,
-    unsigned workarea_size, double cell_rad, unsigned oversampling, double scaling_factor, unsigned support, int nthreads);
+    unsigned workarea_size, double cell_rad, unsigned oversampling, double scaling_factor, unsigned support, int nthreads,
+    double trunc_perc = 0.0, unsigned* trunc_support = nullptr);
 
 /* ---------------- imager: src/stp/imager/imager.h ---------------- */
 struct ImagerResult {
@@ -141,6 +144,26 @@ void estimate_moments_fit(stp_island& isl, double x_bar, double y_bar, double xx
 
 /* ---------------- model visibilities: src/stp/visibility/visibility.cpp ---------------- */
 void model_visibilities(const double* skymodel, size_t n_src, const double* uvw, size_t n_vis, cx* out, int nthreads);
+
+/* ---------------- island fit: src/stp/sourcefind/sourcefind.cpp:719-866, fitting.cpp ---------------- */
+constexpr double LM_MU0 = STP_FIT_MU0, LM_MU_MAX = STP_FIT_MU_MAX, LM_DIAG_MIN = STP_FIT_DIAG_MIN,
+                 LM_DIAG_MAX = STP_FIT_DIAG_MAX, LM_FTOL = STP_FIT_FTOL, LM_XTOL = STP_FIT_XTOL;
+constexpr int LM_MAX_ITERS = STP_FIT_MAX_ITERS;
+struct FitResult {
+    stp_gaussian2d fit;   // constrained
+    double half_cost = 0.0;
+    int iterations = 0;
+    int status = 0;
+};
+// Gaussian2dParams::convert_to_constrained_parameters (fitting.cpp:32-57)
+void fit_constrain(stp_gaussian2d& g);
+// Residual model - data at (x, y) and its Jacobian row (fitting.cpp:59-120); p = (A, x0, y0, sx, sy, theta).
+void gaussian_jacobian(const double* p, double x, double y, double v, double* r, double* J);
+// Least-squares fit of m samples from `start` (unconstrained parameters).
+FitResult lm_fit(const double* xs, const double* ys, const double* vs, size_t m, const double* start);
+// Fits of every island of a source find (label map and image: rows x cols, column-major).
+std::vector<stp_fit_result> fit_islands(const double* data, int rows, int cols, const int32_t* labels,
+    const stp_island* islands, size_t n);
 
 int default_threads();
 

@@ -135,17 +135,25 @@ def fft_c2c(a, forward=True, nthreads=None):
     return out
 
 
-def wproj_kernel_cache(w_value, aa, workarea, cell_rad, oversampling, scaling, support, nthreads=None):
+def wproj_kernel_cache(w_value, aa, workarea, cell_rad, oversampling, scaling, support, nthreads=None,
+                       trunc_perc=0.0, return_support=False):
+    """generate_kernel_cache (aw_projection.cpp:366-452) of one plane; with trunc_perc > 0
+    the support is truncated first (aw_projection.cpp:197-209) and the cache is that size."""
     cs = (oversampling // 2) * 2 + 1
     ks = 2 * support + 1
-    out = np.empty((cs, cs, ks, ks), dtype=np.complex128)  # [y][x] then (col-major ks x ks)
+    out = np.empty((cs, cs, ks, ks), dtype=np.complex128)  # [y][x] then (col-major ks x ks), room for the max
     aa = _f64(aa)
+    ts = ctypes.c_uint()
     _check(lib().oracle_wproj_kernel_cache(ctypes.c_double(w_value), _ptr(aa), ctypes.c_uint(workarea),
                                            ctypes.c_double(cell_rad), ctypes.c_uint(oversampling),
                                            ctypes.c_double(scaling), ctypes.c_uint(support),
-                                           ctypes.c_int(nthreads or threads()), _ptr(out)))
-    # out[y, x] holds a ks x ks column-major matrix -> transpose the last two axes
-    return np.ascontiguousarray(np.swapaxes(out, 2, 3))
+                                           ctypes.c_int(nthreads or threads()), ctypes.c_double(trunc_perc),
+                                           ctypes.byref(ts), _ptr(out)))
+    kt = 2 * ts.value + 1
+    flat = out.reshape(-1)[: cs * cs * kt * kt].reshape(cs, cs, kt, kt)
+    # out[y, x] holds a kt x kt column-major matrix -> transpose the last two axes
+    res = np.ascontiguousarray(np.swapaxes(flat, 2, 3))
+    return (res, ts.value) if return_support else res
 
 
 def average_w_planes(abs_w_sorted, good, planes, median=False):
@@ -281,3 +289,44 @@ def model_visibilities(skymodel, uvw, nthreads=None):
                                            ctypes.c_size_t(uvw.shape[0]), _ptr(out),
                                            nthreads if nthreads is not None else threads()))
     return out
+
+
+def gaussian_jacobian(params, x, y, v):
+    """Residual model - data and its Jacobian row (fitting.cpp:59-120); params =
+    (amplitude, x_centre, y_centre, x_stddev, y_stddev, theta)."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    r = ctypes.c_double()
+    J = np.zeros(6)
+    _check(lib().oracle_gaussian_jacobian(_ptr(p), ctypes.c_double(x), ctypes.c_double(y), ctypes.c_double(v),
+                                          ctypes.byref(r), _ptr(J)))
+    return r.value, J
+
+
+def lm_fit(xs, ys, vs, start):
+    """The island fit's minimiser on samples (xs, ys, vs) from `start`; returns
+    (constrained params (6,), half cost, iterations, status)."""
+    xs, ys, vs = (np.ascontiguousarray(a, dtype=np.float64) for a in (xs, ys, vs))
+    st = np.ascontiguousarray(start, dtype=np.float64)
+    out = np.zeros(6)
+    hc = ctypes.c_double()
+    it = ctypes.c_int()
+    status = ctypes.c_int()
+    _check(lib().oracle_lm_fit(_ptr(xs), _ptr(ys), _ptr(vs), ctypes.c_size_t(xs.size), _ptr(st), _ptr(out),
+                               ctypes.byref(hc), ctypes.byref(it), ctypes.byref(status)))
+    return out, hc.value, it.value, status.value
+
+
+def fit_islands(img, label_map, islands):
+    """IslandParams::leastsq_fit_gaussian_2d (sourcefind.cpp:719-866) for each island
+    (a list of capi.island_tuple dicts or capi.Island records); returns a list of
+    dicts (leastsq_fit, bounding_box, iterations, status)."""
+    img = np.asfortranarray(img, dtype=np.float64)
+    lm = np.asfortranarray(label_map, dtype=np.int32)
+    rows, cols = img.shape
+    islands = list(islands)
+    n = len(islands)
+    recs = capi.islands_to_records(islands)
+    out = (capi.FitResult * max(1, n))()
+    _check(lib().oracle_fit_islands(_ptr(img), ctypes.c_int(rows), ctypes.c_int(cols), _ptr(lm), recs,
+                                    ctypes.c_size_t(n), out))
+    return [capi.fit_tuple(out[i]) for i in range(n)]

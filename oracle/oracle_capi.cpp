@@ -119,11 +119,11 @@ int oracle_fft_c2c(const double* in, int rows, int cols, int forward, double* ou
 }
 
 int oracle_wproj_kernel_cache(double w_value, const double* aa, unsigned workarea, double cell_rad, unsigned os,
-    double scaling, unsigned support, int nthreads, double* out)
+    double scaling, unsigned support, int nthreads, double trunc_perc, unsigned* trunc_support, double* out)
 {
     return guard([&] {
         std::vector<double> a(aa, aa + workarea);
-        auto c = wproj_kernel_cache(w_value, a, workarea, cell_rad, os, scaling, support, nthreads);
+        auto c = wproj_kernel_cache(w_value, a, workarea, cell_rad, os, scaling, support, nthreads, trunc_perc, trunc_support);
         size_t off = 0;
         for (auto& k : c) {
             std::memcpy(out + off, k.data(), k.size() * sizeof(cx));
@@ -243,6 +243,34 @@ int oracle_source_find(const double* img, int rows, int cols, const stp_sourcefi
             islands[i] = r.islands[i];
         if (stats)
             *stats = r.stats;
+    });
+}
+
+int oracle_gaussian_jacobian(const double* p, double x, double y, double v, double* r, double* J)
+{
+    return guard([&] { gaussian_jacobian(p, x, y, v, r, J); });
+}
+
+int oracle_lm_fit(const double* xs, const double* ys, const double* vs, size_t m, const double* start, double* fit6,
+    double* half_cost, int* iterations, int* status)
+{
+    return guard([&] {
+        const FitResult f = lm_fit(xs, ys, vs, m, start);
+        const double v[6] = { f.fit.amplitude, f.fit.x_centre, f.fit.y_centre, f.fit.semimajor, f.fit.semiminor, f.fit.theta };
+        std::memcpy(fit6, v, sizeof(v));
+        *half_cost = f.half_cost;
+        *iterations = f.iterations;
+        *status = f.status;
+    });
+}
+
+int oracle_fit_islands(const double* img, int rows, int cols, const int32_t* labels, const stp_island* islands, size_t n,
+    stp_fit_result* out)
+{
+    return guard([&] {
+        const std::vector<stp_fit_result> r = fit_islands(img, rows, cols, labels, islands, n);
+        for (size_t k = 0; k < n; k++)
+            out[k] = r[k];
     });
 }
 

@@ -10,7 +10,8 @@ import subprocess
 import numpy as np
 import pytest
 
-from fastimaging_amd import capi, simdata
+from fastimaging_amd import capi
+from tests import simdata
 from tests.helpers import peak_rel_err
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))

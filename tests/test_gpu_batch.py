@@ -6,7 +6,8 @@ keep the oracle parity."""
 import numpy as np
 import pytest
 
-from fastimaging_amd import capi, simdata
+from fastimaging_amd import capi
+from tests import simdata
 from tests.helpers import peak_rel_err
 
 pytestmark = pytest.mark.gpu

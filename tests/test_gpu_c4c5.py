@@ -15,7 +15,8 @@ import threading
 import numpy as np
 import pytest
 
-from fastimaging_amd import capi, simdata
+from fastimaging_amd import capi
+from tests import simdata
 from fastimaging_amd.serve import Server
 from tests.helpers import peak_rel_err
 
@@ -39,6 +40,8 @@ def c4(gpu, orc):
         snaps.append((pars, nv))
         oracle.append((oimg, obeam))
         bufs.append(b)
+        if k % 8 == 7:
+            print(f"c4 fixture: {k + 1} of 48 oracle images", flush=True)
     print(f"c4: {sum(nv for _, nv in snaps)} visibilities in 48 snapshots", flush=True)
     yield snaps, oracle, bufs
     for b in bufs:
@@ -174,4 +177,11 @@ def test_c5_served_mix(gpu, orc):
         lm, isl, _ = orc.source_find(img_in, j["sf"])
         assert np.array_equal(b["label_map"], lm), k
         assert [key(i) for i in b["islands"]] == [key(i) for i in isl], k
-        assert [key(i) for i in a["islands"]] == [key(i) for i in b["islands"]], k
+        # alone vs in the mix: the images may differ in the last bits (the order of the
+        # gridder's LDS atomic sums is not fixed), so island ids, positions and areas
+        # are compared exactly and extremum values within the image tolerance
+        shape = lambda i: key(i)[:5]
+        assert [shape(i) for i in a["islands"]] == [shape(i) for i in b["islands"]], k
+        peak = float(np.max(np.abs(a["image"])))
+        assert all(abs(x["extremum_val"] - y["extremum_val"]) <= tol_mix * peak
+                   for x, y in zip(a["islands"], b["islands"])), k

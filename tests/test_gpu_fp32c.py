@@ -7,7 +7,8 @@ same FP32 values."""
 import numpy as np
 import pytest
 
-from fastimaging_amd import capi, simdata
+from fastimaging_amd import capi
+from tests import simdata
 from tests.helpers import compare, peak_rel_err
 
 pytestmark = pytest.mark.gpu

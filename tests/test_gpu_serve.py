@@ -9,7 +9,8 @@ import time
 import numpy as np
 import pytest
 
-from fastimaging_amd import capi, simdata
+from fastimaging_amd import capi
+from tests import simdata
 from fastimaging_amd.serve import Server
 from fastimaging_amd.stp import STPError
 from tests.helpers import peak_rel_err

@@ -9,7 +9,8 @@ import os
 import numpy as np
 import pytest
 
-from fastimaging_amd import capi, simdata
+from fastimaging_amd import capi
+from tests import simdata
 from tests.helpers import ISLAND_EXACT, compare, fftshift, fits_agree, gaussian2d, noise
 
 pytestmark = pytest.mark.gpu

@@ -4,7 +4,8 @@ beams within 1e-10 of the peak; the FP32 forms within 1e-4."""
 import numpy as np
 import pytest
 
-from fastimaging_amd import capi, simdata
+from fastimaging_amd import capi
+from tests import simdata
 from fastimaging_amd.stp import STPError
 from tests.helpers import peak_rel_err
 
@@ -33,6 +34,23 @@ def test_gpu_wproj_small(gpu, orc, planes, support, median):
     e_i, e_b = peak_rel_err(img, oimg), peak_rel_err(beam, obeam)
     print(f"wproj planes={planes} support={support} median={median}: image err {e_i:.3e}, beam err {e_b:.3e}")
     assert e_i < TOL and e_b < TOL
+
+
+@pytest.mark.parametrize("trunc_perc,precision,tol", [(1.0, capi.PRECISION_FP64, TOL), (5.0, capi.PRECISION_FP64, TOL),
+                                                       (1.0, capi.PRECISION_FP32C, 1e-4)])
+def test_gpu_wproj_truncated_kernels(gpu, orc, trunc_perc, precision, tol):
+    """kernel_trunc_perc > 0 (the reference's W-projection configs use 1 %): every plane
+    grids with its own truncated support (aw_projection.cpp:197-209, gridder.h:756-758)."""
+    snap = simdata.snapshot(256, 20.0, 5000, 6, seed=331, field_frac=0.5, w_max=600.0)
+    wp = capi.wproj_pars(6, 14, undersampling_opt=1, kernel_trunc_perc=trunc_perc)
+    img, beam, oimg, obeam = _wcase(gpu, orc, snap, 256, 20.0, wp, precision=precision)
+    # truncation changes the image: the untruncated oracle differs by far more than tol
+    full, _, _, _ = orc.image_visibilities(capi.imager_pars(256, 20.0), snap["vis"], snap["weights"], snap["uvw"],
+                                           wproj=capi.wproj_pars(6, 14, undersampling_opt=1))
+    assert peak_rel_err(oimg, full) > 100 * TOL
+    e_i, e_b = peak_rel_err(img, oimg), peak_rel_err(beam, obeam)
+    print(f"wproj truncated {trunc_perc}%: image err {e_i:.3e}, beam err {e_b:.3e}", flush=True)
+    assert e_i < tol and e_b < tol
 
 
 def test_gpu_wproj_conjugate_and_ties(gpu, orc):

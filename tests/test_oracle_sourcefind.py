@@ -8,7 +8,8 @@ import os
 import numpy as np
 import pytest
 
-from fastimaging_amd import capi, simdata
+from fastimaging_amd import capi
+from tests import simdata
 from tests.helpers import fftshift, gaussian2d, noise
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")

@@ -6,7 +6,7 @@ wrong sign or a swapped coordinate would fail.  The reference's own fixture
 available here; its tolerance, 1e-11 absolute, is the bar used below."""
 import numpy as np
 
-from fastimaging_amd import simdata
+from tests import simdata
 
 TOL = 1e-11  # visibility_test_ModelGeneration.cpp:15 (10e-12 absolute)
 

@@ -9,7 +9,8 @@ gridding smears, at the source's own pixel."""
 import numpy as np
 import pytest
 
-from fastimaging_amd import capi, simdata
+from fastimaging_amd import capi
+from tests import simdata
 from tests.helpers import peak_rel_err
 
 
@@ -96,6 +97,35 @@ def test_wkernel_conjugate_symmetry_and_sum(orc):
         for x in range(os_ + 1):
             assert abs(c[y, x].real.sum() - 1.0) < 1e-12
         assert np.max(np.abs(c[y, y] - c[y, y].T)) < 1e-14
+
+
+@pytest.mark.parametrize("w_value,trunc_perc", [(250.0, 1.0), (900.0, 1.0), (50.0, 5.0), (250.0, 0.01)])
+def test_wkernel_truncation(orc, w_value, trunc_perc):
+    """kernel_trunc_perc (aw_projection.cpp:176-179,197-209, :366-452): the plane's support
+    is the largest s whose tap (row S, column S + s) of the centred kernel exceeds
+    trunc_perc % of its centre tap, in magnitude (that tap is the FFT sample conv(s os, 0)
+    the reference tests, scaled by the kernel's real sum), and each truncated kernel is the
+    centred window of the full one renormalised to a unit real sum."""
+    support, os_, workarea = 12, 8, 32
+    aa = _aa(orc, workarea)
+    cell_rad = 20.0 / 3600 * np.pi / 180
+    scaling = 256.0 / workarea
+    full = orc.wproj_kernel_cache(w_value, aa, workarea, cell_rad, os_, scaling, support)
+    trunc, ts = orc.wproj_kernel_cache(w_value, aa, workarea, cell_rad, os_, scaling, support, trunc_perc=trunc_perc,
+                                       return_support=True)
+    op, S = os_ // 2, support
+    centre = full[op, op]
+    mags = np.abs(centre[S, S:])
+    expect = max([s for s in range(1, S + 1) if mags[s] > trunc_perc / 100.0 * mags[0]] + [0])
+    assert ts == expect and 0 < ts <= S
+    if trunc_perc >= 1.0:
+        assert ts < S  # the test cases do truncate
+    assert trunc.shape[2:] == (2 * ts + 1, 2 * ts + 1)
+    for y in range(os_ + 1):
+        for x in range(os_ + 1):
+            win = full[y, x][S - ts:S + ts + 1, S - ts:S + ts + 1]
+            want = win / win.real.sum()
+            assert np.max(np.abs(trunc[y, x] - want)) < 1e-12 * np.max(np.abs(want)), (y, x)
 
 
 def test_wprojection_refocuses_point_source(orc):
