@@ -31,6 +31,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+# LDS atomic-add rate of one MI355X with every lane of 2048 workgroups adding to random
+# addresses of a 64 x 71 tile (G atomics/s), by precision path: FP64 paths ds_add_f64,
+# two-float ds_add_rtn_f32
+LDS_ATOMIC_PEAK_G = {}
 METRIC = "snapshots/s + p50/p99 latency, FP32-comp vs FP64 GPU at <=1e-4 peak err"
 
 
@@ -50,7 +54,12 @@ def parse():
                          "the headline is fp32c when it runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sourcefind", action="store_true")
-    ap.add_argument("--cpu-snapshots", type=int, default=2)
+    ap.add_argument("--cpu-snapshots", type=int, default=5, help="CPU baseline: snapshots timed after one warm-up")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="c1/c2: also run the step on this many contexts at once, one host thread each "
+                         "(snapshots overlap on the device); 1 = off")
+    ap.add_argument("--no-c4", action="store_true", help="c2 run: skip the batched c4 sub-result")
+    ap.add_argument("--no-c5", action="store_true", help="c2 run: skip the served c5 sub-result")
     return ap.parse_args()
 
 
@@ -104,8 +113,9 @@ def measure(ctx, capi, args, pars, sfp, prec, n, N, bufs, do_sf):
     for _ in range(args.warmup):
         step()
     ctx.synchronize()
-    # entries written by the gridder (for the byte counts): estimate from the tile geometry
-    n_entries = int(1.2 * n)
+    # gridding entries written and good visibilities, as the gridder reports them
+    diag = ctx.image_visibilities_device(pars, n, d_vis, d_w, d_uvw, d_img, d_beam, precision=prec, synchronize=True)
+    n_entries = int(diag.num_grid_entries)
 
     # per-kernel breakdown: every launch bracketed by HIP events (untimed pass)
     ctx.set_timing(True)
@@ -147,6 +157,17 @@ def measure(ctx, capi, args, pars, sfp, prec, n, N, bufs, do_sf):
     dom_bytes = kernel_bytes(dominant, N, rows, n, n_entries, 2, S)
     dom_avg_s = dom_ms * 1e-3 / max(dom_cnt, 1)
     achieved = dom_bytes / dom_avg_s / 1e9
+    # LDS atomic adds of the tile kernel: 3 per tap (re, im, beam) and 49 taps per good
+    # visibility (the conjugate copies near v = 0, under 1 %, are not counted)
+    lds_atomics = None
+    if dominant == "grid_tiles":
+        per_launch = 3.0 * 49.0 * float(diag.num_good_vis) * (2 if prec == capi.PRECISION_FP32_TWOFLOAT else 1)
+        rate = per_launch / dom_avg_s / 1e9
+        lds_atomics = {"per_launch": per_launch, "achieved_G_per_s": round(rate, 1),
+                       "op": "ds_add_rtn_f32 (hi) + ds_add_f32 (lo)" if prec == capi.PRECISION_FP32_TWOFLOAT
+                       else "ds_add_f64",
+                       "peak_G_per_s": LDS_ATOMIC_PEAK_G.get(prec),
+                       "frac": (round(rate / LDS_ATOMIC_PEAK_G[prec], 4) if LDS_ATOMIC_PEAK_G.get(prec) else None)}
     ms_per_step = 1000.0 * t_total / args.steps
     # whole-snapshot compulsory bytes (beam on, 6 source-find passes over the image)
     B = 48 * n + 2 * (2 * rows * N * 2 * S) + 2 * N * N * S + (6 * N * N * S + 4 * N * N if do_sf else 0)
@@ -162,6 +183,7 @@ def measure(ctx, capi, args, pars, sfp, prec, n, N, bufs, do_sf):
         "roofline": {"bound": "hbm", "kernel": dominant, "avg_us": round(dom_avg_s * 1e6, 2),
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "lds_atomics": lds_atomics,
         "path_roofline": {"compulsory_bytes": B, "achieved_GBs": round(B / (ms_per_step * 1e-3) / 1e9, 1),
                           "frac": round(B / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
@@ -170,7 +192,62 @@ def measure(ctx, capi, args, pars, sfp, prec, n, N, bufs, do_sf):
     }
 
 
-def bench_batch(args):
+def measure_pipelined(Context, capi, args, pars, sfp, prec, n, N, in_bufs, do_sf, nstreams):
+    """The same step on `nstreams` contexts (one stream each, own outputs), each driven by
+    its own host thread, so one snapshot's source finding overlaps the next one's imaging
+    on the device.  Returns whole-job throughput and per-snapshot latencies."""
+    import threading
+    d_vis, d_w, d_uvw = in_bufs
+    S = 8 if prec == capi.PRECISION_FP64 else 4
+    ctxs, outs, isl_bufs = [], [], []
+    for _ in range(nstreams):
+        c = Context(0)
+        ctxs.append(c)
+        outs.append([c.malloc(S * N * N), c.malloc(S * N * N), c.malloc(4 * N * N)])
+        isl_bufs.append(c.island_buffer(1 << 16))
+
+    def step(k):
+        c, (d_img, d_beam, d_lab) = ctxs[k], outs[k]
+        c.image_visibilities_device(pars, n, d_vis, d_w, d_uvw, d_img, d_beam, precision=prec, synchronize=not do_sf)
+        if do_sf:
+            c.source_find_device(d_img, N, N, sfp, d_label_map=d_lab, precision=prec, out=isl_bufs[k])
+
+    for k in range(nstreams):
+        for _ in range(max(1, args.warmup)):
+            step(k)
+        ctxs[k].synchronize()
+    lat = [[] for _ in range(nstreams)]
+    share = [args.steps // nstreams + (1 if k < args.steps % nstreams else 0) for k in range(nstreams)]
+    go = threading.Barrier(nstreams + 1)
+
+    def worker(k):
+        go.wait()
+        for _ in range(share[k]):
+            s0 = time.perf_counter()
+            step(k)
+            ctxs[k].synchronize()
+            lat[k].append(time.perf_counter() - s0)
+
+    th = [threading.Thread(target=worker, args=(k,)) for k in range(nstreams)]
+    for t in th:
+        t.start()
+    go.wait()
+    t0 = time.perf_counter()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t0
+    for k in range(nstreams):
+        c = ctxs[k]
+        for b in outs[k]:
+            c.free(b)
+        c.free_host(isl_bufs[k])
+        c.close()
+    all_lat = np.array([x for l in lat for x in l])
+    return {"value": args.steps / wall, "ms_per_step": 1000.0 * wall / args.steps, "streams": nstreams,
+            "p50_ms": float(np.percentile(all_lat, 50) * 1000.0), "p99_ms": float(np.percentile(all_lat, 99) * 1000.0)}
+
+
+def bench_batch(args, emit=True):
     """Workload c4: 48 snapshots (N in {2048, 4096}, 2e4-2e6 visibilities) imaged per
     call of stp_image_batch, FP32 path, autotuned shapes, captured once as a hipGraph
     and replayed; compared with the same batch on the FP64 path run eagerly."""
@@ -196,6 +273,29 @@ def bench_batch(args):
         items.append(it)
         bufs.append(b)
     out = {}
+    # the plain FP64 path: one stp_image_visibilities_device call per snapshot, in order,
+    # no batch API, no autotuning, no graph
+    for k, it in enumerate(items):
+        ctx.image_visibilities_device(it.img_pars, it.n_vis, it.d_vis, it.d_vis_weights, it.d_uvw_lambda, it.d_image,
+                                      it.d_beam, precision=capi.PRECISION_FP64, synchronize=(k == len(items) - 1))
+    reps = max(1, args.steps // 10)
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        for it in items:
+            ctx.image_visibilities_device(it.img_pars, it.n_vis, it.d_vis, it.d_vis_weights, it.d_uvw_lambda,
+                                          it.d_image, it.d_beam, precision=capi.PRECISION_FP64)
+        ctx.synchronize()
+    dt = (time.perf_counter() - t1) / reps
+    out["fp64_plain"] = {"snapshots_per_s": round(len(items) / dt, 2), "ms_per_batch": round(dt * 1000, 3),
+                         "batches_timed": reps}
+    ref64 = {}
+    check = sorted(range(len(items)), key=lambda k: (specs[k]["image_size"], specs[k]["n_vis"]))
+    check = check[:3] + check[-3:]
+    for k in check:
+        N = specs[k]["image_size"]
+        img = np.empty((N, N), order="F")
+        ctx.dtoh(img, items[k].d_image)
+        ref64[k] = img
     for name, prec, flags in (("fp32c_graph", capi.PRECISION_FP32C, capi.BATCH_GRAPH | capi.BATCH_AUTOTUNE),
                               ("fp64_eager", capi.PRECISION_FP64, 0)):
         t1 = time.perf_counter()
@@ -210,20 +310,23 @@ def bench_batch(args):
         dt = (time.perf_counter() - t1) / reps
         out[name] = {"snapshots_per_s": round(len(items) / dt, 2), "ms_per_batch": round(dt * 1000, 3),
                      "first_call_s": round(first, 3), "batches_timed": reps}
-    # accuracy of the FP32 batch against the FP64 oracle on the smallest snapshots
+    # accuracy of the FP32 batch: against the plain FP64 GPU images (themselves within
+    # 1e-15 of the oracle in the tests) on the three smallest and three largest
+    # snapshots, and against the FP64 oracle on the smallest
     acc = []
-    if not args.no_cpu_baseline:
-        from oracle import oracle
-        ctx.image_batch(items, precision=capi.PRECISION_FP32C, flags=capi.BATCH_GRAPH)
-        order = sorted(range(len(items)), key=lambda k: specs[k]["image_size"] ** 2 + 50 * specs[k]["n_vis"])[:3]
-        for k in order:
-            N = specs[k]["image_size"]
-            img = np.empty((N, N), order="F", dtype=np.float32)
-            ctx.dtoh(img, items[k].d_image)
+    ctx.image_batch(items, precision=capi.PRECISION_FP32C, flags=capi.BATCH_GRAPH | capi.BATCH_AUTOTUNE)
+    for k in check:
+        N = specs[k]["image_size"]
+        img = np.empty((N, N), order="F", dtype=np.float32)
+        ctx.dtoh(img, items[k].d_image)
+        a = {"snapshot": k, "N": N, "n_vis": specs[k]["n_vis"],
+             "image_err_vs_fp64_gpu": float(np.max(np.abs(img - ref64[k])) / np.max(np.abs(ref64[k])))}
+        if not args.no_cpu_baseline and k == check[0]:
+            from oracle import oracle
             oimg, _, _, _ = oracle.image_visibilities(items[k].img_pars, snaps[k]["vis"], snaps[k]["weights"],
                                                       snaps[k]["uvw"])
-            acc.append({"snapshot": k, "N": N, "n_vis": specs[k]["n_vis"],
-                        "image_err": float(np.max(np.abs(img - oimg)) / np.max(np.abs(oimg)))})
+            a["image_err_vs_oracle"] = float(np.max(np.abs(img - oimg)) / np.max(np.abs(oimg)))
+        acc.append(a)
     res = {"metric": "snapshots/s (48-snapshot batch, hipGraph replay)", "value": out["fp32c_graph"]["snapshots_per_s"],
            "unit": "snapshots/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
            "data": "synthetic (deterministic SplitMix64 snapshots, inputs resident in HBM)",
@@ -231,14 +334,18 @@ def bench_batch(args):
                                   f"{sum(sp['n_vis'] for sp in specs)}), GaussianSinc s3 os9, beam, imaging only",
                       "n_snapshots": len(items)},
            "paths": out,
+           "speedup_fp32c_graph_vs_fp64_plain": round(out["fp32c_graph"]["snapshots_per_s"] /
+                                                      out["fp64_plain"]["snapshots_per_s"], 3),
            "speedup_fp32c_graph_vs_fp64_eager": round(out["fp32c_graph"]["snapshots_per_s"] /
                                                       out["fp64_eager"]["snapshots_per_s"], 3),
-           "accuracy_vs_oracle": acc, "gen_s": round(gen_s, 2)}
-    print(json.dumps(res), flush=True)
+           "accuracy": acc, "gen_s": round(gen_s, 2)}
     for b in bufs:
         for x in b:
             ctx.free(x)
     ctx.close()
+    if emit:
+        print(json.dumps(res), flush=True)
+    return res
 
 
 FP64_VALU_TFLOPS, FP32_VALU_TFLOPS = 78.6, 157.3  # MI355X vector peaks
@@ -341,7 +448,7 @@ def bench_wproj(args):
     print(json.dumps(res), flush=True)
 
 
-def bench_serve(args):
+def bench_serve(args, emit=True):
     """Workload c5: a served stream of imaging + source-finding requests (N in
     {1024..8192}, 1e4-4e6 visibilities, 30 % W-projection) arriving as a Poisson
     process at ~75 % of the server's measured capacity; reports latency
@@ -409,7 +516,9 @@ def bench_serve(args):
                                   f"W-projection (32 planes, support 32), FP32 path, source finding, "
                                   f"{args.slots} slots", "requests": args.requests, "slots": args.slots},
            "gen_s": round(gen_s, 2)}
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -453,6 +562,9 @@ def main():
     res = {}
     for p in paths:
         res[p] = measure(ctx, capi, args, pars, sfp, precs[p], n, N, bufs, do_sf)
+        if args.streams > 1:
+            res[p]["pipelined"] = measure_pipelined(Context, capi, args, pars, sfp, precs[p], n, N, bufs[:3], do_sf,
+                                                    args.streams)
     head_name = "fp32c" if "fp32c" in res else paths[0]
     head = res[head_name]
 
@@ -480,14 +592,19 @@ def main():
                                   "fp32tf": ", FP32 taps with two-float tile sums, FP32 grids and FFT",
                                   "fp64": ", FP64"}[head_name],
                    "image_size": N, "n_vis": n, "path": head_name},
+        "pipelined": _rounded(head.get("pipelined")),
         "roofline": head["roofline"],
+        "lds_atomics": head["lds_atomics"],
         "path_roofline": head["path_roofline"],
         "kernels": head["kernels"],
     }
     for p, r in res.items():
         if p != head_name:
             result[p] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()
-                         if k in ("value", "ms_per_step", "p50_ms", "p99_ms", "roofline", "path_roofline", "kernels")}
+                         if k in ("value", "ms_per_step", "p50_ms", "p99_ms", "roofline", "lds_atomics",
+                                  "path_roofline", "kernels")}
+            if r.get("pipelined"):
+                result[p]["pipelined"] = _rounded(r["pipelined"])
     for p in ("fp32c", "fp32tf"):
         if "fp64" in res and p in res:
             result[f"speedup_{p}_vs_fp64"] = round(res[p]["value"] / res["fp64"]["value"], 3)
@@ -496,18 +613,30 @@ def main():
     if not args.no_cpu_baseline:
         from oracle import oracle
         nthr = oracle.threads()
-        t0 = time.perf_counter()
+        # one warm-up, then the median of k snapshots, imaging and source finding timed
+        # apart (the stages of common_functions.cpp:172-205)
         k = max(1, args.cpu_snapshots)
         oimg = oisl = None
-        for _ in range(k):
+        t_img, t_sf = [], []
+        for r in range(k + 1):
+            t0 = time.perf_counter()
             oimg, _, _, _ = oracle.image_visibilities(pars, snap["vis"], snap["weights"], snap["uvw"])
+            t1 = time.perf_counter()
             if do_sf:
                 _, oisl, _ = oracle.source_find(oimg, sfp)
-        cpu_s = (time.perf_counter() - t0) / k
+            t2 = time.perf_counter()
+            if r:
+                t_img.append(t1 - t0)
+                t_sf.append(t2 - t1)
+        tot = [a + b for a, b in zip(t_img, t_sf)]
+        cpu_s = float(np.median(tot))
         result["cpu_baseline"] = {"value": round(1.0 / cpu_s, 4), "unit": "snapshots/s", "cores": nthr,
-                                  "kind": "port",
-                                  "sample": f"{k} full snapshot(s) of the same workload through the FP64 CPU oracle "
-                                            f"(threaded gridder/FFT, own radix-2 FFT: FFTW absent)"}
+                                  "kind": "port", "cpu": _cpu_model(),
+                                  "stages_ms": {"imager": round(1000 * float(np.median(t_img)), 1),
+                                                "sourcefind": round(1000 * float(np.median(t_sf)), 1)},
+                                  "sample": f"median of {k} full snapshots of the same workload (after one warm-up) "
+                                            f"through the FP64 CPU oracle: threaded gridder and FFT, its own radix-2 "
+                                            f"FFT (no FFTW here), threaded source finder"}
         peak = float(np.max(np.abs(oimg)))
         acc = {}
         for p, r in res.items():
@@ -527,10 +656,32 @@ def main():
             acc[p] = a
         result["accuracy_vs_oracle"] = acc
     result["gen_s"] = round(gen_s, 2)
-    print(json.dumps(result), flush=True)
     for b in bufs:
         ctx.free(b)
     ctx.close()
+    # the batched and served forms of the FP32 path on their own workloads
+    if args.workload == "c2" and not args.no_c4:
+        c4 = bench_batch(args, emit=False)
+        result["c4"] = {k: c4[k] for k in ("value", "unit", "config", "paths", "speedup_fp32c_graph_vs_fp64_plain",
+                                           "speedup_fp32c_graph_vs_fp64_eager", "accuracy", "gen_s")}
+    if args.workload == "c2" and not args.no_c5:
+        c5 = bench_serve(args, emit=False)
+        result["c5"] = {k: c5[k] for k in c5 if k not in ("metric", "n_gpus", "higher_is_better", "dtype", "data")}
+    print(json.dumps(result), flush=True)
+
+
+def _rounded(d):
+    return None if d is None else {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 if __name__ == "__main__":

@@ -586,7 +586,9 @@ stp_status stp_fft_c2r(stp_ctx* ctx, const double* in, int32_t in_rows, int32_t 
         cplx<double>* g = (cplx<double>*)ws_get(ctx, "unit_fft_in", sizeof(cplx<double>) * ge);
         double* o = (double*)ws_get(ctx, "unit_fft_out", sizeof(double) * size_t(n) * n_cols);
         HIP_CHECK(hipMemcpyAsync(g, in, sizeof(cplx<double>) * ge, hipMemcpyHostToDevice, ctx->stream));
-        STP_REQUIRE(n == n_cols, "c2r currently needs a square output (n_rows == n_cols)");
+        // the output is n x n_cols; the last pass writes columns through the imager's
+        // crop mapping with image_size = n, the identity for n_cols <= n
+        STP_REQUIRE(n_cols <= n, "c2r needs n_cols <= n (rows of the real output)");
         NormSpec ns;
         ns.image_size = n;
         OutPtrs<double> outs{};

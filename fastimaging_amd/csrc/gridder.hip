@@ -1234,7 +1234,7 @@ __device__ __forceinline__ double uni(double x)
     return __longlong_as_double((long long)((unsigned long long)(unsigned)hi << 32 | (unsigned)lo));
 }
 
-template <typename T, typename KT, int CPW>
+template <typename T, typename KT, int CPW, int EB>
 __global__ __launch_bounds__(WTW * 64) void grid_wtiles_kernel(GridGeom g, const Entry* entries, const uint32_t* offsets,
     int ostride, uint32_t cap, const cplx<KT>* cache, cplx<T>* vgrid, cplx<T>* bgrid)
 {
@@ -1260,47 +1260,50 @@ __global__ __launch_bounds__(WTW * 64) void grid_wtiles_kernel(GridGeom g, const
         if (int(threadIdx.x) < cnt)
             ebuf[threadIdx.x] = entries[e0 + threadIdx.x];
         __syncthreads();
-        for (int q = 0; q < cnt; q++) {
-            const int gx = uni(ebuf[q].gx), gy = uni(ebuf[q].gy), cp = uni(ebuf[q].cp), pl = uni(ebuf[q].idx);
-            // kernel column of the wave's first column, mod N; the wave's columns run on from it
-            int j0 = wc - (gx - s);
-            j0 = j0 < 0 ? j0 + N : (j0 >= N ? j0 - N : j0);
-            bool any = false;
+        // EB entries at a time: all their kernel values are loaded before any is summed
+        for (int q0 = 0; q0 < cnt; q0 += EB) {
+            cplx<KT> kv[EB][CPW];
+            bool ok[EB][CPW];
+            bool conj[EB];
+            double w[EB], vr[EB], vi[EB];
 #pragma unroll
-            for (int u = 0; u < CPW; u++) {
-                const int j = j0 + u >= N ? j0 + u - N : j0 + u;
-                any = any || (j < ks && wc + u < N);
-            }
-            if (!any)
-                continue;
-            // kernel row of my grid row: direct, or wrapped from >= N (rows < 0 are skipped)
-            int i = r - (gy - s);
-            if (i < 0 || i >= ks)
-                i += N;
-            const bool rowok = i >= 0 && i < ks && r < g.rows;
-            const bool conj = cp < 0;
-            const int cpx = cp & 0x7ff, cpy = (cp >> 16) & 0x7ff;
-            const cplx<KT>* K = cache + (size_t(pl) * cs * cs + size_t(cpy) * cs + cpx) * kk;
-            const double w = uni(ebuf[q].w), vr = uni(ebuf[q].vr), vi = uni(ebuf[q].vi);
-            cplx<KT> kv[CPW];
-#pragma unroll
-            for (int u = 0; u < CPW; u++) {
-                const int j = j0 + u >= N ? j0 + u - N : j0 + u;
-                const bool ok = rowok && j < ks && wc + u < N;
-                kv[u] = ok ? K[size_t(j) * ks + i] : cplx<KT>{ KT(0), KT(0) };
-            }
-            if (rowok) {
+            for (int b = 0; b < EB; b++) {
+                const int q = min(q0 + b, cnt - 1);
+                const bool valid = q0 + b < cnt;
+                const int gx = uni(ebuf[q].gx), gy = uni(ebuf[q].gy), cp = uni(ebuf[q].cp), pl = uni(ebuf[q].idx);
+                // kernel column of the wave's first column, mod N; the wave's columns run on from it
+                int j0 = wc - (gx - s);
+                j0 = j0 < 0 ? j0 + N : (j0 >= N ? j0 - N : j0);
+                // kernel row of my grid row: direct, or wrapped from >= N (rows < 0 are skipped)
+                int i = r - (gy - s);
+                if (i < 0 || i >= ks)
+                    i += N;
+                const bool rowok = valid && i >= 0 && i < ks && r < g.rows;
+                conj[b] = cp < 0;
+                const int cpx = cp & 0x7ff, cpy = (cp >> 16) & 0x7ff;
+                const cplx<KT>* K = cache + (size_t(pl) * cs * cs + size_t(cpy) * cs + cpx) * kk;
+                w[b] = uni(ebuf[q].w);
+                vr[b] = uni(ebuf[q].vr);
+                vi[b] = uni(ebuf[q].vi);
 #pragma unroll
                 for (int u = 0; u < CPW; u++) {
                     const int j = j0 + u >= N ? j0 + u - N : j0 + u;
-                    if (j >= ks || wc + u >= N)
-                        continue;
-                    const double kr = double(kv[u].x) * w;
-                    const double ki = (conj ? -double(kv[u].y) : double(kv[u].y)) * w;
-                    ar[u] += vr * kr - vi * ki;
-                    ai[u] += vr * ki + vi * kr;
-                    br[u] += kr;
-                    bi[u] += ki;
+                    ok[b][u] = rowok && j < ks && wc + u < N;
+                    kv[b][u] = ok[b][u] ? K[size_t(j) * ks + i] : cplx<KT>{ KT(0), KT(0) };
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < EB; b++) {
+#pragma unroll
+                for (int u = 0; u < CPW; u++) {
+                    // per tap (gridder.h:826-861): kval = K w (conjugate kernel for w < 0)
+                    const double kr = double(kv[b][u].x) * w[b];
+                    const double ki = (conj[b] ? -double(kv[b][u].y) : double(kv[b][u].y)) * w[b];
+                    const double tr = vr[b] * kr - vi[b] * ki, ti = vr[b] * ki + vi[b] * kr;
+                    ar[u] += ok[b][u] ? tr : 0.0;
+                    ai[u] += ok[b][u] ? ti : 0.0;
+                    br[u] += ok[b][u] ? kr : 0.0;
+                    bi[u] += ok[b][u] ? ki : 0.0;
                 }
             }
         }
@@ -1512,8 +1515,10 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         ostride = 1;
     }
     if (wproj) {
-        // eight waves split the tile's columns: 2, 4 or 8 columns per wave
-        auto kern = g.tc == 16 ? grid_wtiles_kernel<T, KT, 2> : (g.tc == 32 ? grid_wtiles_kernel<T, KT, 4> : grid_wtiles_kernel<T, KT, 8>);
+        // eight waves split the tile's columns: 2, 4 or 8 columns per wave; 4 entries'
+        // kernel values in flight per lane
+        auto kern = g.tc == 16 ? grid_wtiles_kernel<T, KT, 2, 4>
+                               : (g.tc == 32 ? grid_wtiles_kernel<T, KT, 4, 4> : grid_wtiles_kernel<T, KT, 8, 2>);
         launch(ctx, "grid_wtiles", kern, dim3(ntiles), dim3(WTW * 64), 0, g, (const Entry*)entries, offsets, ostride, cap,
             (const cplx<KT>*)wcache, d_vis_grid, d_sampling_grid);
     } else {

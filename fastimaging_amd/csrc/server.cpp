@@ -94,10 +94,12 @@ extern "C" uint64_t stp_job_device_bytes(const stp_job* j)
     b += stp_hip::GRID_ENTRY_BYTES * entries;   // binned entries
     b += 4 * 2 * ((rows + 63) / 64) * ((N + 31) / 32) * ((n + 4095) / 4096 + 1); // bin counts + offsets
     b += 2 * I * I * S + 4 * I * I;             // image, beam, label map
-    // source-find workspace: parents, tile component records (4.5 B per pixel), segment
-    // masks and offsets, histograms; island sums and list sized for 1 island per 16 pixels;
-    // sigma-clipping candidates (one value per pixel at most) and their lane table
-    b += 10 * I * I + 128 * I * I / 16 + I * I * S + I * I / 4;
+    // source-find workspace per pixel: parents 4 B; segment masks and offsets 0.3 B; one
+    // component record per 16 pixels (tile regions) plus a pool of one per 256, each
+    // with its record (72 B), island sums (64 B), label/keep words (8 B) and island
+    // slot (96 B): 15.9 B; sigma-clipping candidates S B and their lane table 0.25 B.
+    // Fixed: histograms (up to 256 blocks x 8194 bins), kernel tables and twiddles.
+    b += 21 * I * I + S * I * I + (uint64_t(10) << 20);
     if (wproj) {
         const uint64_t os = std::max<uint32_t>(1, (p.oversampling >> 1) << 1);
         const uint64_t cs = (os / 2) * 2 + 1;
@@ -115,7 +117,9 @@ extern "C" uint64_t stp_job_device_bytes(const stp_job* j)
             while (workarea < N)
                 workarea *= 2;
         const uint64_t arr = workarea * os;
-        b += 2 * arr * arr * 16;
+        // the image-domain and FFT arrays of one group of planes (at most 256 MB each)
+        const uint64_t PB = std::max<uint64_t>(1, std::min<uint64_t>(P, (uint64_t(1) << 28) / (arr * arr * 16)));
+        b += 2 * PB * arr * arr * 16 + 4 * P;
     }
     return b;
 }

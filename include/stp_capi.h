@@ -315,7 +315,8 @@ stp_status stp_convolve_to_grid(stp_ctx* ctx,
 /* stp::fft_fftw_c2r(input, output, r_fft)   src/stp/common/fft.cpp:45-135
  * Unnormalised inverse 2-D complex-to-real FFT of a half-plane grid of
  * in_rows = n/2+1 rows and n_cols columns (column-major complex) into an
- * n x n_cols real matrix, n = 2*(in_rows-1). */
+ * n x n_cols real matrix, n = 2*(in_rows-1); n and n_cols powers of two, 4 to
+ * 16384, n_cols <= n. */
 stp_status stp_fft_c2r(stp_ctx* ctx, const double* in, int32_t in_rows, int32_t n_cols, double* out);
 
 /* stp::fft_fftw_c2c(input, output, r_fft, forward)   src/stp/common/fft.cpp:227-323
