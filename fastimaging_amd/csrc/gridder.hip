@@ -1032,8 +1032,8 @@ __device__ unsigned long long lower_bound_u32(const uint32_t* G, unsigned long l
 
 // average_w_planes (gridder.cpp:68-146) over the |w|-sorted good flags: G is their
 // exclusive prefix sum (G[n] = number of good visibilities).  Writes first[0..P]
-// (first[P] = n), the good count of each plane, and for the median variant the
-// plane |w|.
+// (first[P] = n), first[P+1] (where the last plane's count stopped), the good count
+// of each plane, and for the median variant the plane |w|.
 __global__ void wplane_bounds_kernel(const uint32_t* G, long long n, int P, int median, const unsigned long long* skey,
     uint32_t* first, uint32_t* count, double* w_plane)
 {
@@ -1065,17 +1065,24 @@ __global__ void wplane_bounds_kernel(const uint32_t* G, long long n, int P, int 
         }
         begin = k;
     }
+    // the last plane grids every position up to n (gridder.h:532-533); its count
+    // stopped at `begin`
     first[P] = uint32_t(n);
+    first[P + 1] = uint32_t(begin);
 }
 
-// Mean |w| of each plane's good visibilities (one block per plane).
+// Mean |w| of each plane's counted good visibilities (one block per plane): those in
+// [first[p], first[p+1]), except for the last plane, whose count stops at
+// first[P+1] (with rejected visibilities before it, the reference's plane size,
+// computed from a position rather than a good count, stops short of n).
 __global__ __launch_bounds__(256) void wplane_mean_kernel(const unsigned long long* skey, const uint32_t* flag,
     const uint32_t* first, const uint32_t* count, double* w_plane)
 {
     __shared__ double red[256];
     const int p = blockIdx.x;
+    const unsigned long long end = p + 1 == int(gridDim.x) ? first[p + 2] : first[p + 1];
     double sum = 0.0;
-    for (unsigned long long k = first[p] + threadIdx.x; k < first[p + 1]; k += 256)
+    for (unsigned long long k = first[p] + threadIdx.x; k < end; k += 256)
         if (flag[k])
             sum += __longlong_as_double((long long)skey[k]);
     red[threadIdx.x] = sum;
@@ -1426,7 +1433,7 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         // 2. planes
         uint32_t* flag = (uint32_t*)ws_get(ctx, "w_flag", sizeof(uint32_t) * n);
         uint32_t* G = (uint32_t*)ws_get(ctx, "w_gscan", sizeof(uint32_t) * (n + 1));
-        uint32_t* first = (uint32_t*)ws_get(ctx, "w_first", sizeof(uint32_t) * (P + 1));
+        uint32_t* first = (uint32_t*)ws_get(ctx, "w_first", sizeof(uint32_t) * (P + 2));
         uint32_t* pcount = (uint32_t*)ws_get(ctx, "w_pcount", sizeof(uint32_t) * P);
         double* w_plane = (double*)ws_get(ctx, "w_plane", sizeof(double) * P);
         uint16_t* plane_of = (uint16_t*)ws_get(ctx, "w_planeof", sizeof(uint16_t) * n);

@@ -24,11 +24,15 @@ def _wcase(gpu, orc, snap, N, cell, wp, precision=capi.PRECISION_FP64, **kw):
     return img, beam, oimg, obeam
 
 
-@pytest.mark.parametrize("planes,support,median", [(4, 8, False), (7, 8, True), (1, 5, False), (16, 12, False)])
-def test_gpu_wproj_small(gpu, orc, planes, support, median):
-    """N=256, 4000 visibilities with |w| <= 300 over the full field: plane assignment,
-    W-kernel generation and gridding against the oracle."""
-    snap = simdata.snapshot(256, 20.0, 4000, 6, seed=300 + planes, field_frac=0.5, w_max=300.0)
+@pytest.mark.parametrize("planes,support,median,w_max", [(4, 8, False, 300.0), (7, 8, True, 300.0),
+                                                         (1, 5, False, 300.0), (16, 12, False, 300.0),
+                                                         (6, 14, False, 600.0), (5, 15, True, 600.0)])
+def test_gpu_wproj_small(gpu, orc, planes, support, median, w_max):
+    """N=256, 4000 visibilities with |w| <= w_max over the full field: plane assignment,
+    W-kernel generation and gridding against the oracle.  At supports 14 and 15 the
+    bounds check rejects visibilities at the grid edge, so the last plane's count
+    (hence its mean |w|) stops before the end of the list it grids."""
+    snap = simdata.snapshot(256, 20.0, 4000, 6, seed=300 + planes, field_frac=0.5, w_max=w_max)
     wp = capi.wproj_pars(planes, support, undersampling_opt=1, wplanes_median=median)
     img, beam, oimg, obeam = _wcase(gpu, orc, snap, 256, 20.0, wp)
     e_i, e_b = peak_rel_err(img, oimg), peak_rel_err(beam, obeam)
