@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(_HERE, "lib", "libstp_hip.so")
 KERNEL_TOPHAT, KERNEL_TRIANGLE, KERNEL_SINC, KERNEL_GAUSSIAN, KERNEL_GAUSSIANSINC, KERNEL_PSWF = range(6)
 MEDIAN_ZEROMEDIAN, MEDIAN_BINMEDIAN, MEDIAN_BINAPPROX, MEDIAN_NTHELEMENT = range(4)
 PRECISION_FP64, PRECISION_FP32C, PRECISION_FP32_TWOFLOAT = 0, 1, 2
+# size of the device scratch region of stp_image_visibilities_device_noalloc
+NOALLOC_SCRATCH_BYTES = 128 << 20
 
 STATUS_NAMES = {
     0: "STP_OK", 1: "STP_ERR_INVALID_ARGUMENT", 2: "STP_ERR_UNSUPPORTED", 3: "STP_ERR_DEVICE",
@@ -329,6 +331,11 @@ _SIG = {
                                                      ctypes.c_size_t, _P, _P, _P,
                                                      ctypes.c_int32, _P, _P, ctypes.c_int32,
                                                      ctypes.POINTER(ImagerDiag)]),
+    "stp_image_visibilities_device_noalloc": (ctypes.c_int, [_P, ctypes.POINTER(ImagerPars), ctypes.POINTER(WProjPars),
+                                                             ctypes.POINTER(AProjPars), ctypes.POINTER(KernelSpec),
+                                                             ctypes.c_size_t, _P, _P, _P,
+                                                             ctypes.c_int32, _P, _P, ctypes.c_int32,
+                                                             ctypes.POINTER(ImagerDiag)]),
     "stp_fit_islands": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, ctypes.c_size_t, _P]),
     "stp_fit_islands_device": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, _P,
                                               ctypes.c_size_t, _P]),

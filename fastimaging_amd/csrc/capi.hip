@@ -482,6 +482,48 @@ stp_status stp_image_visibilities_device(stp_ctx* ctx, const stp_imager_pars* p,
     });
 }
 
+stp_status stp_image_visibilities_device_noalloc(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp,
+    const stp_aproj_pars* ap, const stp_kernel_spec* kernel, size_t n, const double* d_vis, const double* d_w,
+    const double* d_uvw, int32_t precision, void* d_image, void* d_beam, int32_t synchronize, stp_imager_diag* diag)
+{
+    return guarded(ctx, [&] {
+        check_imager_pars(p, wp, ap);
+        STP_REQUIRE(d_image != nullptr, "image buffer is NULL");
+        STP_REQUIRE(n == 0 || (d_vis && d_w && d_uvw), "visibility buffers are NULL");
+        STP_REQUIRE(precision >= STP_PRECISION_FP64 && precision <= STP_PRECISION_FP32_TWOFLOAT, "unknown precision");
+        char* base = (char*)scratch_region(ctx->device);
+        // the previous call of this context may still read the region and the host
+        // sources of its table uploads
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        Arena& ar = ctx->arena_state;
+        ar.base = base;
+        ar.cap = STP_NOALLOC_SCRATCH_BYTES;
+        ar.used = 0;
+        ar.slots.clear();
+        ar.twiddles.clear();
+        ar.keep_d.clear();
+        ar.keep_f.clear();
+        // launch timing would create events; it stays off for the call
+        struct Scope {
+            stp_ctx* c;
+            bool timing;
+            ~Scope()
+            {
+                c->arena = nullptr;
+                c->timing = timing;
+            }
+        } scope{ ctx, ctx->timing };
+        ctx->timing = false;
+        ctx->arena = &ar;
+        double* d_total = (double*)ws_get(ctx, "imager_total", sizeof(double) * 3);
+        unsigned long long* d_ngood = (unsigned long long*)(d_total + 1);
+        run_imaging(ctx, p, wp, kernel, n, d_vis, d_w, d_uvw, precision, d_image, d_beam, d_total, d_ngood,
+            (unsigned long long*)(d_total + 2));
+        if (synchronize || diag)
+            read_diag(ctx, p, d_total, diag);
+    });
+}
+
 stp_status stp_image_visibilities(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp,
     const stp_aproj_pars* ap, const stp_kernel_spec* kernel, size_t n, const double* vis, const double* w, const double* uvw, int32_t precision,
     double* image, double* beam, stp_imager_diag* diag)

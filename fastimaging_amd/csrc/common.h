@@ -69,6 +69,19 @@ struct Twiddles {
     int n = 0;
 };
 
+// Fixed device region that a no-allocation call carves its workspace, kernel
+// tables and twiddles from (bump allocation, 256-byte aligned), in place of the
+// context's grow-only buffers.  Set for the duration of one call.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    std::map<std::string, DevBuf> slots;       // workspace and table slots of this call
+    std::map<int, Twiddles> twiddles;          // twiddle tables uploaded by this call
+    std::vector<std::vector<double>> keep_d;   // host sources of queued table uploads
+    std::vector<std::vector<float>> keep_f;
+};
+
 } // namespace stp_hip
 
 struct stp_ctx {
@@ -79,6 +92,11 @@ struct stp_ctx {
     std::map<std::string, stp_hip::DevBuf> host_ws; // pinned host staging (hipHostMalloc)
     std::map<int, stp_hip::Twiddles> twiddles;
     std::map<std::string, stp_hip::DevBuf> tables;
+    // non-null while a no-allocation call runs: every buffer comes from it.  The
+    // state outlives the call so that host sources of queued uploads stay valid
+    // until the next no-allocation call has waited for the stream.
+    stp_hip::Arena* arena = nullptr;
+    stp_hip::Arena arena_state;
     bool timing = false;
     std::vector<std::string> timing_filter; // empty: every launch
     // autotuner: chosen launch configuration per imaging shape; mode 0 off,
@@ -138,6 +156,10 @@ struct LaunchTimer {
 };
 
 void resolve_timings(stp_ctx* ctx);
+
+// Device address of the no-allocation scratch region (STP_NOALLOC_SCRATCH_BYTES)
+// on `device`, the current device; resolved once per device.
+void* scratch_region(int device);
 
 template <typename K, typename... Args>
 inline void launch(stp_ctx* ctx, const char* name, K kernel, dim3 grid, dim3 block, size_t shmem, Args... args)

@@ -6,10 +6,29 @@
 
 namespace stp_hip {
 
+// Next 256-byte aligned piece of the no-allocation region.
+static void* arena_carve(Arena* ar, size_t bytes)
+{
+    const size_t off = (ar->used + 255) & ~size_t(255);
+    if (off > ar->cap || bytes > ar->cap - off)
+        throw Error(STP_ERR_CAPACITY, "no-allocation call needs more than the " + std::to_string(ar->cap) +
+                "-byte device scratch region (" + std::to_string(off + bytes) + " bytes so far)");
+    ar->used = off + bytes;
+    return ar->base + off;
+}
+
 void* ws_get(stp_ctx* ctx, const std::string& name, size_t bytes)
 {
     if (bytes == 0)
         bytes = 256;
+    if (ctx->arena) {
+        DevBuf& s = ctx->arena->slots[name];
+        if (s.bytes < bytes) {
+            s.ptr = arena_carve(ctx->arena, bytes);
+            s.bytes = bytes;
+        }
+        return s.ptr;
+    }
     DevBuf& b = ctx->ws[ctx->ws_slot ? name + "#" + std::to_string(ctx->ws_slot) : name];
     if (b.bytes >= bytes)
         return b.ptr;
@@ -77,6 +96,8 @@ void* host_stage(stp_ctx* ctx, const std::string& name, size_t bytes)
 {
     if (bytes == 0)
         bytes = 256;
+    if (ctx->arena)
+        throw Error(STP_ERR_INTERNAL, "host staging requested inside a no-allocation call");
     DevBuf& b = ctx->host_ws[name];
     if (b.bytes >= bytes)
         return b.ptr;
@@ -93,13 +114,33 @@ void* host_stage(stp_ctx* ctx, const std::string& name, size_t bytes)
     return b.ptr;
 }
 
-bool has_table(stp_ctx* ctx, const std::string& key) { return ctx->tables.count(key) != 0; }
+bool has_table(stp_ctx* ctx, const std::string& key)
+{
+    return ctx->tables.count(key) != 0 || (ctx->arena && ctx->arena->slots.count("table:" + key) != 0);
+}
 
 const double* dev_table(stp_ctx* ctx, const std::string& key, const std::vector<double>& host)
 {
     auto it = ctx->tables.find(key);
     if (it != ctx->tables.end())
         return (const double*)it->second.ptr;
+    if (ctx->arena) {
+        // tables the context does not hold yet are uploaded into the region, on the
+        // context stream, for this call only
+        auto at = ctx->arena->slots.find("table:" + key);
+        if (at != ctx->arena->slots.end())
+            return (const double*)at->second.ptr;
+        DevBuf s;
+        s.bytes = std::max<size_t>(8, host.size() * sizeof(double));
+        s.ptr = arena_carve(ctx->arena, s.bytes);
+        if (!host.empty()) {
+            ctx->arena->keep_d.push_back(host);
+            HIP_CHECK(hipMemcpyAsync(s.ptr, ctx->arena->keep_d.back().data(), host.size() * sizeof(double),
+                hipMemcpyHostToDevice, ctx->stream));
+        }
+        ctx->arena->slots["table:" + key] = s;
+        return (const double*)s.ptr;
+    }
     DevBuf b;
     b.bytes = std::max<size_t>(8, host.size() * sizeof(double));
     HIP_CHECK(hipMalloc(&b.ptr, b.bytes));
@@ -132,6 +173,24 @@ const Twiddles& twiddles_for(stp_ctx* ctx, int n)
         return it->second;
     std::vector<double> hd;
     std::vector<float> hf;
+    if (ctx->arena) {
+        Arena* ar = ctx->arena;
+        auto at = ar->twiddles.find(n);
+        if (at != ar->twiddles.end())
+            return at->second;
+        twiddle_tables(n, hd, hf);
+        Twiddles t;
+        t.n = n;
+        t.d = (double*)arena_carve(ar, hd.size() * sizeof(double));
+        t.f = (float*)arena_carve(ar, hf.size() * sizeof(float));
+        ar->keep_d.push_back(std::move(hd));
+        ar->keep_f.push_back(std::move(hf));
+        HIP_CHECK(hipMemcpyAsync(t.d, ar->keep_d.back().data(), ar->keep_d.back().size() * sizeof(double),
+            hipMemcpyHostToDevice, ctx->stream));
+        HIP_CHECK(hipMemcpyAsync(t.f, ar->keep_f.back().data(), ar->keep_f.back().size() * sizeof(float),
+            hipMemcpyHostToDevice, ctx->stream));
+        return ar->twiddles.emplace(n, t).first->second;
+    }
     twiddle_tables(n, hd, hf);
     Twiddles t;
     t.n = n;

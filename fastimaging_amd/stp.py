@@ -237,9 +237,13 @@ class Context:
         self._check(self.lib.stp_memcpy_dtoh(self.h, _ptr(arr), dptr, arr.nbytes))
 
     def image_visibilities_device(self, img_pars, n, d_vis, d_w, d_uvw, d_image, d_beam, precision=0,
-                                  synchronize=False, w_proj=None, kernel=None):
+                                  synchronize=False, w_proj=None, kernel=None, noalloc=False):
+        """Imaging on device buffers.  noalloc=True calls the entry point that
+        takes every intermediate buffer from the library's fixed device scratch
+        region (capi.NOALLOC_SCRATCH_BYTES) and allocates nothing."""
         diag = capi.ImagerDiag()
-        self._check(self.lib.stp_image_visibilities_device(
+        fn = self.lib.stp_image_visibilities_device_noalloc if noalloc else self.lib.stp_image_visibilities_device
+        self._check(fn(
             self.h, ctypes.byref(img_pars), ctypes.byref(w_proj) if w_proj is not None else None, None,
             ctypes.byref(kernel) if kernel is not None else None, n,
             d_vis, d_w, d_uvw, precision, d_image, d_beam, int(synchronize),

@@ -288,6 +288,36 @@ stp_status stp_image_visibilities_device(stp_ctx* ctx,
     int32_t synchronize,
     stp_imager_diag* diag);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same computation, arguments and results as
+ * stp_image_visibilities_device, with every buffer beyond the caller's inputs and
+ * outputs (grids, gridding entries, kernel tables, FFT twiddles, the
+ * sample-grid total) carved from one fixed device region of
+ * STP_NOALLOC_SCRATCH_BYTES per device that the library holds until the process
+ * ends.  The call makes no hipMalloc / hipFree / host-staging allocation.
+ * A call whose buffers do not fit fails with STP_ERR_CAPACITY, leaving the outputs
+ * unspecified (FP64 at N = 4096 needs 269 MB for its two half-plane grids alone and
+ * is rejected).  No autotuning.  The region keeps the last call's contents; every buffer
+ * that must start at zero is cleared by the call.  The call first waits for the
+ * context stream; every call on one device shares the region, so calls that
+ * overlap on one device (from other contexts or threads) race. */
+#define STP_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+stp_status stp_image_visibilities_device_noalloc(stp_ctx* ctx,
+    const stp_imager_pars* img_pars,
+    const stp_wproj_pars* w_proj,
+    const stp_aproj_pars* a_proj,
+    const stp_kernel_spec* kernel,
+    size_t n_vis,
+    const double* d_vis,
+    const double* d_vis_weights,
+    const double* d_uvw_lambda,
+    int32_t precision,
+    void* d_image,
+    void* d_beam,
+    int32_t synchronize,
+    stp_imager_diag* diag);
+
 /* stp::convolve_to_grid<generateBeam>(kernel_creator, support, image_size, uv_lambda,
  *     vis, vis_weights, kernel_exact, oversampling, shift_uv, halfplane_gridding)
  *   src/stp/gridder/gridder.h:240-997 (no W-projection in this entry point).
