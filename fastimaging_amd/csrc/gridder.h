@@ -21,6 +21,7 @@ struct GridderArgs {
     double cell_rad = 0.0;  // cell size in radians (W-kernel image-domain sampling)
     bool analytic_gcf = false; // image-domain AA kernel of the W-kernels from gcf() (conv_func.h:351-412)
     int tile_cols = 0;      // tile width (16, 32 or 64 grid columns); 0 = default
+    int w_phases = 0;       // W-projection tile launches (phases); 0 = about 6 MB of W-kernel cache each
     unsigned long long* d_nentries = nullptr; // device u64: receives the gridding entries written (may be NULL)
 };
 
@@ -32,6 +33,10 @@ struct GridderArgs {
 template <typename T>
 void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const double* d_uv, const double* d_vis,
     const double* d_w, cplx<T>* d_vis_grid, cplx<T>* d_sampling_grid, double* d_total, unsigned long long* d_ngood);
+
+// W-projection: each W sub-kernel is cached with this many zero columns on both
+// sides (ks rows x (ks + 2 W_KERNEL_PAD_COLS) columns).
+constexpr int W_KERNEL_PAD_COLS = 7;
 
 // Entries (40 bytes each) the gridder's binning reserves for n visibilities: the
 // most tiles one footprint can touch on this grid, times two passes, times n.

@@ -48,8 +48,8 @@ struct GridGeom {
 struct Entry {
     int gx, gy;   // kernel centre on the (shifted) grid for this pass
     int cp;       // cp_x | cp_y << 16 (oversampled kernel indices); W-projection: bit 31 = conjugate kernel
-    int idx;      // visibility index << 1 | pass; W-projection: the w-plane
-    double vr, vi, w;
+    int idx;      // visibility index << 1 | pass; W-projection: the W sub-kernel (plane, cp_y, cp_x)
+    double vr, vi, w; // W-projection: vr, vi are the visibility times w
 };
 static_assert(sizeof(Entry) == 40, "entry layout");
 
@@ -164,7 +164,12 @@ __device__ __forceinline__ Entry make_entry(const GridGeom& g, const Prep& p, in
         const double wl = pass ? -p.wl : p.wl;
         if (wl < 0.0)
             e.cp |= int(0x80000000u);
-        e.idx = int(g.plane[k]);
+        // the W tile kernel takes the sub-kernel index (plane, cp_y, cp_x) and the
+        // visibility times its weight
+        const int cs = (g.os / 2) * 2 + 1;
+        e.idx = (int(g.plane[k]) * cs + cpy) * cs + cpx;
+        e.vr = e.vr * p.w;
+        e.vi = e.vi * p.w;
     } else {
         e.idx = int((k << 1) | pass);
     }
@@ -1179,6 +1184,11 @@ __global__ void wk_trunc_kernel(const cplx<double>* conv, int arr, int os, int s
 // plane (plane_support non-NULL) the kernel of support s_p sits centred in the
 // (2 support + 1)^2 slot, normalised over its own taps, zero outside: gridding
 // those zeros adds nothing, as the reference's smaller loop adds nothing.
+// Each sub-kernel is stored column-major with WPAD zero columns on both sides
+// (ks rows x (ks + 2 WPAD) columns), so that the W tile kernel's column runs
+// starting up to WPAD columns outside the kernel read zeros.
+constexpr int WPAD = W_KERNEL_PAD_COLS;
+
 template <typename KT>
 __global__ __launch_bounds__(256) void wk_extract_kernel(const cplx<double>* conv, int arr, int os, int support,
     const int* plane_support, cplx<KT>* cache)
@@ -1209,10 +1219,11 @@ __global__ __launch_bounds__(256) void wk_extract_kernel(const cplx<double>* con
         __syncthreads();
     }
     const double inv = 1.0 / red[0];
-    cplx<KT>* out = cache + (size_t(blockIdx.y) * cs * cs + blockIdx.x) * ks * ks;
-    for (int e = threadIdx.x; e < ks * ks; e += 256) {
-        const int i = e % ks, j = e / ks;
-        if (i < lo || i > hi || j < lo || j > hi) {
+    const int kw = ks + 2 * WPAD;
+    cplx<KT>* out = cache + (size_t(blockIdx.y) * cs * cs + blockIdx.x) * ks * kw;
+    for (int e = threadIdx.x; e < ks * kw; e += 256) {
+        const int i = e % ks, j = e / ks - WPAD;
+        if (j < 0 || j >= ks || i < lo || i > hi || j < lo || j > hi) {
             out[e] = { KT(0), KT(0) };
             continue;
         }
@@ -1226,12 +1237,28 @@ __global__ __launch_bounds__(256) void wk_extract_kernel(const cplx<double>* con
 // W-projection tile accumulation without atomics.  The tile's 64 rows are the
 // lanes and its columns are split among the workgroup's eight waves (CPW columns
 // each), so every grid cell belongs to exactly one lane, which keeps the cell's
-// sums in registers.  Every wave walks all of the tile's entries (staged in LDS)
-// and adds the taps of its own columns.  Per tap (gridder.h:826-861): kval = K * w
-// with K the plane's complex kernel (conjugated for w < 0), grid += vis * kval,
-// beam += kval, summed in FP64 on every grid precision.  The tile is written once.
+// sums in registers.  Every wave walks the tile's entries (staged in LDS) and adds
+// the taps of its own columns.  Per tap (gridder.h:826-861): kval = K * w with K
+// the plane's complex kernel (conjugated for w < 0), grid += vis * kval,
+// beam += kval, summed in FP64 on every grid precision.
+//
+// Phases: the binning blocks hold the visibilities in |w| order, so a run of
+// blocks covers about one w-plane.  One launch per run of blocks (a phase) keeps
+// the W-kernels that all tiles read during the launch to about one plane's
+// cache, which stays in L2, instead of every plane at once.  A phase takes tile
+// t's entries of blocks [b0, b1) (the block cursors of the binning give their
+// range) and continues the FP64 running sums of the phase before (acc), in the
+// same order as one launch over all entries.
+struct WPhase {
+    const uint32_t* bcur = nullptr; // bcur[b * ntiles + t]: entries of tile t in blocks < b (null: whole tile)
+    const uint32_t* btot = nullptr; // entries of tile t
+    int b0 = 0, b1 = 0, nblk = 0;
+    int first = 1, last = 1;        // first: sums start at zero; last: write the grids
+    cplx<double>* acc_v = nullptr;  // FP64 running sums between phases (rows x N, grid layout)
+    cplx<double>* acc_b = nullptr;
+};
 constexpr int WTW = 8;   // waves per W tile workgroup
-constexpr int WEB = 256; // entries staged per batch
+constexpr double W_PHASE_BYTES = 6.0 * (1 << 20); // W-kernel cache read per phase (measured best on c3)
 
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ double uni(double x)
@@ -1241,89 +1268,176 @@ __device__ __forceinline__ double uni(double x)
     return __longlong_as_double((long long)((unsigned long long)(unsigned)hi << 32 | (unsigned)lo));
 }
 
-template <typename T, typename KT, int CPW, int EB>
-__global__ __launch_bounds__(WTW * 64) void grid_wtiles_kernel(GridGeom g, const Entry* entries, const uint32_t* offsets,
-    int ostride, uint32_t cap, const cplx<KT>* cache, cplx<T>* vgrid, cplx<T>* bgrid)
+constexpr uint32_t W_OOB = 0x80000000u;
+constexpr int BUF_DWORD3 = 0x00020000; // raw (untyped) buffer access
+
+// One W-kernel value per lane through a range-checked buffer resource over the
+// entry's padded sub-kernel (scalar base, scalar column offset): lanes whose row
+// is outside [0, ks) carry offset W_OOB and read zero without a memory access.
+template <typename KT>
+__device__ __forceinline__ cplx<KT> load_kval(const cplx<KT>* sub, int bytes, uint32_t voff, int soff)
 {
-#pragma clang fp contract(off)
-    __shared__ Entry ebuf[WEB];
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)sub, (short)0, bytes, BUF_DWORD3);
+    if constexpr (std::is_same<KT, double>::value) {
+        typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+        const u4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
+        return { __longlong_as_double((long long)((unsigned long long)v.y << 32 | v.x)),
+            __longlong_as_double((long long)((unsigned long long)v.w << 32 | v.z)) };
+    } else {
+        typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+        const u2 v = __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0);
+        return { __uint_as_float(v.x), __uint_as_float(v.y) };
+    }
+}
+
+__device__ __forceinline__ double flip_if(double x, bool neg)
+{
+    return neg ? -x : x;
+}
+
+template <typename T, typename KT, int CPW, int EB>
+__global__ __launch_bounds__(WTW * 64) void grid_wtiles_kernel(GridGeom g, const Entry* __restrict__ entries,
+    const uint32_t* __restrict__ offsets, int ostride, uint32_t cap, const cplx<KT>* __restrict__ cache,
+    cplx<T>* __restrict__ vgrid, cplx<T>* __restrict__ bgrid, WPhase ph)
+{
     const int ks = g.ks, s = g.s, N = g.N;
-    const int cs = (g.os / 2) * 2 + 1;
-    const size_t kk = size_t(ks) * ks;
     const int t = blockIdx.x;
     const int rt = t % g.tiles_r, ct = t / g.tiles_r;
     const int r0 = rt * TILE_R, c0 = ct * g.tc;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = uni(int(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     const int r = r0 + lane;
-    const int wc = c0 + wave * CPW; // first grid column of this wave
+    const int wc = c0 + wave * CPW; // first grid column of this wave (uniform: the loads' buffer resources are scalar)
+    uint32_t eb = offsets[size_t(t) * ostride], ee = offsets[size_t(t + 1) * ostride];
+    if (ph.bcur) {
+        const size_t nt = size_t(g.tiles_r) * g.tiles_c;
+        const uint32_t base = eb;
+        eb = base + (ph.b0 < ph.nblk ? ph.bcur[size_t(ph.b0) * nt + t] : ph.btot[t]);
+        ee = base + (ph.b1 < ph.nblk ? ph.bcur[size_t(ph.b1) * nt + t] : ph.btot[t]);
+    }
+    eb = uni(int(min(eb, cap)));
+    ee = uni(int(min(ee, cap)));
+    // FP32 grids take the sums at the last phase; FP64 grids are their own running sums
+    constexpr bool narrow = !std::is_same<T, double>::value;
+    if (!ph.first && !(narrow && ph.last) && eb >= ee)
+        return; // no entry of this tile in the phase: its sums stay as they are
+    const bool cell = r < g.rows;
     double ar[CPW], ai[CPW], br[CPW], bi[CPW];
 #pragma unroll
-    for (int u = 0; u < CPW; u++)
+    for (int u = 0; u < CPW; u++) {
         ar[u] = ai[u] = br[u] = bi[u] = 0.0;
-    const uint32_t eb = min(offsets[size_t(t) * ostride], cap), ee = min(offsets[size_t(t + 1) * ostride], cap);
-    for (uint32_t e0 = eb; e0 < ee; e0 += WEB) {
-        const int cnt = int(min(uint32_t(WEB), ee - e0));
-        __syncthreads();
-        if (int(threadIdx.x) < cnt)
-            ebuf[threadIdx.x] = entries[e0 + threadIdx.x];
-        __syncthreads();
-        // EB entries at a time: all their kernel values are loaded before any is summed
-        for (int q0 = 0; q0 < cnt; q0 += EB) {
-            cplx<KT> kv[EB][CPW];
-            bool ok[EB][CPW];
-            bool conj[EB];
-            double w[EB], vr[EB], vi[EB];
-#pragma unroll
-            for (int b = 0; b < EB; b++) {
-                const int q = min(q0 + b, cnt - 1);
-                const bool valid = q0 + b < cnt;
-                const int gx = uni(ebuf[q].gx), gy = uni(ebuf[q].gy), cp = uni(ebuf[q].cp), pl = uni(ebuf[q].idx);
-                // kernel column of the wave's first column, mod N; the wave's columns run on from it
-                int j0 = wc - (gx - s);
-                j0 = j0 < 0 ? j0 + N : (j0 >= N ? j0 - N : j0);
-                // kernel row of my grid row: direct, or wrapped from >= N (rows < 0 are skipped)
-                int i = r - (gy - s);
-                if (i < 0 || i >= ks)
-                    i += N;
-                const bool rowok = valid && i >= 0 && i < ks && r < g.rows;
-                conj[b] = cp < 0;
-                const int cpx = cp & 0x7ff, cpy = (cp >> 16) & 0x7ff;
-                const cplx<KT>* K = cache + (size_t(pl) * cs * cs + size_t(cpy) * cs + cpx) * kk;
-                w[b] = uni(ebuf[q].w);
-                vr[b] = uni(ebuf[q].vr);
-                vi[b] = uni(ebuf[q].vi);
-#pragma unroll
-                for (int u = 0; u < CPW; u++) {
-                    const int j = j0 + u >= N ? j0 + u - N : j0 + u;
-                    ok[b][u] = rowok && j < ks && wc + u < N;
-                    kv[b][u] = ok[b][u] ? K[size_t(j) * ks + i] : cplx<KT>{ KT(0), KT(0) };
-                }
-            }
-#pragma unroll
-            for (int b = 0; b < EB; b++) {
-#pragma unroll
-                for (int u = 0; u < CPW; u++) {
-                    // per tap (gridder.h:826-861): kval = K w (conjugate kernel for w < 0)
-                    const double kr = double(kv[b][u].x) * w[b];
-                    const double ki = (conj[b] ? -double(kv[b][u].y) : double(kv[b][u].y)) * w[b];
-                    const double tr = vr[b] * kr - vi[b] * ki, ti = vr[b] * ki + vi[b] * kr;
-                    ar[u] += ok[b][u] ? tr : 0.0;
-                    ai[u] += ok[b][u] ? ti : 0.0;
-                    br[u] += ok[b][u] ? kr : 0.0;
-                    bi[u] += ok[b][u] ? ki : 0.0;
-                }
+        const int c = wc + u;
+        if (!ph.first && cell && c < N && c < c0 + g.tc) {
+            const size_t o = size_t(r) + size_t(c) * g.rows;
+            const cplx<double> v = ph.acc_v[o];
+            ar[u] = v.x;
+            ai[u] = v.y;
+            if (g.beam) {
+                const cplx<double> b = ph.acc_b[o];
+                br[u] = b.x;
+                bi[u] = b.y;
             }
         }
     }
-    if (r < g.rows) {
+    static_assert(CPW - 1 <= WPAD, "column runs must stay within the zero padding");
+    const int kw = ks + 2 * WPAD;
+    const int sub_bytes = int(size_t(ks) * kw * sizeof(cplx<KT>));
+    const int col_bytes = int(ks * sizeof(cplx<KT>));
+    // EB entries at a time (scalar loads; every value uniform): all their kernel
+    // values are loaded before any is summed
+    for (uint32_t e0 = eb; e0 < ee; e0 += EB) {
+        cplx<KT> kv[EB][CPW]; // in the cache precision until summed
+        double ca[EB], cb[EB], cw[EB];
+        bool conj[EB], hit[EB];
+        // the batch's entries: their scalar loads go out together (the empty asm
+        // takes every field in scalar registers, so one wait covers the batch)
+        Entry ent[EB];
+#pragma unroll
+        for (int b = 0; b < EB; b++)
+            ent[b] = entries[min(e0 + b, ee - 1)];
+#pragma unroll
+        for (int b = 0; b < EB; b++)
+            asm volatile("" ::"s"(ent[b].gx), "s"(ent[b].gy), "s"(ent[b].cp), "s"(ent[b].idx), "s"(ent[b].vr),
+                "s"(ent[b].vi), "s"(ent[b].w));
+#pragma unroll
+        for (int b = 0; b < EB; b++) {
+            const Entry& en = ent[b];
+            conj[b] = en.cp < 0;
+            ca[b] = en.vr;
+            cb[b] = en.vi;
+            cw[b] = en.w;
+            // kernel column of the wave's first column, mod N; the wave's columns run
+            // on from it and wrap past N
+            int j0 = wc - (en.gx - s);
+            j0 = j0 < 0 ? j0 + N : (j0 >= N ? j0 - N : j0);
+            const bool wrap = j0 > N - CPW;
+            // byte offset of each column in the padded sub-kernel: the run
+            // j0 - (wrap ? N : 0) + u (columns outside [0, ks) are padding zeros),
+            // or, when the run wraps inside the kernel (grids narrower than
+            // ks + CPW - 1), column by column with the first padding column for
+            // columns outside
+            int soff[CPW];
+            bool any;
+            if (!(wrap && j0 < ks)) {
+                const int jv = wrap ? j0 - N : j0;
+                any = jv < ks && jv + CPW > 0;
+#pragma unroll
+                for (int u = 0; u < CPW; u++)
+                    soff[u] = (jv + u + WPAD) * col_bytes;
+            } else {
+                any = true;
+#pragma unroll
+                for (int u = 0; u < CPW; u++) {
+                    const int j = j0 + u >= N ? j0 + u - N : j0 + u;
+                    soff[u] = (j < ks ? j + WPAD : 0) * col_bytes;
+                }
+            }
+            hit[b] = e0 + b < ee && any;
+            if (!hit[b])
+                continue;
+            // kernel row of my grid row: direct, or wrapped from >= N (rows < 0 are skipped)
+            int i = r - (en.gy - s);
+            if (i < 0)
+                i += N;
+            const uint32_t voff = (i < ks && cell) ? uint32_t(i) * uint32_t(sizeof(cplx<KT>)) : W_OOB;
+            const cplx<KT>* K = cache + size_t(uint32_t(en.idx)) * (size_t(ks) * kw);
+#pragma unroll
+            for (int u = 0; u < CPW; u++)
+                kv[b][u] = load_kval<KT>(K, sub_bytes, voff, soff[u]);
+        }
+#pragma unroll
+        for (int b = 0; b < EB; b++) {
+            if (!hit[b])
+                continue;
+            // per tap (gridder.h:826-861): kval = K w, conjugated for w < 0; grid += vis kval,
+            // beam += kval, as FMAs with the entry's coefficients (vis w, w) and their signs
+            const double sa = flip_if(ca[b], conj[b]), sb = flip_if(cb[b], !conj[b]), sw = flip_if(cw[b], conj[b]);
+#pragma unroll
+            for (int u = 0; u < CPW; u++) {
+                const double kr = double(kv[b][u].x), ki = double(kv[b][u].y);
+                ar[u] = fma(ca[b], kr, ar[u]);
+                ar[u] = fma(sb, ki, ar[u]);
+                ai[u] = fma(cb[b], kr, ai[u]);
+                ai[u] = fma(sa, ki, ai[u]);
+                br[u] = fma(cw[b], kr, br[u]);
+                bi[u] = fma(sw, ki, bi[u]);
+            }
+        }
+    }
+    if (cell) {
 #pragma unroll
         for (int u = 0; u < CPW; u++) {
             const int c = wc + u;
             if (c < N && c < c0 + g.tc) {
                 const size_t o = size_t(r) + size_t(c) * g.rows;
-                vgrid[o] = { T(ar[u]), T(ai[u]) };
-                if (g.beam)
-                    bgrid[o] = { T(br[u]), T(bi[u]) };
+                if (narrow && !ph.last) {
+                    ph.acc_v[o] = { ar[u], ai[u] };
+                    if (g.beam)
+                        ph.acc_b[o] = { br[u], bi[u] };
+                } else {
+                    vgrid[o] = { T(ar[u]), T(ai[u]) };
+                    if (g.beam)
+                        bgrid[o] = { T(br[u]), T(bi[u]) };
+                }
             }
         }
     }
@@ -1471,7 +1585,8 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         const int PB = int(std::max<size_t>(1, std::min<size_t>(size_t(P), (size_t(1) << 28) / (arr2 * sizeof(cplx<double>)))));
         cplx<double>* comb = (cplx<double>*)ws_get(ctx, "w_comb", sizeof(cplx<double>) * arr2 * PB);
         cplx<double>* conv = (cplx<double>*)ws_get(ctx, "w_conv", sizeof(cplx<double>) * arr2 * PB);
-        const size_t per_plane = size_t(cs) * cs * size_t(g.ks) * g.ks;
+        STP_REQUIRE(size_t(P) * cs * cs < (size_t(1) << 31), "too many W sub-kernels (planes x oversampling^2)");
+        const size_t per_plane = size_t(cs) * cs * size_t(g.ks) * (g.ks + 2 * WPAD);
         wcache = (cplx<KT>*)ws_get(ctx, "w_cache", sizeof(cplx<KT>) * per_plane * P);
         // kernel_trunc_perc > 0: every plane grids with its own truncated support
         const bool trunc = a.wproj.kernel_trunc_perc > 0.0;
@@ -1492,6 +1607,8 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
 
     const uint32_t* offsets;
     int ostride;
+    const uint32_t* bcur = nullptr; // block cursors inside each tile (block binning only)
+    const uint32_t* btot = nullptr;
     if (ntiles <= MAX_LDS_TILES) {
         const size_t m = size_t(ntiles) * nblk;
         uint32_t* counts = (uint32_t*)ws_get(ctx, "grid_bcounts", sizeof(uint32_t) * m);
@@ -1508,6 +1625,8 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
             (const uint32_t*)totals, (const uint32_t*)counts, entries, cap, toffs);
         offsets = toffs;
         ostride = 1;
+        bcur = counts;
+        btot = totals;
     } else {
         // very large grids: global per-tile counters
         uint32_t* counts = (uint32_t*)ws_get(ctx, "grid_counts", sizeof(uint32_t) * ntiles);
@@ -1524,10 +1643,40 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     if (wproj) {
         // eight waves split the tile's columns: 2, 4 or 8 columns per wave; 4 entries'
         // kernel values in flight per lane
-        auto kern = g.tc == 16 ? grid_wtiles_kernel<T, KT, 2, 4>
-                               : (g.tc == 32 ? grid_wtiles_kernel<T, KT, 4, 4> : grid_wtiles_kernel<T, KT, 8, 2>);
-        launch(ctx, "grid_wtiles", kern, dim3(ntiles), dim3(WTW * 64), 0, g, (const Entry*)entries, offsets, ostride, cap,
-            (const cplx<KT>*)wcache, d_vis_grid, d_sampling_grid);
+        // eight waves split the tile's columns (2, 4 or 8 columns per wave); two
+        // entries' kernel values in flight per lane (more registers cost occupancy)
+        auto kern = g.tc == 16 ? grid_wtiles_kernel<T, KT, 2, 2>
+                               : (g.tc == 32 ? grid_wtiles_kernel<T, KT, 4, 2> : grid_wtiles_kernel<T, KT, 8, 2>);
+        // phases whose W-kernels (about 6 MB of cache each: one FP64 plane, two FP32
+        // planes) are read by all tiles while mostly held in L2
+        const int P = int(a.wproj.num_wplanes);
+        const double plane_bytes = double(cs) * cs * g.ks * (g.ks + 2 * WPAD) * sizeof(cplx<KT>);
+        int nph = bcur ? int(std::min<double>(double(nblk), std::ceil(double(P) * plane_bytes / W_PHASE_BYTES))) : 1;
+        if (a.w_phases > 0)
+            nph = bcur ? std::min(a.w_phases, int(nblk)) : 1;
+        nph = std::max(1, nph);
+        WPhase ph;
+        if (nph > 1) {
+            ph.bcur = bcur;
+            ph.btot = btot;
+            ph.nblk = int(nblk);
+            if (std::is_same<T, double>::value) {
+                ph.acc_v = (cplx<double>*)d_vis_grid;
+                ph.acc_b = (cplx<double>*)d_sampling_grid;
+            } else {
+                const size_t cells = size_t(g.rows) * N;
+                ph.acc_v = (cplx<double>*)ws_get(ctx, "w_acc", sizeof(cplx<double>) * cells * (g.beam ? 2 : 1));
+                ph.acc_b = g.beam ? ph.acc_v + cells : nullptr;
+            }
+        }
+        for (int f = 0; f < nph; f++) {
+            ph.b0 = int(size_t(f) * nblk / nph);
+            ph.b1 = int(size_t(f + 1) * nblk / nph);
+            ph.first = f == 0;
+            ph.last = f == nph - 1;
+            launch(ctx, "grid_wtiles", kern, dim3(ntiles), dim3(WTW * 64), 0, g, (const Entry*)entries, offsets, ostride, cap,
+                (const cplx<KT>*)wcache, d_vis_grid, d_sampling_grid, ph);
+        }
     } else {
         STP_REQUIRE(a.kernel_exact || cs * g.ks <= KV_DOUBLES, "oversampled kernel table too large");
         const bool twofloat = std::is_same<T, float>::value && a.twofloat;

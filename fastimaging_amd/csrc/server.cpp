@@ -105,7 +105,9 @@ extern "C" uint64_t stp_job_device_bytes(const stp_job* j)
         const uint64_t cs = (os / 2) * 2 + 1;
         const uint64_t P = j->w_proj.num_wplanes;
         b += 26 * n + 16 * n;                   // sort keys, indices, flags, scan, plane ids
-        b += P * cs * cs * ks * ks * 16;        // W-kernel caches (complex FP64)
+        b += P * cs * cs * ks * (ks + 2 * stp_hip::W_KERNEL_PAD_COLS) * 16; // W-kernel caches (complex FP64, padded)
+        if (S == 4)
+            b += 2 * rows * N * 16;             // FP64 running sums between the W tile phases
         // work array (workarea * oversampling, gridder.h:467-486) for the W-kernel FFTs
         const uint32_t uo = j->w_proj.undersampling_opt;
         const uint64_t us = uo > 0 ? (uint64_t(1) << (uo - 1)) : 0;
