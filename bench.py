@@ -31,10 +31,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
-# LDS atomic-add rate of one MI355X with every lane of 2048 workgroups adding to random
-# addresses of a 64 x 71 tile (G atomics/s), by precision path: FP64 paths ds_add_f64,
-# two-float ds_add_rtn_f32
-LDS_ATOMIC_PEAK_G = {}
+# LDS atomic-add rate of one MI355X, measured with every lane of 2048 workgroups of 512
+# threads adding 1024 times to random addresses of a 32 x 71 LDS array (G atomics/s),
+# by precision path: FP64 and FP32C sum with ds_add_f64 (1850 G/s); two-float with
+# ds_add_f32 / ds_add_rtn_f32 (204 G/s, returning or not: FP32 LDS atomics run at a
+# ninth of the FP64 rate on this part)
+LDS_ATOMIC_PEAK_G = {0: 1850.0, 1: 1850.0, 2: 204.0}
 METRIC = "snapshots/s + p50/p99 latency, FP32-comp vs FP64 GPU at <=1e-4 peak err"
 
 
