@@ -1021,27 +1021,40 @@ __global__ void wgood_kernel(GridGeom g, const double* uvw, const double* vis, c
     flag[k] = prep_vis(g, uvw, vis, wts, k).good ? 1u : 0u;
 }
 
-// Smallest k in [lo, hi] with G[k] >= target (G non-decreasing), or hi.
-__device__ unsigned long long lower_bound_u32(const uint32_t* G, unsigned long long lo, unsigned long long hi,
+// Smallest k in [lo, hi) with G[k] >= target (G non-decreasing), or hi, found by
+// one full wave: 64 probes per step narrow the range 64-fold, so a search over 2^21
+// positions takes four dependent loads instead of 21.  Every lane returns the result.
+__device__ unsigned long long lower_bound_wave(const uint32_t* G, unsigned long long lo, unsigned long long hi,
     unsigned long long target)
 {
-    while (lo < hi) {
-        const unsigned long long mid = lo + (hi - lo) / 2;
-        if (G[mid] >= target)
-            hi = mid;
-        else
-            lo = mid + 1;
+    const unsigned long long lane = threadIdx.x & 63;
+    while (hi - lo > 64) {
+        const unsigned long long step = (hi - lo + 63) / 64;
+        const unsigned long long q = lo + lane * step;
+        const unsigned long long m = __ballot(q >= hi || (unsigned long long)G[q] >= target);
+        const int f = m ? __builtin_ctzll(m) : 64;
+        if (f == 0)
+            return lo;
+        // the answer lies in (probe f-1, probe f]
+        const unsigned long long nlo = lo + (unsigned long long)(f - 1) * step + 1;
+        const unsigned long long nhi = f == 64 ? hi : min(hi, lo + (unsigned long long)f * step);
+        lo = nlo;
+        hi = nhi;
     }
-    return lo;
+    const unsigned long long q = lo + lane;
+    const unsigned long long m = __ballot(q >= hi || (unsigned long long)G[q] >= target);
+    return m ? lo + (unsigned long long)__builtin_ctzll(m) : hi;
 }
 
 // average_w_planes (gridder.cpp:68-146) over the |w|-sorted good flags: G is their
-// exclusive prefix sum (G[n] = number of good visibilities).  Writes first[0..P]
+// exclusive prefix sum (G[n] = number of good visibilities).  One wave (the planes
+// are sequential: each starts where the last stopped).  Writes first[0..P]
 // (first[P] = n), first[P+1] (where the last plane's count stopped), the good count
 // of each plane, and for the median variant the plane |w|.
-__global__ void wplane_bounds_kernel(const uint32_t* G, long long n, int P, int median, const unsigned long long* skey,
-    uint32_t* first, uint32_t* count, double* w_plane)
+__global__ __launch_bounds__(64) void wplane_bounds_kernel(const uint32_t* G, long long n, int P, int median,
+    const unsigned long long* skey, uint32_t* first, uint32_t* count, double* w_plane)
 {
+    const bool lead = threadIdx.x == 0;
     const unsigned long long num_gvis = G[n];
     unsigned long long begin = 0;
     auto wv = [&](unsigned long long k) { return __longlong_as_double((long long)skey[k]); };
@@ -1050,11 +1063,13 @@ __global__ void wplane_bounds_kernel(const uint32_t* G, long long n, int P, int 
         unsigned long long k = begin, counter = 0;
         if (plane_size > 0 && begin < (unsigned long long)n) {
             const unsigned long long target = (unsigned long long)G[begin] + plane_size;
-            k = lower_bound_u32(G, begin + 1, (unsigned long long)n, target);
+            k = lower_bound_wave(G, begin + 1, (unsigned long long)n, target);
             counter = (unsigned long long)G[k] - G[begin];
         }
-        first[p] = uint32_t(begin);
-        count[p] = uint32_t(counter);
+        if (lead) {
+            first[p] = uint32_t(begin);
+            count[p] = uint32_t(counter);
+        }
         if (median) {
             unsigned long long middle = plane_size / 2;
             if ((begin + plane_size) > num_gvis)
@@ -1063,31 +1078,39 @@ __global__ void wplane_bounds_kernel(const uint32_t* G, long long n, int P, int 
             if (counter > 0) {
                 // the good element with counter == middle: the first position whose
                 // inclusive good count reaches middle + 1
-                const unsigned long long mid = lower_bound_u32(G, begin + 1, k, (unsigned long long)G[begin] + middle + 1) - 1;
+                const unsigned long long mid = lower_bound_wave(G, begin + 1, k, (unsigned long long)G[begin] + middle + 1) - 1;
                 v = (plane_size % 2 == 0 && mid > 0) ? (wv(mid - 1) + wv(mid)) / 2.0 : wv(mid);
             }
-            w_plane[p] = v;
+            if (lead)
+                w_plane[p] = v;
         }
         begin = k;
     }
     // the last plane grids every position up to n (gridder.h:532-533); its count
     // stopped at `begin`
-    first[P] = uint32_t(n);
-    first[P + 1] = uint32_t(begin);
+    if (lead) {
+        first[P] = uint32_t(n);
+        first[P + 1] = uint32_t(begin);
+    }
 }
 
-// Mean |w| of each plane's counted good visibilities (one block per plane): those in
-// [first[p], first[p+1]), except for the last plane, whose count stops at
-// first[P+1] (with rejected visibilities before it, the reference's plane size,
-// computed from a position rather than a good count, stops short of n).
-__global__ __launch_bounds__(256) void wplane_mean_kernel(const unsigned long long* skey, const uint32_t* flag,
-    const uint32_t* first, const uint32_t* count, double* w_plane)
+// Mean |w| of each plane's counted good visibilities: those in [first[p],
+// first[p+1]), except for the last plane, whose count stops at first[P+1] (with
+// rejected visibilities before it, the reference's plane size, computed from a
+// position rather than a good count, stops short of n).  WMS blocks per plane sum
+// fixed slices (part[p * WMS + slice]); a second kernel adds the slices in order.
+constexpr int WMS = 32;
+__global__ __launch_bounds__(256) void wplane_sum_kernel(const unsigned long long* skey, const uint32_t* flag,
+    const uint32_t* first, double* part)
 {
     __shared__ double red[256];
-    const int p = blockIdx.x;
+    const int p = blockIdx.x, sl = blockIdx.y;
+    const unsigned long long b = first[p];
     const unsigned long long end = p + 1 == int(gridDim.x) ? first[p + 2] : first[p + 1];
+    const unsigned long long len = end > b ? end - b : 0, per = (len + WMS - 1) / WMS;
+    const unsigned long long s0 = b + min(len, sl * per), s1 = b + min(len, (sl + 1) * per);
     double sum = 0.0;
-    for (unsigned long long k = first[p] + threadIdx.x; k < end; k += 256)
+    for (unsigned long long k = s0 + threadIdx.x; k < s1; k += 256)
         if (flag[k])
             sum += __longlong_as_double((long long)skey[k]);
     red[threadIdx.x] = sum;
@@ -1098,7 +1121,18 @@ __global__ __launch_bounds__(256) void wplane_mean_kernel(const unsigned long lo
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        w_plane[p] = count[p] > 0 ? red[0] / double(count[p]) : 0.0;
+        part[p * WMS + sl] = red[0];
+}
+
+__global__ void wplane_mean_kernel(const double* part, const uint32_t* count, int P, double* w_plane)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P)
+        return;
+    double sum = 0.0;
+    for (int sl = 0; sl < WMS; sl++)
+        sum += part[p * WMS + sl];
+    w_plane[p] = count[p] > 0 ? sum / double(count[p]) : 0.0;
 }
 
 __global__ void wplane_ids_kernel(const uint32_t* first, int P, long long n, uint16_t* plane)
@@ -1553,11 +1587,15 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         uint16_t* plane_of = (uint16_t*)ws_get(ctx, "w_planeof", sizeof(uint16_t) * n);
         launch(ctx, "w_good", wgood_kernel, dim3(unsigned(div_up(n, 256))), dim3(256), 0, g, d_uv, d_vis, d_w, flag);
         scan_u32(ctx, flag, n, G);
-        launch(ctx, "w_planes", wplane_bounds_kernel, dim3(1), dim3(1), 0, (const uint32_t*)G, (long long)n, P,
+        launch(ctx, "w_planes", wplane_bounds_kernel, dim3(1), dim3(64), 0, (const uint32_t*)G, (long long)n, P,
             int(a.wproj.wplanes_median != 0), (const unsigned long long*)k0, first, pcount, w_plane);
-        if (!a.wproj.wplanes_median)
-            launch(ctx, "w_plane_mean", wplane_mean_kernel, dim3(unsigned(P)), dim3(256), 0, (const unsigned long long*)k0,
-                (const uint32_t*)flag, (const uint32_t*)first, (const uint32_t*)pcount, w_plane);
+        if (!a.wproj.wplanes_median) {
+            double* wpart = (double*)ws_get(ctx, "w_plane_part", sizeof(double) * size_t(P) * WMS);
+            launch(ctx, "w_plane_sum", wplane_sum_kernel, dim3(unsigned(P), unsigned(WMS)), dim3(256), 0,
+                (const unsigned long long*)k0, (const uint32_t*)flag, (const uint32_t*)first, wpart);
+            launch(ctx, "w_plane_mean", wplane_mean_kernel, dim3(unsigned(div_up(P, 256))), dim3(256), 0,
+                (const double*)wpart, (const uint32_t*)pcount, P, w_plane);
+        }
         launch(ctx, "w_plane_ids", wplane_ids_kernel, dim3(unsigned(div_up(n, 256))), dim3(256), 0, (const uint32_t*)first, P,
             (long long)n, plane_of);
         g.plane = plane_of;
