@@ -236,15 +236,15 @@ def kernel_spec(kernel_function, param, width_normalization_gaussian=None, width
 
 
 def wproj_pars(num_wplanes=0, max_wpconv_support=0, undersampling_opt=1, kernel_trunc_perc=0.0,
-               wplanes_median=False):
+               wplanes_median=False, hankel_opt=False, hankel_proj_slice=False, interp_type=0):
     p = WProjPars()
     p.num_wplanes = num_wplanes
     p.max_wpconv_support = max_wpconv_support
     p.undersampling_opt = undersampling_opt
     p.kernel_trunc_perc = kernel_trunc_perc
-    p.hankel_opt = 0
-    p.hankel_proj_slice = 0
-    p.interp_type = 0
+    p.hankel_opt = int(hankel_opt)
+    p.hankel_proj_slice = int(hankel_proj_slice)
+    p.interp_type = int(interp_type)
     p.wplanes_median = int(wplanes_median)
     return p
 

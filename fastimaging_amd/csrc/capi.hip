@@ -63,8 +63,7 @@ void check_imager_pars(const stp_imager_pars* p, const stp_wproj_pars* wp, const
         throw Error(STP_ERR_UNSUPPORTED, "A-projection is not supported");
     if (wp && wp->num_wplanes > 0) {
         STP_REQUIRE(!p->kernel_exact, "W-projection requires kernel_exact == false (imager.h:287)");
-        if (wp->hankel_opt)
-            throw Error(STP_ERR_UNSUPPORTED, "Hankel-transform W-kernels are not supported");
+        STP_REQUIRE(wp->interp_type >= 0 && wp->interp_type <= 2, "interp_type must be 0 (linear), 1 (cosine) or 2 (cubic)");
     }
 }
 

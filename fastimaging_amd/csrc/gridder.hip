@@ -1190,6 +1190,288 @@ __global__ void wk_fill_kernel(cplx<double>* comb, int arr, int half, const doub
     comb[e] = v;
 }
 
+// Each sub-kernel is stored column-major with WPAD zero columns on both sides
+// (ks rows x (ks + 2 WPAD) columns), so that the W tile kernel's column runs
+// starting up to WPAD columns outside the kernel read zeros.
+constexpr int WPAD = W_KERNEL_PAD_COLS;
+
+// ---------------------------------------------------------------------------
+// Hankel W-kernels (hankel_opt; aw_projection.cpp:211-287,380-411, aw_projection.h:127-213)
+//
+// Per plane: a 1-D profile (the radial screen times the AA kernel, or the
+// projection of the 2-D image-domain kernel onto one axis), its transform (the
+// DHT, or the forward DFT), the truncated support, a natural cubic or linear
+// spline through the transform at radii 0, 1, ..., the spline sampled on the half
+// quadrant {(ii, j): ii <= j < ko}, ko = (s + 1) os, at radius sqrt(ii^2 + j^2), and
+// the (os+1)^2 cuts of that radially symmetric kernel, each normalised by its real
+// sum.  One block per plane for the sequential parts (the spline's tridiagonal
+// solve), one thread per output for the transforms.
+struct HankelArgs {
+    int n;          // transform length: arr / 2 (DHT) or arr (projection slice)
+    int half;       // workarea / 2
+    int arr;        // workarea * os
+    int os, support;
+    int proj_slice, cubic;
+    double scaled_cell, trunc_frac;
+    int kh_stride;  // half-quadrant values per plane (for the maximum support)
+};
+
+// combine_2d_kernel_value (aw_projection.cpp:455-474) at pixel offsets (px, py)
+__device__ cplx<double> wk_combine(const double* aa, int half, int px, int py, double w, double sc)
+{
+    const double dx = double(px) * sc, dy = double(py) * sc;
+    const double r2 = dx * dx + dy * dy;
+    if (r2 >= 1.0)
+        return { aa[half + px] * aa[half + py], 0.0 };
+    const double nn = sqrt(1.0 - r2);
+    const double ph = (-2.0 * M_PI) * w * (nn - 1.0);
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    return { ((cs / nn) * aa[half + px]) * aa[half + py], ((sn / nn) * aa[half + px]) * aa[half + py] };
+}
+
+// Profiles of one plane per block: prof[p * n + i].
+__global__ __launch_bounds__(256) void wk_hankel_profile_kernel(HankelArgs h, const double* aa, const double* w_plane,
+    cplx<double>* prof)
+{
+#pragma clang fp contract(off)
+    const int p = blockIdx.x;
+    const double w = w_plane[p];
+    cplx<double>* out = prof + size_t(p) * h.n;
+    if (!h.proj_slice) {
+        // the radial screen along x times aa(x) (aw_projection.cpp:224-239)
+        for (int i = threadIdx.x; i < h.n; i += 256) {
+            cplx<double> v{ 0.0, 0.0 };
+            if (i < h.half) {
+                const double dx = double(i) * h.scaled_cell;
+                const double r2 = dx * dx;
+                double tr = 1.0, ti = 0.0;
+                if (r2 < 1.0) {
+                    const double nn = sqrt(1.0 - r2);
+                    const double ph = (-2.0 * M_PI) * w * (nn - 1.0);
+                    double sn, cs;
+                    sincos(ph, &sn, &cs);
+                    tr = cs / nn;
+                    ti = sn / nn;
+                }
+                v = { tr * aa[h.half + i], ti * aa[h.half + i] };
+            }
+            out[i] = v;
+        }
+        return;
+    }
+    // generate_projected_image_domain_kernel (aw_projection.cpp:113-171): column sums
+    // of the 2-D kernel, each sum in the reference's order
+    for (int j = threadIdx.x; j < h.n; j += 256) {
+        cplx<double> v{ 0.0, 0.0 };
+        if (j == 0) {
+            // twice the axis values (1 .. half-1) plus the centre
+            double sr = 0.0, si = 0.0;
+            for (int i = 1; i < h.half; i++) {
+                const cplx<double> c = wk_combine(aa, h.half, i, 0, w, h.scaled_cell);
+                sr += c.x;
+                si += c.y;
+            }
+            v = { sr * 2.0 + aa[h.half] * aa[h.half], si * 2.0 };
+        } else {
+            const int jj = j < h.half ? j : (j > h.arr - h.half ? h.arr - j : -1);
+            if (jj > 0) {
+                v = wk_combine(aa, h.half, jj, 0, w, h.scaled_cell);
+                for (int i = 1; i < h.half; i++) {
+                    const cplx<double> c = wk_combine(aa, h.half, i, jj, w, h.scaled_cell);
+                    v.x += c.x * 2.0;
+                    v.y += c.y * 2.0;
+                }
+            }
+        }
+        out[j] = v;
+    }
+}
+
+// y[p * n + i] = sum_j T(j, i) prof[p * n + j], j in order: T = the DHT (tab[j * n + i])
+// or the forward DFT (tab = exp(-2 pi i m / n) interleaved, m = j i mod n).
+__global__ __launch_bounds__(256) void wk_hankel_transform_kernel(HankelArgs h, const double* tab,
+    const cplx<double>* prof, cplx<double>* y)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * 256 + threadIdx.x, p = blockIdx.y;
+    if (i >= h.n)
+        return;
+    const cplx<double>* x = prof + size_t(p) * h.n;
+    double ar = 0.0, ai = 0.0;
+    if (!h.proj_slice) {
+        for (int j = 0; j < h.n; j++) {
+            const double d = tab[size_t(j) * h.n + i];
+            ar += d * x[j].x;
+            ai += d * x[j].y;
+        }
+    } else {
+        unsigned long long m = 0;
+        for (int j = 0; j < h.n; j++) {
+            const double c = tab[2 * m], sn = tab[2 * m + 1];
+            ar += x[j].x * c - x[j].y * sn;
+            ai += x[j].x * sn + x[j].y * c;
+            m += (unsigned long long)i;
+            if (m >= (unsigned long long)h.n)
+                m -= (unsigned long long)h.n;
+        }
+    }
+    y[size_t(p) * h.n + i] = { ar, ai };
+}
+
+// Truncated support (aw_projection.h:132-148) and spline coefficients of one plane:
+// lane 0 the real part, lane 1 the imaginary part.  coef[(p * 2 + part) * 5 + {a, b,
+// c, d, r}][k]; the natural cubic spline on the knots 0, 1, ... solves its
+// tridiagonal system (rows 0 and n-1: 2 b = 0; else (1/3, 4/3, 1/3) against the
+// second differences) by forward elimination and back substitution.
+__global__ __launch_bounds__(64) void wk_hankel_spline_kernel(HankelArgs h, const cplx<double>* y, double* coef, int* psup)
+{
+#pragma clang fp contract(off)
+    const int p = blockIdx.x, part = threadIdx.x;
+    const int n = h.n;
+    const cplx<double>* yp = y + size_t(p) * n;
+    if (part == 0) {
+        int ts = h.support;
+        if (h.trunc_frac > 0.0) {
+            const double min_value = hypot(yp[0].x, yp[0].y) * h.trunc_frac;
+            int idx = h.support * h.os;
+            const double scale = 1.0 / double(h.os);
+            while (idx > h.os) {
+                if (hypot(yp[idx].x, yp[idx].y) > min_value)
+                    break;
+                idx -= h.os;
+            }
+            ts = min(int(ceil(double(idx) * scale)), h.support);
+        }
+        psup[p] = ts;
+    }
+    if (part > 1)
+        return;
+    double* a = coef + (size_t(p) * 2 + part) * 5 * n;
+    double* b = a + n;
+    double* c = b + n;
+    double* d = c + n;
+    double* r = d + n;
+    auto yv = [&](int k) { return part ? yp[k].y : yp[k].x; };
+    if (!h.cubic) {
+        for (int k = 0; k + 1 < n; k++) {
+            c[k] = (yv(k + 1) - yv(k)) / 1.0;
+            a[k] = b[k] = 0.0;
+        }
+        c[n - 1] = c[n - 2];
+        a[n - 1] = b[n - 1] = 0.0;
+        return;
+    }
+    const double lo = 1.0 / 3.0 * 1.0, di = 2.0 / 3.0 * 2.0, up = 1.0 / 3.0 * 1.0;
+    d[0] = 2.0;
+    r[0] = 0.0;
+    for (int k = 1; k < n; k++) {
+        const bool last = k == n - 1;
+        const double lk = last ? 0.0 : lo, dk = last ? 2.0 : di;
+        const double rk = last ? 0.0 : (yv(k + 1) - yv(k)) / 1.0 - (yv(k) - yv(k - 1)) / 1.0;
+        const double m = lk / d[k - 1];
+        d[k] = dk - m * (k == 1 ? 0.0 : up);
+        r[k] = rk - m * r[k - 1];
+    }
+    b[n - 1] = r[n - 1] / d[n - 1];
+    for (int k = n - 2; k >= 0; k--)
+        b[k] = (r[k] - (k == 0 ? 0.0 : up) * b[k + 1]) / d[k];
+    for (int k = 0; k + 1 < n; k++) {
+        a[k] = 1.0 / 3.0 * (b[k + 1] - b[k]) / 1.0;
+        c[k] = (yv(k + 1) - yv(k)) / 1.0 - 1.0 / 3.0 * (2.0 * b[k] + b[k + 1]) * 1.0;
+    }
+    a[n - 1] = 0.0;
+    c[n - 1] = 3.0 * a[n - 2] * 1.0 * 1.0 + 2.0 * b[n - 2] * 1.0 + c[n - 2];
+}
+
+// The spline on the half quadrant of one plane (RadialInterpolate's loop): row ii
+// holds j = ii .. ko-1 at kh[p * stride + ii ko - ii (ii - 1) / 2 + j - ii].
+__global__ __launch_bounds__(256) void wk_hankel_quad_kernel(HankelArgs h, const cplx<double>* y, const double* coef,
+    const int* psup, cplx<double>* kh)
+{
+#pragma clang fp contract(off)
+    const int p = blockIdx.y, n = h.n;
+    const int ko = (psup[p] * 2 + 2) * h.os / 2;
+    const long long total = (long long)ko * (ko + 1) / 2;
+    const cplx<double>* yp = y + size_t(p) * n;
+    const double* cr = coef + size_t(p) * 2 * 5 * n;
+    const double* ci = cr + 5 * size_t(n);
+    for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < total; q += (long long)gridDim.x * 256) {
+        // row ii: the largest with ii ko - ii (ii - 1) / 2 <= q
+        int ii = int((double(2 * ko + 1) - sqrt(double(2 * ko + 1) * double(2 * ko + 1) - 8.0 * double(q))) / 2.0);
+        ii = max(0, min(ii, ko - 1));
+        while (ii > 0 && (long long)ii * ko - (long long)ii * (ii - 1) / 2 > q)
+            ii--;
+        while (ii + 1 < ko && (long long)(ii + 1) * ko - (long long)(ii + 1) * ii / 2 <= q)
+            ii++;
+        const int j = int(q - ((long long)ii * ko - (long long)ii * (ii - 1) / 2)) + ii;
+        const double nx = ii == 0 ? double(j) : sqrt(double((long long)ii * ii + (long long)j * j));
+        int k = int(nx);
+        if (k >= n)
+            k = n - 1;
+        const double hh = nx - double(k);
+        double vr, vi;
+        if (h.cubic) {
+            vr = ((cr[k] * hh + cr[n + k]) * hh + cr[2 * n + k]) * hh + yp[k].x;
+            vi = ((ci[k] * hh + ci[n + k]) * hh + ci[2 * n + k]) * hh + yp[k].y;
+        } else {
+            vr = cr[2 * n + k] * hh + yp[k].x;
+            vi = ci[2 * n + k] * hh + yp[k].y;
+        }
+        kh[size_t(p) * h.kh_stride + q] = { vr, vi };
+    }
+}
+
+// generate_kernel_cache, Hankel branch (aw_projection.cpp:380-411): cut (x, y) of the
+// radially symmetric kernel, element (row tx, column ty) at offsets
+// cx = -ts os + os/2 - x + ty os, cy = ... - y + tx os, divided by the cut's real sum;
+// stored like wk_extract_kernel's cuts (padded columns, truncated kernels centred).
+template <typename KT>
+__global__ __launch_bounds__(256) void wk_hankel_extract_kernel(HankelArgs h, const cplx<double>* kh, const int* psup,
+    int p0, cplx<KT>* cache)
+{
+    __shared__ double red[256];
+    const int p = p0 + int(blockIdx.y);
+    const int cs = (h.os / 2) * 2 + 1;
+    const int x = blockIdx.x % cs, yy = blockIdx.x / cs;
+    const int ks = 2 * h.support + 1, kw = ks + 2 * WPAD;
+    const int ts = psup[p], ko = (ts * 2 + 2) * h.os / 2;
+    const int lo = h.support - ts, hi = h.support + ts;
+    const long long os = h.os, op = h.os / 2;
+    const cplx<double>* khp = kh + size_t(p) * h.kh_stride;
+    auto value = [&](int i, int j) -> cplx<double> {
+        const long long cx = -(long long)ts * os + op - x + (long long)(j - lo) * os;
+        const long long cy = -(long long)ts * os + op - yy + (long long)(i - lo) * os;
+        const long long dcx = cx < 0 ? -cx : cx, dcy = cy < 0 ? -cy : cy;
+        const long long vi = dcx > dcy ? dcy * ko - ((dcy * (dcy + 1)) >> 1) + dcx : dcx * ko - ((dcx * (dcx + 1)) >> 1) + dcy;
+        return khp[vi];
+    };
+    double sum = 0.0;
+    for (int e = threadIdx.x; e < ks * ks; e += 256) {
+        const int i = e % ks, j = e / ks;
+        if (i >= lo && i <= hi && j >= lo && j <= hi)
+            sum += value(i, j).x;
+    }
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    const double ksum = red[0];
+    cplx<KT>* out = cache + (size_t(p) * cs * cs + blockIdx.x) * size_t(ks) * kw;
+    for (int e = threadIdx.x; e < ks * kw; e += 256) {
+        const int i = e % ks, j = e / ks - WPAD;
+        if (j < 0 || j >= ks || i < lo || i > hi || j < lo || j > hi) {
+            out[e] = { KT(0), KT(0) };
+            continue;
+        }
+        const cplx<double> v = value(i, j);
+        out[e] = { KT(v.x / ksum), KT(v.y / ksum) };
+    }
+}
+
 // Truncated support of each plane of a batch (aw_projection.cpp:176-179,197-209):
 // the largest s <= support whose sample conv(s * os, 0) exceeds trunc_frac times
 // |conv(0, 0)|.  A plane truncated to 0 (the reference asserts there, gridder.h:584)
@@ -1218,10 +1500,6 @@ __global__ void wk_trunc_kernel(const cplx<double>* conv, int arr, int os, int s
 // plane (plane_support non-NULL) the kernel of support s_p sits centred in the
 // (2 support + 1)^2 slot, normalised over its own taps, zero outside: gridding
 // those zeros adds nothing, as the reference's smaller loop adds nothing.
-// Each sub-kernel is stored column-major with WPAD zero columns on both sides
-// (ks rows x (ks + 2 WPAD) columns), so that the W tile kernel's column runs
-// starting up to WPAD columns outside the kernel read zeros.
-constexpr int WPAD = W_KERNEL_PAD_COLS;
 
 template <typename KT>
 __global__ __launch_bounds__(256) void wk_extract_kernel(const cplx<double>* conv, int arr, int os, int support,
@@ -1493,7 +1771,6 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     STP_REQUIRE(n < (size_t(1) << 30), "at most 2^30 visibilities per call");
     if (wproj) {
         STP_REQUIRE(!a.kernel_exact, "W-projection requires kernel_exact == false");
-        STP_REQUIRE(a.wproj.hankel_opt == 0, "Hankel W-kernels are not implemented");
         STP_REQUIRE(a.wproj.num_wplanes <= 65535, "at most 65535 w-planes");
         STP_REQUIRE(n < (size_t(1) << 31), "too many visibilities for W-projection");
     }
@@ -1619,27 +1896,70 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
                                                  : dev_table(ctx, key, img_dom_kernel(kf, size_t(workarea), false, a.analytic_gcf));
         // planes in groups of PB: one fill, one batched 2-D c2c and one extraction per
         // group, the image-domain arrays of a group at most 256 MB each
-        const size_t arr2 = size_t(arr) * arr;
-        const int PB = int(std::max<size_t>(1, std::min<size_t>(size_t(P), (size_t(1) << 28) / (arr2 * sizeof(cplx<double>)))));
-        cplx<double>* comb = (cplx<double>*)ws_get(ctx, "w_comb", sizeof(cplx<double>) * arr2 * PB);
-        cplx<double>* conv = (cplx<double>*)ws_get(ctx, "w_conv", sizeof(cplx<double>) * arr2 * PB);
-        STP_REQUIRE(size_t(P) * cs * cs < (size_t(1) << 31), "too many W sub-kernels (planes x oversampling^2)");
         const size_t per_plane = size_t(cs) * cs * size_t(g.ks) * (g.ks + 2 * WPAD);
+        STP_REQUIRE(size_t(P) * cs * cs < (size_t(1) << 31), "too many W sub-kernels (planes x oversampling^2)");
         wcache = (cplx<KT>*)ws_get(ctx, "w_cache", sizeof(cplx<KT>) * per_plane * P);
         // kernel_trunc_perc > 0: every plane grids with its own truncated support
         const bool trunc = a.wproj.kernel_trunc_perc > 0.0;
-        int* psup = trunc ? (int*)ws_get(ctx, "w_psupport", sizeof(int) * P) : nullptr;
-        for (int p0 = 0; p0 < P; p0 += PB) {
-            const int nb = std::min(PB, P - p0);
-            launch(ctx, "w_kernel_fill", wk_fill_kernel, dim3(unsigned(div_up(arr2, 256)), unsigned(nb)), dim3(256), 0, comb,
-                arr, workarea / 2, d_aa, (const double*)w_plane, p0, a.cell_rad * scaling);
-            fft_c2c_2d<double>(ctx, comb, conv, arr, true, nb);
-            if (trunc)
-                launch(ctx, "w_kernel_trunc", wk_trunc_kernel, dim3(div_up(nb, 64)), dim3(64), 0, (const cplx<double>*)conv, arr,
-                    os, g.s, a.wproj.kernel_trunc_perc / 100.0, nb, psup + p0);
-            launch(ctx, "w_kernel_extract", wk_extract_kernel<KT>, dim3(unsigned(cs * cs), unsigned(nb)), dim3(256), 0,
-                (const cplx<double>*)conv, arr, os, g.s, (const int*)(trunc ? psup + p0 : nullptr),
-                wcache + size_t(p0) * per_plane);
+        if (a.wproj.hankel_opt) {
+            // Hankel W-kernels: profile, DHT or DFT, spline, half quadrant, cuts
+            HankelArgs h;
+            h.proj_slice = a.wproj.hankel_proj_slice != 0;
+            h.cubic = a.wproj.interp_type == 2;
+            h.n = h.proj_slice ? arr : arr / 2;
+            h.half = workarea / 2;
+            h.arr = arr;
+            h.os = os;
+            h.support = g.s;
+            h.scaled_cell = a.cell_rad * scaling;
+            h.trunc_frac = trunc ? a.wproj.kernel_trunc_perc / 100.0 : 0.0;
+            const long long ko_max = (long long)(g.s + 1) * os;
+            h.kh_stride = int(ko_max * (ko_max + 1) / 2);
+            STP_REQUIRE(h.n >= 3, "Hankel W-kernels need a transform of at least 3 points");
+            STP_REQUIRE(h.n <= 8192, "Hankel W-kernel transform longer than 8192 points (workarea * oversampling)");
+            const double* tab;
+            char tkey[64];
+            if (!h.proj_slice) {
+                snprintf(tkey, sizeof(tkey), "dht:%d", h.n);
+                tab = has_table(ctx, tkey) ? dev_table(ctx, tkey, {}) : dev_table(ctx, tkey, dht_table(size_t(h.n)));
+            } else {
+                snprintf(tkey, sizeof(tkey), "dft:%d", h.n);
+                tab = has_table(ctx, tkey) ? dev_table(ctx, tkey, {}) : dev_table(ctx, tkey, dft_twiddles(size_t(h.n)));
+            }
+            cplx<double>* prof = (cplx<double>*)ws_get(ctx, "w_hk_prof", sizeof(cplx<double>) * size_t(h.n) * P);
+            cplx<double>* yv = (cplx<double>*)ws_get(ctx, "w_hk_y", sizeof(cplx<double>) * size_t(h.n) * P);
+            double* coef = (double*)ws_get(ctx, "w_hk_coef", sizeof(double) * 10 * size_t(h.n) * P);
+            cplx<double>* kh = (cplx<double>*)ws_get(ctx, "w_hk_quad", sizeof(cplx<double>) * size_t(h.kh_stride) * P);
+            int* psup = (int*)ws_get(ctx, "w_psupport", sizeof(int) * P);
+            launch(ctx, "w_hankel_profile", wk_hankel_profile_kernel, dim3(unsigned(P)), dim3(256), 0, h, d_aa,
+                (const double*)w_plane, prof);
+            launch(ctx, "w_hankel_transform", wk_hankel_transform_kernel, dim3(unsigned(div_up(h.n, 256)), unsigned(P)),
+                dim3(256), 0, h, tab, (const cplx<double>*)prof, yv);
+            launch(ctx, "w_hankel_spline", wk_hankel_spline_kernel, dim3(unsigned(P)), dim3(64), 0, h, (const cplx<double>*)yv,
+                coef, psup);
+            launch(ctx, "w_hankel_quad", wk_hankel_quad_kernel,
+                dim3(unsigned(std::min<long long>(1024, div_up((long long)h.kh_stride, 256))), unsigned(P)), dim3(256), 0, h,
+                (const cplx<double>*)yv, (const double*)coef, (const int*)psup, kh);
+            launch(ctx, "w_kernel_extract", wk_hankel_extract_kernel<KT>, dim3(unsigned(cs * cs), unsigned(P)), dim3(256), 0,
+                h, (const cplx<double>*)kh, (const int*)psup, 0, wcache);
+        } else {
+            const size_t arr2 = size_t(arr) * arr;
+            const int PB = int(std::max<size_t>(1, std::min<size_t>(size_t(P), (size_t(1) << 28) / (arr2 * sizeof(cplx<double>)))));
+            cplx<double>* comb = (cplx<double>*)ws_get(ctx, "w_comb", sizeof(cplx<double>) * arr2 * PB);
+            cplx<double>* conv = (cplx<double>*)ws_get(ctx, "w_conv", sizeof(cplx<double>) * arr2 * PB);
+            int* psup = trunc ? (int*)ws_get(ctx, "w_psupport", sizeof(int) * P) : nullptr;
+            for (int p0 = 0; p0 < P; p0 += PB) {
+                const int nb = std::min(PB, P - p0);
+                launch(ctx, "w_kernel_fill", wk_fill_kernel, dim3(unsigned(div_up(arr2, 256)), unsigned(nb)), dim3(256), 0, comb,
+                    arr, workarea / 2, d_aa, (const double*)w_plane, p0, a.cell_rad * scaling);
+                fft_c2c_2d<double>(ctx, comb, conv, arr, true, nb);
+                if (trunc)
+                    launch(ctx, "w_kernel_trunc", wk_trunc_kernel, dim3(div_up(nb, 64)), dim3(64), 0, (const cplx<double>*)conv, arr,
+                        os, g.s, a.wproj.kernel_trunc_perc / 100.0, nb, psup + p0);
+                launch(ctx, "w_kernel_extract", wk_extract_kernel<KT>, dim3(unsigned(cs * cs), unsigned(nb)), dim3(256), 0,
+                    (const cplx<double>*)conv, arr, os, g.s, (const int*)(trunc ? psup + p0 : nullptr),
+                    wcache + size_t(p0) * per_plane);
+            }
         }
     }
 

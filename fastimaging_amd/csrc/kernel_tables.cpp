@@ -178,4 +178,65 @@ std::vector<double> img_dom_kernel(const KernelFn& k, size_t K, bool normalize, 
     return aa;
 }
 
+// J1 by the rational approximation of Numerical Recipes in C (2nd ed., section 6.5,
+// bessj1), as the reference's besselj1 (aw_projection.cpp:490-508) uses it.
+static double bessel_j1(double x)
+{
+    if (x < 8.0) {
+        const double x2 = x * x;
+        const double num = x * (72362614232.0 + x2 * (-7895059235.0 + x2 * (242396853.1 + x2 * (-2972611.439 +
+            x2 * (15704.48260 + x2 * (-30.16036606))))));
+        const double den = 144725228442.0 + x2 * (2300535178.0 + x2 * (18583304.74 + x2 * (99447.43394 +
+            x2 * (376.9991397 + x2))));
+        return num / den;
+    }
+    const double z = 8.0 / x, z2 = z * z, ph = x - 2.356194491;
+    const double pa = 1.0 + z2 * (0.183105e-2 + z2 * (-0.3516396496e-4 + z2 * (0.2457520174e-5 + z2 * (-0.240337019e-6))));
+    const double pb = 0.04687499995 + z2 * (-0.2002690873e-3 + z2 * (0.8449199096e-5 + z2 * (-0.88228987e-6 +
+        z2 * 0.105787412e-6)));
+    return std::sqrt(0.636619772 / x) * (std::cos(ph) * pa - z * std::sin(ph) * pb);
+}
+
+// DHT(j, i) = P_i(r_j) - P_i(r_{j-1}) with P_i(r) = (2 pi / k_i) r J1(k_i r), the integral
+// of 2 pi r J0(k_i r) from 0 to r, k_i = pi i / n, ring edges r_j = j + 1/2 (the last
+// n - 1); for k_0 = 0 the ring areas pi r_j^2 - pi r_{j-1}^2 (aw_projection.cpp:530-596).
+std::vector<double> dht_table(size_t n)
+{
+    std::vector<double> t(n * n);
+    std::vector<double> r(n);
+    for (size_t j = 0; j < n; j++)
+        r[j] = double(j) + 0.5;
+    r[n - 1] = double(n - 1);
+    double prev = M_PI * r[0] * r[0];
+    t[0] = prev;
+    for (size_t j = 1; j < n; j++) {
+        const double cur = M_PI * r[j] * r[j];
+        t[j * n] = cur - prev;
+        prev = cur;
+    }
+    const double kf = M_PI / double(n);
+    for (size_t i = 1; i < n; i++) {
+        const double k = kf * double(i), kn = (2.0 * M_PI) / k;
+        double hi = kn * r[n - 1] * bessel_j1(k * r[n - 1]);
+        for (size_t j = n - 1; j > 0; j--) {
+            const double lo = kn * r[j - 1] * bessel_j1(k * r[j - 1]);
+            t[j * n + i] = hi - lo;
+            hi = lo;
+        }
+        t[i] = hi;
+    }
+    return t;
+}
+
+std::vector<double> dft_twiddles(size_t n)
+{
+    std::vector<double> t(2 * n);
+    for (size_t m = 0; m < n; m++) {
+        const double a = -2.0 * M_PI * double(m) / double(n);
+        t[2 * m] = std::cos(a);
+        t[2 * m + 1] = std::sin(a);
+    }
+    return t;
+}
+
 } // namespace stp_hip

@@ -34,4 +34,11 @@ std::vector<double> kernel_1d_table(const KernelFn& k, int support, int oversamp
 // directly from the non-zero kernel samples.
 std::vector<double> img_dom_kernel(const KernelFn& k, size_t kernel_size, bool normalize, bool analytic_gcf);
 
+// Hankel W-kernels (src/stp/gridder/aw_projection.cpp:477-596).  The n x n discrete
+// Hankel transform, row-major by output: t[j * n + i] = DHT(j, i), so that threads
+// over outputs i read consecutive words for each input j.
+std::vector<double> dht_table(size_t n);
+// exp(-2 pi i m / n), m = 0..n-1, interleaved (cos, sin): forward DFT twiddles.
+std::vector<double> dft_twiddles(size_t n);
+
 } // namespace stp_hip

@@ -119,9 +119,18 @@ extern "C" uint64_t stp_job_device_bytes(const stp_job* j)
             while (workarea < N)
                 workarea *= 2;
         const uint64_t arr = workarea * os;
-        // the image-domain and FFT arrays of one group of planes (at most 256 MB each)
-        const uint64_t PB = std::max<uint64_t>(1, std::min<uint64_t>(P, (uint64_t(1) << 28) / (arr * arr * 16)));
-        b += 2 * PB * arr * arr * 16 + 4 * P;
+        if (j->w_proj.hankel_opt) {
+            // Hankel kernels: profile, transform, spline coefficients and half quadrant per
+            // plane, and the DHT or DFT table
+            const uint64_t n = j->w_proj.hankel_proj_slice ? arr : arr / 2;
+            const uint64_t ko = (s + 1) * os;
+            b += P * (2 * n * 16 + 10 * n * 8 + ko * (ko + 1) / 2 * 16) + (j->w_proj.hankel_proj_slice ? 16 * n : 8 * n * n);
+        } else {
+            // the image-domain and FFT arrays of one group of planes (at most 256 MB each)
+            const uint64_t PB = std::max<uint64_t>(1, std::min<uint64_t>(P, (uint64_t(1) << 28) / (arr * arr * 16)));
+            b += 2 * PB * arr * arr * 16;
+        }
+        b += 4 * P;
     }
     return b;
 }

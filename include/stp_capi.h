@@ -30,7 +30,7 @@ extern "C" {
 typedef enum stp_status {
     STP_OK = 0,
     STP_ERR_INVALID_ARGUMENT = 1, /* reference: assert(...) in imager.h:257-287, gridder.h:270-295 */
-    STP_ERR_UNSUPPORTED = 2,      /* feature outside this implementation (A-projection, Hankel) */
+    STP_ERR_UNSUPPORTED = 2,      /* feature outside this implementation (A-projection) */
     STP_ERR_DEVICE = 3,           /* HIP runtime error */
     STP_ERR_OUT_OF_MEMORY = 4,    /* device allocation failed or HBM budget exceeded */
     STP_ERR_INTERNAL = 5,         /* internal consistency check failed */
@@ -122,9 +122,9 @@ typedef struct stp_wproj_pars {
     uint32_t max_wpconv_support;
     uint32_t undersampling_opt;
     double kernel_trunc_perc;
-    int32_t hankel_opt;      /* must be 0: Hankel kernels are not implemented */
-    int32_t hankel_proj_slice;
-    int32_t interp_type;
+    int32_t hankel_opt;        /* radially symmetric W-kernels by a Hankel transform (aw_projection.cpp:211-287) */
+    int32_t hankel_proj_slice; /* with hankel_opt: the DFT of the kernel's projection instead of the DHT */
+    int32_t interp_type;       /* with hankel_opt: 0 linear, 1 cosine (as linear), 2 cubic spline */
     int32_t wplanes_median;
 } stp_wproj_pars;
 

@@ -6,6 +6,7 @@
  */
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <numeric>
 #include <stdexcept>
 #include <thread>
@@ -214,6 +215,240 @@ std::vector<std::vector<cx>> wproj_kernel_cache(double w_value, const std::vecto
     return cache;
 }
 
+// ---------------- Hankel W-kernels (wp.hankel_opt) ----------------
+
+// besselj1 (aw_projection.cpp:490-508): J1 by the rational approximation of
+// Numerical Recipes in C, 2nd ed., section 6.5 (bessj1), whose constants the
+// reference uses; |error| < 1e-7.
+double besselj1(double x)
+{
+    if (x < 8.0) {
+        const double y = x * x;
+        const double p = x * (72362614232.0 + y * (-7895059235.0 + y * (242396853.1 + y * (-2972611.439 + y * (15704.48260 + y * (-30.16036606))))));
+        const double q = 144725228442.0 + y * (2300535178.0 + y * (18583304.74 + y * (99447.43394 + y * (376.9991397 + y))));
+        return p / q;
+    }
+    const double z = 8.0 / x, y = z * z, xx = x - 2.356194491;
+    const double p = 1.0 + y * (0.183105e-2 + y * (-0.3516396496e-4 + y * (0.2457520174e-5 + y * (-0.240337019e-6))));
+    const double q = 0.04687499995 + y * (-0.2002690873e-3 + y * (0.8449199096e-5 + y * (-0.88228987e-6 + y * 0.105787412e-6)));
+    return std::sqrt(0.636619772 / x) * (std::cos(xx) * p - z * std::sin(xx) * q);
+}
+
+// dht (aw_projection.cpp:530-596): the n x n discrete Hankel transform, column-major
+// D[j + i n] = DHT(j, i): row j of column i is the integral of 2 pi r J0(k_i r) over
+// the ring [rn_{j-1}, rn_j] (rn_j = j + 1/2, the last one n - 1), from the primitive
+// (2 pi / k) r J1(k r); column 0 (k = 0) holds the ring areas.
+std::vector<double> dht_matrix(size_t n)
+{
+    std::vector<double> k(n), rn(n), kn(n), D(n * n);
+    const double kf = M_PI / double(n);
+    kn[0] = 0.0;
+    k[0] = 0.0;
+    rn[0] = 0.5;
+    for (size_t i = 1; i < n; i++) {
+        k[i] = kf * double(i);
+        rn[i] = double(i) + 0.5;
+        kn[i] = (2.0 * M_PI) / k[i];
+    }
+    rn[n - 1] = double(n - 1);
+    D[0] = M_PI * rn[0] * rn[0];
+    double prev = D[0];
+    for (size_t j = 1; j < n; j++) {
+        const double cur = M_PI * rn[j] * rn[j];
+        D[j] = cur - prev;
+        prev = cur;
+    }
+    for (size_t i = 1; i < n; i++) {
+        double* col = D.data() + i * n;
+        col[n - 1] = kn[i] * rn[n - 1] * besselj1(k[i] * rn[n - 1]);
+        for (size_t j = n - 1; j > 0; j--) {
+            col[j - 1] = kn[i] * rn[j - 1] * besselj1(k[i] * rn[j - 1]);
+            col[j] -= col[j - 1];
+        }
+    }
+    return D;
+}
+
+// tk::spline on the knots x_i = i (spline.h, the reference's vendored cubic/linear
+// spline): natural cubic (zero curvature at both ends: the tridiagonal system
+// solved by forward elimination and back substitution) or linear; coefficients a,
+// b, c with f(x) = ((a h + b) h + c) h + y_k, h = x - k.
+void spline_coefs(const double* y, size_t n, bool cubic, double* a, double* b, double* c)
+{
+    if (!cubic) {
+        for (size_t i = 0; i + 1 < n; i++) {
+            c[i] = (y[i + 1] - y[i]) / 1.0;
+            a[i] = b[i] = 0.0;
+        }
+        c[n - 1] = c[n - 2];
+        a[n - 1] = b[n - 1] = 0.0;
+        return;
+    }
+    // rows 1..n-2: (1/3) b_{i-1} + (2/3)(2) b_i + (1/3) b_{i+1} = (y_{i+1} - y_i) - (y_i - y_{i-1});
+    // rows 0 and n-1: 2 b = 0
+    const double lo = 1.0 / 3.0 * 1.0, di = 2.0 / 3.0 * 2.0, up = 1.0 / 3.0 * 1.0;
+    std::vector<double> d(n), r(n);
+    d[0] = 2.0;
+    r[0] = 0.0;
+    for (size_t i = 1; i < n; i++) {
+        const bool last = i == n - 1;
+        const double li = last ? 0.0 : lo, dii = last ? 2.0 : di;
+        const double ri = last ? 0.0 : (y[i + 1] - y[i]) / 1.0 - (y[i] - y[i - 1]) / 1.0;
+        const double upper_prev = (i == 1) ? 0.0 : up; // A(i-1, i)
+        const double m = li / d[i - 1];
+        d[i] = dii - m * upper_prev;
+        r[i] = ri - m * r[i - 1];
+    }
+    b[n - 1] = r[n - 1] / d[n - 1];
+    for (size_t i = n - 1; i-- > 0;) {
+        const double upper = (i == 0) ? 0.0 : up;
+        b[i] = (r[i] - upper * b[i + 1]) / d[i];
+    }
+    for (size_t i = 0; i + 1 < n; i++) {
+        a[i] = 1.0 / 3.0 * (b[i + 1] - b[i]) / 1.0;
+        c[i] = (y[i + 1] - y[i]) / 1.0 - 1.0 / 3.0 * (2.0 * b[i] + b[i + 1]) * 1.0;
+    }
+    a[n - 1] = 0.0;
+    c[n - 1] = 3.0 * a[n - 2] * 1.0 * 1.0 + 2.0 * b[n - 2] * 1.0 + c[n - 2];
+}
+
+double spline_eval(const double* a, const double* b, const double* c, const double* y, size_t k, double x, bool cubic)
+{
+    const double h = x - double(k);
+    return cubic ? ((a[k] * h + b[k]) * h + c[k]) * h + y[k] : c[k] * h + y[k];
+}
+
+// Hankel W-kernel cache of one plane: generate_convolution_kernel_wproj's Hankel
+// branch (aw_projection.cpp:211-287), RadialInterpolate (aw_projection.h:127-213,
+// non-diagonal) and generate_kernel_cache's Hankel branch (aw_projection.cpp:380-411).
+// proj_slice: the 1-D projection of the 2-D image-domain kernel
+// (generate_projected_image_domain_kernel, aw_projection.cpp:113-171) and its forward
+// DFT instead of the radial profile and the DHT.
+std::vector<std::vector<cx>> hankel_kernel_cache(double w_value, const std::vector<double>& aa, unsigned workarea_size,
+    double cell_rad, unsigned oversampling, double scaling_factor, unsigned max_support, bool proj_slice, bool cubic,
+    double trunc_perc, unsigned* trunc_support, std::vector<cx>* radius_out)
+{
+    const size_t arr = size_t(workarea_size) * oversampling;
+    const size_t half_arr = arr / 2, half_k = workarea_size / 2;
+    const double scaled_cell = cell_rad * scaling_factor;
+    std::vector<cx> y;
+    if (!proj_slice) {
+        std::vector<cx> prof(half_arr, cx(0.0, 0.0));
+        for (size_t i = 0; i < half_k; i++) {
+            const double dx = double(i) * scaled_cell;
+            const double r2 = dx * dx;
+            cx t;
+            if (r2 >= 1.0) {
+                t = cx(1.0, 0.0);
+            } else {
+                const double nn = std::sqrt(1.0 - r2);
+                const double ph = (-2.0 * M_PI) * w_value * (nn - 1.0);
+                t = cx(std::cos(ph), std::sin(ph)) / nn;
+            }
+            prof[i] = t * aa[half_k + i];
+        }
+        static std::map<size_t, std::vector<double>> dht_cache;
+        auto it = dht_cache.find(half_arr);
+        if (it == dht_cache.end())
+            it = dht_cache.emplace(half_arr, dht_matrix(half_arr)).first;
+        const std::vector<double>& D = it->second;
+        y.resize(half_arr);
+        for (size_t i = 0; i < half_arr; i++) {
+            double a1 = 0.0, a2 = 0.0;
+            for (size_t j = 0; j < half_arr; j++) {
+                a1 += D[j + i * half_arr] * prof[j].real();
+                a2 += D[j + i * half_arr] * prof[j].imag();
+            }
+            y[i] = cx(a1, a2);
+        }
+    } else {
+        std::vector<cx> proj(arr, cx(0.0, 0.0));
+        for (size_t i = 1; i < half_k; i++)
+            proj[i] = combine_2d_kernel_value(aa, i, 0, w_value, scaled_cell, unsigned(half_k));
+        cx s0(0.0, 0.0);
+        for (size_t i = 0; i < arr; i++)
+            s0 += proj[i];
+        proj[0] = s0 * cx(2.0, 0.0);
+        proj[0] += aa[half_k] * aa[half_k];
+        for (size_t j = 1; j < half_k; j++)
+            for (size_t i = 1; i < half_k; i++)
+                proj[j] += combine_2d_kernel_value(aa, i, j, w_value, scaled_cell, unsigned(half_k)) * cx(2.0, 0.0);
+        for (size_t i = 1; i < half_k; i++)
+            proj[arr - i] = proj[i];
+        // forward DFT (fft_fftw_dft_c2c_1d, FFTW_FORWARD)
+        y.resize(arr);
+        for (size_t k = 0; k < arr; k++) {
+            cx acc(0.0, 0.0);
+            for (size_t j = 0; j < arr; j++) {
+                const double ang = -2.0 * M_PI * double((j * k) % arr) / double(arr);
+                acc += proj[j] * cx(std::cos(ang), std::sin(ang));
+            }
+            y[k] = acc;
+        }
+    }
+    if (radius_out)
+        *radius_out = y;
+    const size_t n = y.size();
+    // truncation (aw_projection.h:132-148)
+    unsigned ts = max_support;
+    if (trunc_perc > 0.0) {
+        const double min_value = std::abs(y[0]) * (trunc_perc / 100.0);
+        size_t idx = size_t(max_support) * oversampling;
+        const double scale = 1.0 / double(oversampling);
+        while (idx > oversampling) {
+            if (std::abs(y[idx]) > min_value)
+                break;
+            idx -= oversampling;
+        }
+        ts = std::min(unsigned(std::ceil(double(idx) * scale)), max_support);
+    }
+    if (trunc_support)
+        *trunc_support = ts;
+    // RadialInterpolate: the half quadrant {(ii, j): ii <= j < ko} row by row
+    const size_t ko = (size_t(ts) * 2 + 2) * oversampling / 2;
+    std::vector<double> yr(n), yi(n), ar(n), br(n), cr(n), ai(n), bi(n), ci(n);
+    for (size_t i = 0; i < n; i++) {
+        yr[i] = y[i].real();
+        yi[i] = y[i].imag();
+    }
+    spline_coefs(yr.data(), n, cubic, ar.data(), br.data(), cr.data());
+    spline_coefs(yi.data(), n, cubic, ai.data(), bi.data(), ci.data());
+    std::vector<cx> kh(ko + ko * (ko - 1) / 2);
+    size_t q = 0;
+    for (size_t ii = 0; ii < ko; ii++)
+        for (size_t j = ii; j < ko; j++) {
+            const double nx = (ii == 0) ? double(j) : std::sqrt(double(ii * ii + j * j));
+            size_t k = size_t(nx);
+            if (k >= n)
+                k = n - 1;
+            kh[q++] = cx(spline_eval(ar.data(), br.data(), cr.data(), yr.data(), k, nx, cubic),
+                spline_eval(ai.data(), bi.data(), ci.data(), yi.data(), k, nx, cubic));
+        }
+    // generate_kernel_cache, Hankel branch
+    const long long os = oversampling, op = os / 2, S = ts;
+    const size_t cache_size = size_t(2 * op + 1), tcs = size_t(2 * S + 1);
+    std::vector<std::vector<cx>> cache(cache_size * cache_size, std::vector<cx>(tcs * tcs));
+    for (long long x = 0; x < (long long)cache_size; x++)
+        for (long long yv = 0; yv < (long long)cache_size; yv++) {
+            std::vector<cx>& tmp = cache[size_t(yv) * cache_size + size_t(x)];
+            double ksum = 0.0;
+            size_t ty = 0;
+            for (long long cxv = -S * os + op - x; cxv < S * os + op - x + 1; cxv += os, ty++) {
+                size_t tx = 0;
+                for (long long cyv = -S * os + op - yv; cyv < S * os + op - yv + 1; cyv += os, tx++) {
+                    const size_t dcx = size_t(cxv < 0 ? -cxv : cxv), dcy = size_t(cyv < 0 ? -cyv : cyv);
+                    const size_t vi = dcx > dcy ? dcy * ko - ((dcy * (dcy + 1)) >> 1) + dcx
+                                                : dcx * ko - ((dcx * (dcx + 1)) >> 1) + dcy;
+                    tmp[tx + ty * tcs] = kh[vi];
+                    ksum += kh[vi].real();
+                }
+            }
+            for (auto& v : tmp)
+                v = cx(v.real() / ksum, v.imag() / ksum);
+        }
+    return cache;
+}
+
 GridderOutput convolve_to_grid(const KernelFunc& kf, unsigned support, int image_size,
     const double* uv_in, const cx* vis_in, const double* weights_in, size_t n,
     bool kernel_exact, unsigned oversampling, bool shift_uv, bool halfplane, bool generate_beam,
@@ -226,8 +461,6 @@ GridderOutput convolve_to_grid(const KernelFunc& kf, unsigned support, int image
     if ((N % 2) != 0)
         throw std::invalid_argument("image_size must be even");
     if (use_wproj) {
-        if (wp.hankel)
-            throw std::invalid_argument("Hankel W-kernels are not restated");
         conv_support = int(wp.max_wpconv_support);
         kernel_exact = false;
     }
@@ -352,8 +585,12 @@ GridderOutput convolve_to_grid(const KernelFunc& kf, unsigned support, int image
             int cs_plane = conv_support;
             if (use_wproj) {
                 unsigned ts = 0;
-                cache = wproj_kernel_cache(w_avg[pi], aa, workarea, cell_rad, oversampling, scaling, unsigned(conv_support), nthreads,
-                    wp.kernel_trunc_perc, &ts);
+                if (wp.hankel)
+                    cache = hankel_kernel_cache(w_avg[pi], aa, workarea, cell_rad, oversampling, scaling, unsigned(conv_support),
+                        wp.hankel_proj_slice, wp.interp_cubic, wp.kernel_trunc_perc, &ts, nullptr);
+                else
+                    cache = wproj_kernel_cache(w_avg[pi], aa, workarea, cell_rad, oversampling, scaling, unsigned(conv_support),
+                        nthreads, wp.kernel_trunc_perc, &ts);
                 // the plane grids with its truncated support (gridder.h:756-758)
                 cs_plane = int(ts);
             }

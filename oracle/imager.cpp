@@ -60,6 +60,8 @@ ImagerResult image_visibilities(const stp_imager_pars& p, const stp_wproj_pars* 
         wp.undersampling_opt = wpp->undersampling_opt;
         wp.kernel_trunc_perc = wpp->kernel_trunc_perc;
         wp.hankel = wpp->hankel_opt != 0;
+        wp.hankel_proj_slice = wpp->hankel_proj_slice != 0;
+        wp.interp_cubic = wpp->interp_type == 2;
         wp.wplanes_median = wpp->wplanes_median != 0;
         if (p.kernel_exact)
             throw std::invalid_argument("W-projection requires kernel_exact == false (imager.h:287)");

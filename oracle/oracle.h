@@ -86,6 +86,7 @@ struct WProjPars {
     unsigned num_wplanes = 0, max_wpconv_support = 0, undersampling_opt = 1;
     double kernel_trunc_perc = 0.0;
     bool hankel = false, wplanes_median = false;
+    bool hankel_proj_slice = false, interp_cubic = false; // Hankel kernels: projection slice; cubic spline (interp_type 2)
 };
 
 GridderOutput convolve_to_grid(const KernelFunc& kf, unsigned support, int image_size,
@@ -100,6 +101,17 @@ GridderOutput convolve_to_grid(const KernelFunc& kf, unsigned support, int image
 std::vector<std::vector<cx>> wproj_kernel_cache(double w_value, const std::vector<double>& aa_kernel_img,
     unsigned workarea_size, double cell_rad, unsigned oversampling, double scaling_factor, unsigned support, int nthreads,
     double trunc_perc = 0.0, unsigned* trunc_support = nullptr);
+
+// Hankel W-kernels (aw_projection.cpp:211-287,380-411,477-596; aw_projection.h:127-213).
+double besselj1(double x);
+std::vector<double> dht_matrix(size_t n);
+void spline_coefs(const double* y, size_t n, bool cubic, double* a, double* b, double* c);
+double spline_eval(const double* a, const double* b, const double* c, const double* y, size_t k, double x, bool cubic);
+// Cache of one plane, laid out as wproj_kernel_cache's; radius_out (may be NULL)
+// receives the transformed radial profile (DHT) or the DFT of the projection.
+std::vector<std::vector<cx>> hankel_kernel_cache(double w_value, const std::vector<double>& aa_kernel_img,
+    unsigned workarea_size, double cell_rad, unsigned oversampling, double scaling_factor, unsigned max_support,
+    bool proj_slice, bool cubic, double trunc_perc, unsigned* trunc_support, std::vector<cx>* radius_out);
 
 /* ---------------- imager: src/stp/imager/imager.h ---------------- */
 struct ImagerResult {

@@ -156,6 +156,53 @@ def wproj_kernel_cache(w_value, aa, workarea, cell_rad, oversampling, scaling, s
     return (res, ts.value) if return_support else res
 
 
+def hankel_kernel_cache(w_value, aa, workarea, cell_rad, oversampling, scaling, support, proj_slice=False, cubic=False,
+                        trunc_perc=0.0):
+    """Hankel W-kernel cache of one plane (aw_projection.cpp:211-287,380-411): returns
+    (cache[y, x, row, col], truncated support, transformed radial profile)."""
+    cs = (oversampling // 2) * 2 + 1
+    ks = 2 * support + 1
+    out = np.empty((cs, cs, ks, ks), dtype=np.complex128)
+    aa = _f64(aa)
+    ts = ctypes.c_uint()
+    cap = workarea * oversampling
+    rad = np.empty(cap, dtype=np.complex128)
+    rn = ctypes.c_size_t()
+    _check(lib().oracle_hankel_kernel_cache(ctypes.c_double(w_value), _ptr(aa), ctypes.c_uint(workarea),
+                                            ctypes.c_double(cell_rad), ctypes.c_uint(oversampling),
+                                            ctypes.c_double(scaling), ctypes.c_uint(support), ctypes.c_int(int(proj_slice)),
+                                            ctypes.c_int(int(cubic)), ctypes.c_double(trunc_perc), ctypes.byref(ts),
+                                            _ptr(out), _ptr(rad), ctypes.c_size_t(cap), ctypes.byref(rn)))
+    kt = 2 * ts.value + 1
+    flat = out.reshape(-1)[: cs * cs * kt * kt].reshape(cs, cs, kt, kt)
+    return np.ascontiguousarray(np.swapaxes(flat, 2, 3)), ts.value, rad[: rn.value].copy()
+
+
+def besselj1(x):
+    """besselj1 (aw_projection.cpp:490-508)."""
+    x = _f64(np.atleast_1d(x))
+    out = np.empty_like(x)
+    _check(lib().oracle_besselj1(_ptr(x), ctypes.c_size_t(x.size), _ptr(out)))
+    return out
+
+
+def dht_matrix(n):
+    """dht (aw_projection.cpp:530-596): D[j, i] (numpy row j, column i)."""
+    out = np.empty(n * n)
+    _check(lib().oracle_dht_matrix(ctypes.c_size_t(n), _ptr(out)))
+    return out.reshape(n, n).T.copy()  # column-major D[j + i n]
+
+
+def spline(y, x, cubic=True):
+    """tk::spline (natural cubic or linear) through (k, y[k]), k = 0..n-1, evaluated at x."""
+    y = _f64(y)
+    x = _f64(np.atleast_1d(x))
+    out = np.empty_like(x)
+    _check(lib().oracle_spline(_ptr(y), ctypes.c_size_t(y.size), ctypes.c_int(int(cubic)), _ptr(x), ctypes.c_size_t(x.size),
+                               _ptr(out)))
+    return out
+
+
 def average_w_planes(abs_w_sorted, good, planes, median=False):
     """average_w_planes (gridder.cpp:68-146) on |w| already in ascending order."""
     w = _f64(abs_w_sorted)

@@ -94,6 +94,8 @@ int oracle_convolve_to_grid(const stp_kernel_spec* k, unsigned support, int imag
             wp.undersampling_opt = wpp->undersampling_opt;
             wp.kernel_trunc_perc = wpp->kernel_trunc_perc;
             wp.hankel = wpp->hankel_opt != 0;
+            wp.hankel_proj_slice = wpp->hankel_proj_slice != 0;
+            wp.interp_cubic = wpp->interp_type == 2;
             wp.wplanes_median = wpp->wplanes_median != 0;
         }
         GridderOutput g = convolve_to_grid(make_kernel_func(*k), support, image_size, uv,
@@ -128,6 +130,59 @@ int oracle_wproj_kernel_cache(double w_value, const double* aa, unsigned workare
         for (auto& k : c) {
             std::memcpy(out + off, k.data(), k.size() * sizeof(cx));
             off += 2 * k.size();
+        }
+    });
+}
+
+int oracle_hankel_kernel_cache(double w_value, const double* aa, unsigned workarea, double cell_rad, unsigned os,
+    double scaling, unsigned support, int proj_slice, int cubic, double trunc_perc, unsigned* trunc_support, double* out,
+    double* radius, size_t radius_cap, size_t* radius_n)
+{
+    return guard([&] {
+        std::vector<double> a(aa, aa + workarea);
+        std::vector<cx> rad;
+        auto c = hankel_kernel_cache(w_value, a, workarea, cell_rad, os, scaling, support, proj_slice != 0, cubic != 0,
+            trunc_perc, trunc_support, &rad);
+        size_t off = 0;
+        for (auto& k : c) {
+            std::memcpy(out + off, k.data(), k.size() * sizeof(cx));
+            off += 2 * k.size();
+        }
+        if (radius_n)
+            *radius_n = rad.size();
+        if (radius)
+            std::memcpy(radius, rad.data(), std::min(radius_cap, rad.size()) * sizeof(cx));
+    });
+}
+
+int oracle_besselj1(const double* x, size_t n, double* out)
+{
+    return guard([&] {
+        for (size_t i = 0; i < n; i++)
+            out[i] = besselj1(x[i]);
+    });
+}
+
+int oracle_dht_matrix(size_t n, double* out)
+{
+    return guard([&] {
+        const std::vector<double> d = dht_matrix(n);
+        std::memcpy(out, d.data(), d.size() * sizeof(double));
+    });
+}
+
+int oracle_spline(const double* y, size_t n, int cubic, const double* x, size_t m, double* out)
+{
+    return guard([&] {
+        if (n < 3)
+            throw std::invalid_argument("spline needs at least 3 points");
+        std::vector<double> a(n), b(n), c(n);
+        spline_coefs(y, n, cubic != 0, a.data(), b.data(), c.data());
+        for (size_t i = 0; i < m; i++) {
+            size_t k = x[i] < 0.0 ? 0 : size_t(x[i]);
+            if (k >= n)
+                k = n - 1;
+            out[i] = spline_eval(a.data(), b.data(), c.data(), y, k, x[i], cubic != 0);
         }
     });
 }

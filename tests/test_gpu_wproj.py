@@ -88,17 +88,38 @@ def test_gpu_wproj_fp32(gpu, orc, precision):
 
 
 def test_gpu_wproj_errors(gpu):
-    """Exact kernels and Hankel kernels are rejected with status codes."""
+    """Exact kernels and an unknown interpolation type are rejected with status codes."""
     snap = simdata.snapshot(64, 20.0, 100, 1, seed=5, w_max=10.0)
     wp = capi.wproj_pars(2, 4)
     with pytest.raises(STPError) as e:
         gpu.image_visibilities(snap["vis"], snap["weights"], snap["uvw"], capi.imager_pars(64, 20.0, kernel_exact=True),
                                w_proj=wp)
     assert e.value.status == capi.STATUS_CODES["STP_ERR_INVALID_ARGUMENT"]
-    wp.hankel_opt = 1
+    wp = capi.wproj_pars(2, 4, hankel_opt=True, interp_type=3)
     with pytest.raises(STPError) as e:
         gpu.image_visibilities(snap["vis"], snap["weights"], snap["uvw"], capi.imager_pars(64, 20.0), w_proj=wp)
-    assert e.value.status == capi.STATUS_CODES["STP_ERR_UNSUPPORTED"]
+    assert e.value.status == capi.STATUS_CODES["STP_ERR_INVALID_ARGUMENT"]
+
+
+@pytest.mark.parametrize("proj_slice,interp,trunc", [(False, 0, 0.0), (False, 2, 0.0), (True, 0, 0.0), (True, 2, 0.0),
+                                                     (False, 2, 2.0), (True, 1, 2.0)])
+def test_gpu_wproj_hankel(gpu, orc, proj_slice, interp, trunc):
+    """Hankel W-kernels (aw_projection.cpp:211-287,380-411): DHT of the radial profile or
+    DFT of the projection, linear / cosine (linear) / cubic spline, with and without
+    truncation, against the oracle at N=256; both FP64 and the FP32 path."""
+    snap = simdata.snapshot(256, 20.0, 4000, 6, seed=341, field_frac=0.5, w_max=400.0)
+    wp = capi.wproj_pars(5, 10, undersampling_opt=1, kernel_trunc_perc=trunc, hankel_opt=True,
+                         hankel_proj_slice=proj_slice, interp_type=interp)
+    img, beam, oimg, obeam = _wcase(gpu, orc, snap, 256, 20.0, wp)
+    e_i, e_b = peak_rel_err(img, oimg), peak_rel_err(beam, obeam)
+    img32, beam32, _, _ = _wcase(gpu, orc, snap, 256, 20.0, wp, precision=capi.PRECISION_FP32C)
+    e32 = max(peak_rel_err(img32, oimg), peak_rel_err(beam32, obeam))
+    # the Hankel kernels differ from the 2-D FFT ones: the images must differ by far more
+    _, _, fimg, _ = _wcase(gpu, orc, snap, 256, 20.0, capi.wproj_pars(5, 10, undersampling_opt=1))
+    print(f"hankel slice={proj_slice} interp={interp} trunc={trunc}: image err {e_i:.3e}, beam err {e_b:.3e}, "
+          f"fp32 {e32:.3e}, vs 2-D kernels {peak_rel_err(oimg, fimg):.2e}", flush=True)
+    assert peak_rel_err(oimg, fimg) > 100 * TOL
+    assert e_i < TOL and e_b < TOL and e32 < 1e-4
 
 
 C3_PARS = dict(kernel_support=3, oversampling=8)
