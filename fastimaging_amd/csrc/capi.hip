@@ -45,6 +45,8 @@ stp_status guarded(stp_ctx* ctx, F f)
     }
 }
 
+constexpr size_t MAX_BATCH_GRAPHS = 16; // captured batch graphs kept per context
+
 bool is_pow2(uint32_t x) { return x && !(x & (x - 1)); }
 
 // Parameter checks of image_visibilities (imager.h:257-287).
@@ -924,7 +926,19 @@ stp_status stp_image_batch(stp_ctx* ctx, size_t n_items, const stp_batch_item* i
                 const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(graph);
                 HIP_CHECK(ie);
+                // a bounded cache: batches with new buffers or sizes on every call would
+                // otherwise keep one graph each; the oldest goes once its launches are done
+                if (ctx->graphs.size() >= MAX_BATCH_GRAPHS) {
+                    HIP_CHECK(hipStreamSynchronize(main));
+                    auto old = ctx->graphs.find(ctx->graph_order.front());
+                    if (old != ctx->graphs.end()) {
+                        (void)hipGraphExecDestroy(old->second);
+                        ctx->graphs.erase(old);
+                    }
+                    ctx->graph_order.pop_front();
+                }
                 ctx->graphs[sig] = exec;
+                ctx->graph_order.push_back(sig);
             }
         } else {
             enqueue();

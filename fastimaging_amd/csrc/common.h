@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <deque>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -109,6 +110,7 @@ struct stp_ctx {
     std::vector<hipStream_t> branch_streams;
     std::vector<hipEvent_t> branch_events;
     std::map<std::string, hipGraphExec_t> graphs;
+    std::deque<std::string> graph_order; // capture order: the oldest graph is dropped beyond MAX_BATCH_GRAPHS
     std::vector<stp_hip::PendingTiming> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, stp_hip::TimingAcc> timings;

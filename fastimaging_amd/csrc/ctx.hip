@@ -90,6 +90,7 @@ void invalidate_graphs(stp_ctx* ctx)
     for (auto& kv : ctx->graphs)
         (void)hipGraphExecDestroy(kv.second);
     ctx->graphs.clear();
+    ctx->graph_order.clear();
 }
 
 void* host_stage(stp_ctx* ctx, const std::string& name, size_t bytes)

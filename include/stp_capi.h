@@ -506,8 +506,9 @@ typedef struct stp_batch_item {
  * context (0 = 4), each stream with its own workspace, so small snapshots fill
  * the device together.  With STP_BATCH_GRAPH the first call with a given batch
  * (parameters, sizes and buffer addresses) runs eagerly and records the whole
- * batch into one hipGraph; later calls with the same batch launch that graph.
- * sample_grid_totals (host, n_items, or NULL) receives 2 * sum of good weights
+ * batch into one hipGraph; later calls with the same batch launch that graph
+ * (the flag is for fixed, reused buffers: a context keeps the 16 most recent
+ * graphs).  sample_grid_totals (host, n_items, or NULL) receives 2 * sum of good weights
  * per snapshot; the call returns after the batch has finished. */
 stp_status stp_image_batch(stp_ctx* ctx, size_t n_items, const stp_batch_item* items, const stp_kernel_spec* kernel,
     int32_t precision, uint32_t max_concurrent, uint32_t flags, double* sample_grid_totals);

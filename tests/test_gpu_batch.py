@@ -108,6 +108,47 @@ def test_batch_matches_single_and_graph_replays(gpu, orc, precision):
             got = b.images()
             for (a, ab), (c, cb) in zip(eager, got):
                 assert peak_rel_err(c, a) <= tol and peak_rel_err(cb, ab) <= tol, rep
+        # new visibilities in the same buffers: the replay images the new data (nothing
+        # of the first data is baked into the graph)
+        for k, (N, cell, nv, ns, seed, wp, wmax) in enumerate(_specs()):
+            sn = simdata.snapshot(N, cell, nv, ns + 2, seed=seed + 50, field_frac=0.5 if wp is not None else 0.25,
+                                  w_max=wmax)
+            gpu.htod(b.bufs[k][0], np.ascontiguousarray(sn["vis"]))
+            gpu.htod(b.bufs[k][1], np.ascontiguousarray(sn["weights"]))
+            gpu.htod(b.bufs[k][2], np.asfortranarray(sn["uvw"]))
+        b.clear()
+        gpu.image_batch(b.items, precision=precision, max_concurrent=3, flags=capi.BATCH_GRAPH)
+        replayed = b.images()
+        for it in b.items:
+            gpu.image_visibilities_device(it.img_pars, it.n_vis, it.d_vis, it.d_vis_weights, it.d_uvw_lambda,
+                                          it.d_image, it.d_beam, precision=precision, synchronize=True,
+                                          w_proj=it.w_proj if it.w_proj.num_wplanes else None)
+        fresh = b.images()
+        for (a, ab), (c, cb), (o, _) in zip(fresh, replayed, eager):
+            assert peak_rel_err(c, a) <= tol and peak_rel_err(cb, ab) <= tol
+            assert peak_rel_err(a, o) > 1e-3  # the data did change
+    finally:
+        b.free()
+
+
+def test_batch_graph_cache_is_bounded(gpu):
+    """More distinct batches than the context keeps graphs for: the oldest graphs are
+    dropped and re-recorded when their batch comes back, with the same images."""
+    b = _Batch(gpu, [(256, 4.0, 5_000, 3, 711, None, 0.0)], capi.PRECISION_FP64)
+    try:
+        it = b.items[0]
+        first = None
+        for k in range(20):
+            it.n_vis = 5_000 - k  # a new batch signature each call
+            b.clear()
+            gpu.image_batch(b.items, precision=capi.PRECISION_FP64, flags=capi.BATCH_GRAPH)
+            if k == 0:
+                first = b.images()[0][0]
+        it.n_vis = 5_000
+        for _ in range(2):
+            b.clear()
+            gpu.image_batch(b.items, precision=capi.PRECISION_FP64, flags=capi.BATCH_GRAPH)
+            assert peak_rel_err(b.images()[0][0], first) <= 1e-12
     finally:
         b.free()
 
