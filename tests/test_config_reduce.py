@@ -42,6 +42,14 @@ def test_wproj_and_aproj_config():
     assert c.pbeam_coefs == [0, 0, 0, 0, 1, 0, 1, 0, 0]
 
 
+def test_hankel_config():
+    c = ConfigurationFile(os.path.join(CFG, "fastimg_config_wproj_HANKEL_size11.json"))
+    p, w = c.img_pars, c.w_proj
+    assert (p.image_size, p.cell_size, p.kernel_function, p.oversampling) == (2048, 60.0, capi.KERNEL_GAUSSIAN, 8)
+    assert (w.num_wplanes, w.max_wpconv_support, w.kernel_trunc_perc) == (128, 127, 1.0)
+    assert (w.hankel_opt, w.hankel_proj_slice, w.interp_type) == (1, 1, 2)
+
+
 def test_pipeline_test_configs():
     g = ConfigurationFile(os.path.join(CFG, "fastimg_oversampling_gaussiansinc_config.json"))
     assert (g.img_pars.kernel_function, g.img_pars.oversampling, g.img_pars.generate_beam) == (capi.KERNEL_GAUSSIANSINC, 8, 1)

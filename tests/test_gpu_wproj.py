@@ -122,6 +122,28 @@ def test_gpu_wproj_hankel(gpu, orc, proj_slice, interp, trunc):
     assert e_i < TOL and e_b < TOL and e32 < 1e-4
 
 
+def test_gpu_wproj_hankel_reference_config(gpu, orc):
+    """The reference's Hankel configuration (configs/reduce-wproj/
+    fastimg_config_wproj_HANKEL_size11.json: N=2048, 60" cells, Gaussian s3, os 8,
+    support 127, projection slice, cubic spline, 1 % truncation) with 16 of its 128
+    planes, against the oracle."""
+    import os
+    from fastimaging_amd.config import ConfigurationFile
+    c = ConfigurationFile(os.path.join(os.path.dirname(__file__), "golden", "configs",
+                                       "fastimg_config_wproj_HANKEL_size11.json"))
+    c.w_proj.num_wplanes = 16
+    N, cell = c.img_pars.image_size, c.img_pars.cell_size
+    snap = simdata.snapshot(N, cell, 3000, 6, seed=351, field_frac=0.5, w_max=800.0)
+    pars = c.img_pars
+    img, beam, diag = gpu.image_visibilities(snap["vis"], snap["weights"], snap["uvw"], pars, w_proj=c.w_proj,
+                                             return_diag=True)
+    oimg, obeam, ototal, ongood = orc.image_visibilities(pars, snap["vis"], snap["weights"], snap["uvw"], wproj=c.w_proj)
+    assert diag.num_good_vis == ongood > 0
+    e = peak_rel_err(img, oimg)
+    print(f"hankel reference config (16 planes, support 127): image err {e:.3e}", flush=True)
+    assert e < TOL
+
+
 C3_PARS = dict(kernel_support=3, oversampling=8)
 
 
