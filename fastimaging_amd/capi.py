@@ -154,7 +154,7 @@ class BatchItem(ctypes.Structure):
     ]
 
 
-BATCH_GRAPH, BATCH_AUTOTUNE = 1, 2
+BATCH_GRAPH, BATCH_AUTOTUNE, BATCH_FUSED = 1, 2, 4
 
 
 class ServerCfg(ctypes.Structure):

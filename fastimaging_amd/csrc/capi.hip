@@ -4,7 +4,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 #include <exception>
 
@@ -110,6 +112,50 @@ std::string tune_key(const stp_imager_pars* p, const stp_wproj_pars* wp, size_t 
     return k;
 }
 
+// Gridder arguments of one imaging call (imager.h:233-376) with the autotuned
+// launch shape of its imaging shape; the FFT's u split in *fft_l1.
+GridderArgs imager_args(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp, const stp_kernel_spec& kern,
+    size_t n, int precision, int* fft_l1)
+{
+    const int N = int(p->padded_image_size);
+    GridderArgs a;
+    a.image_size = N;
+    a.support = int(p->kernel_support);
+    a.oversampling = int(p->oversampling);
+    a.kernel_exact = p->kernel_exact != 0;
+    a.shift_uv = true;
+    a.halfplane = true;
+    a.generate_beam = p->generate_beam != 0;
+    a.uv_scale = ((p->cell_size / 3600.0) * (M_PI / 180.0)) * double(N); // imager.h:293
+    a.kernel = kern;
+    a.twofloat = precision == STP_PRECISION_FP32_TWOFLOAT;
+    if (wp && wp->num_wplanes > 0)
+        a.wproj = *wp;
+    a.cell_rad = (p->cell_size / 3600.0) * (M_PI / 180.0);
+    a.analytic_gcf = p->analytic_gcf != 0;
+    *fft_l1 = 0;
+    auto tc = ctx->tune.find(tune_key(p, wp, n, precision));
+    if (tc != ctx->tune.end()) {
+        a.tile_cols = tc->second.tile_cols;
+        *fft_l1 = tc->second.fft_l1;
+    }
+    return a;
+}
+
+// Gridding correction table of the padded size (imager.h:96-213), or NULL when
+// the call has none.
+const double* gcf_table(stp_ctx* ctx, const stp_imager_pars* p, const stp_kernel_spec& kern)
+{
+    if (!p->gridding_correction)
+        return nullptr;
+    const int N = int(p->padded_image_size);
+    const KernelFn kf = kernel_fn(kern);
+    char key[192];
+    snprintf(key, sizeof(key), "gcf:%d:%.17g:%.17g:%.17g:%d:%d", kf.kind, kf.param, kf.width_g, kf.width_s, N, p->analytic_gcf);
+    return has_table(ctx, key) ? dev_table(ctx, key, {})
+                               : dev_table(ctx, key, img_dom_kernel(kf, size_t(N), false, p->analytic_gcf != 0));
+}
+
 template <typename T, typename OT>
 void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp, const stp_kernel_spec* kspec, size_t n,
     const double* d_vis, const double* d_w, const double* d_uvw, OT* d_image, OT* d_beam, double* d_total,
@@ -120,28 +166,11 @@ void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp
     const int N = int(p->padded_image_size);
     const bool beam = p->generate_beam != 0;
     STP_REQUIRE(!beam || d_beam != nullptr, "beam buffer is NULL with generate_beam");
-    GridderArgs a;
-    a.image_size = N;
-    a.support = int(p->kernel_support);
-    a.oversampling = int(p->oversampling);
-    a.kernel_exact = p->kernel_exact != 0;
-    a.shift_uv = true;
-    a.halfplane = true;
-    a.generate_beam = beam;
-    a.uv_scale = ((p->cell_size / 3600.0) * (M_PI / 180.0)) * double(N); // imager.h:293
-    a.kernel = kern;
-    a.twofloat = twofloat;
-    if (wp && wp->num_wplanes > 0)
-        a.wproj = *wp;
-    a.cell_rad = (p->cell_size / 3600.0) * (M_PI / 180.0);
-    a.analytic_gcf = p->analytic_gcf != 0;
-    a.d_nentries = d_nentries;
     int fft_l1 = 0;
-    auto tc = ctx->tune.find(tune_key(p, wp, n, std::is_same<T, double>::value ? 0 : (twofloat ? 2 : 1)));
-    if (tc != ctx->tune.end()) {
-        a.tile_cols = tc->second.tile_cols;
-        fft_l1 = tc->second.fft_l1;
-    }
+    const int precision = std::is_same<T, double>::value ? STP_PRECISION_FP64
+                                                          : (twofloat ? STP_PRECISION_FP32_TWOFLOAT : STP_PRECISION_FP32C);
+    GridderArgs a = imager_args(ctx, p, wp, kern, n, precision, &fft_l1);
+    a.d_nentries = d_nentries;
     const int rows = N / 2 + 1;
     const size_t ge = size_t(rows) * N;
     cplx<T>* grids = (cplx<T>*)ws_get(ctx, "imager_grids", sizeof(cplx<T>) * ge * (beam ? 2 : 1));
@@ -150,18 +179,108 @@ void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp
     ns.image_size = int(p->image_size);
     ns.total_ptr = d_total;
     ns.u_l1 = fft_l1;
-    if (p->gridding_correction) {
-        const KernelFn kf = kernel_fn(kern);
-        char key[192];
-        snprintf(key, sizeof(key), "gcf:%d:%.17g:%.17g:%.17g:%d:%d", kf.kind, kf.param, kf.width_g, kf.width_s, N,
-            p->analytic_gcf);
-        ns.gcf = has_table(ctx, key) ? dev_table(ctx, key, {})
-                                     : dev_table(ctx, key, img_dom_kernel(kf, size_t(N), false, p->analytic_gcf != 0));
-    }
+    ns.gcf = gcf_table(ctx, p, kern);
     OutPtrs<OT> outs{};
     outs.p[0] = d_image;
     outs.p[1] = d_beam;
     fft_c2r_2d<T, OT>(ctx, grids, beam ? 2 : 1, N, N, outs, ns);
+}
+
+// Whether a batch item can join the fused launches (see grid_batch_eligible).
+bool batch_fusable(stp_ctx* ctx, const stp_batch_item& it, const stp_kernel_spec* kspec, int precision)
+{
+    if (it.w_proj.num_wplanes > 0)
+        return false;
+    const stp_kernel_spec kern = kspec ? *kspec : dispatch_kernel(&it.img_pars);
+    int l1;
+    return grid_batch_eligible(imager_args(ctx, &it.img_pars, nullptr, kern, it.n_vis, precision, &l1), it.n_vis);
+}
+
+// The fused items of a batch: one gridding launch per stage over all of them
+// (grid_batch), then one c2r per group of equal transform shape (padded size,
+// image size, gcf, u split, beam) over the group's grids, which lie contiguous.
+template <typename T, typename OT>
+void run_batch_fused(stp_ctx* ctx, const stp_batch_item* items, const std::vector<size_t>& idx, const stp_kernel_spec* kspec,
+    int precision, double* totals)
+{
+    struct Shape {
+        int N, image, l1, beam;
+        const double* gcf;
+        bool operator<(const Shape& o) const
+        {
+            return std::tie(N, image, l1, beam, gcf) < std::tie(o.N, o.image, o.l1, o.beam, o.gcf);
+        }
+    };
+    std::vector<BatchGridItem> bg(idx.size());
+    std::map<Shape, std::vector<size_t>> groups; // shape -> positions in bg
+    for (size_t q = 0; q < idx.size(); q++) {
+        const stp_batch_item& it = items[idx[q]];
+        const stp_imager_pars* p = &it.img_pars;
+        const stp_kernel_spec kern = kspec ? *kspec : dispatch_kernel(p);
+        check_kernel_spec(&kern);
+        int l1 = 0;
+        BatchGridItem& b = bg[q];
+        b.a = imager_args(ctx, p, nullptr, kern, it.n_vis, precision, &l1);
+        b.n = it.n_vis;
+        b.uv = it.d_uvw_lambda;
+        b.vis = it.d_vis;
+        b.wts = it.d_vis_weights;
+        b.total = totals + 2 * idx[q];
+        b.ngood = (unsigned long long*)(totals + 2 * idx[q] + 1);
+        groups[Shape{ int(p->padded_image_size), int(p->image_size), l1, p->generate_beam ? 1 : 0, gcf_table(ctx, p, kern) }]
+            .push_back(q);
+    }
+    // grids: each group's grids contiguous in item order (vis grid, then beam grid)
+    size_t elems = 0;
+    for (auto& gr : groups)
+        elems += size_t(gr.first.N / 2 + 1) * gr.first.N * (gr.first.beam ? 2 : 1) * gr.second.size();
+    cplx<T>* grids = (cplx<T>*)ws_get(ctx, "bat_grids", sizeof(cplx<T>) * std::max<size_t>(elems, 1));
+    // per group: output and total pointer per grid, in the uploaded table
+    std::vector<uint8_t> extra;
+    struct Run {
+        Shape s;
+        cplx<T>* grids;
+        int nbatch;
+        size_t out_off, tot_off;
+    };
+    std::vector<Run> runs;
+    size_t cur = 0;
+    for (auto& gr : groups) {
+        const Shape& s = gr.first;
+        const size_t ge = size_t(s.N / 2 + 1) * s.N;
+        Run r{ s, grids + cur, int(gr.second.size() * (s.beam ? 2 : 1)), 0, 0 };
+        std::vector<OT*> outs;
+        std::vector<const double*> tots;
+        for (size_t q : gr.second) {
+            const stp_batch_item& it = items[idx[q]];
+            bg[q].vgrid = grids + cur;
+            cur += ge;
+            outs.push_back((OT*)it.d_image);
+            tots.push_back(bg[q].total);
+            if (s.beam) {
+                bg[q].bgrid = grids + cur;
+                cur += ge;
+                outs.push_back((OT*)it.d_beam);
+                tots.push_back(bg[q].total);
+            }
+        }
+        r.out_off = extra.size();
+        extra.insert(extra.end(), (const uint8_t*)outs.data(), (const uint8_t*)(outs.data() + outs.size()));
+        r.tot_off = extra.size();
+        extra.insert(extra.end(), (const uint8_t*)tots.data(), (const uint8_t*)(tots.data() + tots.size()));
+        runs.push_back(r);
+    }
+    const uint8_t* d_extra = (const uint8_t*)grid_batch<T>(ctx, bg, extra);
+    for (const Run& r : runs) {
+        NormSpec ns;
+        ns.image_size = r.s.image;
+        ns.total_tab = (const double* const*)(d_extra + r.tot_off);
+        ns.u_l1 = r.s.l1;
+        ns.gcf = r.s.gcf;
+        OutPtrs<OT> outs{};
+        outs.tab = (OT* const*)(d_extra + r.out_off);
+        fft_c2r_2d<T, OT>(ctx, r.grids, r.nbatch, r.s.N, r.s.N, outs, ns);
+    }
 }
 
 // Imaging on device buffers in the requested precision.
@@ -339,6 +458,8 @@ stp_status stp_ctx_destroy(stp_ctx* ctx)
         (void)hipEventDestroy(ctx->copy_event);
     if (ctx->skymodel_event)
         (void)hipEventDestroy(ctx->skymodel_event);
+    if (ctx->batch_tab_event)
+        (void)hipEventDestroy(ctx->batch_tab_event);
     for (auto& kv : ctx->ws)
         if (kv.second.ptr)
             (void)hipFree(kv.second.ptr);
@@ -831,17 +952,6 @@ stp_status stp_image_batch(stp_ctx* ctx, size_t n_items, const stp_batch_item* i
         }
         if (n_items == 0)
             return;
-        const int K = int(std::min<size_t>(n_items, max_concurrent ? std::min(max_concurrent, 16u) : 4u));
-        while (int(ctx->branch_streams.size()) < K) {
-            hipStream_t st;
-            HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-            ctx->branch_streams.push_back(st);
-        }
-        while (ctx->branch_events.size() < size_t(K) + 1) {
-            hipEvent_t e;
-            HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            ctx->branch_events.push_back(e);
-        }
         const stp_wproj_pars* none = nullptr;
         auto wp_of = [&](const stp_batch_item& it) { return it.w_proj.num_wplanes > 0 ? &it.w_proj : none; };
         // per-snapshot totals and good counts live in the base workspace
@@ -852,16 +962,56 @@ stp_status stp_image_batch(stp_ctx* ctx, size_t n_items, const stp_batch_item* i
                 autotune_shape(ctx, &it.img_pars, wp_of(it), kernel, it.n_vis, it.d_vis, it.d_vis_weights, it.d_uvw_lambda,
                     precision, it.d_image, it.d_beam, totals + 2 * i, (unsigned long long*)(totals + 2 * i + 1));
             }
+        // fused snapshots (one launch per stage, main stream) and the rest (streams)
+        std::vector<size_t> fused, rest;
+        for (size_t i = 0; i < n_items; i++)
+            ((flags & STP_BATCH_FUSED) && batch_fusable(ctx, items[i], kernel, precision) ? fused : rest).push_back(i);
         hipStream_t main = ctx->stream;
-        auto enqueue = [&]() {
+        auto run_fused = [&]() {
+            if (fused.empty())
+                return;
+            if (precision == STP_PRECISION_FP64)
+                run_batch_fused<double, double>(ctx, items, fused, kernel, precision, totals);
+            else
+                run_batch_fused<float, float>(ctx, items, fused, kernel, precision, totals);
+        };
+        auto finish = [&]() {
+            if (sample_grid_totals) {
+                std::vector<double> t(2 * n_items);
+                HIP_CHECK(hipMemcpyAsync(t.data(), totals, sizeof(double) * 2 * n_items, hipMemcpyDeviceToHost, main));
+                HIP_CHECK(hipStreamSynchronize(main));
+                for (size_t i = 0; i < n_items; i++)
+                    sample_grid_totals[i] = t[2 * i];
+            } else {
+                HIP_CHECK(hipStreamSynchronize(main));
+            }
+        };
+        if (rest.empty()) {
+            run_fused();
+            finish();
+            return;
+        }
+        const int K = int(std::min<size_t>(rest.size(), max_concurrent ? std::min(max_concurrent, 16u) : 4u));
+        while (int(ctx->branch_streams.size()) < K) {
+            hipStream_t st;
+            HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            ctx->branch_streams.push_back(st);
+        }
+        while (ctx->branch_events.size() < size_t(K) + 1) {
+            hipEvent_t e;
+            HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            ctx->branch_events.push_back(e);
+        }
+        auto enqueue = [&](bool with_fused) {
             HIP_CHECK(hipEventRecord(ctx->branch_events[K], main));
             for (int b = 0; b < K; b++)
                 HIP_CHECK(hipStreamWaitEvent(ctx->branch_streams[b], ctx->branch_events[K], 0));
             try {
-                for (size_t i = 0; i < n_items; i++) {
-                    const int b = int(i % size_t(K));
+                for (size_t r = 0; r < rest.size(); r++) {
+                    const int b = int(r % size_t(K));
                     ctx->stream = ctx->branch_streams[b];
                     ctx->ws_slot = b + 1;
+                    const size_t i = rest[r];
                     const stp_batch_item& it = items[i];
                     run_imaging(ctx, &it.img_pars, wp_of(it), kernel, it.n_vis, it.d_vis, it.d_vis_weights, it.d_uvw_lambda,
                         precision, it.d_image, it.d_beam, totals + 2 * i, (unsigned long long*)(totals + 2 * i + 1));
@@ -873,6 +1023,8 @@ stp_status stp_image_batch(stp_ctx* ctx, size_t n_items, const stp_batch_item* i
             }
             ctx->stream = main;
             ctx->ws_slot = 0;
+            if (with_fused)
+                run_fused(); // on the main stream, beside the branches
             for (int b = 0; b < K; b++) {
                 HIP_CHECK(hipEventRecord(ctx->branch_events[b], ctx->branch_streams[b]));
                 HIP_CHECK(hipStreamWaitEvent(main, ctx->branch_events[b], 0));
@@ -887,7 +1039,7 @@ stp_status stp_image_batch(stp_ctx* ctx, size_t n_items, const stp_batch_item* i
                     kernel->width_normalization_gaussian, kernel->width_normalization_sinc);
                 sig += buf;
             }
-            for (size_t i = 0; i < n_items; i++) {
+            for (size_t i : rest) {
                 const stp_batch_item& it = items[i];
                 const stp_imager_pars& q = it.img_pars;
                 snprintf(buf, sizeof(buf), "|%u,%u,%.17g,%.17g,%d,%u,%d,%u,%d,%d,%d,%u,%u,%u,%.17g,%d,%zu,%p,%p,%p,%p,%p", q.image_size,
@@ -898,20 +1050,22 @@ stp_status stp_image_batch(stp_ctx* ctx, size_t n_items, const stp_batch_item* i
                     (const void*)it.d_uvw_lambda, it.d_image, it.d_beam);
                 sig += buf;
             }
+            // the fused snapshots first, eagerly (their tables change with every batch)
+            run_fused();
             auto g = ctx->graphs.find(sig);
             if (g != ctx->graphs.end()) {
                 HIP_CHECK(hipGraphLaunch(g->second, main));
             } else {
                 // eager run: sizes every workspace and uploads every table, so the
                 // capture below records kernel launches only
-                enqueue();
+                enqueue(false);
                 HIP_CHECK(hipStreamSynchronize(main));
                 const bool timing = ctx->timing;
                 ctx->timing = false;
                 hipGraph_t graph = nullptr;
                 HIP_CHECK(hipStreamBeginCapture(main, hipStreamCaptureModeRelaxed));
                 try {
-                    enqueue();
+                    enqueue(false);
                 } catch (...) {
                     hipGraph_t dropped = nullptr;
                     (void)hipStreamEndCapture(main, &dropped);
@@ -941,17 +1095,9 @@ stp_status stp_image_batch(stp_ctx* ctx, size_t n_items, const stp_batch_item* i
                 ctx->graph_order.push_back(sig);
             }
         } else {
-            enqueue();
+            enqueue(true);
         }
-        if (sample_grid_totals) {
-            std::vector<double> t(2 * n_items);
-            HIP_CHECK(hipMemcpyAsync(t.data(), totals, sizeof(double) * 2 * n_items, hipMemcpyDeviceToHost, main));
-            HIP_CHECK(hipStreamSynchronize(main));
-            for (size_t i = 0; i < n_items; i++)
-                sample_grid_totals[i] = t[2 * i];
-        } else {
-            HIP_CHECK(hipStreamSynchronize(main));
-        }
+        finish();
     });
 }
 

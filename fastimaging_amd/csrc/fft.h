@@ -13,15 +13,19 @@ struct NormSpec {
     const double* total_ptr = nullptr; // device scalar; NULL = no scaling
     const double* gcf = nullptr;       // device array of the padded size; NULL = none
     int u_l1 = 0;                      // four-step split of the u axis, N = L1 * (N / L1); 0 = balanced
+    const double* const* total_tab = nullptr; // device array: grid g's scalar (replaces total_ptr)
 };
 
+// Outputs of the batch's grids: p[g] for up to 4 grids, or the device array tab.
 template <typename OT>
 struct OutPtrs {
     OT* p[4];
+    OT* const* tab = nullptr;
 };
 
 // Unnormalised 2-D c2r of nbatch half-plane grids ((n_rows/2+1) x N each,
-// contiguous) into nbatch outputs; the grids are overwritten.
+// contiguous) into nbatch outputs (at most 4 unless outs.tab is set); the grids
+// are overwritten.
 template <typename T, typename OT>
 void fft_c2r_2d(stp_ctx* ctx, cplx<T>* grids, int nbatch, int n_rows, int N, const OutPtrs<OT>& outs, const NormSpec& ns);
 

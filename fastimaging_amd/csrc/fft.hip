@@ -333,8 +333,9 @@ __global__ __launch_bounds__(WG) void fft_v_c2r(const cplx<T>* grids, size_t gri
     const cplx<T>* g = grids + size_t(gi) * grid_elems;
     cplx<T>* b = buf + t * (M + S::PAD);
     const int hi = ns.image_size / 2, hp = n / 2;
-    const bool scale = ns.total_ptr != nullptr;
-    const double factor = scale ? (1.0 / ns.total_ptr[0]) : 1.0;
+    const double* tp = ns.total_tab ? ns.total_tab[gi] : ns.total_ptr;
+    const bool scale = tp != nullptr;
+    const double factor = scale ? (1.0 / tp[0]) : 1.0;
     const int x = blockIdx.x * S::NT + t;
     const bool active = x < N;
     const int slot = active ? ((x % L1) * L2 + x / L1) : 0;
@@ -387,7 +388,7 @@ __global__ __launch_bounds__(WG) void fft_v_c2r(const cplx<T>* grids, size_t gri
     if (j < 0)
         return;
     const int joff = (j < hi) ? j : j - ns.image_size;
-    OT* out = outs.p[gi] + size_t(j) * ns.image_size;
+    OT* out = (outs.tab ? outs.tab[gi] : outs.p[gi]) + size_t(j) * ns.image_size;
     for (int m = lt; m < M; m += S::TPT) {
         const cplx<T> z = b[m];
         const int ys[2] = { 2 * m, 2 * m + 1 };
@@ -518,7 +519,7 @@ void fft_c2r_2d(stp_ctx* ctx, cplx<T>* grids, int nbatch, int n_rows, int N, con
     split_len<T>(N, L1, L2, ns.u_l1);
     const Twiddles& twv = twiddles_for(ctx, n_rows);
     const T* tvn = std::is_same<T, double>::value ? (const T*)twv.d : (const T*)twv.f;
-    STP_REQUIRE(nbatch >= 1 && nbatch <= 4, "1 to 4 grids per FFT launch");
+    STP_REQUIRE(nbatch >= 1 && nbatch <= (outs.tab ? 65535 : 4), "1 to 4 grids per FFT launch (65535 with an output table)");
     const int M = n_rows / 2;
     switch (M) {
 #define VCASE(MM)                                                                                              \

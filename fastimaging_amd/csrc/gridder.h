@@ -34,6 +34,33 @@ template <typename T>
 void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const double* d_uv, const double* d_vis,
     const double* d_w, cplx<T>* d_vis_grid, cplx<T>* d_sampling_grid, double* d_total, unsigned long long* d_ngood);
 
+// One snapshot of a fused batch (grid_batch): its arguments, inputs, grids and
+// the device scalars that receive sample_grid_total and the good count.
+struct BatchGridItem {
+    GridderArgs a;
+    size_t n = 0;
+    const double* uv = nullptr;
+    const double* vis = nullptr;
+    const double* wts = nullptr;
+    void* vgrid = nullptr;   // cplx<T> grids; bgrid NULL without a beam
+    void* bgrid = nullptr;
+    double* total = nullptr;
+    unsigned long long* ngood = nullptr;
+};
+
+// Whether a snapshot can join a fused batch: no W-projection, oversampled
+// kernels of at most 16 taps, at most 24576 tiles and 4096 binning blocks
+// (16.7 M visibilities).
+bool grid_batch_eligible(const GridderArgs& a, size_t n);
+
+// Grids every item of a batch with one launch per stage (binning counts, column
+// scans, scatter, the tile kernel per kernel width class, totals): each stage's
+// workgroups span all snapshots.  `extra` (caller tables, e.g. the FFT's per-grid
+// pointers) is uploaded with the batch's own tables in one copy; returns its
+// device address (valid until the next fused batch of the context).
+template <typename T>
+const void* grid_batch(stp_ctx* ctx, const std::vector<BatchGridItem>& items, const std::vector<uint8_t>& extra);
+
 // W-projection: each W sub-kernel is cached with this many zero columns on both
 // sides (ks rows x (ks + 2 W_KERNEL_PAD_COLS) columns).
 constexpr int W_KERNEL_PAD_COLS = 7;

@@ -16,6 +16,7 @@
 // beam += kval (gridder.h:854-858); the oversampled kernel is the separable
 // product ky[row] * kx[col] of normalised 1-D kernels (gridder.h:84-102).
 #include <cmath>
+#include <cstring>
 #include <type_traits>
 
 #include "gridder.h"
@@ -425,8 +426,8 @@ constexpr int BWG = 1024;
 constexpr int VPB = CANON_BLOCK; // visibilities per binning block (also the canonical sum block)
 constexpr int MAX_LDS_TILES = 24576;
 
-__global__ __launch_bounds__(BWG) void grid_bin_count_kernel(GridGeom g, const double* uv, const double* vis, const double* wts,
-    uint32_t* counts, int nblk, double* wpart)
+__device__ __forceinline__ void bin_count_body(const GridGeom& g, const double* uv, const double* vis, const double* wts,
+    uint32_t* counts, double* wpart, int bid)
 {
 #pragma clang fp contract(off)
     extern __shared__ uint32_t hist[];
@@ -436,7 +437,7 @@ __global__ __launch_bounds__(BWG) void grid_bin_count_kernel(GridGeom g, const d
     for (int t = threadIdx.x; t < ntiles; t += BWG)
         hist[t] = 0;
     __syncthreads();
-    const long long base = (long long)blockIdx.x * VPB;
+    const long long base = (long long)bid * VPB;
     unsigned long long ng = 0;
     VisIn in[VPB / BWG];
 #pragma unroll
@@ -488,20 +489,26 @@ __global__ __launch_bounds__(BWG) void grid_bin_count_kernel(GridGeom g, const d
         unsigned long long c = 0;
         for (int k = 0; k < BWG / 64; k++)
             c += wng[k];
-        wpart[2 * blockIdx.x] = wl[0];
-        wpart[2 * blockIdx.x + 1] = double(c);
+        wpart[2 * bid] = wl[0];
+        wpart[2 * bid + 1] = double(c);
     }
     for (int t = threadIdx.x; t < ntiles; t += BWG)
-        counts[size_t(blockIdx.x) * ntiles + t] = hist[t];
+        counts[size_t(bid) * ntiles + t] = hist[t];
+}
+
+__global__ __launch_bounds__(BWG) void grid_bin_count_kernel(GridGeom g, const double* uv, const double* vis,
+    const double* wts, uint32_t* counts, double* wpart)
+{
+    bin_count_body(g, uv, vis, wts, counts, wpart, int(blockIdx.x));
 }
 
 // Per tile t: counts[b][t] -> entries of tile t in blocks before b (in place); totals[t].
 // A workgroup takes 64 tiles (lanes) x 16 slices of the blocks (waves).
-__global__ __launch_bounds__(1024) void bin_colscan_kernel(uint32_t* counts, int nblk, int ntiles, uint32_t* totals)
+__device__ __forceinline__ void colscan_body(uint32_t* counts, int nblk, int ntiles, uint32_t* totals, int bid)
 {
     __shared__ uint32_t part[16][64];
     const int lane = threadIdx.x & 63, sl = threadIdx.x >> 6;
-    const int t = blockIdx.x * 64 + lane;
+    const int t = bid * 64 + lane;
     const int per = (nblk + 15) / 16, b0 = min(nblk, sl * per), b1 = min(nblk, b0 + per);
     uint32_t s = 0;
     if (t < ntiles)
@@ -535,6 +542,11 @@ __global__ __launch_bounds__(1024) void bin_colscan_kernel(uint32_t* counts, int
             run += v[k];
         }
     }
+}
+
+__global__ __launch_bounds__(1024) void bin_colscan_kernel(uint32_t* counts, int nblk, int ntiles, uint32_t* totals)
+{
+    colscan_body(counts, nblk, ntiles, totals, int(blockIdx.x));
 }
 
 // Exclusive scan of the tile totals into out[0..ntiles) (LDS) by the BWG threads of a
@@ -582,24 +594,24 @@ __device__ uint32_t scan_tile_totals(const uint32_t* totals, int ntiles, uint32_
 // Block b's entries go to cursors tile offset + block offset; the tile offsets (the
 // scan of the tile totals) are computed by every block in LDS, and block 0 also
 // writes them to tile_off[0..ntiles] for the tile kernel.
-__global__ __launch_bounds__(BWG) void grid_bin_scatter_kernel(GridGeom g, const double* uv, const double* vis, const double* wts,
-    const uint32_t* totals, const uint32_t* block_off, Entry* entries, uint32_t cap, uint32_t* tile_off)
+__device__ __forceinline__ void bin_scatter_body(const GridGeom& g, const double* uv, const double* vis, const double* wts,
+    const uint32_t* totals, const uint32_t* block_off, Entry* entries, uint32_t cap, uint32_t* tile_off, int bid)
 {
     extern __shared__ uint32_t cur[];
     __shared__ uint32_t wsum[BWG / 64];
     const int ntiles = g.tiles_r * g.tiles_c;
     const uint32_t total = scan_tile_totals(totals, ntiles, cur, wsum);
     __syncthreads();
-    if (blockIdx.x == 0) {
+    if (bid == 0) {
         for (int t = threadIdx.x; t < ntiles; t += BWG)
             tile_off[t] = cur[t];
         if (threadIdx.x == 0)
             tile_off[ntiles] = total;
     }
     for (int t = threadIdx.x; t < ntiles; t += BWG)
-        cur[t] += block_off[size_t(blockIdx.x) * ntiles + t];
+        cur[t] += block_off[size_t(bid) * ntiles + t];
     __syncthreads();
-    const long long base = (long long)blockIdx.x * VPB;
+    const long long base = (long long)bid * VPB;
     VisIn in[VPB / BWG];
 #pragma unroll
     for (int j = 0; j < VPB / BWG; j++) {
@@ -624,6 +636,12 @@ __global__ __launch_bounds__(BWG) void grid_bin_scatter_kernel(GridGeom g, const
             });
         }
     }
+}
+
+__global__ __launch_bounds__(BWG) void grid_bin_scatter_kernel(GridGeom g, const double* uv, const double* vis,
+    const double* wts, const uint32_t* totals, const uint32_t* block_off, Entry* entries, uint32_t cap, uint32_t* tile_off)
+{
+    bin_scatter_body(g, uv, vis, wts, totals, block_off, entries, cap, tile_off, int(blockIdx.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -845,8 +863,8 @@ __global__ __launch_bounds__(TWG) void grid_tiles_kernel(GridGeom g, const Entry
 // without branches; taps that miss the tile issue no LDS atomic (the LDS FP64
 // atomic rate bounds this kernel).
 template <typename T, typename Acc, int KSM>
-__global__ __launch_bounds__(TWG) void grid_tiles_os_kernel(GridGeom g, const Entry* entries, const uint32_t* offsets,
-    int ostride, uint32_t cap, const double* ktab, cplx<T>* vgrid, cplx<T>* bgrid)
+__device__ __forceinline__ void tiles_os_body(const GridGeom& g, const Entry* entries, const uint32_t* offsets, int ostride,
+    uint32_t cap, const double* ktab, cplx<T>* vgrid, cplx<T>* bgrid, int t)
 {
 #pragma clang fp contract(off)
     using AS = typename Acc::S;
@@ -863,7 +881,6 @@ __global__ __launch_bounds__(TWG) void grid_tiles_os_kernel(GridGeom g, const En
         kt[k] = T(ktab[k]);
     for (int k = threadIdx.x; k < Acc::arrays * plane; k += TWG)
         acc[k] = AS(0);
-    const int t = blockIdx.x;
     const int rt = t % g.tiles_r, ct = t / g.tiles_r;
     const int r0 = rt * TILE_R, c0 = ct * g.tc;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -929,6 +946,13 @@ __global__ __launch_bounds__(TWG) void grid_tiles_os_kernel(GridGeom g, const En
                 bgrid[o] = { T(Acc::value(acc, plane, 2, a)), T(0) };
         }
     }
+}
+
+template <typename T, typename Acc, int KSM>
+__global__ __launch_bounds__(TWG) void grid_tiles_os_kernel(GridGeom g, const Entry* entries, const uint32_t* offsets,
+    int ostride, uint32_t cap, const double* ktab, cplx<T>* vgrid, cplx<T>* bgrid)
+{
+    tiles_os_body<T, Acc, KSM>(g, entries, offsets, ostride, cap, ktab, vgrid, bgrid, int(blockIdx.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -1755,11 +1779,98 @@ __global__ __launch_bounds__(WTW * 64) void grid_wtiles_kernel(GridGeom g, const
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused batches: the snapshots of a batch are gridded by one launch per stage.
+// Each stage's workgroups are numbered across the snapshots; pre[k] is the first
+// workgroup of snapshot k (pre[0] = 0, non-decreasing, snapshots without work
+// in a stage repeat the next start) and a workgroup finds its snapshot by a
+// binary search over pre (uniform, so scalar loads).
+struct BItem {
+    GridGeom g;
+    const double* uv;
+    const double* vis;
+    const double* wts;
+    const double* ktab;        // oversampled 1-D kernel table
+    uint32_t* counts;          // nblk x ntiles block counts, then block cursors
+    uint32_t* totals;          // entries per tile
+    uint32_t* toffs;           // tile entry offsets (ntiles + 1)
+    Entry* entries;
+    double* wpart;             // canonical weight / good-count pairs, one per block
+    void* vgrid;
+    void* bgrid;
+    double* total;
+    unsigned long long* ngood;
+    uint32_t cap;
+    int nblk;
+};
+
+__device__ __forceinline__ int batch_item(const uint32_t* __restrict__ pre, int K, uint32_t b)
+{
+    int lo = 0, hi = K - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(BWG) void batch_count_kernel(const BItem* __restrict__ items, const uint32_t* __restrict__ pre,
+    int K)
+{
+    const int k = batch_item(pre, K, blockIdx.x);
+    const BItem& it = items[k];
+    bin_count_body(it.g, it.uv, it.vis, it.wts, it.counts, it.wpart, int(blockIdx.x - pre[k]));
+}
+
+__global__ __launch_bounds__(1024) void batch_colscan_kernel(const BItem* __restrict__ items, const uint32_t* __restrict__ pre,
+    int K)
+{
+    const int k = batch_item(pre, K, blockIdx.x);
+    const BItem& it = items[k];
+    colscan_body(it.counts, it.nblk, it.g.tiles_r * it.g.tiles_c, it.totals, int(blockIdx.x - pre[k]));
+}
+
+__global__ __launch_bounds__(BWG) void batch_scatter_kernel(const BItem* __restrict__ items, const uint32_t* __restrict__ pre,
+    int K)
+{
+    const int k = batch_item(pre, K, blockIdx.x);
+    const BItem& it = items[k];
+    bin_scatter_body(it.g, it.uv, it.vis, it.wts, it.totals, it.counts, it.entries, it.cap, it.toffs,
+        int(blockIdx.x - pre[k]));
+}
+
+template <typename T, typename Acc, int KSM>
+__global__ __launch_bounds__(TWG) void batch_tiles_kernel(const BItem* __restrict__ items, const uint32_t* __restrict__ pre,
+    int K)
+{
+    const int k = batch_item(pre, K, blockIdx.x);
+    const BItem& it = items[k];
+    tiles_os_body<T, Acc, KSM>(it.g, it.entries, it.toffs, 1, it.cap, it.ktab, (cplx<T>*)it.vgrid, (cplx<T>*)it.bgrid,
+        int(blockIdx.x - pre[k]));
+}
+
+// One workgroup per snapshot: the last canonical level of its weight pairs (at
+// most CANON_BLOCK blocks), sample_grid_total and the good count.
+__global__ __launch_bounds__(CANON_LANES) void batch_total_kernel(const BItem* __restrict__ items)
+{
+    __shared__ double lds[2 * CANON_LANES];
+    const BItem& it = items[blockIdx.x];
+    double a, b;
+    canon_last_level(it.wpart, unsigned(it.nblk), a, b, lds);
+    if (threadIdx.x == 0) {
+        it.total[0] = a * 2.0;
+        if (it.ngood)
+            *it.ngood = (unsigned long long)b;
+    }
+}
+
 } // namespace
 
-template <typename T>
-void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const double* d_uv, const double* d_vis,
-    const double* d_w, cplx<T>* d_vis_grid, cplx<T>* d_sampling_grid, double* d_total, unsigned long long* d_ngood)
+// Grid geometry of one call (validated); the kernel functor in kf.
+static GridGeom make_geom(const GridderArgs& a, size_t n, bool beam, KernelFn& kf)
 {
     const int N = a.image_size;
     const bool wproj = a.wproj.num_wplanes > 0;
@@ -1793,19 +1904,29 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     g.halfplane = a.halfplane;
     g.shift = a.shift_uv;
     g.exact = a.kernel_exact;
-    g.beam = a.generate_beam && d_sampling_grid != nullptr;
+    g.beam = beam;
     g.tiles_r = int(div_up(g.rows, TILE_R));
     g.tc = a.tile_cols > 0 ? a.tile_cols : TILE_C;
     STP_REQUIRE(g.tc == 16 || g.tc == 32 || g.tc == 64, "tile width must be 16, 32 or 64 columns");
     g.tiles_c = int(div_up(N, g.tc));
     g.scale = a.uv_scale;
     g.n = (long long)n;
-    const KernelFn kf = kernel_fn(a.kernel);
+    kf = kernel_fn(a.kernel);
     g.kfunc = kf.kind;
     g.kparam = kf.param;
     g.kwg = kf.width_g;
     g.kws = kf.width_s;
+    return g;
+}
 
+template <typename T>
+void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const double* d_uv, const double* d_vis,
+    const double* d_w, cplx<T>* d_vis_grid, cplx<T>* d_sampling_grid, double* d_total, unsigned long long* d_ngood)
+{
+    const bool wproj = a.wproj.num_wplanes > 0;
+    KernelFn kf;
+    GridGeom g = make_geom(a, n, a.generate_beam && d_sampling_grid != nullptr, kf);
+    const int N = g.N, os = g.os;
     const int ntiles = g.tiles_r * g.tiles_c;
     // kernel table (oversampled 1-D kernels), computed on the host once per configuration
     const double* d_tab = nullptr;
@@ -1976,7 +2097,7 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
         HIP_CHECK(hipFuncSetAttribute((const void*)grid_bin_count_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(hist_bytes)));
         HIP_CHECK(hipFuncSetAttribute((const void*)grid_bin_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(hist_bytes)));
         launch(ctx, "grid_count", grid_bin_count_kernel, dim3(unsigned(nblk)), dim3(BWG), hist_bytes, g, d_uv, d_vis, d_w, counts,
-            int(nblk), wpart);
+            wpart);
         launch(ctx, "grid_colscan", bin_colscan_kernel, dim3(div_up(ntiles, 64)), dim3(1024), 0, counts, int(nblk), int(ntiles),
             totals);
         launch(ctx, "grid_scatter", grid_bin_scatter_kernel, dim3(unsigned(nblk)), dim3(BWG), hist_bytes, g, d_uv, d_vis, d_w,
@@ -2064,6 +2185,136 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
     launch(ctx, "grid_total", grid_total_kernel, dim3(1), dim3(CANON_LANES), 0, fin, unsigned(mfin), d_total, d_ngood,
         offsets + size_t(ntiles) * ostride, a.d_nentries);
 }
+
+bool grid_batch_eligible(const GridderArgs& a, size_t n)
+{
+    if (a.wproj.num_wplanes > 0 || a.kernel_exact || a.support < 1 || 2 * a.support + 1 > 16)
+        return false;
+    const int N = a.image_size;
+    const int tc = a.tile_cols > 0 ? a.tile_cols : TILE_C;
+    const int rows = a.halfplane ? N / 2 + 1 : N;
+    const long long ntiles = (long long)div_up(rows, TILE_R) * div_up(N, tc);
+    const int os = std::max(1, (a.oversampling >> 1) << 1);
+    const int cs = (os / 2) * 2 + 1;
+    return N >= 4 && ntiles <= MAX_LDS_TILES && std::max<size_t>(1, div_up(n, VPB)) <= size_t(CANON_BLOCK) &&
+           cs * (2 * a.support + 1) <= KV_DOUBLES;
+}
+
+template <typename T>
+const void* grid_batch(stp_ctx* ctx, const std::vector<BatchGridItem>& items, const std::vector<uint8_t>& extra)
+{
+    const int K = int(items.size());
+    STP_REQUIRE(K >= 1 && K <= 65535, "1 to 65535 snapshots per fused batch");
+    const bool twofloat = std::is_same<T, float>::value && items[0].a.twofloat;
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    // per snapshot: geometry, kernel table, workspace offsets inside one pool
+    struct Place {
+        size_t counts, totals, toffs, wpart, entries;
+    };
+    std::vector<BItem> bi(K);
+    std::vector<Place> pl(K);
+    std::vector<uint32_t> pre_b(K), pre_c(K), pre_t[2] = { std::vector<uint32_t>(K), std::vector<uint32_t>(K) };
+    size_t nb = 0, nc = 0, nt[2] = { 0, 0 }, pool = 0, hist_bytes = 4, tile_bytes[2] = { 0, 0 };
+    for (int k = 0; k < K; k++) {
+        const BatchGridItem& it = items[k];
+        STP_REQUIRE(grid_batch_eligible(it.a, it.n), "snapshot not eligible for a fused batch");
+        STP_REQUIRE((std::is_same<T, float>::value && it.a.twofloat) == twofloat, "one tile-sum form per fused batch");
+        KernelFn kf;
+        BItem& b = bi[k];
+        b.g = make_geom(it.a, it.n, it.a.generate_beam && it.bgrid != nullptr, kf);
+        const GridGeom& g = b.g;
+        const int ntiles = g.tiles_r * g.tiles_c;
+        const size_t nblk = std::max<size_t>(1, div_up(it.n, VPB));
+        const size_t cap64 = std::max<size_t>(1, 2 * footprint_tiles_max(g) * it.n);
+        STP_REQUIRE(cap64 < (size_t(1) << 32), "too many gridding entries for one snapshot");
+        char key[160];
+        snprintf(key, sizeof(key), "k1d:%d:%.17g:%.17g:%.17g:%d:%d", kf.kind, kf.param, kf.width_g, kf.width_s, it.a.support,
+            g.os);
+        b.ktab = has_table(ctx, key) ? dev_table(ctx, key, {}) : dev_table(ctx, key, kernel_1d_table(kf, it.a.support, g.os));
+        b.uv = it.uv;
+        b.vis = it.vis;
+        b.wts = it.wts;
+        b.vgrid = it.vgrid;
+        b.bgrid = g.beam ? it.bgrid : nullptr;
+        b.total = it.total;
+        b.ngood = it.ngood;
+        b.cap = uint32_t(cap64);
+        b.nblk = int(nblk);
+        Place& p = pl[k];
+        p.counts = pool;
+        pool = al(pool + sizeof(uint32_t) * nblk * ntiles);
+        p.totals = pool;
+        pool = al(pool + sizeof(uint32_t) * ntiles);
+        p.toffs = pool;
+        pool = al(pool + sizeof(uint32_t) * (ntiles + 1));
+        p.wpart = pool;
+        pool = al(pool + sizeof(double) * (2 * nblk + 2));
+        p.entries = pool;
+        pool = al(pool + sizeof(Entry) * cap64);
+        pre_b[k] = uint32_t(nb);
+        nb += nblk;
+        pre_c[k] = uint32_t(nc);
+        nc += div_up(ntiles, 64);
+        const int cls = g.ks <= 8 ? 0 : 1;
+        for (int c = 0; c < 2; c++)
+            pre_t[c][k] = uint32_t(nt[c]);
+        nt[cls] += ntiles;
+        hist_bytes = std::max(hist_bytes, sizeof(uint32_t) * size_t(ntiles));
+        const int S = TILE_R + (g.ks <= 15 ? g.ks : 1);
+        const size_t acc = (twofloat ? 6 * sizeof(float) : 3 * sizeof(double)) * (size_t(g.tc) * S);
+        const int cs = (g.os / 2) * 2 + 1;
+        tile_bytes[cls] = std::max(tile_bytes[cls], ((acc + 7) / 8) * 8 + sizeof(double) * size_t(cs) * g.ks);
+    }
+    STP_REQUIRE(nb < (size_t(1) << 31) && nt[0] < (size_t(1) << 31) && nt[1] < (size_t(1) << 31),
+        "too many workgroups for one fused batch");
+    uint8_t* base = (uint8_t*)ws_get(ctx, "bat_pool", pool);
+    for (int k = 0; k < K; k++) {
+        bi[k].counts = (uint32_t*)(base + pl[k].counts);
+        bi[k].totals = (uint32_t*)(base + pl[k].totals);
+        bi[k].toffs = (uint32_t*)(base + pl[k].toffs);
+        bi[k].wpart = (double*)(base + pl[k].wpart);
+        bi[k].entries = (Entry*)(base + pl[k].entries);
+    }
+    // tables: items, the stage prefixes, the caller's tables; one upload from the
+    // page-locked stage once the previous batch's upload has left it
+    size_t o_items = 0, o_b = al(sizeof(BItem) * K), o_c = o_b + al(4 * K), o_t0 = o_c + al(4 * K), o_t1 = o_t0 + al(4 * K);
+    const size_t o_x = o_t1 + al(4 * K), tab_bytes = o_x + std::max<size_t>(extra.size(), 1);
+    if (!ctx->batch_tab_event)
+        HIP_CHECK(hipEventCreateWithFlags(&ctx->batch_tab_event, hipEventDisableTiming));
+    HIP_CHECK(hipEventSynchronize(ctx->batch_tab_event));
+    uint8_t* h = (uint8_t*)host_stage(ctx, "bat_tabs", tab_bytes);
+    uint8_t* d = (uint8_t*)ws_get(ctx, "bat_tabs", tab_bytes);
+    memcpy(h + o_items, bi.data(), sizeof(BItem) * K);
+    memcpy(h + o_b, pre_b.data(), 4 * K);
+    memcpy(h + o_c, pre_c.data(), 4 * K);
+    memcpy(h + o_t0, pre_t[0].data(), 4 * K);
+    memcpy(h + o_t1, pre_t[1].data(), 4 * K);
+    if (!extra.empty())
+        memcpy(h + o_x, extra.data(), extra.size());
+    HIP_CHECK(hipMemcpyAsync(d, h, tab_bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_CHECK(hipEventRecord(ctx->batch_tab_event, ctx->stream));
+    const BItem* d_items = (const BItem*)(d + o_items);
+    const uint32_t* d_pre_b = (const uint32_t*)(d + o_b);
+    HIP_CHECK(hipFuncSetAttribute((const void*)batch_count_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(hist_bytes)));
+    HIP_CHECK(hipFuncSetAttribute((const void*)batch_scatter_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(hist_bytes)));
+    launch(ctx, "batch_count", batch_count_kernel, dim3(unsigned(nb)), dim3(BWG), hist_bytes, d_items, d_pre_b, K);
+    launch(ctx, "batch_colscan", batch_colscan_kernel, dim3(unsigned(nc)), dim3(1024), 0, d_items, (const uint32_t*)(d + o_c), K);
+    launch(ctx, "batch_scatter", batch_scatter_kernel, dim3(unsigned(nb)), dim3(BWG), hist_bytes, d_items, d_pre_b, K);
+    for (int c = 0; c < 2; c++) {
+        if (nt[c] == 0)
+            continue;
+        auto kern = c == 0 ? (twofloat ? batch_tiles_kernel<T, AccTwoFloat, 8> : batch_tiles_kernel<T, AccF64, 8>)
+                           : (twofloat ? batch_tiles_kernel<T, AccTwoFloat, 16> : batch_tiles_kernel<T, AccF64, 16>);
+        HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(tile_bytes[c])));
+        launch(ctx, "batch_tiles", kern, dim3(unsigned(nt[c])), dim3(TWG), tile_bytes[c], d_items,
+            (const uint32_t*)(d + (c == 0 ? o_t0 : o_t1)), K);
+    }
+    launch(ctx, "batch_total", batch_total_kernel, dim3(unsigned(K)), dim3(CANON_LANES), 0, d_items);
+    return d + o_x;
+}
+
+template const void* grid_batch<double>(stp_ctx*, const std::vector<BatchGridItem>&, const std::vector<uint8_t>&);
+template const void* grid_batch<float>(stp_ctx*, const std::vector<BatchGridItem>&, const std::vector<uint8_t>&);
 
 size_t gridder_entry_capacity(int N, bool halfplane, int support, bool exact, int tile_cols, size_t n)
 {

@@ -501,6 +501,7 @@ typedef struct stp_batch_item {
 
 #define STP_BATCH_GRAPH 1u       /* capture the batch once, replay the hipGraph on later calls */
 #define STP_BATCH_AUTOTUNE 2u    /* tune every new shape of the batch before running it */
+#define STP_BATCH_FUSED 4u       /* one launch per stage over all eligible snapshots */
 
 /* Images n_items snapshots.  They run on up to max_concurrent streams of the
  * context (0 = 4), each stream with its own workspace, so small snapshots fill
@@ -508,7 +509,16 @@ typedef struct stp_batch_item {
  * (parameters, sizes and buffer addresses) runs eagerly and records the whole
  * batch into one hipGraph; later calls with the same batch launch that graph
  * (the flag is for fixed, reused buffers: a context keeps the 16 most recent
- * graphs).  sample_grid_totals (host, n_items, or NULL) receives 2 * sum of good weights
+ * graphs).  With STP_BATCH_FUSED the snapshots without W-projection whose
+ * kernels are oversampled with at most 16 taps (and at most 24576 grid tiles and
+ * 16.7 M visibilities) are imaged together, eagerly and with no capture: one
+ * binning, one column scan, one scatter and one tile launch per kernel width class
+ * span all of their tiles, one normalisation launch all of their totals, and one
+ * FFT per group of equal transform shape all of the group's grids, so any new
+ * batch runs in a dozen launches; the other snapshots take the streams (and the
+ * graph) as above, beside them.  The fused snapshots' entry, count and grid
+ * buffers are held at once (worst-case entries, as for single calls).
+ * sample_grid_totals (host, n_items, or NULL) receives 2 * sum of good weights
  * per snapshot; the call returns after the batch has finished. */
 stp_status stp_image_batch(stp_ctx* ctx, size_t n_items, const stp_batch_item* items, const stp_kernel_spec* kernel,
     int32_t precision, uint32_t max_concurrent, uint32_t flags, double* sample_grid_totals);

@@ -20,7 +20,9 @@ class _Batch:
         self.gpu, self.prec = gpu, precision
         self.snaps, self.items, self.bufs = [], [], []
         osz = 8 if precision == capi.PRECISION_FP64 else 4
-        for N, cell, nv, ns, seed, wp, wmax in specs:
+        for spec in specs:
+            N, cell, nv, ns, seed, wp, wmax = spec[:7]
+            kw = spec[7] if len(spec) > 7 else {}
             snap = simdata.snapshot(N, cell, nv, ns, seed=seed, field_frac=0.5 if wp is not None else 0.25, w_max=wmax)
             b = [gpu.malloc(16 * nv), gpu.malloc(8 * nv), gpu.malloc(24 * nv), gpu.malloc(osz * N * N),
                  gpu.malloc(osz * N * N)]
@@ -28,7 +30,7 @@ class _Batch:
             gpu.htod(b[1], np.ascontiguousarray(snap["weights"]))
             gpu.htod(b[2], np.asfortranarray(snap["uvw"]))
             it = capi.BatchItem()
-            it.img_pars = capi.imager_pars(N, cell)
+            it.img_pars = capi.imager_pars(N, cell, **kw)
             if wp is not None:
                 it.w_proj = wp
             it.n_vis = nv
@@ -179,3 +181,105 @@ def test_autotune(gpu, orc, tmp_path):
             assert peak_rel_err(img, oimg) < 1e-10 and peak_rel_err(beam, obeam) < 1e-10, t
     finally:
         gpu.set_autotune(0, None)
+
+
+# Fused batches: default kernels (7 taps), a 15-tap kernel (the wide tile class),
+# no beam, padding 2 with gridding correction, a tiny snapshot, and two that take
+# the streams beside them (W-projection, 17 taps).
+FUSED_SPECS = [
+    (512, 2.0, 20_000, 5, 721, None, 0.0),
+    (1024, 1.0, 120_000, 12, 722, None, 0.0, {"kernel_support": 7, "oversampling": 8}),
+    (256, 4.0, 5_000, 3, 723, None, 0.0, {"generate_beam": False}),
+    (512, 2.0, 30_000, 8, 724, None, 0.0, {"padding_factor": 2.0, "gridding_correction": True}),
+    (256, 20.0, 4_000, 6, 725, "w", 300.0),
+    (256, 4.0, 6_000, 4, 726, None, 0.0, {"kernel_support": 8}),
+    (2048, 1.0, 300_000, 40, 727, None, 0.0),
+    (512, 2.0, 100, 1, 728, None, 0.0),
+]
+STREAMED = (4, 5)
+
+
+def _fused_specs(order=None):
+    sp = [(s[0], s[1], s[2], s[3], s[4], capi.wproj_pars(4, 8) if s[5] else None) + tuple(s[6:]) for s in FUSED_SPECS]
+    return [sp[k] for k in order] if order is not None else sp
+
+
+def _compare(items, got, want, tol):
+    for k, (it, (a, ab), (c, cb)) in enumerate(zip(items, want, got)):
+        assert peak_rel_err(c, a) <= tol, k
+        if it.img_pars.generate_beam:
+            assert peak_rel_err(cb, ab) <= tol, k
+
+
+@pytest.mark.parametrize("precision", [capi.PRECISION_FP64, capi.PRECISION_FP32C, capi.PRECISION_FP32_TWOFLOAT])
+def test_fused_batch_matches_single(gpu, orc, precision):
+    """STP_BATCH_FUSED: the eligible snapshots are gridded by one launch per stage
+    (kernel timings count the launches) and give each snapshot's single-call image
+    and beam (FP64 within 1e-12 of the peak, FP32 within 1e-6) with bit-identical
+    sample_grid_totals; the streamed ones run beside them; a different batch next
+    (other sizes and order) rebuilds the tables; graph mode replays the streamed part
+    while the fused part runs eagerly; FP64 matches the oracle."""
+    b = _Batch(gpu, _fused_specs(), precision)
+    try:
+        tol = 1e-12 if precision == capi.PRECISION_FP64 else 1e-6
+        single, stotals = [], []
+        for it in b.items:
+            d = gpu.image_visibilities_device(it.img_pars, it.n_vis, it.d_vis, it.d_vis_weights, it.d_uvw_lambda,
+                                              it.d_image, it.d_beam, precision=precision, synchronize=True,
+                                              w_proj=it.w_proj if it.w_proj.num_wplanes else None)
+            stotals.append(d.sample_grid_total)
+        single = b.images()
+        b.clear()
+        gpu.set_timing(True)
+        gpu.reset_timings()
+        try:
+            totals = gpu.image_batch(b.items, precision=precision, max_concurrent=2, flags=capi.BATCH_FUSED)
+            t = gpu.timings()
+        finally:
+            gpu.set_timing(False)
+        print({k: v for k, v in t.items() if k.startswith("batch_") or k.startswith("grid_")}, flush=True)
+        assert t["batch_count"][1] == 1 and t["batch_scatter"][1] == 1 and t["batch_total"][1] == 1
+        assert t["batch_tiles"][1] == 2  # the 8-tap and the 16-tap class
+        assert t["grid_tiles"][1] == 1 and t["grid_wtiles"][1] >= 1  # 17 taps and W-projection on the streams
+        _compare(b.items, b.images(), single, tol)
+        assert list(totals) == stotals
+        if precision == capi.PRECISION_FP64:
+            for k in (1, 3):
+                it, s = b.items[k], b.snaps[k]
+                oimg, obeam, _, _ = orc.image_visibilities(it.img_pars, s["vis"], s["weights"], s["uvw"])
+                assert peak_rel_err(single[k][0], oimg) < 1e-10 and peak_rel_err(single[k][1], obeam) < 1e-10
+        # another batch: a subset in another order, then the first batch again
+        order = [6, 3, 0, 7]
+        sub = _Batch.__new__(_Batch)
+        sub.gpu, sub.prec = gpu, precision
+        sub.items = [b.items[k] for k in order]
+        sub.snaps = [b.snaps[k] for k in order]
+        sub.bufs = []
+        sub.clear()
+        totals = gpu.image_batch(sub.items, precision=precision, flags=capi.BATCH_FUSED)
+        _compare(sub.items, sub.images(), [single[k] for k in order], tol)
+        assert list(totals) == [stotals[k] for k in order]
+        for rep in range(3):
+            b.clear()
+            totals = gpu.image_batch(b.items, precision=precision, max_concurrent=2,
+                                     flags=capi.BATCH_FUSED | capi.BATCH_GRAPH)
+            _compare(b.items, b.images(), single, tol)
+            assert list(totals) == stotals, rep
+    finally:
+        b.free()
+
+
+def test_fused_batch_only_eligible_and_autotuned(gpu, orc):
+    """Every snapshot fused (no streams), with autotuned tile widths and FFT splits
+    per shape: the FP32 images stay within 1e-4 of the oracle."""
+    specs = [s for k, s in enumerate(_fused_specs()) if k not in STREAMED]
+    b = _Batch(gpu, specs, capi.PRECISION_FP32C)
+    try:
+        gpu.image_batch(b.items, precision=capi.PRECISION_FP32C, flags=capi.BATCH_FUSED | capi.BATCH_AUTOTUNE)
+        got = b.images()
+        for k in (0, 1, 4):
+            it, s = b.items[k], b.snaps[k]
+            oimg, obeam, _, _ = orc.image_visibilities(it.img_pars, s["vis"], s["weights"], s["uvw"])
+            assert peak_rel_err(got[k][0], oimg) < 1e-4 and peak_rel_err(got[k][1], obeam) < 1e-4, k
+    finally:
+        b.free()
