@@ -251,8 +251,10 @@ def measure_pipelined(Context, capi, args, pars, sfp, prec, n, N, in_bufs, do_sf
 
 def bench_batch(args, emit=True):
     """Workload c4: 48 snapshots (N in {2048, 4096}, 2e4-2e6 visibilities) imaged per
-    call of stp_image_batch, FP32 path, autotuned shapes, captured once as a hipGraph
-    and replayed; compared with the same batch on the FP64 path run eagerly."""
+    call of stp_image_batch, FP32 path with autotuned shapes, fused (one launch per
+    stage across the snapshots) and as a replayed hipGraph of per-snapshot streams;
+    compared with the FP64 path fused, per-snapshot streams run eagerly, and plain
+    per-snapshot calls.  `value` is the faster FP32 form."""
     from fastimaging_amd import capi
     from tests import simdata
     from fastimaging_amd.stp import Context
@@ -298,7 +300,9 @@ def bench_batch(args, emit=True):
         img = np.empty((N, N), order="F")
         ctx.dtoh(img, items[k].d_image)
         ref64[k] = img
-    for name, prec, flags in (("fp32c_graph", capi.PRECISION_FP32C, capi.BATCH_GRAPH | capi.BATCH_AUTOTUNE),
+    for name, prec, flags in (("fp32c_fused", capi.PRECISION_FP32C, capi.BATCH_FUSED | capi.BATCH_AUTOTUNE),
+                              ("fp32c_graph", capi.PRECISION_FP32C, capi.BATCH_GRAPH | capi.BATCH_AUTOTUNE),
+                              ("fp64_fused", capi.PRECISION_FP64, capi.BATCH_FUSED),
                               ("fp64_eager", capi.PRECISION_FP64, 0)):
         t1 = time.perf_counter()
         ctx.image_batch(items, precision=prec, flags=flags)  # tunes and records on the first call
@@ -316,7 +320,7 @@ def bench_batch(args, emit=True):
     # 1e-15 of the oracle in the tests) on the three smallest and three largest
     # snapshots, and against the FP64 oracle on the smallest
     acc = []
-    ctx.image_batch(items, precision=capi.PRECISION_FP32C, flags=capi.BATCH_GRAPH | capi.BATCH_AUTOTUNE)
+    ctx.image_batch(items, precision=capi.PRECISION_FP32C, flags=capi.BATCH_FUSED | capi.BATCH_AUTOTUNE)
     for k in check:
         N = specs[k]["image_size"]
         img = np.empty((N, N), order="F", dtype=np.float32)
@@ -329,7 +333,8 @@ def bench_batch(args, emit=True):
                                                       snaps[k]["uvw"])
             a["image_err_vs_oracle"] = float(np.max(np.abs(img - oimg)) / np.max(np.abs(oimg)))
         acc.append(a)
-    res = {"metric": "snapshots/s (48-snapshot batch, hipGraph replay)", "value": out["fp32c_graph"]["snapshots_per_s"],
+    best = max(("fp32c_fused", "fp32c_graph"), key=lambda k: out[k]["snapshots_per_s"])
+    res = {"metric": "snapshots/s (48-snapshot batch)", "value": out[best]["snapshots_per_s"], "path": best,
            "unit": "snapshots/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
            "data": "synthetic (deterministic SplitMix64 snapshots, inputs resident in HBM)",
            "config": {"workload": f"c4: {len(items)} snapshots, N in {{2048, 4096}}, 2e4-2e6 vis (total "
@@ -340,6 +345,10 @@ def bench_batch(args, emit=True):
                                                       out["fp64_plain"]["snapshots_per_s"], 3),
            "speedup_fp32c_graph_vs_fp64_eager": round(out["fp32c_graph"]["snapshots_per_s"] /
                                                       out["fp64_eager"]["snapshots_per_s"], 3),
+           "speedup_fp32c_fused_vs_fp64_plain": round(out["fp32c_fused"]["snapshots_per_s"] /
+                                                      out["fp64_plain"]["snapshots_per_s"], 3),
+           "speedup_fp32c_fused_vs_fp64_fused": round(out["fp32c_fused"]["snapshots_per_s"] /
+                                                      out["fp64_fused"]["snapshots_per_s"], 3),
            "accuracy": acc, "gen_s": round(gen_s, 2)}
     for b in bufs:
         for x in b:
@@ -668,8 +677,9 @@ def main():
     # the batched and served forms of the FP32 path on their own workloads
     if args.workload == "c2" and not args.no_c4:
         c4 = bench_batch(args, emit=False)
-        result["c4"] = {k: c4[k] for k in ("value", "unit", "config", "paths", "speedup_fp32c_graph_vs_fp64_plain",
-                                           "speedup_fp32c_graph_vs_fp64_eager", "accuracy", "gen_s")}
+        result["c4"] = {k: c4[k] for k in ("value", "unit", "path", "config", "paths", "speedup_fp32c_graph_vs_fp64_plain",
+                                           "speedup_fp32c_graph_vs_fp64_eager", "speedup_fp32c_fused_vs_fp64_plain",
+                                           "speedup_fp32c_fused_vs_fp64_fused", "accuracy", "gen_s")}
     if args.workload == "c2" and not args.no_c5:
         c5 = bench_serve(args, emit=False)
         result["c5"] = {k: c5[k] for k in c5 if k not in ("metric", "n_gpus", "higher_is_better", "dtype", "data")}

@@ -54,7 +54,7 @@ struct BatchGridItem {
 bool grid_batch_eligible(const GridderArgs& a, size_t n);
 
 // Grids every item of a batch with one launch per stage (binning counts, column
-// scans, scatter, the tile kernel per kernel width class, totals): each stage's
+// scans, scatter, the tile kernel per kernel and tile width class, totals): each stage's
 // workgroups span all snapshots.  `extra` (caller tables, e.g. the FFT's per-grid
 // pointers) is uploaded with the batch's own tables in one copy; returns its
 // device address (valid until the next fused batch of the context).

@@ -15,8 +15,12 @@
 // Per tap the arithmetic is the reference's: kval = K * w, grid += vis * kval,
 // beam += kval (gridder.h:854-858); the oversampled kernel is the separable
 // product ky[row] * kx[col] of normalised 1-D kernels (gridder.h:84-102).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
 
 #include "gridder.h"
@@ -285,10 +289,24 @@ int max_span_tiles(int N, int last, int s, int ks, int exact, int w, bool rows)
     return best;
 }
 
+// Memoised per geometry: the scan over every centre takes tens of microseconds
+// of host time per call at N = 4096.
 size_t footprint_tiles_max(const GridGeom& g)
 {
-    return size_t(max_span_tiles(g.N, g.rows - 1, g.s, g.ks, g.exact, TILE_R, true)) *
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int, int, int, int>, size_t> memo;
+    const auto key = std::make_tuple(g.N, g.rows, g.s, g.ks, g.exact, g.tc);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = memo.find(key);
+        if (it != memo.end())
+            return it->second;
+    }
+    const size_t v = size_t(max_span_tiles(g.N, g.rows - 1, g.s, g.ks, g.exact, TILE_R, true)) *
         size_t(max_span_tiles(g.N, g.rows - 1, g.s, g.ks, g.exact, g.tc, false));
+    std::lock_guard<std::mutex> lk(mu);
+    memo[key] = v;
+    return v;
 }
 
 // wpart: per block the canonical sum of the good weights and the good count.
@@ -2213,8 +2231,11 @@ const void* grid_batch(stp_ctx* ctx, const std::vector<BatchGridItem>& items, co
     };
     std::vector<BItem> bi(K);
     std::vector<Place> pl(K);
-    std::vector<uint32_t> pre_b(K), pre_c(K), pre_t[2] = { std::vector<uint32_t>(K), std::vector<uint32_t>(K) };
-    size_t nb = 0, nc = 0, nt[2] = { 0, 0 }, pool = 0, hist_bytes = 4, tile_bytes[2] = { 0, 0 };
+    // tile launches per class: kernel width (up to 8 or 16 taps) x tile width (16, 32,
+    // 64 columns), so each launch reserves the LDS its own tiles need
+    constexpr int NCLS = 6;
+    std::vector<uint32_t> pre_b(K), pre_c(K), pre_t(size_t(NCLS) * K);
+    size_t nb = 0, nc = 0, nt[NCLS] = {}, pool = 0, hist_bytes = 4, tile_bytes[NCLS] = {};
     for (int k = 0; k < K; k++) {
         const BatchGridItem& it = items[k];
         STP_REQUIRE(grid_batch_eligible(it.a, it.n), "snapshot not eligible for a fused batch");
@@ -2255,9 +2276,9 @@ const void* grid_batch(stp_ctx* ctx, const std::vector<BatchGridItem>& items, co
         nb += nblk;
         pre_c[k] = uint32_t(nc);
         nc += div_up(ntiles, 64);
-        const int cls = g.ks <= 8 ? 0 : 1;
-        for (int c = 0; c < 2; c++)
-            pre_t[c][k] = uint32_t(nt[c]);
+        const int cls = (g.ks <= 8 ? 0 : 3) + (g.tc == 16 ? 0 : (g.tc == 32 ? 1 : 2));
+        for (int c = 0; c < NCLS; c++)
+            pre_t[size_t(c) * K + k] = uint32_t(nt[c]);
         nt[cls] += ntiles;
         hist_bytes = std::max(hist_bytes, sizeof(uint32_t) * size_t(ntiles));
         const int S = TILE_R + (g.ks <= 15 ? g.ks : 1);
@@ -2265,7 +2286,7 @@ const void* grid_batch(stp_ctx* ctx, const std::vector<BatchGridItem>& items, co
         const int cs = (g.os / 2) * 2 + 1;
         tile_bytes[cls] = std::max(tile_bytes[cls], ((acc + 7) / 8) * 8 + sizeof(double) * size_t(cs) * g.ks);
     }
-    STP_REQUIRE(nb < (size_t(1) << 31) && nt[0] < (size_t(1) << 31) && nt[1] < (size_t(1) << 31),
+    STP_REQUIRE(nb < (size_t(1) << 31) && *std::max_element(nt, nt + NCLS) < (size_t(1) << 31),
         "too many workgroups for one fused batch");
     uint8_t* base = (uint8_t*)ws_get(ctx, "bat_pool", pool);
     for (int k = 0; k < K; k++) {
@@ -2277,8 +2298,8 @@ const void* grid_batch(stp_ctx* ctx, const std::vector<BatchGridItem>& items, co
     }
     // tables: items, the stage prefixes, the caller's tables; one upload from the
     // page-locked stage once the previous batch's upload has left it
-    size_t o_items = 0, o_b = al(sizeof(BItem) * K), o_c = o_b + al(4 * K), o_t0 = o_c + al(4 * K), o_t1 = o_t0 + al(4 * K);
-    const size_t o_x = o_t1 + al(4 * K), tab_bytes = o_x + std::max<size_t>(extra.size(), 1);
+    const size_t o_items = 0, o_b = al(sizeof(BItem) * K), o_c = o_b + al(4 * K), o_t = o_c + al(4 * K);
+    const size_t o_x = o_t + al(4 * size_t(NCLS) * K), tab_bytes = o_x + std::max<size_t>(extra.size(), 1);
     if (!ctx->batch_tab_event)
         HIP_CHECK(hipEventCreateWithFlags(&ctx->batch_tab_event, hipEventDisableTiming));
     HIP_CHECK(hipEventSynchronize(ctx->batch_tab_event));
@@ -2287,8 +2308,7 @@ const void* grid_batch(stp_ctx* ctx, const std::vector<BatchGridItem>& items, co
     memcpy(h + o_items, bi.data(), sizeof(BItem) * K);
     memcpy(h + o_b, pre_b.data(), 4 * K);
     memcpy(h + o_c, pre_c.data(), 4 * K);
-    memcpy(h + o_t0, pre_t[0].data(), 4 * K);
-    memcpy(h + o_t1, pre_t[1].data(), 4 * K);
+    memcpy(h + o_t, pre_t.data(), 4 * size_t(NCLS) * K);
     if (!extra.empty())
         memcpy(h + o_x, extra.data(), extra.size());
     HIP_CHECK(hipMemcpyAsync(d, h, tab_bytes, hipMemcpyHostToDevice, ctx->stream));
@@ -2300,14 +2320,14 @@ const void* grid_batch(stp_ctx* ctx, const std::vector<BatchGridItem>& items, co
     launch(ctx, "batch_count", batch_count_kernel, dim3(unsigned(nb)), dim3(BWG), hist_bytes, d_items, d_pre_b, K);
     launch(ctx, "batch_colscan", batch_colscan_kernel, dim3(unsigned(nc)), dim3(1024), 0, d_items, (const uint32_t*)(d + o_c), K);
     launch(ctx, "batch_scatter", batch_scatter_kernel, dim3(unsigned(nb)), dim3(BWG), hist_bytes, d_items, d_pre_b, K);
-    for (int c = 0; c < 2; c++) {
+    for (int c = 0; c < NCLS; c++) {
         if (nt[c] == 0)
             continue;
-        auto kern = c == 0 ? (twofloat ? batch_tiles_kernel<T, AccTwoFloat, 8> : batch_tiles_kernel<T, AccF64, 8>)
+        auto kern = c < 3 ? (twofloat ? batch_tiles_kernel<T, AccTwoFloat, 8> : batch_tiles_kernel<T, AccF64, 8>)
                            : (twofloat ? batch_tiles_kernel<T, AccTwoFloat, 16> : batch_tiles_kernel<T, AccF64, 16>);
         HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(tile_bytes[c])));
         launch(ctx, "batch_tiles", kern, dim3(unsigned(nt[c])), dim3(TWG), tile_bytes[c], d_items,
-            (const uint32_t*)(d + (c == 0 ? o_t0 : o_t1)), K);
+            (const uint32_t*)(d + o_t) + size_t(c) * K, K);
     }
     launch(ctx, "batch_total", batch_total_kernel, dim3(unsigned(K)), dim3(CANON_LANES), 0, d_items);
     return d + o_x;

@@ -512,7 +512,7 @@ typedef struct stp_batch_item {
  * graphs).  With STP_BATCH_FUSED the snapshots without W-projection whose
  * kernels are oversampled with at most 16 taps (and at most 24576 grid tiles and
  * 16.7 M visibilities) are imaged together, eagerly and with no capture: one
- * binning, one column scan, one scatter and one tile launch per kernel width class
+ * binning, one column scan, one scatter and one tile launch per kernel and tile width class
  * span all of their tiles, one normalisation launch all of their totals, and one
  * FFT per group of equal transform shape all of the group's grids, so any new
  * batch runs in a dozen launches; the other snapshots take the streams (and the

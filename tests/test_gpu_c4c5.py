@@ -4,7 +4,8 @@ sizes, checked snapshot by snapshot against the FP64 oracle.
 c4: the 48 snapshots of simdata.batch_specs() (N in {2048, 4096}, 2e4-2e6
 visibilities log-uniform, 1-40 sources) through stp_image_batch: FP64 eager within
 1e-10 of each snapshot's peak, FP32 (FP64 tile sums) captured as a hipGraph with
-autotuning and replayed, within 1e-4.
+autotuning and replayed, within 1e-4; the same two bars for the fused batch (one
+launch per stage across the 48 snapshots).
 c5: a served mix with an 8192^2 image, a 4M-visibility job and a 32-plane
 support-32 W-projection job, each the same alone and in the mix and against the
 oracle (images; island lists and label maps bit-exact against the oracle's source
@@ -103,6 +104,28 @@ def test_c4_fp32_graph_replay(gpu, c4):
                 gpu.htod(it.d_image, np.zeros(N * N, dtype=np.float32))
             gpu.image_batch(items, precision=capi.PRECISION_FP32C, max_concurrent=4, flags=flags)
             _check(gpu, items, oracle, np.float32, 1e-4, f"FP32 graph call {rep}")
+    finally:
+        for o in outs:
+            for x in o:
+                gpu.free(x)
+
+
+@pytest.mark.parametrize("precision,dt,tol", [(capi.PRECISION_FP64, np.float64, 1e-10),
+                                              (capi.PRECISION_FP32C, np.float32, 1e-4)])
+def test_c4_fused(gpu, c4, precision, dt, tol):
+    """c4 through STP_BATCH_FUSED (all 48 snapshots in one launch per stage, with
+    autotuned shapes on the FP32 path), twice in a row, against the oracle."""
+    snaps, oracle, bufs = c4
+    items, outs = _items(gpu, snaps, bufs, 8 if precision == capi.PRECISION_FP64 else 4)
+    flags = capi.BATCH_FUSED | (capi.BATCH_AUTOTUNE if precision != capi.PRECISION_FP64 else 0)
+    try:
+        for rep in range(2):
+            for it in items:
+                N = it.img_pars.image_size
+                gpu.htod(it.d_image, np.zeros(N * N, dtype=dt))
+            totals = gpu.image_batch(items, precision=precision, flags=flags)
+            assert np.all(totals > 0)
+            _check(gpu, items, oracle, dt, tol, f"fused precision {precision} call {rep}")
     finally:
         for o in outs:
             for x in o:
