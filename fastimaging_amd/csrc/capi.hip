@@ -1,5 +1,6 @@
 // extern "C" entry points of the gfx950 STP library (include/stp_capi.h).
 // Every entry point converts exceptions into a status code and a message.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -201,7 +202,7 @@ bool batch_fusable(stp_ctx* ctx, const stp_batch_item& it, const stp_kernel_spec
 // image size, gcf, u split, beam) over the group's grids, which lie contiguous.
 template <typename T, typename OT>
 void run_batch_fused(stp_ctx* ctx, const stp_batch_item* items, const std::vector<size_t>& idx, const stp_kernel_spec* kspec,
-    int precision, double* totals)
+    int precision, double* totals, hipEvent_t grid_after = nullptr, hipEvent_t gridded = nullptr)
 {
     struct Shape {
         int N, image, l1, beam;
@@ -270,7 +271,12 @@ void run_batch_fused(stp_ctx* ctx, const stp_batch_item* items, const std::vecto
         extra.insert(extra.end(), (const uint8_t*)tots.data(), (const uint8_t*)(tots.data() + tots.size()));
         runs.push_back(r);
     }
+    // the gridding starts after `grid_after` and records `gridded` before the FFTs
+    if (grid_after)
+        HIP_CHECK(hipStreamWaitEvent(ctx->stream, grid_after, 0));
     const uint8_t* d_extra = (const uint8_t*)grid_batch<T>(ctx, bg, extra);
+    if (gridded)
+        HIP_CHECK(hipEventRecord(gridded, ctx->stream));
     for (const Run& r : runs) {
         NormSpec ns;
         ns.image_size = r.s.image;
@@ -458,8 +464,8 @@ stp_status stp_ctx_destroy(stp_ctx* ctx)
         (void)hipEventDestroy(ctx->copy_event);
     if (ctx->skymodel_event)
         (void)hipEventDestroy(ctx->skymodel_event);
-    if (ctx->batch_tab_event)
-        (void)hipEventDestroy(ctx->batch_tab_event);
+    for (auto& kv : ctx->batch_tab_events)
+        (void)hipEventDestroy(kv.second);
     for (auto& kv : ctx->ws)
         if (kv.second.ptr)
             (void)hipFree(kv.second.ptr);
@@ -967,13 +973,67 @@ stp_status stp_image_batch(stp_ctx* ctx, size_t n_items, const stp_batch_item* i
         for (size_t i = 0; i < n_items; i++)
             ((flags & STP_BATCH_FUSED) && batch_fusable(ctx, items[i], kernel, precision) ? fused : rest).push_back(i);
         hipStream_t main = ctx->stream;
+        // streams: K for the rest, two more for the fused snapshots' chunks
+        const int K = rest.empty() ? 0
+                                   : int(std::min<size_t>(rest.size(), max_concurrent ? std::min(max_concurrent, 16u) : 4u));
+        while (int(ctx->branch_streams.size()) < K + 2) {
+            hipStream_t st;
+            HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            ctx->branch_streams.push_back(st);
+        }
+        while (ctx->branch_events.size() < size_t(K) + 7) {
+            hipEvent_t e;
+            HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            ctx->branch_events.push_back(e);
+        }
         auto run_fused = [&]() {
             if (fused.empty())
                 return;
-            if (precision == STP_PRECISION_FP64)
-                run_batch_fused<double, double>(ctx, items, fused, kernel, precision, totals);
-            else
-                run_batch_fused<float, float>(ctx, items, fused, kernel, precision, totals);
+            // chunks of about equal work (visibilities, and grid cells for the FFT),
+            // pipelined on two streams: chunk c grids once chunk c-1 has gridded, so
+            // its tile sums (LDS-bound) overlap chunk c-1's FFTs (HBM-bound)
+            const int P = fused.size() >= 8 ? 4 : (fused.size() >= 4 ? 2 : 1);
+            std::vector<std::vector<size_t>> part(P);
+            auto cost = [&](size_t i) {
+                const double N = double(items[i].img_pars.padded_image_size);
+                return double(items[i].n_vis) + 1.1e6 * (N / 4096.0) * (N / 4096.0);
+            };
+            std::vector<size_t> ord(fused);
+            std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return cost(a) > cost(b); });
+            std::vector<double> load(P, 0.0);
+            for (size_t i : ord) {
+                const int j = int(std::min_element(load.begin(), load.end()) - load.begin());
+                part[j].push_back(i);
+                load[j] += cost(i);
+            }
+            for (auto& pt : part)
+                std::sort(pt.begin(), pt.end());
+            hipEvent_t fork = ctx->branch_events[K + 1], join = ctx->branch_events[K + 2];
+            HIP_CHECK(hipEventRecord(fork, main));
+            for (int s = 0; s < 2; s++)
+                HIP_CHECK(hipStreamWaitEvent(ctx->branch_streams[K + s], fork, 0));
+            try {
+                for (int c = 0; c < P; c++) {
+                    ctx->stream = ctx->branch_streams[K + (c & 1)];
+                    ctx->ws_slot = K + 1 + c;
+                    hipEvent_t after = c > 0 ? ctx->branch_events[K + 2 + c] : nullptr;
+                    hipEvent_t done = ctx->branch_events[K + 3 + c];
+                    if (precision == STP_PRECISION_FP64)
+                        run_batch_fused<double, double>(ctx, items, part[c], kernel, precision, totals, after, done);
+                    else
+                        run_batch_fused<float, float>(ctx, items, part[c], kernel, precision, totals, after, done);
+                }
+            } catch (...) {
+                ctx->stream = main;
+                ctx->ws_slot = 0;
+                throw;
+            }
+            ctx->stream = main;
+            ctx->ws_slot = 0;
+            for (int s = 0; s < 2; s++) {
+                HIP_CHECK(hipEventRecord(join, ctx->branch_streams[K + s]));
+                HIP_CHECK(hipStreamWaitEvent(main, join, 0));
+            }
         };
         auto finish = [&]() {
             if (sample_grid_totals) {
@@ -990,17 +1050,6 @@ stp_status stp_image_batch(stp_ctx* ctx, size_t n_items, const stp_batch_item* i
             run_fused();
             finish();
             return;
-        }
-        const int K = int(std::min<size_t>(rest.size(), max_concurrent ? std::min(max_concurrent, 16u) : 4u));
-        while (int(ctx->branch_streams.size()) < K) {
-            hipStream_t st;
-            HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-            ctx->branch_streams.push_back(st);
-        }
-        while (ctx->branch_events.size() < size_t(K) + 1) {
-            hipEvent_t e;
-            HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            ctx->branch_events.push_back(e);
         }
         auto enqueue = [&](bool with_fused) {
             HIP_CHECK(hipEventRecord(ctx->branch_events[K], main));

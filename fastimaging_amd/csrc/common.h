@@ -123,8 +123,8 @@ struct stp_ctx {
     hipEvent_t copy_event = nullptr;
     // the last upload of sky-model terms from the page-locked stage
     hipEvent_t skymodel_event = nullptr;
-    // the last upload of a fused batch's tables from the page-locked stage
-    hipEvent_t batch_tab_event = nullptr;
+    // per workspace slot: the last upload of a fused batch's tables from its page-locked stage
+    std::map<int, hipEvent_t> batch_tab_events;
 };
 
 namespace stp_hip {

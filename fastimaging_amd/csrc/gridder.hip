@@ -2300,10 +2300,11 @@ const void* grid_batch(stp_ctx* ctx, const std::vector<BatchGridItem>& items, co
     // page-locked stage once the previous batch's upload has left it
     const size_t o_items = 0, o_b = al(sizeof(BItem) * K), o_c = o_b + al(4 * K), o_t = o_c + al(4 * K);
     const size_t o_x = o_t + al(4 * size_t(NCLS) * K), tab_bytes = o_x + std::max<size_t>(extra.size(), 1);
-    if (!ctx->batch_tab_event)
-        HIP_CHECK(hipEventCreateWithFlags(&ctx->batch_tab_event, hipEventDisableTiming));
-    HIP_CHECK(hipEventSynchronize(ctx->batch_tab_event));
-    uint8_t* h = (uint8_t*)host_stage(ctx, "bat_tabs", tab_bytes);
+    hipEvent_t& up = ctx->batch_tab_events[ctx->ws_slot];
+    if (!up)
+        HIP_CHECK(hipEventCreateWithFlags(&up, hipEventDisableTiming));
+    HIP_CHECK(hipEventSynchronize(up));
+    uint8_t* h = (uint8_t*)host_stage(ctx, "bat_tabs#" + std::to_string(ctx->ws_slot), tab_bytes);
     uint8_t* d = (uint8_t*)ws_get(ctx, "bat_tabs", tab_bytes);
     memcpy(h + o_items, bi.data(), sizeof(BItem) * K);
     memcpy(h + o_b, pre_b.data(), 4 * K);
@@ -2312,7 +2313,7 @@ const void* grid_batch(stp_ctx* ctx, const std::vector<BatchGridItem>& items, co
     if (!extra.empty())
         memcpy(h + o_x, extra.data(), extra.size());
     HIP_CHECK(hipMemcpyAsync(d, h, tab_bytes, hipMemcpyHostToDevice, ctx->stream));
-    HIP_CHECK(hipEventRecord(ctx->batch_tab_event, ctx->stream));
+    HIP_CHECK(hipEventRecord(up, ctx->stream));
     const BItem* d_items = (const BItem*)(d + o_items);
     const uint32_t* d_pre_b = (const uint32_t*)(d + o_b);
     HIP_CHECK(hipFuncSetAttribute((const void*)batch_count_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(hist_bytes)));

@@ -214,7 +214,7 @@ def _compare(items, got, want, tol):
 @pytest.mark.parametrize("precision", [capi.PRECISION_FP64, capi.PRECISION_FP32C, capi.PRECISION_FP32_TWOFLOAT])
 def test_fused_batch_matches_single(gpu, orc, precision):
     """STP_BATCH_FUSED: the eligible snapshots are gridded by one launch per stage
-    (kernel timings count the launches) and give each snapshot's single-call image
+    for each chunk (kernel timings count the launches) and give each snapshot's single-call image
     and beam (FP64 within 1e-12 of the peak, FP32 within 1e-6) with bit-identical
     sample_grid_totals; the streamed ones run beside them; a different batch next
     (other sizes and order) rebuilds the tables; graph mode replays the streamed part
@@ -238,8 +238,9 @@ def test_fused_batch_matches_single(gpu, orc, precision):
         finally:
             gpu.set_timing(False)
         print({k: v for k, v in t.items() if k.startswith("batch_") or k.startswith("grid_")}, flush=True)
-        assert t["batch_count"][1] == 1 and t["batch_scatter"][1] == 1 and t["batch_total"][1] == 1
-        assert t["batch_tiles"][1] == 2  # the 8-tap and the 16-tap class
+        # six fused snapshots: two chunks, one launch per stage each
+        assert t["batch_count"][1] == 2 and t["batch_scatter"][1] == 2 and t["batch_total"][1] == 2
+        assert 2 <= t["batch_tiles"][1] <= 4  # the 8-tap and the 16-tap class over the chunks
         assert t["grid_tiles"][1] == 1 and t["grid_wtiles"][1] >= 1  # 17 taps and W-projection on the streams
         _compare(b.items, b.images(), single, tol)
         assert list(totals) == stotals
