@@ -79,6 +79,12 @@ def kernel_bytes(name, N, rows, nvis, n_entries, nb, S):
         return 48 * nvis                # read u, v, w(unused), vis, weight
     if name == "grid_scatter":
         return 48 * nvis + 40 * n_entries
+    if name in ("sf_sum", "sel_hist1", "sel_hist2", "sel_gather", "sf_clip"):
+        return img                      # one pass over the image
+    if name == "ccl_tile":
+        return img + 4 * N * N          # read the image, write the 32-bit pixel keys
+    if name == "ccl_label_map":
+        return 8 * N * N                # read the pixel keys, write the 32-bit label map
     return None
 
 
