@@ -97,10 +97,13 @@ def pmc_traffic(workload, path, kernel):
         return None
     for line in open(fn):
         f = line.split()
-        if line.startswith("#") or len(f) != 4:
+        if line.startswith("#") or len(f) < 4:
             continue
-        if f[0] == kernel:
-            return int(f[2]) + int(f[3])
+        # the profile names the kernel function ("grid_tiles_os_kernel<float, AccF64, 8>"),
+        # the library's timing names the launch ("grid_tiles")
+        name = " ".join(f[:-3])
+        if name == kernel or name.startswith(kernel + "_") or name.startswith(kernel + "<"):
+            return int(f[-2]) + int(f[-1])
     return None
 
 
