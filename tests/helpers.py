@@ -71,3 +71,30 @@ def compare(gpu_res, orc_res):
         for f in ISLAND_EXACT:
             assert a[f] == b[f], f
         assert fits_agree(a["moments_fit"], b["moments_fit"]), (a, b)
+
+
+def islands_agree_banded(isl, oisl, det_pos, det_neg, band):
+    """Islands found on an FP32 image against the oracle's islands on the FP64 image.
+    The two images differ by at most `band` per pixel, so an island whose extremum lies
+    within `band` of its detection threshold (det_pos, or det_neg for negative islands)
+    may be found on one side only.  Every island beyond the band must be on both sides
+    with the same sign and extremum pixel, and its extremum value within `band`.
+    Returns (islands beyond the band, islands inside it)."""
+    def key(i):
+        return (i["sign"], i["extremum_x_idx"], i["extremum_y_idx"])
+
+    def clear(i):
+        e = i["extremum_val"]
+        return e > det_pos + band if i["sign"] > 0 else e < det_neg - band
+
+    g = {key(i): i for i in isl}
+    o = {key(i): i for i in oisl}
+    strict = 0
+    for mine, other in ((g, o), (o, g)):
+        for k, i in mine.items():
+            if clear(i):
+                assert k in other, ("island beyond the guard band found on one side only", k, i["extremum_val"])
+                assert abs(i["extremum_val"] - other[k]["extremum_val"]) <= band, k
+                strict += 1
+    banded = len(set(g) | set(o)) - len({k for k, i in g.items() if clear(i)} | {k for k, i in o.items() if clear(i)})
+    return strict // 2, banded
