@@ -197,13 +197,13 @@ bool batch_fusable(stp_ctx* ctx, const stp_batch_item& it, const stp_kernel_spec
     return grid_batch_eligible(imager_args(ctx, &it.img_pars, nullptr, kern, it.n_vis, precision, &l1), it.n_vis);
 }
 
-// The fused items of a batch: one gridding launch per stage over all of them
-// (grid_batch), then one c2r per group of equal transform shape (padded size,
-// image size, gcf, u split, beam) over the group's grids, which lie contiguous.
+// A chunk of the fused items of a batch: one gridding launch per stage over all of
+// them (grid_batch_bin, then grid_batch_tiles), then one c2r per group of equal
+// transform shape (padded size, image size, gcf, u split, beam) over the group's
+// grids, which lie contiguous.  The three phases run on the context stream current
+// at each call, so a caller can give them different, ordered streams.
 template <typename T, typename OT>
-void run_batch_fused(stp_ctx* ctx, const stp_batch_item* items, const std::vector<size_t>& idx, const stp_kernel_spec* kspec,
-    int precision, double* totals, hipEvent_t grid_after = nullptr, hipEvent_t gridded = nullptr)
-{
+struct FusedChunk {
     struct Shape {
         int N, image, l1, beam;
         const double* gcf;
@@ -212,32 +212,6 @@ void run_batch_fused(stp_ctx* ctx, const stp_batch_item* items, const std::vecto
             return std::tie(N, image, l1, beam, gcf) < std::tie(o.N, o.image, o.l1, o.beam, o.gcf);
         }
     };
-    std::vector<BatchGridItem> bg(idx.size());
-    std::map<Shape, std::vector<size_t>> groups; // shape -> positions in bg
-    for (size_t q = 0; q < idx.size(); q++) {
-        const stp_batch_item& it = items[idx[q]];
-        const stp_imager_pars* p = &it.img_pars;
-        const stp_kernel_spec kern = kspec ? *kspec : dispatch_kernel(p);
-        check_kernel_spec(&kern);
-        int l1 = 0;
-        BatchGridItem& b = bg[q];
-        b.a = imager_args(ctx, p, nullptr, kern, it.n_vis, precision, &l1);
-        b.n = it.n_vis;
-        b.uv = it.d_uvw_lambda;
-        b.vis = it.d_vis;
-        b.wts = it.d_vis_weights;
-        b.total = totals + 2 * idx[q];
-        b.ngood = (unsigned long long*)(totals + 2 * idx[q] + 1);
-        groups[Shape{ int(p->padded_image_size), int(p->image_size), l1, p->generate_beam ? 1 : 0, gcf_table(ctx, p, kern) }]
-            .push_back(q);
-    }
-    // grids: each group's grids contiguous in item order (vis grid, then beam grid)
-    size_t elems = 0;
-    for (auto& gr : groups)
-        elems += size_t(gr.first.N / 2 + 1) * gr.first.N * (gr.first.beam ? 2 : 1) * gr.second.size();
-    cplx<T>* grids = (cplx<T>*)ws_get(ctx, "bat_grids", sizeof(cplx<T>) * std::max<size_t>(elems, 1));
-    // per group: output and total pointer per grid, in the uploaded table
-    std::vector<uint8_t> extra;
     struct Run {
         Shape s;
         cplx<T>* grids;
@@ -245,47 +219,113 @@ void run_batch_fused(stp_ctx* ctx, const stp_batch_item* items, const std::vecto
         size_t out_off, tot_off;
     };
     std::vector<Run> runs;
-    size_t cur = 0;
-    for (auto& gr : groups) {
-        const Shape& s = gr.first;
-        const size_t ge = size_t(s.N / 2 + 1) * s.N;
-        Run r{ s, grids + cur, int(gr.second.size() * (s.beam ? 2 : 1)), 0, 0 };
-        std::vector<OT*> outs;
-        std::vector<const double*> tots;
-        for (size_t q : gr.second) {
+    BatchPlan plan;
+
+    void bin(stp_ctx* ctx, const stp_batch_item* items, const std::vector<size_t>& idx, const stp_kernel_spec* kspec,
+        int precision, double* totals)
+    {
+        std::vector<BatchGridItem> bg(idx.size());
+        std::map<Shape, std::vector<size_t>> groups; // shape -> positions in bg
+        for (size_t q = 0; q < idx.size(); q++) {
             const stp_batch_item& it = items[idx[q]];
-            bg[q].vgrid = grids + cur;
-            cur += ge;
-            outs.push_back((OT*)it.d_image);
-            tots.push_back(bg[q].total);
-            if (s.beam) {
-                bg[q].bgrid = grids + cur;
-                cur += ge;
-                outs.push_back((OT*)it.d_beam);
-                tots.push_back(bg[q].total);
-            }
+            const stp_imager_pars* p = &it.img_pars;
+            const stp_kernel_spec kern = kspec ? *kspec : dispatch_kernel(p);
+            check_kernel_spec(&kern);
+            int l1 = 0;
+            BatchGridItem& b = bg[q];
+            b.a = imager_args(ctx, p, nullptr, kern, it.n_vis, precision, &l1);
+            b.n = it.n_vis;
+            b.uv = it.d_uvw_lambda;
+            b.vis = it.d_vis;
+            b.wts = it.d_vis_weights;
+            b.total = totals + 2 * idx[q];
+            b.ngood = (unsigned long long*)(totals + 2 * idx[q] + 1);
+            groups[Shape{ int(p->padded_image_size), int(p->image_size), l1, p->generate_beam ? 1 : 0,
+                       gcf_table(ctx, p, kern) }]
+                .push_back(q);
         }
-        r.out_off = extra.size();
-        extra.insert(extra.end(), (const uint8_t*)outs.data(), (const uint8_t*)(outs.data() + outs.size()));
-        r.tot_off = extra.size();
-        extra.insert(extra.end(), (const uint8_t*)tots.data(), (const uint8_t*)(tots.data() + tots.size()));
-        runs.push_back(r);
+        // grids: each group's grids contiguous in item order (vis grid, then beam grid)
+        size_t elems = 0;
+        for (auto& gr : groups)
+            elems += size_t(gr.first.N / 2 + 1) * gr.first.N * (gr.first.beam ? 2 : 1) * gr.second.size();
+        cplx<T>* grids = (cplx<T>*)ws_get(ctx, "bat_grids", sizeof(cplx<T>) * std::max<size_t>(elems, 1));
+        // per group: output and total pointer per grid, in the uploaded table
+        std::vector<uint8_t> extra;
+        runs.clear();
+        size_t cur = 0;
+        for (auto& gr : groups) {
+            const Shape& s = gr.first;
+            const size_t ge = size_t(s.N / 2 + 1) * s.N;
+            Run r{ s, grids + cur, int(gr.second.size() * (s.beam ? 2 : 1)), 0, 0 };
+            std::vector<OT*> outs;
+            std::vector<const double*> tots;
+            for (size_t q : gr.second) {
+                const stp_batch_item& it = items[idx[q]];
+                bg[q].vgrid = grids + cur;
+                cur += ge;
+                outs.push_back((OT*)it.d_image);
+                tots.push_back(bg[q].total);
+                if (s.beam) {
+                    bg[q].bgrid = grids + cur;
+                    cur += ge;
+                    outs.push_back((OT*)it.d_beam);
+                    tots.push_back(bg[q].total);
+                }
+            }
+            r.out_off = extra.size();
+            extra.insert(extra.end(), (const uint8_t*)outs.data(), (const uint8_t*)(outs.data() + outs.size()));
+            r.tot_off = extra.size();
+            extra.insert(extra.end(), (const uint8_t*)tots.data(), (const uint8_t*)(tots.data() + tots.size()));
+            runs.push_back(r);
+        }
+        plan = grid_batch_bin<T>(ctx, bg, extra);
     }
-    // the gridding starts after `grid_after` and records `gridded` before the FFTs
-    if (grid_after)
-        HIP_CHECK(hipStreamWaitEvent(ctx->stream, grid_after, 0));
-    const uint8_t* d_extra = (const uint8_t*)grid_batch<T>(ctx, bg, extra);
-    if (gridded)
-        HIP_CHECK(hipEventRecord(gridded, ctx->stream));
-    for (const Run& r : runs) {
-        NormSpec ns;
-        ns.image_size = r.s.image;
-        ns.total_tab = (const double* const*)(d_extra + r.tot_off);
-        ns.u_l1 = r.s.l1;
-        ns.gcf = r.s.gcf;
-        OutPtrs<OT> outs{};
-        outs.tab = (OT* const*)(d_extra + r.out_off);
-        fft_c2r_2d<T, OT>(ctx, r.grids, r.nbatch, r.s.N, r.s.N, outs, ns);
+
+    void tiles(stp_ctx* ctx) { grid_batch_tiles<T>(ctx, plan); }
+
+    void fft(stp_ctx* ctx)
+    {
+        const uint8_t* d_extra = (const uint8_t*)plan.d_extra;
+        for (const Run& r : runs) {
+            NormSpec ns;
+            ns.image_size = r.s.image;
+            ns.total_tab = (const double* const*)(d_extra + r.tot_off);
+            ns.u_l1 = r.s.l1;
+            ns.gcf = r.s.gcf;
+            OutPtrs<OT> outs{};
+            outs.tab = (OT* const*)(d_extra + r.out_off);
+            fft_c2r_2d<T, OT>(ctx, r.grids, r.nbatch, r.s.N, r.s.N, outs, ns);
+        }
+    }
+};
+
+// Fused items in chunks on two streams: every chunk's binning, then every chunk's
+// tile sums, in turn on the first stream, and each chunk's FFTs on the second once
+// its tile sums are done, so the tile sums of chunk c (LDS-bound) overlap the FFTs
+// of chunk c-1 (HBM-bound), while the binning, whose 1024-thread workgroups wait
+// long for whole CUs beside other kernels, runs alone.  (Two streams, not three:
+// the device's few hardware queues may put a third on the queue of one of them.)
+// events[c] marks the end of chunk c's tile sums.
+template <typename T, typename OT>
+void run_fused_chunks(stp_ctx* ctx, const stp_batch_item* items, const std::vector<std::vector<size_t>>& part,
+    const stp_kernel_spec* kspec, int precision, double* totals, const hipStream_t* streams, const hipEvent_t* events,
+    int slot0)
+{
+    const int P = int(part.size());
+    std::vector<FusedChunk<T, OT>> ch(P);
+    ctx->stream = streams[0];
+    for (int c = 0; c < P; c++) {
+        ctx->ws_slot = slot0 + c;
+        ch[c].bin(ctx, items, part[c], kspec, precision, totals);
+    }
+    for (int c = 0; c < P; c++) {
+        ctx->stream = streams[0];
+        ctx->ws_slot = slot0 + c;
+        ch[c].tiles(ctx);
+        HIP_CHECK(hipEventRecord(events[c], streams[0]));
+        HIP_CHECK(hipStreamWaitEvent(streams[1], events[c], 0));
+        ctx->stream = streams[1];
+        ch[c].fft(ctx);
     }
 }
 
@@ -989,9 +1029,7 @@ stp_status stp_image_batch(stp_ctx* ctx, size_t n_items, const stp_batch_item* i
         auto run_fused = [&]() {
             if (fused.empty())
                 return;
-            // chunks of about equal work (visibilities, and grid cells for the FFT),
-            // pipelined on two streams: chunk c grids once chunk c-1 has gridded, so
-            // its tile sums (LDS-bound) overlap chunk c-1's FFTs (HBM-bound)
+            // chunks of about equal work (visibilities, and grid cells for the FFT)
             const int P = fused.size() >= 8 ? 4 : (fused.size() >= 4 ? 2 : 1);
             std::vector<std::vector<size_t>> part(P);
             auto cost = [&](size_t i) {
@@ -1008,21 +1046,16 @@ stp_status stp_image_batch(stp_ctx* ctx, size_t n_items, const stp_batch_item* i
             }
             for (auto& pt : part)
                 std::sort(pt.begin(), pt.end());
-            hipEvent_t fork = ctx->branch_events[K + 1], join = ctx->branch_events[K + 2];
-            HIP_CHECK(hipEventRecord(fork, main));
+            const hipStream_t* st = &ctx->branch_streams[K];
+            const hipEvent_t* ev = &ctx->branch_events[K + 3];
+            HIP_CHECK(hipEventRecord(ctx->branch_events[K + 1], main));
             for (int s = 0; s < 2; s++)
-                HIP_CHECK(hipStreamWaitEvent(ctx->branch_streams[K + s], fork, 0));
+                HIP_CHECK(hipStreamWaitEvent(st[s], ctx->branch_events[K + 1], 0));
             try {
-                for (int c = 0; c < P; c++) {
-                    ctx->stream = ctx->branch_streams[K + (c & 1)];
-                    ctx->ws_slot = K + 1 + c;
-                    hipEvent_t after = c > 0 ? ctx->branch_events[K + 2 + c] : nullptr;
-                    hipEvent_t done = ctx->branch_events[K + 3 + c];
-                    if (precision == STP_PRECISION_FP64)
-                        run_batch_fused<double, double>(ctx, items, part[c], kernel, precision, totals, after, done);
-                    else
-                        run_batch_fused<float, float>(ctx, items, part[c], kernel, precision, totals, after, done);
-                }
+                if (precision == STP_PRECISION_FP64)
+                    run_fused_chunks<double, double>(ctx, items, part, kernel, precision, totals, st, ev, K + 1);
+                else
+                    run_fused_chunks<float, float>(ctx, items, part, kernel, precision, totals, st, ev, K + 1);
             } catch (...) {
                 ctx->stream = main;
                 ctx->ws_slot = 0;
@@ -1031,8 +1064,8 @@ stp_status stp_image_batch(stp_ctx* ctx, size_t n_items, const stp_batch_item* i
             ctx->stream = main;
             ctx->ws_slot = 0;
             for (int s = 0; s < 2; s++) {
-                HIP_CHECK(hipEventRecord(join, ctx->branch_streams[K + s]));
-                HIP_CHECK(hipStreamWaitEvent(main, join, 0));
+                HIP_CHECK(hipEventRecord(ctx->branch_events[K + 2], st[s]));
+                HIP_CHECK(hipStreamWaitEvent(main, ctx->branch_events[K + 2], 0));
             }
         };
         auto finish = [&]() {

@@ -61,6 +61,25 @@ bool grid_batch_eligible(const GridderArgs& a, size_t n);
 template <typename T>
 const void* grid_batch(stp_ctx* ctx, const std::vector<BatchGridItem>& items, const std::vector<uint8_t>& extra);
 
+// The two halves of grid_batch, for callers that put the tile launches on another
+// stream: grid_batch_bin uploads the tables and launches the binning on the
+// context stream; grid_batch_tiles launches the tile and total kernels on the
+// context stream, which must by then be ordered after the binning.
+struct BatchPlan {
+    static constexpr int CLASSES = 6;  // kernel width (8 or 16 taps) x tile width (16, 32, 64)
+    const void* d_tab = nullptr;       // device tables (snapshot records first)
+    const void* d_extra = nullptr;     // the caller's tables
+    int items = 0;
+    bool twofloat = false;
+    size_t pre_t_off = 0;              // byte offset of the tile-class prefixes in d_tab
+    size_t tiles[CLASSES] = {};
+    size_t tile_lds[CLASSES] = {};
+};
+template <typename T>
+BatchPlan grid_batch_bin(stp_ctx* ctx, const std::vector<BatchGridItem>& items, const std::vector<uint8_t>& extra);
+template <typename T>
+void grid_batch_tiles(stp_ctx* ctx, const BatchPlan& plan);
+
 // W-projection: each W sub-kernel is cached with this many zero columns on both
 // sides (ks rows x (ks + 2 W_KERNEL_PAD_COLS) columns).
 constexpr int W_KERNEL_PAD_COLS = 7;

@@ -2219,7 +2219,7 @@ bool grid_batch_eligible(const GridderArgs& a, size_t n)
 }
 
 template <typename T>
-const void* grid_batch(stp_ctx* ctx, const std::vector<BatchGridItem>& items, const std::vector<uint8_t>& extra)
+BatchPlan grid_batch_bin(stp_ctx* ctx, const std::vector<BatchGridItem>& items, const std::vector<uint8_t>& extra)
 {
     const int K = int(items.size());
     STP_REQUIRE(K >= 1 && K <= 65535, "1 to 65535 snapshots per fused batch");
@@ -2233,7 +2233,7 @@ const void* grid_batch(stp_ctx* ctx, const std::vector<BatchGridItem>& items, co
     std::vector<Place> pl(K);
     // tile launches per class: kernel width (up to 8 or 16 taps) x tile width (16, 32,
     // 64 columns), so each launch reserves the LDS its own tiles need
-    constexpr int NCLS = 6;
+    constexpr int NCLS = BatchPlan::CLASSES;
     std::vector<uint32_t> pre_b(K), pre_c(K), pre_t(size_t(NCLS) * K);
     size_t nb = 0, nc = 0, nt[NCLS] = {}, pool = 0, hist_bytes = 4, tile_bytes[NCLS] = {};
     for (int k = 0; k < K; k++) {
@@ -2321,19 +2321,48 @@ const void* grid_batch(stp_ctx* ctx, const std::vector<BatchGridItem>& items, co
     launch(ctx, "batch_count", batch_count_kernel, dim3(unsigned(nb)), dim3(BWG), hist_bytes, d_items, d_pre_b, K);
     launch(ctx, "batch_colscan", batch_colscan_kernel, dim3(unsigned(nc)), dim3(1024), 0, d_items, (const uint32_t*)(d + o_c), K);
     launch(ctx, "batch_scatter", batch_scatter_kernel, dim3(unsigned(nb)), dim3(BWG), hist_bytes, d_items, d_pre_b, K);
+    BatchPlan plan;
+    plan.d_tab = d;
+    plan.d_extra = d + o_x;
+    plan.items = K;
+    plan.twofloat = twofloat;
+    plan.pre_t_off = o_t;
     for (int c = 0; c < NCLS; c++) {
-        if (nt[c] == 0)
-            continue;
-        auto kern = c < 3 ? (twofloat ? batch_tiles_kernel<T, AccTwoFloat, 8> : batch_tiles_kernel<T, AccF64, 8>)
-                           : (twofloat ? batch_tiles_kernel<T, AccTwoFloat, 16> : batch_tiles_kernel<T, AccF64, 16>);
-        HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(tile_bytes[c])));
-        launch(ctx, "batch_tiles", kern, dim3(unsigned(nt[c])), dim3(TWG), tile_bytes[c], d_items,
-            (const uint32_t*)(d + o_t) + size_t(c) * K, K);
+        plan.tiles[c] = nt[c];
+        plan.tile_lds[c] = tile_bytes[c];
     }
-    launch(ctx, "batch_total", batch_total_kernel, dim3(unsigned(K)), dim3(CANON_LANES), 0, d_items);
-    return d + o_x;
+    return plan;
 }
 
+template <typename T>
+void grid_batch_tiles(stp_ctx* ctx, const BatchPlan& plan)
+{
+    const BItem* d_items = (const BItem*)plan.d_tab;
+    const int K = plan.items;
+    for (int c = 0; c < BatchPlan::CLASSES; c++) {
+        if (plan.tiles[c] == 0)
+            continue;
+        auto kern = c < 3 ? (plan.twofloat ? batch_tiles_kernel<T, AccTwoFloat, 8> : batch_tiles_kernel<T, AccF64, 8>)
+                           : (plan.twofloat ? batch_tiles_kernel<T, AccTwoFloat, 16> : batch_tiles_kernel<T, AccF64, 16>);
+        HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(plan.tile_lds[c])));
+        launch(ctx, "batch_tiles", kern, dim3(unsigned(plan.tiles[c])), dim3(TWG), plan.tile_lds[c], d_items,
+            (const uint32_t*)((const uint8_t*)plan.d_tab + plan.pre_t_off) + size_t(c) * K, K);
+    }
+    launch(ctx, "batch_total", batch_total_kernel, dim3(unsigned(K)), dim3(CANON_LANES), 0, d_items);
+}
+
+template <typename T>
+const void* grid_batch(stp_ctx* ctx, const std::vector<BatchGridItem>& items, const std::vector<uint8_t>& extra)
+{
+    const BatchPlan plan = grid_batch_bin<T>(ctx, items, extra);
+    grid_batch_tiles<T>(ctx, plan);
+    return plan.d_extra;
+}
+
+template BatchPlan grid_batch_bin<double>(stp_ctx*, const std::vector<BatchGridItem>&, const std::vector<uint8_t>&);
+template BatchPlan grid_batch_bin<float>(stp_ctx*, const std::vector<BatchGridItem>&, const std::vector<uint8_t>&);
+template void grid_batch_tiles<double>(stp_ctx*, const BatchPlan&);
+template void grid_batch_tiles<float>(stp_ctx*, const BatchPlan&);
 template const void* grid_batch<double>(stp_ctx*, const std::vector<BatchGridItem>&, const std::vector<uint8_t>&);
 template const void* grid_batch<float>(stp_ctx*, const std::vector<BatchGridItem>&, const std::vector<uint8_t>&);
 

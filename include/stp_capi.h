@@ -512,13 +512,13 @@ typedef struct stp_batch_item {
  * graphs).  With STP_BATCH_FUSED the snapshots without W-projection whose
  * kernels are oversampled with at most 16 taps (and at most 24576 grid tiles and
  * 16.7 M visibilities) are imaged together, eagerly and with no capture, in up
- * to four chunks of about equal work pipelined on two streams (a chunk grids
- * while the one before it runs its FFTs): in each chunk one binning, one column
- * scan, one scatter and one tile launch per kernel and tile width class span all
- * of its tiles, one launch all of its totals, and one FFT per group of equal
- * transform shape all of the group's grids, so any new batch runs in a few dozen
- * launches; the other snapshots take the streams (and the graph) as above,
- * beside them.  The fused snapshots' entry, count and grid
+ * to four chunks of about equal work: in each chunk one binning, one column scan,
+ * one scatter and one tile launch per kernel and tile width class span all of its
+ * tiles, one launch all of its totals, and one FFT per group of equal transform
+ * shape all of the group's grids, so any new batch runs in a few dozen launches.
+ * The chunks' binning runs first, then their tile sums on one stream beside their
+ * FFTs on a second (chunk c's tile sums overlap chunk c-1's FFTs).  The other
+ * snapshots take the streams (and the graph) as above, beside them.  The fused snapshots' entry, count and grid
  * buffers are held at once (worst-case entries, as for single calls).
  * sample_grid_totals (host, n_items, or NULL) receives 2 * sum of good weights
  * per snapshot; the call returns after the batch has finished. */
