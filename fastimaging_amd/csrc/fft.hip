@@ -389,6 +389,21 @@ __global__ __launch_bounds__(WG) void fft_v_c2r(const cplx<T>* grids, size_t gri
         return;
     const int joff = (j < hi) ? j : j - ns.image_size;
     OT* out = (outs.tab ? outs.tab[gi] : outs.p[gi]) + size_t(j) * ns.image_size;
+    if (ns.image_size == n && !ns.gcf && (reinterpret_cast<uintptr_t>(out) & (2 * sizeof(OT) - 1)) == 0) {
+        // no crop, no correction, aligned column: output rows 2m, 2m + 1 are z.x, z.y,
+        // stored as one pair
+        using V2 = typename std::conditional<std::is_same<OT, float>::value, float2, double2>::type;
+        for (int m = lt; m < M; m += S::TPT) {
+            const cplx<T> z = b[m];
+            const double v0 = scale ? double(z.x) * factor : double(z.x);
+            const double v1 = scale ? double(z.y) * factor : double(z.y);
+            V2 pr;
+            pr.x = OT(v0);
+            pr.y = OT(v1);
+            reinterpret_cast<V2*>(out)[m] = pr;
+        }
+        return;
+    }
     for (int m = lt; m < M; m += S::TPT) {
         const cplx<T> z = b[m];
         const int ys[2] = { 2 * m, 2 * m + 1 };
