@@ -172,9 +172,12 @@ void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp
                                                           : (twofloat ? STP_PRECISION_FP32_TWOFLOAT : STP_PRECISION_FP32C);
     GridderArgs a = imager_args(ctx, p, wp, kern, n, precision, &fft_l1);
     a.d_nentries = d_nentries;
+    // without W-projection the sampling grid is real: packed, half the bytes in the
+    // grid and the FFT
+    a.beam_packed = beam && a.wproj.num_wplanes == 0 && N >= 8;
     const int rows = N / 2 + 1;
-    const size_t ge = size_t(rows) * N;
-    cplx<T>* grids = (cplx<T>*)ws_get(ctx, "imager_grids", sizeof(cplx<T>) * ge * (beam ? 2 : 1));
+    const size_t ge = size_t(rows) * N, gb = a.beam_packed ? ge / N * (N / 2) : ge;
+    cplx<T>* grids = (cplx<T>*)ws_get(ctx, "imager_grids", sizeof(cplx<T>) * (ge + (beam ? gb : 0)));
     grid_visibilities<T>(ctx, a, n, d_uvw, d_vis, d_w, grids, beam ? grids + ge : nullptr, d_total, d_ngood);
     NormSpec ns;
     ns.image_size = int(p->image_size);
@@ -183,8 +186,15 @@ void run_imager(stp_ctx* ctx, const stp_imager_pars* p, const stp_wproj_pars* wp
     ns.gcf = gcf_table(ctx, p, kern);
     OutPtrs<OT> outs{};
     outs.p[0] = d_image;
-    outs.p[1] = d_beam;
-    fft_c2r_2d<T, OT>(ctx, grids, beam ? 2 : 1, N, N, outs, ns);
+    if (a.beam_packed) {
+        fft_c2r_2d<T, OT>(ctx, grids, 1, N, N, outs, ns);
+        OutPtrs<OT> bouts{};
+        bouts.p[0] = d_beam;
+        fft_c2r_2d_real<T, OT>(ctx, grids + ge, 1, N, bouts, ns);
+    } else {
+        outs.p[1] = d_beam;
+        fft_c2r_2d<T, OT>(ctx, grids, beam ? 2 : 1, N, N, outs, ns);
+    }
 }
 
 // Whether a batch item can join the fused launches (see grid_batch_eligible).
@@ -216,6 +226,7 @@ struct FusedChunk {
         Shape s;
         cplx<T>* grids;
         int nbatch;
+        bool real;       // packed real beam grids
         size_t out_off, tot_off;
     };
     std::vector<Run> runs;
@@ -234,6 +245,7 @@ struct FusedChunk {
             int l1 = 0;
             BatchGridItem& b = bg[q];
             b.a = imager_args(ctx, p, nullptr, kern, it.n_vis, precision, &l1);
+            b.a.beam_packed = p->generate_beam != 0;  // eligible snapshots: no W-projection, N >= 8
             b.n = it.n_vis;
             b.uv = it.d_uvw_lambda;
             b.vis = it.d_vis;
@@ -244,39 +256,39 @@ struct FusedChunk {
                        gcf_table(ctx, p, kern) }]
                 .push_back(q);
         }
-        // grids: each group's grids contiguous in item order (vis grid, then beam grid)
+        // grids: each group's vis grids contiguous in item order, then its packed beam grids
         size_t elems = 0;
-        for (auto& gr : groups)
-            elems += size_t(gr.first.N / 2 + 1) * gr.first.N * (gr.first.beam ? 2 : 1) * gr.second.size();
+        for (auto& gr : groups) {
+            const size_t rows = size_t(gr.first.N / 2 + 1);
+            elems += rows * (gr.first.N + (gr.first.beam ? gr.first.N / 2 : 0)) * gr.second.size();
+        }
         cplx<T>* grids = (cplx<T>*)ws_get(ctx, "bat_grids", sizeof(cplx<T>) * std::max<size_t>(elems, 1));
-        // per group: output and total pointer per grid, in the uploaded table
+        // per run: output and total pointer per grid, in the uploaded table
         std::vector<uint8_t> extra;
         runs.clear();
         size_t cur = 0;
-        for (auto& gr : groups) {
-            const Shape& s = gr.first;
-            const size_t ge = size_t(s.N / 2 + 1) * s.N;
-            Run r{ s, grids + cur, int(gr.second.size() * (s.beam ? 2 : 1)), 0, 0 };
+        auto add_run = [&](const Shape& s, bool real, const std::vector<size_t>& members) {
+            const size_t ge = size_t(s.N / 2 + 1) * (real ? s.N / 2 : s.N);
+            Run r{ s, grids + cur, int(members.size()), real, 0, 0 };
             std::vector<OT*> outs;
             std::vector<const double*> tots;
-            for (size_t q : gr.second) {
+            for (size_t q : members) {
                 const stp_batch_item& it = items[idx[q]];
-                bg[q].vgrid = grids + cur;
+                (real ? bg[q].bgrid : bg[q].vgrid) = grids + cur;
                 cur += ge;
-                outs.push_back((OT*)it.d_image);
+                outs.push_back((OT*)(real ? it.d_beam : it.d_image));
                 tots.push_back(bg[q].total);
-                if (s.beam) {
-                    bg[q].bgrid = grids + cur;
-                    cur += ge;
-                    outs.push_back((OT*)it.d_beam);
-                    tots.push_back(bg[q].total);
-                }
             }
             r.out_off = extra.size();
             extra.insert(extra.end(), (const uint8_t*)outs.data(), (const uint8_t*)(outs.data() + outs.size()));
             r.tot_off = extra.size();
             extra.insert(extra.end(), (const uint8_t*)tots.data(), (const uint8_t*)(tots.data() + tots.size()));
             runs.push_back(r);
+        };
+        for (auto& gr : groups) {
+            add_run(gr.first, false, gr.second);
+            if (gr.first.beam)
+                add_run(gr.first, true, gr.second);
         }
         plan = grid_batch_bin<T>(ctx, bg, extra);
     }
@@ -294,7 +306,10 @@ struct FusedChunk {
             ns.gcf = r.s.gcf;
             OutPtrs<OT> outs{};
             outs.tab = (OT* const*)(d_extra + r.out_off);
-            fft_c2r_2d<T, OT>(ctx, r.grids, r.nbatch, r.s.N, r.s.N, outs, ns);
+            if (r.real)
+                fft_c2r_2d_real<T, OT>(ctx, r.grids, r.nbatch, r.s.N, outs, ns);
+            else
+                fft_c2r_2d<T, OT>(ctx, r.grids, r.nbatch, r.s.N, r.s.N, outs, ns);
         }
     }
 };

@@ -29,6 +29,13 @@ struct OutPtrs {
 template <typename T, typename OT>
 void fft_c2r_2d(stp_ctx* ctx, cplx<T>* grids, int nbatch, int n_rows, int N, const OutPtrs<OT>& outs, const NormSpec& ns);
 
+// Unnormalised 2-D c2r of nbatch real half-plane grids of an N x N transform, each
+// stored packed as (N/2+1) x (N/2) complex with element (v, u') = G(v, 2u') +
+// i G(v, 2u' + 1) (contiguous); the grids are overwritten.  Half the bytes of
+// fft_c2r_2d on the same grid stored complex.
+template <typename T, typename OT>
+void fft_c2r_2d_real(stp_ctx* ctx, cplx<T>* grids, int nbatch, int N, const OutPtrs<OT>& outs, const NormSpec& ns);
+
 // 2-D c2c of nbatch N x N column-major matrices (contiguous); `data` is
 // overwritten, results in `out`.
 template <typename T>

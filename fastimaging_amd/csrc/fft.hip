@@ -428,6 +428,173 @@ __global__ __launch_bounds__(WG) void fft_v_c2r(const cplx<T>* grids, size_t gri
     }
 }
 
+// V for a real beam grid stored packed (square n x n transform): the grid holds
+// z(v, u') = G(v, 2u') + i G(v, 2u' + 1), u' < U = n/2, already transformed along u'
+// (length U, slot order of the L1 x L2 split) into Z(v, k).  Real-FFT split: with
+// s = (Z(k) + conj Z(U-k)) / 2 and d = -i e^{2 pi i k / n} (Z(k) - conj Z(U-k)) / 2,
+// the u-transform of the real rows is H(v, k) = s + d and H(v, U-k) = conj(s - d).
+// A workgroup slot takes the column pair (k, U - k), k <= U/2: one c2r along v per
+// column, and the image column n - c of each column c other than 0 and U from the
+// point symmetry of a real even transform, B(y, n - c) = B(n - y mod n, c).  Output
+// normalised, cropped and corrected as fft_v_c2r.
+// Column buffers of fft_v_c2r_real: PAR (two columns in at most 40 KB of LDS; more
+// would cost occupancy: FP64 at n = 4096 went 59 -> 68 us) gives each
+// column of a pair its own threads, so both c2r run at once; otherwise a slot runs
+// the pair's two c2r in turn.
+template <typename T, int M>
+struct RealVShape {
+    using S = Shape<M>;
+    static constexpr bool PAR = S::NT >= 2 || 2 * (M + S::PAD) * sizeof(cplx<T>) <= 40 * 1024;
+    static constexpr int SLOTS = PAR ? (S::NT >= 2 ? S::NT : 2) : S::NT;
+    static constexpr int PAIRS = PAR ? SLOTS / 2 : S::NT;
+    static constexpr int THREADS = SLOTS * S::TPT;
+};
+
+template <typename T, typename OT, int M>
+__global__ __launch_bounds__(2 * WG) void fft_v_c2r_real(const cplx<T>* grids, size_t grid_elems, int L1, int L2,
+    const T* tw_n, OutPtrs<OT> outs, NormSpec ns)
+{
+    using S = Shape<M>;
+    using RS = RealVShape<T, M>;
+    constexpr int n = 2 * M;
+    constexpr int U = M;             // packed columns
+    constexpr int rows = M + 1;
+    constexpr int QN = (n / 4 > 0) ? n / 4 : 1;
+    constexpr int QLOG = Log2<QN>::value;
+    constexpr bool SPLIT = std::is_same<T, float>::value || M >= 8192;
+    constexpr int BLOG = QLOG < 5 ? QLOG : 5;
+    constexpr int NTW = SPLIT ? (1 << BLOG) + (QN >> BLOG) : QN;
+    constexpr int KI = (M / 2) / S::TPT + 1;
+    __shared__ cplx<T> buf[RS::SLOTS * (M + S::PAD)];
+    __shared__ cplx<T> s_tw[NTW];
+    const int tid = threadIdx.x;
+    for (int r = tid; r < NTW; r += RS::THREADS)
+        s_tw[r] = load_tw(tw_n, (SPLIT && r >= (1 << BLOG)) ? (r - (1 << BLOG)) << BLOG : r, true);
+    const C2rTw<T, SPLIT> tw{ s_tw, QLOG, BLOG };
+    const int t = tid / S::TPT, lt = tid % S::TPT;
+    const int pair = RS::PAR ? t >> 1 : t;
+    const int first = RS::PAR ? (t & 1) : 0;  // the member this slot runs first (PAR: its only one)
+    const int gi = blockIdx.y;
+    const cplx<T>* g = grids + size_t(gi) * grid_elems;
+    cplx<T>* b = buf + t * (M + S::PAD);
+    const int hi = ns.image_size / 2, hp = n / 2;
+    const double* tp = ns.total_tab ? ns.total_tab[gi] : ns.total_ptr;
+    const bool scale = tp != nullptr;
+    const double factor = scale ? (1.0 / tp[0]) : 1.0;
+    const int x = blockIdx.x * RS::PAIRS + pair;
+    const bool active = x <= U / 2;
+    const int xb = active ? (U - x) % U : 0;
+    const cplx<T>* ca = g + size_t(active ? (x % L1) * L2 + x / L1 : 0) * rows;
+    const cplx<T>* cc = g + size_t((xb % L1) * L2 + xb / L1) * rows;
+    // both columns' terms at rows k and M - k first (all loads in flight together)
+    cplx<T> a0[KI], a1[KI], c0[KI], c1[KI];
+#pragma unroll
+    for (int i = 0; i < KI; i++) {
+        const int k = lt + i * S::TPT;
+        if (active && k <= M / 2) {
+            a0[i] = ca[k];
+            a1[i] = ca[M - k];
+            c0[i] = cc[k];
+            c1[i] = cc[M - k];
+        }
+    }
+    __syncthreads();  // the table is ready
+    const cplx<T> w = tw.at(active ? x : 0);
+    // H(k) = s + d (member 0), conj(s - d) (member 1)
+    auto split = [&](cplx<T> a, cplx<T> c, int member) -> cplx<T> {
+        const cplx<T> sv = { (a.x + c.x) * T(0.5), (a.y - c.y) * T(0.5) };
+        const cplx<T> q = { (a.x - c.x) * T(0.5), (a.y + c.y) * T(0.5) };
+        const cplx<T> wq = cmul(w, q);
+        const cplx<T> dv = { wq.y, -wq.x };  // -i * w * q
+        return member == 0 ? cplx<T>{ sv.x + dv.x, sv.y + dv.y } : cplx<T>{ sv.x - dv.x, -(sv.y - dv.y) };
+    };
+    auto store = [&](int xc, bool refl) {
+        const int xo = refl ? n - xc : xc;  // padded output column
+        int j = -1;
+        if (xo < hi)
+            j = xo;
+        else if (xo >= n - (ns.image_size - hi))
+            j = xo - n + ns.image_size;
+        if (j < 0)
+            return;
+        const int joff = (j < hi) ? j : j - ns.image_size;
+        OT* out = (outs.tab ? outs.tab[gi] : outs.p[gi]) + size_t(j) * ns.image_size;
+        if (ns.image_size == n && !ns.gcf && (reinterpret_cast<uintptr_t>(out) & (2 * sizeof(OT) - 1)) == 0) {
+            // no crop, no correction, aligned column: rows 2p, 2p + 1 as one pair; the
+            // reflected column's pair p holds B(n - 2p mod n), B(n - 2p - 1)
+            using V2 = typename std::conditional<std::is_same<OT, float>::value, float2, double2>::type;
+            for (int p = lt; p < M; p += S::TPT) {
+                const T v0 = refl ? b[(M - p) & (M - 1)].x : b[p].x;
+                const T v1 = refl ? b[M - 1 - p].y : b[p].y;
+                V2 pr;
+                pr.x = OT(scale ? double(v0) * factor : double(v0));
+                pr.y = OT(scale ? double(v1) * factor : double(v1));
+                reinterpret_cast<V2*>(out)[p] = pr;
+            }
+            return;
+        }
+        for (int m = lt; m < M; m += S::TPT) {
+            const cplx<T> z = b[m];
+            const int ys[2] = { 2 * m, 2 * m + 1 };
+            const T vals[2] = { z.x, z.y };
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int y = refl ? (n - ys[q]) & (n - 1) : ys[q];
+                int i;
+                if (y < hi)
+                    i = y;
+                else if (y >= n - (ns.image_size - hi))
+                    i = y - n + ns.image_size;
+                else
+                    continue;
+                double val = scale ? double(vals[q]) * factor : double(vals[q]);
+                if (ns.gcf) {
+                    const int ioff = (i < hi) ? i : i - ns.image_size;
+                    val = val / (ns.gcf[hp + ioff] * ns.gcf[hp + joff]);
+                }
+                out[i] = OT(val);
+            }
+        }
+    };
+#pragma unroll
+    for (int mi = 0; mi < (RS::PAR ? 1 : 2); mi++) {
+        const int member = first + mi;
+        const int xc = member == 0 ? x : U - x;
+        const bool on = active && (member == 0 || U - x != x);
+        // pack Z[k] = E[k] + i O[k] of this column's c2r (as fft_v_c2r); thread pair (k, M - k)
+#pragma unroll
+        for (int i = 0; i < KI; i++) {
+            const int k = lt + i * S::TPT;
+            if (!on || k > M / 2)
+                continue;
+            const int km = M - k;
+            const cplx<T> a = split(a0[i], c0[i], member), c = split(a1[i], c1[i], member);
+            if (k == 0) {
+                b[0] = { a.x + c.x, a.x - c.x };
+                continue;
+            }
+            const cplx<T> e1 = { a.x + c.x, a.y - c.y };
+            const cplx<T> d1 = { a.x - c.x, a.y + c.y };
+            const cplx<T> o1 = cmul(d1, tw.at(k));
+            b[k] = { e1.x - o1.y, e1.y + o1.x };
+            if (km != k) {
+                const cplx<T> e2 = { c.x + a.x, c.y - a.y };
+                const cplx<T> d2 = { c.x - a.x, c.y + a.y };
+                const cplx<T> o2 = cmul(d2, tw.at(km));
+                b[km] = { e2.x - o2.y, e2.y + o2.x };
+            }
+        }
+        __syncthreads();
+        lds_fft<T, M, S::TPT>(b, lt, tw, true);
+        if (on) {
+            store(xc, false);
+            if (xc != 0 && xc != U)
+                store(xc, true);
+        }
+        __syncthreads();  // the next member's pack overwrites the column
+    }
+}
+
 // Column c2c of length L (contiguous) reading column slot of a four-step row
 // transform and writing natural column x of out.
 template <typename T, int L>
@@ -550,6 +717,35 @@ void fft_c2r_2d(stp_ctx* ctx, cplx<T>* grids, int nbatch, int n_rows, int N, con
     }
 }
 
+template <typename T, typename OT>
+void fft_c2r_2d_real(stp_ctx* ctx, cplx<T>* grids, int nbatch, int N, const OutPtrs<OT>& outs, const NormSpec& ns)
+{
+    STP_REQUIRE(N >= 8 && (N & (N - 1)) == 0, "real-grid FFT size must be a power of two >= 8");
+    STP_REQUIRE(nbatch >= 1 && nbatch <= (outs.tab ? 65535 : 4), "1 to 4 grids per FFT launch (65535 with an output table)");
+    const int rows = N / 2 + 1, U = N / 2;
+    const size_t grid_elems = size_t(rows) * U;
+    const Twiddles& twu = twiddles_for(ctx, U);
+    const T* tw = std::is_same<T, double>::value ? (const T*)twu.d : (const T*)twu.f;
+    launch_u_passes<T>(ctx, grids, nbatch, rows, U, grid_elems, tw, true, 0);
+    int L1, L2;
+    split_len<T>(U, L1, L2, 0);
+    const Twiddles& twv = twiddles_for(ctx, N);
+    const T* tvn = std::is_same<T, double>::value ? (const T*)twv.d : (const T*)twv.f;
+    const int M = N / 2;
+    switch (M) {
+#define RVCASE(MM)                                                                                                    \
+    case MM:                                                                                                          \
+        launch(ctx, "fft_v_c2r_real", fft_v_c2r_real<T, OT, MM>, dim3(div_up(U / 2 + 1, RealVShape<T, MM>::PAIRS), nbatch), \
+            dim3(RealVShape<T, MM>::THREADS), 0, (const cplx<T>*)grids, grid_elems, L1, L2, tvn, outs, ns);            \
+        break;
+        RVCASE(4) RVCASE(8) RVCASE(16) RVCASE(32) RVCASE(64) RVCASE(128) RVCASE(256) RVCASE(512) RVCASE(1024)
+        RVCASE(2048) RVCASE(4096) RVCASE(8192)
+#undef RVCASE
+    default:
+        throw Error(STP_ERR_UNSUPPORTED, "real-grid c2r length not supported: " + std::to_string(N));
+    }
+}
+
 template <typename T>
 void fft_c2c_2d(stp_ctx* ctx, cplx<T>* data, cplx<T>* out, int N, bool forward, int nbatch)
 {
@@ -578,6 +774,8 @@ void fft_c2c_2d(stp_ctx* ctx, cplx<T>* data, cplx<T>* out, int N, bool forward, 
 template void fft_c2r_2d<double, double>(stp_ctx*, cplx<double>*, int, int, int, const OutPtrs<double>&, const NormSpec&);
 template void fft_c2r_2d<float, float>(stp_ctx*, cplx<float>*, int, int, int, const OutPtrs<float>&, const NormSpec&);
 template void fft_c2r_2d<float, double>(stp_ctx*, cplx<float>*, int, int, int, const OutPtrs<double>&, const NormSpec&);
+template void fft_c2r_2d_real<double, double>(stp_ctx*, cplx<double>*, int, int, const OutPtrs<double>&, const NormSpec&);
+template void fft_c2r_2d_real<float, float>(stp_ctx*, cplx<float>*, int, int, const OutPtrs<float>&, const NormSpec&);
 template void fft_c2c_2d<double>(stp_ctx*, cplx<double>*, cplx<double>*, int, bool, int);
 
 } // namespace stp_hip

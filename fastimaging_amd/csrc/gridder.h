@@ -22,6 +22,8 @@ struct GridderArgs {
     bool analytic_gcf = false; // image-domain AA kernel of the W-kernels from gcf() (conv_func.h:351-412)
     int tile_cols = 0;      // tile width (16, 32 or 64 grid columns); 0 = default
     int w_phases = 0;       // W-projection tile launches (phases); 0 = about 6 MB of W-kernel cache each
+    bool beam_packed = false; // sampling grid as packed real values: rows x N/2 complex, element (v, u') =
+                              // (G(v, 2u'), G(v, 2u'+1)) (the kernels are real without W-projection)
     unsigned long long* d_nentries = nullptr; // device u64: receives the gridding entries written (may be NULL)
 };
 
@@ -49,7 +51,7 @@ struct BatchGridItem {
 };
 
 // Whether a snapshot can join a fused batch: no W-projection, oversampled
-// kernels of at most 16 taps, at most 24576 tiles and 4096 binning blocks
+// kernels of at most 16 taps, N >= 8, at most 24576 tiles and 4096 binning blocks
 // (16.7 M visibilities).
 bool grid_batch_eligible(const GridderArgs& a, size_t n);
 

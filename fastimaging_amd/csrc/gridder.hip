@@ -38,6 +38,7 @@ constexpr int GWG = 256;
 struct GridGeom {
     int N, rows, half, s, ks, os;
     int halfplane, shift, exact, beam;
+    int bpack = 0; // beam grid packed real (rows x N/2 complex: columns 2u', 2u'+1)
     int tiles_r, tiles_c;
     int tc;       // tile width in grid columns (TILE_R rows per tile)
     double scale;
@@ -750,6 +751,35 @@ struct AccTwoFloat {
     }
 };
 
+// Writes a tile's sums once: the vis grid complex; the beam grid complex (imaginary
+// part 0) or, with g.bpack, packed real: cell (r, c) is component c & 1 of element
+// (r, c / 2) of a rows x N/2 complex array (tile columns come in even pairs).
+template <typename T, typename Acc>
+__device__ __forceinline__ void write_tile(const GridGeom& g, const typename Acc::S* acc, int plane, int S, int r0, int c0,
+    cplx<T>* vgrid, cplx<T>* bgrid)
+{
+    for (int k = threadIdx.x; k < g.tc * TILE_R; k += TWG) {
+        const int lr = k % TILE_R, lc = k / TILE_R;
+        const int r = r0 + lr, c = c0 + lc;
+        if (r < g.rows && c < g.N) {
+            const size_t o = size_t(r) + size_t(c) * g.rows;
+            const int a = lc * S + lr;
+            vgrid[o] = { T(Acc::value(acc, plane, 0, a)), T(Acc::value(acc, plane, 1, a)) };
+            if (g.beam && !g.bpack)
+                bgrid[o] = { T(Acc::value(acc, plane, 2, a)), T(0) };
+        }
+    }
+    if (g.beam && g.bpack)
+        for (int k = threadIdx.x; k < (g.tc / 2) * TILE_R; k += TWG) {
+            const int lr = k % TILE_R, lp = k / TILE_R;
+            const int r = r0 + lr, c = c0 + 2 * lp;
+            if (r < g.rows && c < g.N) {
+                const int a = 2 * lp * S + lr;
+                bgrid[size_t(r) + size_t(c / 2) * g.rows] = { T(Acc::value(acc, plane, 2, a)), T(Acc::value(acc, plane, 2, a + S)) };
+            }
+        }
+}
+
 template <typename T, typename Acc>
 __global__ __launch_bounds__(TWG) void grid_tiles_kernel(GridGeom g, const Entry* entries, const uint32_t* offsets, int ostride,
     uint32_t cap, const double* ktab, const double* uv, cplx<T>* vgrid, cplx<T>* bgrid)
@@ -862,17 +892,7 @@ __global__ __launch_bounds__(TWG) void grid_tiles_kernel(GridGeom g, const Entry
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < g.tc * TILE_R; k += TWG) {
-        const int lr = k % TILE_R, lc = k / TILE_R;
-        const int r = r0 + lr, c = c0 + lc;
-        if (r < g.rows && c < g.N) {
-            const size_t o = size_t(r) + size_t(c) * g.rows;
-            const int a = lc * S + lr;
-            vgrid[o] = { T(Acc::value(acc, plane, 0, a)), T(Acc::value(acc, plane, 1, a)) };
-            if (g.beam)
-                bgrid[o] = { T(Acc::value(acc, plane, 2, a)), T(0) };
-        }
-    }
+    write_tile<T, Acc>(g, acc, plane, S, r0, c0, vgrid, bgrid);
 }
 
 // Oversampled-kernel tiles with ks <= KSM taps per side (the common case): the
@@ -953,17 +973,7 @@ __device__ __forceinline__ void tiles_os_body(const GridGeom& g, const Entry* en
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < g.tc * TILE_R; k += TWG) {
-        const int lr = k % TILE_R, lc = k / TILE_R;
-        const int r = r0 + lr, c = c0 + lc;
-        if (r < g.rows && c < g.N) {
-            const size_t o = size_t(r) + size_t(c) * g.rows;
-            const int a = lc * S + lr;
-            vgrid[o] = { T(Acc::value(acc, plane, 0, a)), T(Acc::value(acc, plane, 1, a)) };
-            if (g.beam)
-                bgrid[o] = { T(Acc::value(acc, plane, 2, a)), T(0) };
-        }
-    }
+    write_tile<T, Acc>(g, acc, plane, S, r0, c0, vgrid, bgrid);
 }
 
 template <typename T, typename Acc, int KSM>
@@ -1923,6 +1933,8 @@ static GridGeom make_geom(const GridderArgs& a, size_t n, bool beam, KernelFn& k
     g.shift = a.shift_uv;
     g.exact = a.kernel_exact;
     g.beam = beam;
+    g.bpack = beam && a.beam_packed ? 1 : 0;
+    STP_REQUIRE(!g.bpack || (!wproj && a.halfplane), "a packed real beam grid needs the half-plane grid without W-projection");
     g.tiles_r = int(div_up(g.rows, TILE_R));
     g.tc = a.tile_cols > 0 ? a.tile_cols : TILE_C;
     STP_REQUIRE(g.tc == 16 || g.tc == 32 || g.tc == 64, "tile width must be 16, 32 or 64 columns");
@@ -2214,7 +2226,7 @@ bool grid_batch_eligible(const GridderArgs& a, size_t n)
     const long long ntiles = (long long)div_up(rows, TILE_R) * div_up(N, tc);
     const int os = std::max(1, (a.oversampling >> 1) << 1);
     const int cs = (os / 2) * 2 + 1;
-    return N >= 4 && ntiles <= MAX_LDS_TILES && std::max<size_t>(1, div_up(n, VPB)) <= size_t(CANON_BLOCK) &&
+    return N >= 8 && ntiles <= MAX_LDS_TILES && std::max<size_t>(1, div_up(n, VPB)) <= size_t(CANON_BLOCK) &&
            cs * (2 * a.support + 1) <= KV_DOUBLES;
 }
 

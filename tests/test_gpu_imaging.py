@@ -206,3 +206,33 @@ def test_gpu_imager_c2_full_size(gpu, orc):
     assert peak_rel_err(beam, obeam) < TOL
     # beam peak at the phase centre is 1 after normalisation
     assert abs(beam[0, 0] - 1.0) < 1e-12
+
+
+_ORACLE_BEAMS = {}
+
+
+@pytest.mark.parametrize("precision", [capi.PRECISION_FP64, capi.PRECISION_FP32C])
+@pytest.mark.parametrize("N,padding,gcf", [(8, 1.0, False), (16, 1.0, True), (32, 2.0, True), (64, 1.0, False),
+                                           (2048, 1.0, False), (8192, 1.0, False), (16384, 1.0, False)])
+def test_gpu_beam_real_transform(gpu, orc, N, padding, gcf, precision):
+    """Without W-projection the sampling grid is real and is transformed packed (two
+    columns per complex element, half the bytes): every transform size from the
+    smallest to 16384 (one and two columns of a pair per thread group), padding crop
+    and grid correction, against the oracle's beam and the symmetry of a real even
+    transform, B(y, x) = B(-y, -x) (padded size without crop)."""
+    image = int(N / padding)
+    snap = simdata.snapshot(image, 20.0 * 64 / max(image, 64), 3000 if N <= 64 else 20000, 3, seed=90 + N)
+    pars = capi.imager_pars(image, snap["cell_arcsec"], padding_factor=padding, gridding_correction=gcf)
+    img, beam = gpu.image_visibilities(snap["vis"], snap["weights"], snap["uvw"], pars, precision=precision)
+    key = (N, padding, gcf)
+    if key not in _ORACLE_BEAMS:  # the oracle once per case, for both precisions
+        _ORACLE_BEAMS.clear()
+        _ORACLE_BEAMS[key] = orc.image_visibilities(pars, snap["vis"], snap["weights"], snap["uvw"])[:2]
+    oimg, obeam = _ORACLE_BEAMS[key]
+    tol = TOL if precision == capi.PRECISION_FP64 else 1e-4
+    e_b, e_i = peak_rel_err(beam, obeam), peak_rel_err(img, oimg)
+    print(f"N={N} padding {padding} gcf {gcf} precision {precision}: beam err {e_b:.2e}, image err {e_i:.2e}", flush=True)
+    assert e_b < tol and e_i < tol
+    if padding == 1.0:
+        refl = np.roll(np.roll(beam[::-1, ::-1], 1, axis=0), 1, axis=1)  # B(-y, -x)
+        assert np.max(np.abs(refl - beam)) <= (1e-12 if precision == capi.PRECISION_FP64 else 1e-5) * np.max(np.abs(beam))
