@@ -66,15 +66,21 @@ def parse():
 
 
 def kernel_bytes(name, N, rows, nvis, n_entries, nb, S):
-    """Algorithmic HBM bytes per launch (S = bytes per real of the grid and image)."""
+    """Algorithmic HBM bytes per launch (S = bytes per real of the grid and image).  With a
+    beam (nb = 2) the beam grid is real, stored packed in half the bytes of the vis grid, and
+    its FFT passes are launches of their own under the same names (u1, u2: the mean of the
+    vis and beam launches)."""
     grid = rows * N * 2 * S  # one complex half-plane grid
     img = N * N * S
+    grids = grid + (grid // 2 if nb == 2 else 0)
     if name in ("fft_u1", "fft_u2"):
-        return 2 * grid * nb            # read + write every grid once
+        return 2 * grids // (2 if nb == 2 else 1)  # read + write every grid once, per launch
     if name == "fft_v_c2r":
-        return grid * nb + img * nb     # read grids, write images
+        return grid + img               # read the vis grid, write the image
+    if name == "fft_v_c2r_real":
+        return grid // 2 + img          # read the packed beam grid, write the beam
     if name == "grid_tiles":
-        return grid * nb + 40 * n_entries  # write every grid element once, read the binned entries
+        return grids + 40 * n_entries   # write every grid element once, read the binned entries
     if name == "grid_count":
         return 48 * nvis                # read u, v, w(unused), vis, weight
     if name == "grid_scatter":
