@@ -1083,7 +1083,8 @@ struct IslandSum {
 
 struct SegIds {
     TileGeom g;
-    const unsigned long long* mask;  // roots per segment: [0, nseg) positive, [nseg, 2 nseg) negative
+    const unsigned long long* mask;  // roots per segment: [0, nseg) positive, [nseg, 2 nseg) negative;
+                                     // then [2 nseg, 3 nseg): the foreground pixels of each segment
     const uint32_t* offs;            // exclusive scan of the popcounts of mask, 2 nseg + 1 entries
 };
 
@@ -1394,6 +1395,7 @@ __global__ __launch_bounds__(CT_THREADS) void ccl_tile(const T* __restrict__ x, 
         const unsigned seg = seg_index(g, cv0 + lane, ty);
         rmask[seg] = s_rmp[lane];
         rmask[g.nseg + seg] = s_rmn[lane];
+        rmask[2 * size_t(g.nseg) + seg] = s_mask[lane];  // foreground of either sign (for the label map)
     }
     const uint32_t pool_base = s_base;
     const bool ok = total <= rec.rt || size_t(pool_base) + size_t(total - rec.rt) <= size_t(rec.pool_cap);
@@ -1684,11 +1686,19 @@ __global__ __launch_bounds__(LM_THREADS) void ccl_label_map(const uint32_t* __re
     const uint32_t kp = s.offs[g.nseg];
     // Each phase only issues loads into registers (none waits on another of its
     // phase): parents, roots of open components, segment masks and offsets, extrema.
+    // a pixel's parent is read only where the tile labelling found foreground (the
+    // segment's mask, one uniform load per column): background needs no load
     uint32_t pv[LM_PPT], rv[LM_PPT];
+    unsigned long long fm[LM_PPT];
 #pragma unroll
     for (int j = 0; j < LM_PPT; j++) {
         const int c = w + LM_WAVES * j;
-        pv[j] = c < nc ? P[uint32_t(cv0 + c) * uint32_t(g.R) + uint32_t(rbase + lane)] : P_BG;
+        fm[j] = c < nc ? s.mask[2 * size_t(g.nseg) + seg_index(g, cv0 + c, ty)] : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < LM_PPT; j++) {
+        const int c = w + LM_WAVES * j;
+        pv[j] = ((fm[j] >> lane) & 1ull) ? P[uint32_t(cv0 + c) * uint32_t(g.R) + uint32_t(rbase + lane)] : P_BG;
     }
 #pragma unroll
     for (int j = 0; j < LM_PPT; j++)
@@ -2191,7 +2201,7 @@ void source_find_impl(stp_ctx* ctx, const T* x, int rows, int cols, const stp_so
     const unsigned ntiles = unsigned(tg.ntx) * unsigned(tg.nty);
     const size_t nseg2 = 2 * size_t(tg.nseg);
     uint32_t* P = wsp<uint32_t>(ctx, "ccl_parent", n);
-    unsigned long long* rmask = wsp<unsigned long long>(ctx, "ccl_rmask", nseg2);
+    unsigned long long* rmask = wsp<unsigned long long>(ctx, "ccl_rmask", nseg2 + tg.nseg);
     uint32_t* segoff = wsp<uint32_t>(ctx, "ccl_segoff", nseg2 + 1);
     const int find_neg = int(p.find_negative_sources != 0), conn4 = int(p.ccl_4connectivity != 0);
     // component records: a region of CT_RT per tile, then a pool for tiles with more
