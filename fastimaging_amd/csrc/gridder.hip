@@ -1065,6 +1065,26 @@ __global__ __launch_bounds__(64) void rs_scatter_kernel(const unsigned long long
     }
 }
 
+// The inputs in |w| order, gathered once: uv (u, v, w columns), vis and weights of
+// sorted position k from original position perm[k], so the later passes read them
+// in sequence instead of gathering each time.
+__global__ void wpermute_kernel(const double* uvw, const double* vis, const double* wts, const uint32_t* perm, long long n,
+    double* uvw_s, double* vis_s, double* wts_s)
+{
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n)
+        return;
+    const long long i = perm[k];
+    const double u = uvw[i], v = uvw[n + i], w = uvw[2 * n + i];
+    const double vr = vis[2 * i], vi = vis[2 * i + 1], wt = wts[i];
+    uvw_s[k] = u;
+    uvw_s[n + k] = v;
+    uvw_s[2 * n + k] = w;
+    vis_s[2 * k] = vr;
+    vis_s[2 * k + 1] = vi;
+    wts_s[k] = wt;
+}
+
 __global__ void wgood_kernel(GridGeom g, const double* uvw, const double* vis, const double* wts, uint32_t* flag)
 {
     const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -2005,7 +2025,17 @@ void grid_visibilities(stp_ctx* ctx, const GridderArgs& a, size_t n, const doubl
             std::swap(k0, k1);
             std::swap(i0, i1);
         }
-        g.perm = i0;
+        {
+            double* uvw_s = (double*)ws_get(ctx, "w_uvw_sorted", sizeof(double) * 3 * n);
+            double* vis_s = (double*)ws_get(ctx, "w_vis_sorted", sizeof(double) * 2 * n);
+            double* wts_s = (double*)ws_get(ctx, "w_wts_sorted", sizeof(double) * n);
+            launch(ctx, "w_permute", wpermute_kernel, dim3(unsigned(div_up(n, 256))), dim3(256), 0, d_uv, d_vis, d_w,
+                (const uint32_t*)i0, (long long)n, uvw_s, vis_s, wts_s);
+            d_uv = uvw_s;
+            d_vis = vis_s;
+            d_w = wts_s;
+            g.perm = nullptr;  // position k of the pass is sorted position k
+        }
         // 2. planes
         uint32_t* flag = (uint32_t*)ws_get(ctx, "w_flag", sizeof(uint32_t) * n);
         uint32_t* G = (uint32_t*)ws_get(ctx, "w_gscan", sizeof(uint32_t) * (n + 1));
