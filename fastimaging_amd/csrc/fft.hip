@@ -137,6 +137,32 @@ __device__ __forceinline__ void bfly4(cplx<T>* v, bool inverse)
     v[3] = csub(a1, a3);
 }
 
+// Radix-8 DFT, y_k = sum_r v_r w^{rk}, w = exp(+-i pi/4): y_{2k} is the radix-4 DFT
+// of a_r = v_r + v_{r+4}, y_{2k+1} that of b_r = (v_r - v_{r+4}) w^r (r < 4).
+template <typename T>
+__device__ __forceinline__ void bfly8(cplx<T>* v, bool inverse)
+{
+    const T h = T(0.70710678118654752440084436210484904L);  // cos(pi/4)
+    cplx<T> a[4], b[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        a[r] = cadd(v[r], v[r + 4]);
+        b[r] = csub(v[r], v[r + 4]);
+    }
+    // w = (h, +-h), w^2 = +-i, w^3 = (-h, +-h)
+    const cplx<T> d1 = b[1], d2 = b[2], d3 = b[3];
+    b[1] = inverse ? cplx<T>{ h * (d1.x - d1.y), h * (d1.y + d1.x) } : cplx<T>{ h * (d1.x + d1.y), h * (d1.y - d1.x) };
+    b[2] = inverse ? cplx<T>{ -d2.y, d2.x } : cplx<T>{ d2.y, -d2.x };
+    b[3] = inverse ? cplx<T>{ -h * (d3.x + d3.y), h * (d3.x - d3.y) } : cplx<T>{ h * (d3.y - d3.x), -h * (d3.x + d3.y) };
+    bfly4(a, inverse);
+    bfly4(b, inverse);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        v[2 * k] = a[k];
+        v[2 * k + 1] = b[k];
+    }
+}
+
 // One Stockham stage of radix R on a length-L transform held at buf.
 template <typename T, int L, int TPT, int R, typename TW>
 __device__ __forceinline__ void stockham_stage(cplx<T>* buf, int lt, int Ns, const TW& tw, bool inverse)
@@ -165,7 +191,9 @@ __device__ __forceinline__ void stockham_stage(cplx<T>* buf, int lt, int Ns, con
                 for (int r = 1; r < R; r++)
                     v[b][r] = cmul(v[b][r], tw(r * k * step));
             }
-            if (R == 4)
+            if constexpr (R == 8)
+                bfly8(v[b], inverse);
+            else if constexpr (R == 4)
                 bfly4(v[b], inverse);
             else
                 bfly2(v[b]);
@@ -179,19 +207,24 @@ __device__ __forceinline__ void stockham_stage(cplx<T>* buf, int lt, int Ns, con
 }
 
 // In-place unnormalised FFT of length L at buf, TPT cooperating threads (lt in [0, TPT)).
-// tw: a twiddle source (exponents below 3L/4); inverse selects the + sign.
+// tw: a twiddle source (exponents below 7L/8); inverse selects the + sign.
 template <typename T, int L, int TPT, typename TW>
 __device__ __forceinline__ void lds_fft(cplx<T>* buf, int lt, const TW& tw, bool inverse)
 {
+    // radix-8 stages after one radix-2 or radix-4 stage for the remainder of log2 L
+    // mod 3 (fewer passes through LDS and barriers than radix 4 alone)
     constexpr int LOG = Log2<L>::value;
     int Ns = 1;
-    if (LOG & 1) {
+    if constexpr (LOG % 3 == 1) {
         stockham_stage<T, L, TPT, 2>(buf, lt, Ns, tw, inverse);
         Ns = 2;
+    } else if constexpr (LOG % 3 == 2) {
+        stockham_stage<T, L, TPT, 4>(buf, lt, Ns, tw, inverse);
+        Ns = 4;
     }
-    if constexpr (L >= 4) {
-        for (; Ns < L; Ns *= 4)
-            stockham_stage<T, L, TPT, 4>(buf, lt, Ns, tw, inverse);
+    if constexpr (L >= 8) {
+        for (; Ns < L; Ns *= 8)
+            stockham_stage<T, L, TPT, 8>(buf, lt, Ns, tw, inverse);
     }
 }
 
