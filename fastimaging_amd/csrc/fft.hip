@@ -348,10 +348,10 @@ __global__ __launch_bounds__(WG) void fft_v_c2r(const cplx<T>* grids, size_t gri
     constexpr int rows = M + 1;
     constexpr int QN = (n / 4 > 0) ? n / 4 : 1;  // quarter wave: r < n/4
     constexpr int QLOG = Log2<QN>::value;
-    // FP32: two small tables (a complex multiply per twiddle; the smaller LDS footprint
-    // lets more workgroups share a CU: 80 -> 74 us at c2); FP64: the quarter table,
-    // except at n = 16384, whose 128 KB column leaves no room for it
-    constexpr bool SPLIT = std::is_same<T, float>::value || M >= 8192;
+    // two small tables (a complex multiply per twiddle) where the smaller LDS footprint
+    // lets more workgroups share a CU (FP32: 80 -> 74 us at c2; FP64 from n = 2048: 68 ->
+    // 58 us at c2); the quarter table for short FP64 columns
+    constexpr bool SPLIT = std::is_same<T, float>::value || M >= 1024;
     constexpr int BLOG = QLOG < 5 ? QLOG : 5;
     constexpr int NTW = SPLIT ? (1 << BLOG) + (QN >> BLOG) : QN;
     constexpr int KI = (M / 2) / S::TPT + 1;      // pack pairs (k, M - k) per thread
@@ -494,6 +494,8 @@ __global__ __launch_bounds__(2 * WG) void fft_v_c2r_real(const cplx<T>* grids, s
     constexpr int rows = M + 1;
     constexpr int QN = (n / 4 > 0) ? n / 4 : 1;
     constexpr int QLOG = Log2<QN>::value;
+    // FP64 keeps the quarter table here up to n = 8192 (the split tables took this pass
+    // from 49 to 54 us at c2: its occupancy is bound by registers, not LDS)
     constexpr bool SPLIT = std::is_same<T, float>::value || M >= 8192;
     constexpr int BLOG = QLOG < 5 ? QLOG : 5;
     constexpr int NTW = SPLIT ? (1 << BLOG) + (QN >> BLOG) : QN;
