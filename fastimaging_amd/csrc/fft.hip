@@ -9,8 +9,12 @@
 //            of length n/2 (even/odd packing); the imaginary parts of the v=0
 //            and v=n/2 rows are ignored, as FFTW's c2r does.  Normalisation,
 //            padding crop and grid correction of the imager are fused here.
-// Each transform runs in LDS with 64-wide waves; a workgroup holds NT
-// transforms of length L, TPT = 256/NT threads per transform.
+// A real grid (the imager's sampling grid without W-projection) is stored packed,
+// two real columns per complex one: U1, U2 run at half the length, and the V pass
+// splits each column pair into the two u-transformed columns first (fft_v_c2r_real).
+// Each transform runs in LDS with 64-wide waves (radix-8 Stockham stages after
+// one radix-2 or radix-4 stage); a workgroup holds NT transforms of length L,
+// TPT = 256/NT threads per transform.
 #include <type_traits>
 
 #include "common.h"
