@@ -191,9 +191,23 @@ __device__ __forceinline__ void stockham_stage(cplx<T>* buf, int lt, int Ns, con
             const int k = j & (Ns - 1);
             if (Ns > 1) {
                 const int step = L / (Ns * R);
+                if constexpr (R == 8) {
+                    // three table twiddles, the other four as their products (a few
+                    // ulp, fewer LDS loads)
+                    const cplx<T> w1 = tw(k * step), w2 = tw(2 * k * step), w4 = tw(4 * k * step);
+                    const cplx<T> w3 = cmul(w1, w2);
+                    v[b][1] = cmul(v[b][1], w1);
+                    v[b][2] = cmul(v[b][2], w2);
+                    v[b][3] = cmul(v[b][3], w3);
+                    v[b][4] = cmul(v[b][4], w4);
+                    v[b][5] = cmul(v[b][5], cmul(w4, w1));
+                    v[b][6] = cmul(v[b][6], cmul(w4, w2));
+                    v[b][7] = cmul(v[b][7], cmul(w4, w3));
+                } else {
 #pragma unroll
-                for (int r = 1; r < R; r++)
-                    v[b][r] = cmul(v[b][r], tw(r * k * step));
+                    for (int r = 1; r < R; r++)
+                        v[b][r] = cmul(v[b][r], tw(r * k * step));
+                }
             }
             if constexpr (R == 8)
                 bfly8(v[b], inverse);
